@@ -1,0 +1,481 @@
+"""FP64 CPU oracle: a numpy restatement of PCAone's randomized-SVD path.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module, and only as the checker.  The
+product (pcaone_amd) never calls it.
+
+Every function cites the reference source it restates (PCAone v0.7.2, paths
+relative to the reference root).  Linear algebra: numpy/LAPACK FP64.  LAPACK's
+Householder QR (dgeqrf/dorgqr) uses the same reflector sign convention as
+Eigen's HouseholderQR (beta = -sign(alpha)*||x||), so thin-Q factors match the
+reference's up to rounding; the SVD of B may differ from Eigen's JacobiSVD by
+column signs, which the reference's own outputs never fix on the plain path
+(outputs are compared sign-aligned).
+
+Pinning (see tests/test_oracle.py):
+  decode / allele frequency / centering  - 256-byte KAT + brute force + golden .mbim
+  standardize_E                          - closed form
+  omega / shuffle                        - libstdc++ calls (oracle/rng.cpp); pinned by the
+                                           --maxp 0 golden run (U after one pass from Omega)
+  NormalOp / FancyOp / compute_usv       - golden outputs of the reference binary
+                                           (tests/golden/*), dense SVD at convergence
+  flip_omg / mev / flip_uv               - closed form + brute force
+  fit_with_pi (EMU, incl. perm quirk)    - golden EMU outputs (-d 1 and -d 2)
+The out-of-core block plan (src/Data.cpp:42-84) is not restated here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+BED2GENO = np.array([1.0, -9.0, 0.5, 0.0])  # src/Common.hpp:58-59
+VAR_TOL = 1e-9  # src/Data.hpp:7
+
+_rng = None
+
+
+def _rnglib():
+    global _rng
+    if _rng is None:
+        path = HERE / "_build" / "liboracle_rng.so"
+        if not path.exists():
+            raise ImportError(f"{path} missing; run python oracle/build_oracle.py")
+        _rng = C.CDLL(str(path))
+        _rng.oracle_omega.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, C.c_void_p]
+        _rng.oracle_shuffle.argtypes = [C.c_long, C.c_void_p]
+    return _rng
+
+
+def omega(seed: int, gaussian: bool, n: int, l: int) -> np.ndarray:
+    """src/Halko.cpp:15-23, src/RSVD.hpp:25-59: N x L, column-major fill."""
+    out = np.empty(n * l, dtype=np.float64)
+    _rnglib().oracle_omega(int(seed), 1 if gaussian else 0, n, l, out.ctypes.data)
+    return out.reshape(l, n).T.copy()
+
+
+def shuffle_indices(m: int) -> np.ndarray:
+    """src/RSVD.hpp:61-70: permutation indices (permuted column i = original column idx[i])."""
+    out = np.empty(m, dtype=np.int32)
+    _rnglib().oracle_shuffle(m, out.ctypes.data)
+    return out
+
+
+# ---------------------------------------------------------------- PLINK decode
+def read_plink(prefix: str):
+    """src/FilePlink.hpp:10-30: N = lines(.fam), M = lines(.bim), magic 6c 1b 01."""
+    prefix = str(prefix)
+    with open(prefix + ".fam") as f:
+        n = sum(1 for _ in f)
+    with open(prefix + ".bim") as f:
+        m = sum(1 for _ in f)
+    raw = np.fromfile(prefix + ".bed", dtype=np.uint8)
+    if raw.size < 3 or raw[0] != 0x6C or raw[1] != 0x1B or raw[2] != 0x01:
+        raise ValueError("Incorrect magic number in plink bed file.")
+    bps = (n + 3) >> 2
+    packed = raw[3:3 + bps * m].reshape(m, bps)
+    return packed, n, m
+
+
+def decode_codes(packed: np.ndarray, n: int) -> np.ndarray:
+    """2-bit codes, M x N (sample j at bits 2*(j%4) of byte j/4; src/FilePlink.cpp:74-83)."""
+    shifts = np.array([0, 2, 4, 6], dtype=np.uint8)
+    codes = (packed[:, :, None] >> shifts[None, None, :]) & 3
+    return codes.reshape(packed.shape[0], -1)[:, :n]
+
+
+def allele_freq(codes: np.ndarray) -> np.ndarray:
+    """src/FilePlink.cpp:37-55: mean of BED2GENO over non-missing; 0 if none observed."""
+    vals = BED2GENO[codes]
+    obs = codes != 1
+    cnt = obs.sum(axis=1)
+    s = np.where(obs, vals, 0.0).sum(axis=1)
+    return np.where(cnt > 0, s / np.maximum(cnt, 1), 0.0)
+
+
+@dataclass
+class Params:
+    """Subset of src/Cmd.hpp Param used on the path, with src/Cmd.cpp:171-238 rules."""
+    svd: int = 2
+    k: int = 10
+    oversamples: int = 10
+    maxp: int = 20
+    tol: float = 1e-4
+    seed: int = 112
+    rand: int = 1
+    bands: int = 64
+    noshuffle: bool = False
+    emu: bool = False
+    maxiter: int = 100
+    tolem: float = 1e-5
+    scale: int = -9
+    ploidy: int = 2
+    maf: float = 0.0
+
+    def __post_init__(self):
+        self.oversamples = max(self.oversamples, self.k)  # src/Cmd.cpp:216
+        if not self.emu:
+            self.maxiter = 0  # src/Cmd.cpp:230-232
+        if self.bands < 4 or self.bands % 2 != 0:  # src/Cmd.cpp:235-236
+            raise ValueError("the -w/--batches must be a power of 2 and the minimun is 4.")
+        self.perm = self.svd == 2 and not self.noshuffle  # src/Cmd.cpp:238
+        self.missme = self.emu
+
+
+@dataclass
+class Data:
+    """In-core Data for a PLINK input (src/Data.hpp, src/FilePlink.cpp:26-120)."""
+    G: np.ndarray  # N x M centered (missing -> 0)
+    F: np.ndarray
+    C: np.ndarray | None  # M x N missing mask (missme only)
+    params: Params
+    perm: np.ndarray | None = None
+    p_miss: float = 0.0
+    keep: np.ndarray | None = None
+
+    @property
+    def nsamples(self):
+        return self.G.shape[0]
+
+    @property
+    def nsnps(self):
+        return self.G.shape[1]
+
+    @classmethod
+    def from_packed(cls, packed: np.ndarray, n: int, params: Params) -> "Data":
+        codes = decode_codes(packed, n)
+        F = allele_freq(codes)
+        keep = None
+        if params.maf > 0:  # src/Data.cpp:88-105 (MAF(a) = a > 0.5 ? 1-a : a, keep if MAF > maf)
+            maf = np.where(F > 0.5, 1 - F, F)
+            keep = np.nonzero(maf > params.maf)[0]
+            codes = codes[keep]
+            F = F[keep]
+        vals = BED2GENO[codes]
+        G = np.where(codes == 1, 0.0, vals - F[:, None]).T.copy()  # src/FilePlink.cpp:102-109
+        C = (codes == 1) if params.missme else None
+        p_miss = float(C.mean()) if C is not None else 0.0
+        return cls(G=G, F=F, C=C, params=params, p_miss=p_miss, keep=keep)
+
+    def standardize_E(self):
+        """src/Data.cpp:351-362."""
+        if self.params.scale != -9:
+            return
+        sd = np.sqrt(self.F * (1.0 - self.F))
+        fac = np.where(sd > VAR_TOL, math.sqrt(self.params.ploidy) / np.where(sd > VAR_TOL, sd, 1.0), 1.0)
+        self.G *= fac[None, :]
+
+    def fit_with_pi(self, U, S, VT):
+        """src/Data.cpp:318-348 (EMU).  With the in-core shuffle the mask is read
+        through perm (src/Data.cpp:322) while the clamp uses F(i) unpermuted (:328)."""
+        if not self.params.emu:
+            return
+        pred = (U * S[None, :]) @ VT  # N x M
+        if self.params.perm:
+            mask = self.C[self.perm].T
+        else:
+            mask = self.C.T
+        lo = -self.F[None, :]
+        hi = 1.0 - self.F[None, :]
+        imp = np.minimum(np.maximum(pred, lo), hi)  # fmin(fmax(x, -F), 1-F)
+        self.G = np.where(mask, imp, self.G)
+
+
+# ---------------------------------------------------------------- helpers
+def thin_q(X: np.ndarray) -> np.ndarray:
+    """HouseholderQR(X).householderQ() * Identity(rows, cols) (src/Halko.cpp:121-122)."""
+    return np.linalg.qr(X, mode="reduced")[0]
+
+
+def flip_omg(omg2: np.ndarray, omg: np.ndarray) -> np.ndarray:
+    """src/RSVD.hpp:80-89; returns the new Omg (Omg2 := Omg is the caller's job)."""
+    a = np.abs(omg2 - omg).sum(axis=0)
+    b = np.abs(omg2 + omg).sum(axis=0)
+    sgn = np.where(a > 2.0 * b, -1.0, 1.0)
+    return omg * sgn[None, :]
+
+
+def mev(X: np.ndarray, Y: np.ndarray) -> float:
+    """src/Utils.cpp:194-200."""
+    return float(np.linalg.norm(X.T @ Y, axis=0).sum() / X.shape[1])
+
+
+def flip_uv(U: np.ndarray, V: np.ndarray, ubase: bool = False):
+    """src/Utils.cpp:118-154 (column layout: U N x k, V M x k)."""
+    U = U.copy()
+    V = V.copy()
+    base = U if ubase else V
+    for i in range(base.shape[1]):
+        x = int(np.argmax(np.abs(base[:, i])))  # first max, as Eigen maxCoeff
+        if base[x, i] < 0:
+            U[:, i] *= -1
+            V[:, i] *= -1
+    return U, V
+
+
+# ---------------------------------------------------------------- operators
+class RsvdOp:
+    """RsvdOpData (src/Halko.hpp:6-42, src/Halko.cpp:15-97)."""
+
+    def __init__(self, data: Data, k: int, os: int):
+        self.data = data
+        self.k = k
+        self.os = os
+        self.update = False
+        self.standardize = False
+        self.U = self.V = self.S = None
+        self.trace: list[float] = []  # diff of every checked epoch
+        self.epochs = 0
+        self.init_omg()
+
+    @property
+    def L(self):
+        return self.k + self.os
+
+    def init_omg(self):
+        p = self.data.params
+        self.Omg = omega(p.seed, p.rand != 0, self.data.nsamples, self.L)
+        self.Omg2 = self.Omg.copy()
+
+    def set_flags(self, update, standardize):
+        self.update, self.standardize = update, standardize
+
+    def compute_gh(self, G, H, pi):
+        raise NotImplementedError
+
+    def compute_usv(self, p: int, tol: float):
+        """src/Halko.cpp:46-97."""
+        params = self.data.params
+        k = self.k
+        G = np.zeros((self.data.nsnps, self.L))
+        H = np.zeros((self.data.nsamples, self.L))
+        Upre = Ucur = None
+        pi = 0
+        while pi <= p:
+            G, H = self.compute_gh(G, H, pi)
+            Q1, R1 = np.linalg.qr(G, mode="reduced")
+            Q2, R2 = np.linalg.qr(Q1, mode="reduced")
+            R = R2 @ R1
+            G = Q2
+            B = np.linalg.solve(R.T, H.T)
+            UB, s, VBt = np.linalg.svd(B, full_matrices=False)
+            Ucur = VBt.T[:, :k]
+            self.epochs = pi + 1
+            if pi > 0:
+                diff = 1.0 - mev(Ucur, Upre)
+                self.trace.append(diff)
+                if diff < tol or pi == p:
+                    if params.svd == 2 and 2.0 ** pi < params.bands:
+                        p = int(math.log2(params.bands))
+                    else:
+                        self.U = Ucur
+                        self.V = G @ UB[:, :k]
+                        self.S = s[:k]
+                        break
+                else:
+                    Upre = Ucur
+            else:
+                Upre = Ucur
+            pi += 1
+        self.U = Ucur
+
+
+class NormalOp(RsvdOp):
+    """NormalRsvdOpData::computeGandH in-core (src/Halko.cpp:99-127)."""
+
+    def compute_gh(self, G, H, pi):
+        d = self.data
+        if pi == 0:
+            self.init_omg()
+            if self.update:
+                d.fit_with_pi(self.U, self.S, self.V.T)
+            if self.standardize:
+                d.standardize_E()
+        if pi > 0:
+            self.Omg = flip_omg(self.Omg2, thin_q(H))
+            self.Omg2 = self.Omg.copy()
+        G = d.G.T @ self.Omg
+        H = d.G @ G
+        return G, H
+
+
+class FancyOp(RsvdOp):
+    """FancyRsvdOpData::computeGandH in-core (src/Halko.cpp:155-224).  Windows that
+    start past the last SNP (possible when M < bands^2) are empty here; the
+    reference computes a negative block size for them."""
+
+    def __init__(self, data, k, os):
+        super().__init__(data, k, os)
+        self.H1 = np.zeros((data.nsamples, self.L))
+        self.H2 = np.zeros((data.nsamples, self.L))
+        self.bandsize = 1
+        self.blocksize = 0
+        self.omega_updates = 0
+
+    def compute_gh(self, G, H, pi):
+        d = self.data
+        bands = d.params.bands
+        if pi == 0:
+            self.init_omg()
+        if 2.0 ** pi >= bands:
+            self.H1[:] = 0
+            self.H2[:] = 0
+        if pi == 0:
+            if self.update:
+                d.fit_with_pi(self.U, self.S, self.V.T)
+            if self.standardize:
+                d.standardize_E()
+            self.bandsize = 1
+            self.blocksize = int(math.ceil(d.nsnps / bands))
+            if d.params.perm and not (self.update and d.params.emu):
+                d.perm = shuffle_indices(d.nsnps)
+                d.G = d.G[:, d.perm].copy()
+        self.bandsize = min(self.bandsize * 2, bands)
+        bs = self.bandsize
+        i = 1
+        for b in range(bands):
+            start = b * self.blocksize
+            stop = d.nsnps - 1 if (b + 1) * self.blocksize >= d.nsnps else (b + 1) * self.blocksize - 1
+            if start <= stop:
+                blk = slice(start, stop + 1)
+                G[blk] = d.G[:, blk].T @ self.Omg
+                if i <= bs // 2:
+                    self.H1 += d.G[:, blk] @ G[blk]
+                else:
+                    self.H2 += d.G[:, blk] @ G[blk]
+            adjacent = pi > 0 and (b + 1) == 2 ** (pi - 1) and 2.0 ** pi < bands
+            if (b + 1) < bs and not adjacent:
+                i += 1
+                continue
+            if not (i == bs or i == bs // 2 or adjacent):
+                i += 1
+                continue
+            H = self.H1 + self.H2
+            self.Omg = flip_omg(self.Omg2, thin_q(H))
+            self.Omg2 = self.Omg.copy()
+            self.omega_updates += 1
+            if i == bs:
+                self.H1[:] = 0
+                i = 0
+            else:
+                self.H2[:] = 0
+            i += 1
+        return G, H
+
+
+@dataclass
+class Result:
+    U: np.ndarray
+    S: np.ndarray
+    V: np.ndarray
+    E: np.ndarray
+    epochs: int
+    trace: list = field(default_factory=list)
+    em_trace: list = field(default_factory=list)
+    perm: np.ndarray | None = None
+
+
+def run_pca_with_halko(data: Data) -> Result:
+    """src/Halko.cpp:271-345 (in-core, genetic input, no LD)."""
+    p = data.params
+    op = FancyOp(data, p.k, p.oversamples) if p.svd == 2 else NormalOp(data, p.k, p.oversamples)
+    em_trace = []
+    if not p.missme:
+        op.set_flags(False, True)
+        op.compute_usv(p.maxp, p.tol)
+    else:
+        op.set_flags(False, False)
+        op.compute_usv(p.maxp, p.tol)
+        op.U, op.V = flip_uv(op.U, op.V, False)
+        for _ in range(p.maxiter):
+            op.set_flags(True, False)
+            Vpre = op.V
+            op.compute_usv(p.maxp, p.tol)
+            op.U, op.V = flip_uv(op.U, op.V, False)
+            diff = 1.0 - mev(op.V, Vpre)
+            em_trace.append(diff)
+            if diff < p.tolem:
+                break
+        if p.emu:
+            op.set_flags(True, True)
+            op.compute_usv(p.maxp, p.tol)
+            op.U, op.V = flip_uv(op.U, op.V, False)
+    S = op.S if op.S is not None else np.zeros(0)
+    return Result(U=op.U, S=S, V=op.V if op.V is not None else np.zeros((0, 0)), E=S ** 2 / data.nsnps,
+                  epochs=op.epochs, trace=op.trace, em_trace=em_trace, perm=data.perm)
+
+
+def run_plink(prefix: str, params: Params) -> Result:
+    packed, n, _ = read_plink(prefix)
+    return run_pca_with_halko(Data.from_packed(packed, n, params))
+
+
+# ---------------------------------------------------------------- outputs
+def fmt6(x: float) -> str:
+    """Eigen IOFormat(6, ...) element: ostream with precision 6 (src/Data.cpp:215)."""
+    return f"{x:.6g}"
+
+
+def write_eigs_files(res: Result, data: Data, out: str, printv: bool = False) -> None:
+    """src/Data.cpp:211-240 (.sigvals/.eigvals/.eigvecs[/.loadings])."""
+    with open(out + ".sigvals", "w") as f:
+        f.write(f"#{res.U.shape[0]},{res.V.shape[0]}\n")
+        f.write("\n".join(fmt6(v) for v in res.S) + "\n")
+    with open(out + ".eigvals", "w") as f:
+        f.write("\n".join(fmt6(v) for v in res.E) + "\n")
+    with open(out + ".eigvecs", "w") as f:
+        f.write("\n".join("\t".join(fmt6(v) for v in row) for row in res.U) + "\n")
+    if printv:
+        V = res.V
+        if data.params.perm and data.perm is not None:
+            V = np.empty_like(res.V)
+            V[data.perm] = res.V  # original SNP perm[i] <- permuted row i
+        with open(out + ".loadings", "w") as f:
+            f.write("\n".join("\t".join(fmt6(v) for v in row) for row in V) + "\n")
+
+
+# ---------------------------------------------------------------- exact oracle
+def dense_pca(data: Data, k: int):
+    """Exact top-k PCs of the standardized matrix (the reference's --svd 3
+    quantity, src/Main.cpp:180-217): U (N x k), S, E = S^2/M, V (M x k)."""
+    U, s, Vt = np.linalg.svd(data.G, full_matrices=False)
+    return U[:, :k], s[:k], s[:k] ** 2 / data.nsnps, Vt[:k].T
+
+
+# ---------------------------------------------------------------- comparison metrics
+def sign_align(ref: np.ndarray, x: np.ndarray) -> np.ndarray:
+    sgn = np.sign(np.sum(ref * x, axis=0))
+    sgn[sgn == 0] = 1
+    return x * sgn[None, :]
+
+
+def pc_correlations(ref: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """|Pearson correlation| of each PC column pair."""
+    out = []
+    for i in range(ref.shape[1]):
+        a = ref[:, i] - ref[:, i].mean()
+        b = x[:, i] - x[:, i].mean()
+        out.append(abs(float(a @ b) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)))
+    return np.array(out)
+
+
+def principal_angles(ref: np.ndarray, x: np.ndarray) -> np.ndarray:
+    qa = np.linalg.qr(ref)[0]
+    qb = np.linalg.qr(x)[0]
+    s = np.linalg.svd(qa.T @ qb, compute_uv=False)
+    return np.arccos(np.clip(s, -1.0, 1.0))
+
+
+def read_matrix(path: str) -> np.ndarray:
+    rows = []
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line.startswith("#"):
+                continue
+            rows.append([float(t) for t in line.split("\t")])
+    return np.array(rows)

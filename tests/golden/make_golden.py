@@ -1,0 +1,121 @@
+"""Regenerates the golden fixtures under tests/golden/ from the reference binary.
+
+Run in the build container (the reference sources are needed to build
+oracle/_ref/pcaone_ref):
+
+    python oracle/build_oracle.py && python tests/golden/make_golden.py
+
+Inputs are synthetic Balding-Nichols PLINK sets.  The small ones are committed
+under tests/golden/data/; the 2,000 x 20,000 one is regenerated from its spec
+and checked against the recorded SHA-256.  For every case the reference's
+own output files (.eigvals/.sigvals/.eigvecs[/.loadings/.mbim]) are copied
+and meta.json records the command, the epoch count and the per-epoch
+convergence trace parsed from the log.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+
+from pcaone_amd import synth  # noqa: E402
+
+REF = ROOT / "oracle" / "_ref" / "pcaone_ref"
+DATA = HERE / "data"
+
+INPUTS = {
+    "f1": dict(spec=synth.Spec(seed=2001, nsamples=200, nsnps=2000, npops=3), commit=True),
+    "f4": dict(spec=synth.Spec(seed=2004, nsamples=300, nsnps=3000, npops=3, miss_rate=0.1), commit=True),
+    "cfg1": dict(spec=synth.config(1), commit=False),
+}
+
+CASES = {
+    "f1_d1": ("f1", "-d 1 -k 5 --maxp 7 -V"),
+    "f1_d1_p0": ("f1", "-d 1 -k 5 --maxp 0"),
+    "f1_d1_unif": ("f1", "-d 1 -k 5 --maxp 3 --rand 0 --seed 7 --tol-rsvd 0"),
+    "f1_d1_maf": ("f1", "-d 1 -k 4 --maf 0.1 --maxp 5 -V"),
+    "f1_d2": ("f1", "-d 2 -w 16 -k 5 --maxp 7 -V"),
+    "f1_d2_noshuf": ("f1", "-d 2 -w 16 -k 5 --maxp 7 -V -S"),
+    "f4_d1_miss": ("f4", "-d 1 -k 3 --maxp 5 -V"),
+    "f4_emu_d1": ("f4", "-d 1 -k 3 --emu --maxiter 5 -V"),
+    "f4_emu_d2": ("f4", "-d 2 -w 16 -k 3 --emu --maxiter 5 -V"),
+    "cfg1_d1": ("cfg1", "-d 1 -k 10 --oversamples 10 --maxp 7 --tol-rsvd 0"),
+    "cfg1_d2": ("cfg1", "-d 2 -w 64 -k 10 --maxp 7 --tol-rsvd 0"),
+}
+
+EPOCH_RE = re.compile(r"running of epoch =\s*(\d+), diff = ([-0-9.e+]+)")
+STOP_RE = re.compile(r"stops at epoch =\s*(\d+)")
+EM_RE = re.compile(r"estimated iter =\s*(\d+), diff = ([-0-9.e+]+)")
+
+
+def sha256(path: Path) -> str:
+    return hashlib.sha256(path.read_bytes()).hexdigest()
+
+
+def make_inputs(tmp: Path) -> dict:
+    DATA.mkdir(exist_ok=True)
+    out = {}
+    for name, d in INPUTS.items():
+        spec = d["spec"]
+        prefix = (DATA if d["commit"] else tmp) / name
+        synth.write_plink(spec, str(prefix))
+        out[name] = dict(prefix=str(prefix), sha256=sha256(Path(str(prefix) + ".bed")), spec=spec.__dict__,
+                         committed=d["commit"])
+    return out
+
+
+def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
+    dest = HERE / name
+    if dest.exists():
+        shutil.rmtree(dest)
+    dest.mkdir()
+    outp = tmp / name
+    cmd = [str(REF), "-b", inp["prefix"], *args.split(), "-o", str(outp), "-v", "1"]
+    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    res = subprocess.run(cmd, capture_output=True, text=True, env=env, check=True)
+    log = res.stdout
+    trace = [float(m.group(2)) for m in EPOCH_RE.finditer(log)]
+    stop = STOP_RE.findall(log)
+    em = [float(m.group(2)) for m in EM_RE.finditer(log)]
+    for suf in (".eigvals", ".sigvals", ".eigvecs", ".loadings", ".mbim"):
+        src = Path(str(outp) + suf)
+        if src.exists():
+            shutil.copy(src, dest / suf[1:])
+    meta = dict(input=inp_name_of(inp), args=args, trace=trace, stop_epochs=[int(s) for s in stop], em_trace=em)
+    (dest / "meta.json").write_text(json.dumps(meta, indent=1) + "\n")
+    print(f"{name}: {len(trace)} checked epochs, stops {stop}, em {len(em)}")
+
+
+_names = {}
+
+
+def inp_name_of(inp):
+    return _names[inp["prefix"]]
+
+
+def main() -> None:
+    if not REF.exists():
+        raise SystemExit(f"{REF} missing: python oracle/build_oracle.py")
+    with tempfile.TemporaryDirectory() as td:
+        tmp = Path(td)
+        inputs = make_inputs(tmp)
+        for n, inp in inputs.items():
+            _names[inp["prefix"]] = n
+        manifest = {n: dict(sha256=i["sha256"], spec=i["spec"], committed=i["committed"]) for n, i in inputs.items()}
+        (HERE / "inputs.json").write_text(json.dumps(manifest, indent=1) + "\n")
+        for name, (inp, args) in CASES.items():
+            run_case(name, inputs[inp], args, tmp)
+
+
+if __name__ == "__main__":
+    main()

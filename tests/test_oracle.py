@@ -1,0 +1,169 @@
+"""Pins the FP64 oracle (oracle/pcaone_oracle.py) before anything is checked against it."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_cases, load_meta, params_from_args
+from oracle import pcaone_oracle as O
+
+
+# --------------------------------------------------------------- decode KATs
+@pytest.mark.parametrize("n", [1021, 1022, 1023, 1024])
+def test_decode_all_byte_values(n):
+    row = np.arange(256, dtype=np.uint8)
+    packed = row[None, :]
+    codes = O.decode_codes(packed, n)
+    assert codes.shape == (1, n)
+    for j in range(n):
+        assert codes[0, j] == (int(row[j // 4]) >> (2 * (j % 4))) & 3
+
+
+def test_bed2geno_table():
+    # 00 -> 2 copies of A1 (1.0), 01 -> missing, 10 -> het (0.5), 11 -> 0 copies
+    codes = np.array([[0, 1, 2, 3]], dtype=np.uint8)
+    assert list(O.BED2GENO[codes[0]]) == [1.0, -9.0, 0.5, 0.0]
+
+
+def test_allele_freq_brute_force():
+    rng = np.random.default_rng(0)
+    codes = rng.integers(0, 4, size=(50, 37)).astype(np.uint8)
+    codes[3] = 1  # all missing -> F = 0
+    F = O.allele_freq(codes)
+    for j in range(codes.shape[0]):
+        vals = [O.BED2GENO[c] for c in codes[j] if c != 1]
+        assert F[j] == (sum(vals) / len(vals) if vals else 0.0)
+
+
+def test_centering_and_standardize_closed_form():
+    codes = np.array([[0, 2, 3, 1, 2]], dtype=np.uint8)
+    packed = np.zeros((1, 2), dtype=np.uint8)
+    for j, c in enumerate(codes[0]):
+        packed[0, j // 4] |= c << (2 * (j % 4))
+    d = O.Data.from_packed(packed, 5, O.Params(svd=1, k=1))
+    f = (1.0 + 0.5 + 0.0 + 0.5) / 4
+    assert d.F[0] == f
+    np.testing.assert_array_equal(d.G[:, 0], [1 - f, 0.5 - f, -f, 0.0, 0.5 - f])
+    d.standardize_E()
+    s = np.sqrt(2.0) / np.sqrt(f * (1 - f))
+    np.testing.assert_allclose(d.G[:, 0], s * np.array([1 - f, 0.5 - f, -f, 0.0, 0.5 - f]), rtol=1e-15)
+
+
+def test_standardize_skips_monomorphic():
+    packed = np.full((1, 1), 0b11111111, dtype=np.uint8)
+    d = O.Data.from_packed(packed, 4, O.Params(svd=1, k=1))
+    d.standardize_E()
+    assert d.F[0] == 0.0 and np.all(d.G == 0.0)
+
+
+# --------------------------------------------------------------- small closed forms
+def test_flip_omg_brute_force():
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((40, 6))
+    b = a.copy()
+    b[:, [1, 4]] *= -1
+    b[:, 2] = rng.standard_normal(40)
+    out = O.flip_omg(a, b)
+    for i in range(6):
+        flip = np.abs(a[:, i] - b[:, i]).sum() > 2 * np.abs(a[:, i] + b[:, i]).sum()
+        np.testing.assert_array_equal(out[:, i], -b[:, i] if flip else b[:, i])
+    np.testing.assert_array_equal(out[:, [1, 4]], a[:, [1, 4]])
+
+
+def test_mev_closed_form():
+    q = np.linalg.qr(np.random.default_rng(2).standard_normal((30, 4)))[0]
+    assert abs(O.mev(q, q) - 1.0) < 1e-14
+    assert abs(O.mev(q, -q) - 1.0) < 1e-14
+    q2 = np.linalg.qr(np.random.default_rng(3).standard_normal((30, 8)))[0]
+    assert abs(O.mev(q2[:, :4], q2[:, 4:])) < 1e-14
+
+
+def test_flip_uv_makes_max_abs_loading_positive():
+    rng = np.random.default_rng(4)
+    U, V = rng.standard_normal((10, 3)), rng.standard_normal((20, 3))
+    U2, V2 = O.flip_uv(U, V, False)
+    for i in range(3):
+        x = np.argmax(np.abs(V2[:, i]))
+        assert V2[x, i] > 0
+        np.testing.assert_array_equal(U2[:, i] * V2[:, i][0], U[:, i] * V[:, i][0])
+
+
+def test_omega_is_deterministic_and_columnwise():
+    a = O.omega(112, True, 5, 3)
+    b = O.omega(112, True, 15, 1)
+    np.testing.assert_array_equal(a.T.ravel(), b.ravel())  # column-major fill of one stream
+    assert not np.array_equal(O.omega(113, True, 5, 3), a)
+    u = O.omega(7, False, 100, 2)
+    assert u.min() >= -1 and u.max() <= 1
+
+
+def test_shuffle_is_permutation():
+    p = O.shuffle_indices(1000)
+    assert sorted(p.tolist()) == list(range(1000))
+    np.testing.assert_array_equal(p, O.shuffle_indices(1000))
+
+
+# --------------------------------------------------------------- golden (reference binary) runs
+def _outputs_match(case, res, data, printv, tol_vec=2e-6):
+    d = GOLDEN / case
+    E = O.read_matrix(d / "eigvals").ravel() if (d / "eigvals").stat().st_size > 1 else np.zeros(0)
+    if E.size:
+        np.testing.assert_allclose(res.E, E, rtol=1e-5)
+        S = O.read_matrix(d / "sigvals").ravel()
+        np.testing.assert_allclose(res.S, S, rtol=1e-5)
+    U = O.read_matrix(d / "eigvecs")
+    assert U.shape == res.U.shape
+    np.testing.assert_allclose(O.sign_align(U, res.U), U, atol=tol_vec)
+    if printv and (d / "loadings").exists():
+        V = O.read_matrix(d / "loadings")
+        Vo = res.V
+        if data.params.perm and data.perm is not None:
+            Vo = np.empty_like(res.V)
+            Vo[data.perm] = res.V
+        np.testing.assert_allclose(O.sign_align(V, Vo), V, atol=tol_vec)
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_oracle_matches_reference_binary(case, golden_inputs):
+    meta = load_meta(case)
+    params, printv = params_from_args(meta["args"])
+    packed, n, _ = O.read_plink(golden_inputs[meta["input"]])
+    data = O.Data.from_packed(packed, n, params)
+    res = O.run_pca_with_halko(data)
+    if meta["trace"]:
+        np.testing.assert_allclose(res.trace, meta["trace"], atol=1.5e-6)
+    if meta["stop_epochs"]:
+        assert res.epochs == meta["stop_epochs"][-1]
+    if meta["em_trace"]:
+        np.testing.assert_allclose(res.em_trace, meta["em_trace"], atol=1.5e-6)
+    _outputs_match(case, res, data, printv)
+
+
+def test_golden_mbim_allele_frequencies(golden_inputs):
+    packed, n, m = O.read_plink(golden_inputs["f1"])
+    F = O.allele_freq(O.decode_codes(packed, n))
+    mb = [float(line.split("\t")[6]) for line in (GOLDEN / "f1_d1" / "mbim").read_text().splitlines()]
+    np.testing.assert_allclose(F, mb, rtol=5e-6)
+
+
+def test_planted_mistake_is_caught(golden_inputs, monkeypatch):
+    """Disabling flipOmg in the oracle must break the winSVD golden trace."""
+    meta = load_meta("f1_d2")
+    params, _ = params_from_args(meta["args"])
+    packed, n, _ = O.read_plink(golden_inputs["f1"])
+    monkeypatch.setattr(O, "flip_omg", lambda a, b: -b)
+    res = O.run_pca_with_halko(O.Data.from_packed(packed, n, params))
+    assert not np.allclose(res.trace, meta["trace"], atol=1.5e-6)
+
+
+def test_dense_oracle_agrees_at_convergence(golden_inputs):
+    """cfg1 (2,000 x 20,000, 3 populations): the two structured PCs of the RSVD
+    match the exact SVD of the standardized matrix."""
+    meta = load_meta("cfg1_d1")
+    params, _ = params_from_args(meta["args"])
+    packed, n, _ = O.read_plink(golden_inputs["cfg1"])
+    data = O.Data.from_packed(packed, n, params)
+    data.standardize_E()
+    U, S, E, V = O.dense_pca(data, params.k)
+    Eref = O.read_matrix(GOLDEN / "cfg1_d1" / "eigvals").ravel()
+    Uref = O.read_matrix(GOLDEN / "cfg1_d1" / "eigvecs")
+    np.testing.assert_allclose(Eref[:2], E[:2], rtol=1e-5)
+    assert np.all(O.pc_correlations(U[:, :2], Uref[:, :2]) > 0.9999)
