@@ -1,12 +1,17 @@
 /*
- * pcaone_amd: MI355X-native randomized-SVD hot path of PCAone (--svd 1 / --svd 2 / --emu)
+ * pcaone_amd: MI355X-native randomized-SVD hot path of PCAone (v0.7.2):
+ * --svd 1 (single-pass RSVD), --svd 2 (window-based RSVD) and --emu.
  *
  * C ABI.  Every entry point returns 0 on success or a non-zero status
- * (PCAONE_ERR_*) and leaves a message readable with pcaone_last_error().
+ * (PCAONE_ERR_*); the message is readable with pcaone_last_error().  This
+ * replaces the reference's cao.error() throw (src/Logger.hpp:85-94) and
+ * Param's exit(EXIT_FAILURE) (src/Cmd.cpp:240-264).
+ *
  * Matrices are column-major FP64 host arrays with leading dimension = rows,
- * the layout of the reference's Eigen::MatrixXd (src/Common.hpp:29).
- * Host pointers are borrowed for the duration of the call only; the context
+ * the layout of the reference's Mat2D = Eigen::MatrixXd (src/Common.hpp:29).
+ * Host pointers are borrowed for the duration of a call only; the context
  * owns every device buffer.  A context is single-threaded and stream-ordered.
+ * Reference paths below are relative to the PCAone source tree.
  */
 #ifndef PCAONE_AMD_H_
 #define PCAONE_AMD_H_
@@ -27,13 +32,51 @@ extern "C" {
 #define PCAONE_ERR_NO_DEVICE 6
 #define PCAONE_ERR_INTERNAL 7
 
-/* Arithmetic form of the two genotype products G = A*Omega and H = A^T*G. */
-#define PCAONE_PREC_FP64 0   /* FP64 MFMA (oracle-grade form)                          */
-#define PCAONE_PREC_FP32 1   /* FP32 MFMA, plain form                                   */
-#define PCAONE_PREC_BF16 2   /* BF16 MFMA, exact genotypes, hi/lo split of Omega and G  */
-#define PCAONE_PREC_BF16X1 3 /* BF16 MFMA, exact genotypes, single-piece Omega and G    */
+/* Arithmetic form of the genotype products G = A*Omega and H = A^T*G.
+ * Genotypes enter every form exactly; Omega and G are rounded to the form's
+ * operand type (split into BF16 hi/lo pieces for PCAONE_PREC_BF16). */
+#define PCAONE_PREC_FP64 0   /* FP64 MFMA, FP64 accumulation                 */
+#define PCAONE_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation (plain form)    */
+#define PCAONE_PREC_BF16 2   /* BF16 MFMA, hi/lo split of Omega and G        */
+#define PCAONE_PREC_BF16X1 3 /* BF16 MFMA, single BF16 piece of Omega and G  */
 
 typedef struct pcaone_ctx pcaone_ctx;
+
+/* Run settings: the fields of the reference's Param used on this path
+ * (src/Cmd.hpp:16-98; defaults from pcaone_params_default match it). */
+typedef struct pcaone_params {
+  int svd;              /* -d/--svd: 1 sSVD (Halko), 2 winSVD (default 2)          */
+  uint32_t k;           /* -k/--pc (10)                                             */
+  uint32_t oversamples; /* --oversamples (10); raised to k as src/Cmd.cpp:216 does  */
+  uint32_t maxp;        /* --maxp (20)                                              */
+  uint32_t bands;       /* -w/--batches (64)                                        */
+  uint32_t maxiter;     /* --maxiter (100), EM iterations with --emu                */
+  double tol;           /* --tol-rsvd (1e-4)                                        */
+  double tolem;         /* --tol-em (1e-5)                                          */
+  double maf;           /* --maf (0: no filter)                                     */
+  int seed;             /* --seed (112)                                             */
+  int rand;             /* --rand: 1 Gaussian, 0 uniform (1)                        */
+  int ploidy;           /* 2 (1 with --haploid)                                     */
+  int noshuffle;        /* -S/--no-shuffle                                          */
+  int emu;              /* --emu                                                    */
+  int printv;           /* -V/--printv: also return / write loadings                */
+  int verbose;          /* -v (0 here: silent)                                      */
+  int precision;        /* PCAONE_PREC_* (PCAONE_PREC_FP32)                         */
+  int wm_g, wm_h;       /* GEMM wave tiles (0 = tuned default)                      */
+  uint64_t kchunk_g;    /* split-K chunk of the G product in codes (0 = auto)       */
+  uint64_t kchunk_h;    /* split-K chunk of the H product in codes (0 = auto)       */
+} pcaone_params;
+
+typedef struct pcaone_run_info {
+  int epochs;        /* epochs of the last computeUSV                */
+  int total_epochs;  /* epochs over all computeUSV calls (EMU)      */
+  int em_iters;      /* EM iterations run                           */
+  int n_trace;       /* entries available from pcaone_run_trace     */
+  int n_sv;          /* singular values returned (0 with --maxp 0)  */
+  double seconds;    /* wall time of the run (inputs resident)      */
+  double g_ms, h_ms; /* summed device time of the two products      */
+  uint64_t g_launches, h_launches;
+} pcaone_run_info;
 
 /* Synthetic Balding-Nichols PLINK data (test and benchmark input; no
  * counterpart in the reference). */
@@ -49,18 +92,94 @@ typedef struct pcaone_synth_spec {
   double high_miss_rate;   /* extra missing rate of those samples         */
 } pcaone_synth_spec;
 
-/* ---- errors ------------------------------------------------------------ */
-/* Message of the last failure on `ctx`, or of the calling thread's last
- * context-free failure when ctx is NULL.  Replaces cao.error()'s throw
- * (src/Logger.hpp:85-94). */
+/* ---- errors and context ------------------------------------------------ */
+/* Message of the last failure on ctx, or of the calling thread's last
+ * context-free failure when ctx is NULL. */
 const char* pcaone_last_error(const pcaone_ctx* ctx);
+int pcaone_device_count(int* n);
+int pcaone_open(int device, pcaone_ctx** out);
+int pcaone_close(pcaone_ctx* ctx);
+void pcaone_params_default(pcaone_params* p);
 
-/* ---- synthetic data ---------------------------------------------------- */
-/* packed_out: nsnps x ceil(nsamples/4) bytes, the .bed payload after its
- * 3-byte magic.  attempts_out (nullable): per-SNP redraw counter. */
+/* ---- Data boundary (FileBed / Data) ------------------------------------ */
+/* Replaces FileBed::FileBed + read_all (src/FilePlink.hpp:10-30,
+ * src/FilePlink.cpp:26-120) and the --maf filter (src/Data.cpp:88-105): reads
+ * <prefix>.fam/.bim/.bed, checks the magic, moves the payload to HBM
+ * undecoded and counts allele frequencies on the device. */
+int pcaone_bed_read(pcaone_ctx* ctx, const char* prefix, double maf, uint32_t* nsamples, uint64_t* nsnps);
+/* The same from memory: reserve, then upload raw .bed rows (ceil(N/4) bytes
+ * per SNP, the payload after the 3-byte magic), then finalize. */
+int pcaone_bed_reserve(pcaone_ctx* ctx, uint32_t nsamples, uint64_t nsnps);
+int pcaone_bed_upload(pcaone_ctx* ctx, const uint8_t* rows, uint64_t snp_begin, uint64_t nsnps);
+int pcaone_bed_finalize(pcaone_ctx* ctx);
+/* Synthetic data generated into the context (reserve + upload + finalize). */
+int pcaone_bed_generate(pcaone_ctx* ctx, const pcaone_synth_spec* spec);
+int pcaone_dims(const pcaone_ctx* ctx, uint32_t* nsamples, uint64_t* nsnps, uint64_t* nmissing);
+/* Data::F (src/FilePlink.cpp:37-55), unpermuted SNP order, nsnps values. */
+int pcaone_allele_freq(const pcaone_ctx* ctx, double* F);
+/* Data::standardize_E (src/Data.cpp:351-362) with scale -9. */
+int pcaone_standardize(pcaone_ctx* ctx, int ploidy);
+/* permute_matrix (src/RSVD.hpp:61-78): column i <- column perm[i]. */
+int pcaone_permute_snps(pcaone_ctx* ctx, const int32_t* perm);
+/* Data::fit_with_pi, EMU branch (src/Data.cpp:318-348): missing entries take
+ * clamp(U S V^T, -F, 1-F) in every later product.  U N x k, S k, V M x k
+ * (V in the current column order). */
+int pcaone_set_impute(pcaone_ctx* ctx, const double* U, const double* S, const double* V, uint32_t k);
+
+/* ---- RsvdOpData boundary (src/Halko.hpp:6-93) -------------------------- */
+/* Workspace for L = k + oversamples columns in the given form. */
+int pcaone_configure(pcaone_ctx* ctx, int precision, uint32_t L);
+/* RsvdOpData::initOmg (src/Halko.cpp:15-23): Omega = Omg2 = omega (N x L). */
+int pcaone_set_omega(pcaone_ctx* ctx, const double* omega);
+/* The products of computeGandH for SNP rows [snp_begin, snp_end):
+ * G rows = A Omega, then h_mode 0: H = A^T G (src/Halko.cpp:125-126),
+ * 1: H1 += ..., 2: H2 += ... (src/Halko.cpp:195-203). */
+int pcaone_sketch(pcaone_ctx* ctx, uint64_t snp_begin, uint64_t snp_end, int h_mode);
+/* Omega <- thin-Q(src) then flipOmg (src/Halko.cpp:121-123, 212-215):
+ * source 0: H, 1: H = H1 + H2.  CholeskyQR2 with Householder-convention
+ * column signs. */
+int pcaone_update_omega(pcaone_ctx* ctx, int source);
+int pcaone_zero_h(pcaone_ctx* ctx, int which); /* 1: H1, 2: H2 */
+/* One epoch of computeUSV after computeGandH (src/Halko.cpp:55-77): QR of G
+ * twice, B = R^-T H^T, SVD of B via B B^T, Ucur, diff = 1 - mev(Ucur, Upre)
+ * when have_prev.  sigma: L values (nullable). */
+int pcaone_finish_epoch(pcaone_ctx* ctx, uint32_t k, int have_prev, double* sigma, double* diff);
+int pcaone_save_prev(pcaone_ctx* ctx); /* Upre = Ucur */
+int pcaone_get_u(pcaone_ctx* ctx, uint32_t k, double* U); /* Ucur, N x k */
+int pcaone_get_v(pcaone_ctx* ctx, uint32_t k, double* V); /* G U_B, M x k (src/Halko.cpp:84) */
+
+/* ---- drivers ------------------------------------------------------------ */
+/* run_pca_with_halko (src/Halko.cpp:271-345) on the resident data.  U: N x k,
+ * S and E (= S^2/M): k, V (nullable; filled with printv or emu): M x k in the
+ * original SNP order. */
+int pcaone_run_halko(pcaone_ctx* ctx, const pcaone_params* p, double* U, double* S, double* E, double* V,
+                     pcaone_run_info* info);
+/* diff values of every checked epoch / EM iteration of the last run. */
+int pcaone_run_trace(const pcaone_ctx* ctx, int em, double* out, int cap, int* n);
+/* The PLINK path of src/Main.cpp:150-231 for --svd 1/2: read <bfile>, run,
+ * write <out>.sigvals/.eigvals/.eigvecs/.eigvecs2 (+ .loadings/.mbim with
+ * printv). */
+int pcaone_run_bfile(pcaone_ctx* ctx, const char* bfile, const char* out, const pcaone_params* p,
+                     pcaone_run_info* info);
+/* Kernel timing with HIP events around the two products (on/off). */
+int pcaone_time_kernels(pcaone_ctx* ctx, int on);
+
+/* ---- synthetic data ----------------------------------------------------- */
+/* packed_out: nsnps x ceil(nsamples/4) bytes; attempts_out (nullable): per-SNP
+ * redraw counter. */
 int pcaone_synth_packed(const pcaone_synth_spec* spec, uint8_t* packed_out, uint32_t* attempts_out);
-/* Writes <prefix>.bed/.bim/.fam. */
 int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix);
+
+/* ---- diagnostics (no reference counterpart) ---------------------------- */
+/* Y = D(X) W with X raw .bed rows (nrows x ceil(ncodes/4) bytes), D mapping
+ * codes 00/01/10/11 to 2/0/0/-2, W ncodes x L, Y nrows x L (column-major),
+ * computed by the packed MFMA kernel of the given form. */
+int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
+                       const double* W, uint32_t L, double* Y);
+/* Host FP64 helpers of the L x L steps, row-major n x n. */
+int pcaone_host_sym_eig(int n, const double* A, double* w, double* V);
+int pcaone_host_householder_signs(int n, const double* Y, double* s);
+int pcaone_host_chol_upper(int n, const double* A, double* R);
 
 #ifdef __cplusplus
 }

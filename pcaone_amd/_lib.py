@@ -12,6 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpcaone_amd.so"
 
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x1": PREC_BF16X1}
+ERR_NO_DEVICE = 6
 
 
 class PcaoneError(RuntimeError):
@@ -26,7 +27,60 @@ class SynthSpec(C.Structure):
                 ("miss_rate", C.c_double), ("high_miss_frac", C.c_double), ("high_miss_rate", C.c_double)]
 
 
+class Params(C.Structure):
+    _fields_ = [("svd", C.c_int), ("k", C.c_uint32), ("oversamples", C.c_uint32), ("maxp", C.c_uint32),
+                ("bands", C.c_uint32), ("maxiter", C.c_uint32), ("tol", C.c_double), ("tolem", C.c_double),
+                ("maf", C.c_double), ("seed", C.c_int), ("rand", C.c_int), ("ploidy", C.c_int),
+                ("noshuffle", C.c_int), ("emu", C.c_int), ("printv", C.c_int), ("verbose", C.c_int),
+                ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("kchunk_g", C.c_uint64),
+                ("kchunk_h", C.c_uint64)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("total_epochs", C.c_int), ("em_iters", C.c_int), ("n_trace", C.c_int),
+                ("n_sv", C.c_int), ("seconds", C.c_double), ("g_ms", C.c_double), ("h_ms", C.c_double),
+                ("g_launches", C.c_uint64), ("h_launches", C.c_uint64)]
+
+
 _lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "pcaone_last_error": (C.c_char_p, [_P]),
+    "pcaone_device_count": (C.c_int, [_P]),
+    "pcaone_open": (C.c_int, [C.c_int, _P]),
+    "pcaone_close": (C.c_int, [_P]),
+    "pcaone_params_default": (None, [_P]),
+    "pcaone_bed_read": (C.c_int, [_P, C.c_char_p, C.c_double, _P, _P]),
+    "pcaone_bed_reserve": (C.c_int, [_P, C.c_uint32, C.c_uint64]),
+    "pcaone_bed_upload": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
+    "pcaone_bed_finalize": (C.c_int, [_P]),
+    "pcaone_bed_generate": (C.c_int, [_P, _P]),
+    "pcaone_dims": (C.c_int, [_P, _P, _P, _P]),
+    "pcaone_allele_freq": (C.c_int, [_P, _P]),
+    "pcaone_standardize": (C.c_int, [_P, C.c_int]),
+    "pcaone_permute_snps": (C.c_int, [_P, _P]),
+    "pcaone_set_impute": (C.c_int, [_P, _P, _P, _P, C.c_uint32]),
+    "pcaone_configure": (C.c_int, [_P, C.c_int, C.c_uint32]),
+    "pcaone_set_omega": (C.c_int, [_P, _P]),
+    "pcaone_sketch": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int]),
+    "pcaone_update_omega": (C.c_int, [_P, C.c_int]),
+    "pcaone_zero_h": (C.c_int, [_P, C.c_int]),
+    "pcaone_finish_epoch": (C.c_int, [_P, C.c_uint32, C.c_int, _P, _P]),
+    "pcaone_save_prev": (C.c_int, [_P]),
+    "pcaone_get_u": (C.c_int, [_P, C.c_uint32, _P]),
+    "pcaone_get_v": (C.c_int, [_P, C.c_uint32, _P]),
+    "pcaone_run_halko": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "pcaone_run_trace": (C.c_int, [_P, C.c_int, _P, C.c_int, _P]),
+    "pcaone_run_bfile": (C.c_int, [_P, C.c_char_p, C.c_char_p, _P, _P]),
+    "pcaone_time_kernels": (C.c_int, [_P, C.c_int]),
+    "pcaone_synth_packed": (C.c_int, [_P, _P, _P]),
+    "pcaone_synth_write_plink": (C.c_int, [_P, C.c_char_p]),
+    "pcaone_packed_gemm": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.c_uint32, _P]),
+    "pcaone_host_sym_eig": (C.c_int, [C.c_int, _P, _P, _P]),
+    "pcaone_host_householder_signs": (C.c_int, [C.c_int, _P, _P]),
+    "pcaone_host_chol_upper": (C.c_int, [C.c_int, _P, _P]),
+}
 
 
 def lib() -> C.CDLL:
@@ -35,19 +89,12 @@ def lib() -> C.CDLL:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is not built; run `python -m pcaone_amd.build` (or __graft_entry__.build())")
         L = C.CDLL(str(LIB_PATH))
-        _declare(L)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
         _lib = L
     return _lib
-
-
-def _declare(L: C.CDLL) -> None:
-    p = C.c_void_p
-    L.pcaone_last_error.restype = C.c_char_p
-    L.pcaone_last_error.argtypes = [p]
-    L.pcaone_synth_packed.argtypes = [C.POINTER(SynthSpec), p, p]
-    L.pcaone_synth_write_plink.argtypes = [C.POINTER(SynthSpec), C.c_char_p]
-    for name in ("pcaone_synth_packed", "pcaone_synth_write_plink"):
-        getattr(L, name).restype = C.c_int
 
 
 def check(status: int, ctx=None) -> None:

@@ -1,0 +1,200 @@
+"""Python view of the C ABI: a device context holding one genotype matrix.
+
+Every call goes to libpcaone_amd.so; there is no CPU fallback.  Arrays are
+numpy, column-major where the C ABI is (U, V, Omega).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import PRECISIONS, Params, RunInfo, check, lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data if a is not None else None
+
+
+def _f64(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().pcaone_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().pcaone_device_count(C.byref(n)))
+    return n.value
+
+
+@dataclass
+class Result:
+    U: np.ndarray
+    S: np.ndarray
+    E: np.ndarray
+    V: np.ndarray | None
+    info: dict
+    trace: list = field(default_factory=list)
+    em_trace: list = field(default_factory=list)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().pcaone_open(device, C.byref(h)))
+        self._h = h
+
+    # -- lifecycle
+    def close(self):
+        if self._h:
+            lib().pcaone_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        check(st, self._h)
+
+    # -- data
+    def read_bed(self, prefix: str, maf: float = 0.0):
+        n, m = C.c_uint32(), C.c_uint64()
+        self._check(lib().pcaone_bed_read(self._h, str(prefix).encode(), maf, C.byref(n), C.byref(m)))
+        return n.value, m.value
+
+    def upload_packed(self, packed: np.ndarray, nsamples: int):
+        packed = np.ascontiguousarray(packed, dtype=np.uint8)
+        m = packed.shape[0]
+        assert packed.shape[1] == (nsamples + 3) // 4
+        self._check(lib().pcaone_bed_reserve(self._h, nsamples, m))
+        self._check(lib().pcaone_bed_upload(self._h, _ptr(packed), 0, m))
+        self._check(lib().pcaone_bed_finalize(self._h))
+
+    def generate(self, spec):
+        cs = spec.c()
+        self._check(lib().pcaone_bed_generate(self._h, C.byref(cs)))
+
+    def dims(self):
+        n, m, miss = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        self._check(lib().pcaone_dims(self._h, C.byref(n), C.byref(m), C.byref(miss)))
+        return n.value, m.value, miss.value
+
+    def allele_freq(self) -> np.ndarray:
+        _, m, _ = self.dims()
+        F = np.empty(m)
+        self._check(lib().pcaone_allele_freq(self._h, _ptr(F)))
+        return F
+
+    def standardize(self, ploidy: int = 2):
+        self._check(lib().pcaone_standardize(self._h, ploidy))
+
+    def permute_snps(self, perm):
+        p = np.ascontiguousarray(perm, dtype=np.int32)
+        self._check(lib().pcaone_permute_snps(self._h, _ptr(p)))
+
+    def set_impute(self, U, S, V):
+        U, S, V = _f64(U), np.ascontiguousarray(S, dtype=np.float64), _f64(V)
+        self._check(lib().pcaone_set_impute(self._h, _ptr(U), _ptr(S), _ptr(V), S.size))
+
+    # -- operator steps
+    def configure(self, precision, L: int):
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        self._check(lib().pcaone_configure(self._h, precision, L))
+
+    def set_omega(self, omega: np.ndarray):
+        o = _f64(omega)
+        self._check(lib().pcaone_set_omega(self._h, _ptr(o)))
+
+    def sketch(self, j0: int, j1: int, h_mode: int = 0):
+        self._check(lib().pcaone_sketch(self._h, j0, j1, h_mode))
+
+    def update_omega(self, source: int = 0):
+        self._check(lib().pcaone_update_omega(self._h, source))
+
+    def zero_h(self, which: int):
+        self._check(lib().pcaone_zero_h(self._h, which))
+
+    def finish_epoch(self, k: int, L: int, have_prev: bool = False):
+        sig = np.empty(L)
+        diff = C.c_double(0.0)
+        self._check(lib().pcaone_finish_epoch(self._h, k, int(have_prev), _ptr(sig), C.byref(diff)))
+        return sig, diff.value
+
+    def save_prev(self):
+        self._check(lib().pcaone_save_prev(self._h))
+
+    def get_u(self, k: int) -> np.ndarray:
+        n, _, _ = self.dims()
+        U = np.empty((n, k), order="F")
+        self._check(lib().pcaone_get_u(self._h, k, _ptr(U)))
+        return U
+
+    def get_v(self, k: int) -> np.ndarray:
+        _, m, _ = self.dims()
+        V = np.empty((m, k), order="F")
+        self._check(lib().pcaone_get_v(self._h, k, _ptr(V)))
+        return V
+
+    def time_kernels(self, on: bool = True):
+        self._check(lib().pcaone_time_kernels(self._h, int(on)))
+
+    # -- drivers
+    def run(self, params: Params | None = None, want_v: bool = False, **kw) -> Result:
+        p = params if params is not None else default_params(**kw)
+        n, m, _ = self.dims()
+        k = p.k
+        U = np.empty((n, k), order="F")
+        S = np.zeros(k)
+        E = np.zeros(k)
+        V = np.empty((m, k), order="F") if want_v else None
+        info = RunInfo()
+        self._check(lib().pcaone_run_halko(self._h, C.byref(p), _ptr(U), _ptr(S), _ptr(E), _ptr(V), C.byref(info)))
+        d = {f: getattr(info, f) for f, _ in RunInfo._fields_}
+        return Result(U=U, S=S[: info.n_sv], E=E[: info.n_sv], V=V, info=d, trace=self._trace(0, info.n_trace),
+                      em_trace=self._trace(1, max(info.em_iters, 0)))
+
+    def _trace(self, em: int, n: int):
+        buf = np.empty(max(n, 1))
+        cnt = C.c_int(0)
+        self._check(lib().pcaone_run_trace(self._h, em, _ptr(buf), n, C.byref(cnt)))
+        return buf[: min(n, cnt.value)].tolist()
+
+    def run_bfile(self, bfile: str, out: str, params: Params | None = None, **kw) -> dict:
+        p = params if params is not None else default_params(**kw)
+        info = RunInfo()
+        self._check(lib().pcaone_run_bfile(self._h, str(bfile).encode(), str(out).encode(), C.byref(p),
+                                           C.byref(info)))
+        return {f: getattr(info, f) for f, _ in RunInfo._fields_}
+
+    def packed_gemm(self, precision, packed: np.ndarray, ncodes: int, W: np.ndarray) -> np.ndarray:
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        packed = np.ascontiguousarray(packed, dtype=np.uint8)
+        W = _f64(W)
+        Y = np.empty((packed.shape[0], W.shape[1]), order="F")
+        self._check(lib().pcaone_packed_gemm(self._h, precision, _ptr(packed), packed.shape[0], ncodes, _ptr(W),
+                                             W.shape[1], _ptr(Y)))
+        return Y

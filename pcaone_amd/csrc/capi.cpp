@@ -1,22 +1,32 @@
-// extern "C" entry points: argument checks, exception -> status translation.
+// extern "C" entry points: argument checks and exception -> status translation.
+#include <cstring>
+#include <memory>
 #include <string>
 
 #include "../../include/pcaone_amd.h"
 #include "common.hpp"
+#include "context.hpp"
+#include "kernels.hpp"
+#include "linalg_host.hpp"
+#include "plink_io.hpp"
+#include "rsvd.hpp"
 #include "synth_host.hpp"
 
 namespace pcaone {
-
 namespace {
 thread_local std::string g_error;
 }
-
 void set_global_error(const std::string& msg) { g_error = msg; }
 const char* global_error() { return g_error.c_str(); }
-
 }  // namespace pcaone
 
 using namespace pcaone;
+
+struct pcaone_ctx {
+  std::unique_ptr<Context> ctx;
+  RunInfo last;
+  std::string err;
+};
 
 namespace {
 
@@ -37,10 +47,85 @@ int guard_global(F&& f) {
   }
 }
 
+template <class F>
+int guard(pcaone_ctx* c, F&& f) {
+  if (!c || !c->ctx) {
+    set_global_error("null context");
+    return kErrArgument;
+  }
+  try {
+    PCAONE_HIP(hipSetDevice(c->ctx->device));
+    f(*c->ctx);
+    return kOk;
+  } catch (const Error& e) {
+    c->err = e.what();
+    set_global_error(c->err);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    c->err = "host out of memory";
+    set_global_error(c->err);
+    return kErrInternal;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    set_global_error(c->err);
+    return kErrInternal;
+  }
+}
+
 synth::Spec to_spec(const pcaone_synth_spec* s) {
   require(s != nullptr, kErrArgument, "null synth spec");
   return synth::Spec{s->seed,   s->nsamples, s->npops,     s->nsnps,          s->freq_lo,       s->freq_hi,
                      s->fst_lo, s->fst_hi,   s->miss_rate, s->high_miss_frac, s->high_miss_rate};
+}
+
+Form to_form(int p) {
+  require(p >= 0 && p <= 3, kErrArgument, "unknown precision");
+  return static_cast<Form>(p);
+}
+
+Param to_param(const pcaone_params* p) {
+  require(p != nullptr, kErrArgument, "null params");
+  Param q;
+  q.svd = p->svd;
+  q.k = p->k;
+  q.oversamples = p->oversamples;
+  q.maxp = p->maxp;
+  q.bands = p->bands;
+  q.maxiter = p->maxiter;
+  q.tol = p->tol;
+  q.tolem = p->tolem;
+  q.maf = p->maf;
+  q.seed = p->seed;
+  q.rand = p->rand;
+  q.ploidy = p->ploidy;
+  q.noshuffle = p->noshuffle != 0;
+  q.emu = p->emu != 0;
+  q.printv = p->printv != 0;
+  q.verbose = p->verbose;
+  q.precision = to_form(p->precision);
+  q.tuning.wm_g = p->wm_g;
+  q.tuning.wm_h = p->wm_h;
+  q.tuning.kchunk_g = p->kchunk_g;
+  q.tuning.kchunk_h = p->kchunk_h;
+  require(q.k >= 1, kErrArgument, "k must be >= 1");
+  require(q.ploidy == 1 || q.ploidy == 2, kErrArgument, "ploidy must be 1 or 2");
+  q.derive();
+  return q;
+}
+
+void fill_info(pcaone_run_info* info, const RunInfo& r, const Context& c, int nsv) {
+  if (!info) return;
+  std::memset(info, 0, sizeof(*info));
+  info->epochs = r.epochs;
+  info->total_epochs = r.total_epochs;
+  info->em_iters = r.em_iters;
+  info->n_trace = (int)r.trace.size();
+  info->n_sv = nsv;
+  info->seconds = r.seconds;
+  info->g_ms = c.ktimes.g_ms;
+  info->h_ms = c.ktimes.h_ms;
+  info->g_launches = c.ktimes.g_launches;
+  info->h_launches = c.ktimes.h_launches;
 }
 
 }  // namespace
@@ -48,8 +133,217 @@ synth::Spec to_spec(const pcaone_synth_spec* s) {
 extern "C" {
 
 const char* pcaone_last_error(const pcaone_ctx* ctx) {
-  (void)ctx;
+  if (ctx && !ctx->err.empty()) return ctx->err.c_str();
   return global_error();
+}
+
+int pcaone_device_count(int* n) {
+  return guard_global([&] {
+    require(n != nullptr, kErrArgument, "null output");
+    int c = 0;
+    *n = hipGetDeviceCount(&c) == hipSuccess ? c : 0;
+  });
+}
+
+int pcaone_open(int device, pcaone_ctx** out) {
+  return guard_global([&] {
+    require(out != nullptr, kErrArgument, "null output");
+    *out = nullptr;
+    auto c = std::make_unique<pcaone_ctx>();
+    c->ctx = std::make_unique<Context>(device);
+    *out = c.release();
+  });
+}
+
+int pcaone_close(pcaone_ctx* ctx) {
+  return guard_global([&] { delete ctx; });
+}
+
+void pcaone_params_default(pcaone_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->svd = 2;
+  p->k = 10;
+  p->oversamples = 10;
+  p->maxp = 20;
+  p->bands = 64;
+  p->maxiter = 100;
+  p->tol = 1e-4;
+  p->tolem = 1e-5;
+  p->seed = 112;
+  p->rand = 1;
+  p->ploidy = 2;
+  p->precision = PCAONE_PREC_FP32;
+}
+
+int pcaone_bed_read(pcaone_ctx* c, const char* prefix, double maf, uint32_t* nsamples, uint64_t* nsnps) {
+  return guard(c, [&](Context& x) {
+    require(prefix != nullptr, kErrArgument, "null prefix");
+    Param p;
+    p.maf = maf;
+    p.svd = 1;
+    p.derive();
+    read_plink(x, prefix, p);
+    if (nsamples) *nsamples = x.N;
+    if (nsnps) *nsnps = x.M;
+  });
+}
+
+int pcaone_bed_reserve(pcaone_ctx* c, uint32_t nsamples, uint64_t nsnps) {
+  return guard(c, [&](Context& x) { x.reserve(nsamples, nsnps); });
+}
+
+int pcaone_bed_upload(pcaone_ctx* c, const uint8_t* rows, uint64_t snp_begin, uint64_t n) {
+  return guard(c, [&](Context& x) {
+    require(rows != nullptr || n == 0, kErrArgument, "null rows");
+    x.upload_rows(rows, snp_begin, n);
+  });
+}
+
+int pcaone_bed_finalize(pcaone_ctx* c) {
+  return guard(c, [&](Context& x) { x.finalize(); });
+}
+
+int pcaone_bed_generate(pcaone_ctx* c, const pcaone_synth_spec* spec) {
+  return guard(c, [&](Context& x) { x.generate(to_spec(spec)); });
+}
+
+int pcaone_dims(const pcaone_ctx* c, uint32_t* nsamples, uint64_t* nsnps, uint64_t* nmissing) {
+  return guard(const_cast<pcaone_ctx*>(c), [&](Context& x) {
+    if (nsamples) *nsamples = x.N;
+    if (nsnps) *nsnps = x.M;
+    if (nmissing) *nmissing = x.nmiss;
+  });
+}
+
+int pcaone_allele_freq(const pcaone_ctx* c, double* F) {
+  return guard(const_cast<pcaone_ctx*>(c), [&](Context& x) {
+    require(F != nullptr, kErrArgument, "null output");
+    require(!x.F.empty(), kErrState, "no data");
+    std::memcpy(F, x.F.data(), x.F.size() * sizeof(double));
+  });
+}
+
+int pcaone_standardize(pcaone_ctx* c, int ploidy) {
+  return guard(c, [&](Context& x) { x.standardize(ploidy); });
+}
+
+int pcaone_permute_snps(pcaone_ctx* c, const int32_t* perm) {
+  return guard(c, [&](Context& x) {
+    require(perm != nullptr, kErrArgument, "null permutation");
+    x.permute_columns(std::vector<int>(perm, perm + x.M));
+  });
+}
+
+int pcaone_set_impute(pcaone_ctx* c, const double* U, const double* S, const double* V, uint32_t k) {
+  return guard(c, [&](Context& x) {
+    require(U && S && V, kErrArgument, "null U/S/V");
+    x.set_impute(U, S, V, (int)k);
+  });
+}
+
+int pcaone_configure(pcaone_ctx* c, int precision, uint32_t L) {
+  return guard(c, [&](Context& x) { x.configure(to_form(precision), (int)L); });
+}
+
+int pcaone_set_omega(pcaone_ctx* c, const double* omega) {
+  return guard(c, [&](Context& x) {
+    require(omega != nullptr, kErrArgument, "null omega");
+    x.set_omega(omega);
+  });
+}
+
+int pcaone_sketch(pcaone_ctx* c, uint64_t b, uint64_t e, int h_mode) {
+  return guard(c, [&](Context& x) {
+    require(h_mode >= 0 && h_mode <= 2, kErrArgument, "h_mode must be 0, 1 or 2");
+    x.sketch(b, e, h_mode);
+  });
+}
+
+int pcaone_update_omega(pcaone_ctx* c, int source) {
+  return guard(c, [&](Context& x) {
+    require(source == 0 || source == 1, kErrArgument, "source must be 0 or 1");
+    x.omega_from(source);
+  });
+}
+
+int pcaone_zero_h(pcaone_ctx* c, int which) {
+  return guard(c, [&](Context& x) {
+    require(which == 1 || which == 2, kErrArgument, "which must be 1 or 2");
+    x.zero_h(which);
+  });
+}
+
+int pcaone_finish_epoch(pcaone_ctx* c, uint32_t k, int have_prev, double* sigma, double* diff) {
+  return guard(c, [&](Context& x) {
+    const EpochResult r = x.finish_epoch((int)k, have_prev != 0);
+    if (sigma) std::memcpy(sigma, r.sigma.data(), r.sigma.size() * sizeof(double));
+    if (diff) *diff = r.diff;
+  });
+}
+
+int pcaone_save_prev(pcaone_ctx* c) {
+  return guard(c, [&](Context& x) { x.save_prev(); });
+}
+
+int pcaone_get_u(pcaone_ctx* c, uint32_t k, double* U) {
+  return guard(c, [&](Context& x) {
+    require(U != nullptr && k >= 1 && (int)k <= x.L, kErrArgument, "bad U request");
+    x.download_u((int)k, U);
+  });
+}
+
+int pcaone_get_v(pcaone_ctx* c, uint32_t k, double* V) {
+  return guard(c, [&](Context& x) {
+    require(V != nullptr && k >= 1 && (int)k <= x.L, kErrArgument, "bad V request");
+    x.compute_v((int)k);
+    x.download_v((int)k, V);
+  });
+}
+
+int pcaone_run_halko(pcaone_ctx* c, const pcaone_params* p, double* U, double* S, double* E, double* V,
+                     pcaone_run_info* info) {
+  return guard(c, [&](Context& x) {
+    Param q = to_param(p);
+    if (V) q.printv = true;
+    require(x.M > 0 && !x.F.empty(), kErrState, "no genotype data (pcaone_bed_read / upload first)");
+    c->last = RunInfo{};
+    x.ktimes = Context::KernelTimes{};
+    const PcaResult r = run_pca_with_halko(x, q, &c->last);
+    const uint64_t N = x.N, M = x.M;
+    if (U) std::memcpy(U, r.U.data(), N * q.k * sizeof(double));
+    if (S) std::memcpy(S, r.S.data(), r.S.size() * sizeof(double));
+    if (E) std::memcpy(E, r.E.data(), r.E.size() * sizeof(double));
+    if (V && !r.V.empty()) std::memcpy(V, r.V.data(), M * q.k * sizeof(double));
+    fill_info(info, c->last, x, (int)r.S.size());
+  });
+}
+
+int pcaone_run_trace(const pcaone_ctx* c, int em, double* out, int cap, int* n) {
+  return guard(const_cast<pcaone_ctx*>(c), [&](Context&) {
+    const auto& v = em ? c->last.em_trace : c->last.trace;
+    if (n) *n = (int)v.size();
+    if (out)
+      for (int i = 0; i < cap && i < (int)v.size(); ++i) out[i] = v[i];
+  });
+}
+
+int pcaone_run_bfile(pcaone_ctx* c, const char* bfile, const char* out, const pcaone_params* p,
+                     pcaone_run_info* info) {
+  return guard(c, [&](Context& x) {
+    require(bfile && out, kErrArgument, "null path");
+    const Param q = to_param(p);
+    const PlinkInfo pi = read_plink(x, bfile, q);
+    c->last = RunInfo{};
+    x.ktimes = Context::KernelTimes{};
+    const PcaResult r = run_pca_with_halko(x, q, &c->last);
+    write_outputs(x, r, q, bfile, out, pi);
+    fill_info(info, c->last, x, (int)r.S.size());
+  });
+}
+
+int pcaone_time_kernels(pcaone_ctx* c, int on) {
+  return guard(c, [&](Context& x) { x.time_kernels = on != 0; });
 }
 
 int pcaone_synth_packed(const pcaone_synth_spec* spec, uint8_t* packed_out, uint32_t* attempts_out) {
@@ -63,6 +357,90 @@ int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix) 
   return guard_global([&] {
     require(prefix != nullptr, kErrArgument, "null prefix");
     synth::write_plink(to_spec(spec), prefix);
+  });
+}
+
+int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
+                       const double* W, uint32_t L, double* Y) {
+  return guard(c, [&](Context& x) {
+    require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
+    const Form f = to_form(precision);
+    const int lp = (int)round_up(L, 16);
+    const int wm = gemm_default_wm(f, lp);
+    const uint64_t tile = gemm_row_tile(f, wm);
+    const uint64_t rows_pad = round_up(nrows, tile);
+    const uint64_t K = round_up(ncodes, kKStep), pitch = K / 16, bps = bytes_per_snp(ncodes);
+    DevBuf<uint8_t> raw(nrows * bps);
+    DevBuf<uint32_t> X(rows_pad * pitch);
+    launch_fill_rows(X.get(), pitch, 0, rows_pad, x.stream);
+    PCAONE_HIP(hipMemcpyAsync(raw.get(), rows, nrows * bps, hipMemcpyHostToDevice, x.stream));
+    launch_interleave(raw.get(), nrows, bps, ncodes, X.get(), pitch, x.stream);
+    std::vector<double> w(K * lp, 0.0);
+    for (uint64_t k = 0; k < ncodes; ++k)
+      for (uint32_t l = 0; l < L; ++l) w[k * lp + l] = W[(uint64_t)l * ncodes + k];
+    DevBuf<double> dw(K * lp);
+    PCAONE_HIP(hipMemcpyAsync(dw.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, x.stream));
+    const int P = form_pieces(f);
+    DevBuf<unsigned char> wt((uint64_t)P * lp * K * form_wbytes(f));
+    launch_make_pieces(f, dw.get(), ncodes, K, lp, nullptr, nullptr, wt.get(), K, (uint64_t)lp * K, nullptr,
+                       nullptr, x.stream);
+    DevBuf<unsigned char> out(rows_pad * lp * form_accbytes(f));
+    GemmArgs a{};
+    a.X = X.get();
+    a.xpitch = pitch;
+    a.rows = rows_pad;
+    a.Wt = wt.get();
+    a.wpitch = K;
+    a.wpiece = (uint64_t)lp * K;
+    a.Y = out.get();
+    a.ychunk = rows_pad * lp;
+    a.k_begin = 0;
+    a.k_end = K;
+    a.kchunk = K;
+    a.lp = lp;
+    launch_packed_gemm(f, wm, a, x.stream);
+    std::vector<unsigned char> h(out.bytes());
+    PCAONE_HIP(hipMemcpyAsync(h.data(), out.get(), h.size(), hipMemcpyDeviceToHost, x.stream));
+    PCAONE_HIP(hipStreamSynchronize(x.stream));
+    for (uint64_t r = 0; r < nrows; ++r)
+      for (uint32_t l = 0; l < L; ++l) {
+        const uint64_t idx = r * lp + l;
+        Y[(uint64_t)l * nrows + r] =
+            f == Form::FP64 ? reinterpret_cast<const double*>(h.data())[idx] : reinterpret_cast<const float*>(h.data())[idx];
+      }
+  });
+}
+
+int pcaone_host_sym_eig(int n, const double* A, double* w, double* V) {
+  return guard_global([&] {
+    require(n >= 1 && A && w && V, kErrArgument, "bad arguments");
+    la::Mat a(n);
+    std::memcpy(a.a.data(), A, (size_t)n * n * sizeof(double));
+    std::vector<double> ww;
+    la::Mat vv;
+    la::sym_eig_desc(a, ww, vv);
+    std::memcpy(w, ww.data(), n * sizeof(double));
+    std::memcpy(V, vv.a.data(), (size_t)n * n * sizeof(double));
+  });
+}
+
+int pcaone_host_householder_signs(int n, const double* Y, double* s) {
+  return guard_global([&] {
+    require(n >= 1 && Y && s, kErrArgument, "bad arguments");
+    la::Mat y(n);
+    std::memcpy(y.a.data(), Y, (size_t)n * n * sizeof(double));
+    const auto v = la::householder_signs(y);
+    std::memcpy(s, v.data(), n * sizeof(double));
+  });
+}
+
+int pcaone_host_chol_upper(int n, const double* A, double* R) {
+  return guard_global([&] {
+    require(n >= 1 && A && R, kErrArgument, "bad arguments");
+    la::Mat a(n);
+    std::memcpy(a.a.data(), A, (size_t)n * n * sizeof(double));
+    const la::Mat r = la::chol_upper(a);
+    std::memcpy(R, r.a.data(), (size_t)n * n * sizeof(double));
   });
 }
 

@@ -1,0 +1,618 @@
+// Device context implementation (see context.hpp for the HBM layout).
+#include "context.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+#include "synth_host.hpp"
+
+namespace pcaone {
+
+namespace {
+constexpr uint64_t kRowPad = 256;  // largest GEMM row tile
+constexpr double kVarTol = 1e-9;   // src/Data.hpp:7
+
+uint64_t bytes_of(Form f) { return form_wbytes(f); }
+}  // namespace
+
+Context::Context(int dev) : device(dev) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) fail(kErrNoDevice, "no HIP device available");
+  require(dev >= 0 && dev < n, kErrArgument, "device index out of range");
+  PCAONE_HIP(hipSetDevice(dev));
+  PCAONE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+Context::~Context() {
+  if (stream) {
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+// ------------------------------------------------------------------ data
+void Context::reserve(uint32_t nsamples, uint64_t nsnps) {
+  require(nsamples > 0 && nsnps > 0, kErrArgument, "empty genotype matrix");
+  require(nsnps < (1ull << 32), kErrArgument, "more than 2^32-1 SNPs");
+  N = nsamples;
+  M = nsnps;
+  bps_ = bytes_per_snp(N);
+  ks_ = round_up(N, kKStep);
+  kt_ = round_up(M, kKStep);
+  pitch_s_ = ks_ / 16;
+  pitch_t_ = kt_ / 16;
+  rows_s_ = round_up(M, kRowPad) + kRowPad;
+  rows_t_ = round_up(N, kRowPad);
+  xs_.resize(rows_s_ * pitch_s_);
+  launch_fill_rows(xs_.get(), pitch_s_, 0, rows_s_, stream);
+  nmiss = 0;
+  F.clear();
+  perm.clear();
+  L = lp = 0;
+}
+
+void Context::upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n) {
+  require(N > 0, kErrState, "reserve() first");
+  require(j0 + n <= M, kErrArgument, "upload beyond the reserved SNP count");
+  const uint64_t chunk = std::max<uint64_t>(1, (256ull << 20) / std::max<uint64_t>(bps_, 1));
+  raw_.resize(std::min(n, chunk) * bps_);
+  for (uint64_t done = 0; done < n; done += chunk) {
+    const uint64_t m = std::min(chunk, n - done);
+    PCAONE_HIP(hipMemcpyAsync(raw_.get(), rows + done * bps_, m * bps_, hipMemcpyHostToDevice, stream));
+    launch_interleave(raw_.get(), m, bps_, N, xs_.get() + (j0 + done) * pitch_s_, pitch_s_, stream);
+    PCAONE_HIP(hipStreamSynchronize(stream));
+  }
+}
+
+void Context::generate(const synth::Spec& spec) {
+  // Host writes the packed rows in bounded chunks (same generator as the
+  // .bed writer), then uploads them.
+  synth::validate(spec);
+  reserve(spec.nsamples, spec.nsnps);
+  const auto fst = synth::pop_fst(spec);
+  const auto flags = synth::sample_flags(spec);
+  const synth::MissSpec ms = synth::miss_spec(spec);
+  const uint64_t chunk = std::max<uint64_t>(1, (512ull << 20) / bps_);
+  std::vector<uint8_t> buf(std::min<uint64_t>(chunk, M) * bps_);
+  for (uint64_t j0 = 0; j0 < M; j0 += chunk) {
+    const uint64_t m = std::min(chunk, M - j0);
+    bool bad = false;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t jj = 0; jj < (int64_t)m; ++jj) {
+      const uint64_t j = j0 + (uint64_t)jj;
+      std::vector<uint64_t> thr(spec.npops);
+      uint32_t t = 0;
+      for (;; ++t) {
+        synth::snp_thresholds(spec, fst, j, t, thr.data());
+        if (synth::snp_row(spec, ms, flags, j, t, thr.data(), buf.data() + (uint64_t)jj * bps_)) break;
+        if (t == 63) {
+#pragma omp atomic write
+          bad = true;
+          break;
+        }
+      }
+    }
+    require(!bad, kErrNumeric, "synth: a SNP stayed monomorphic after 64 attempts");
+    upload_rows(buf.data(), j0, m);
+  }
+  finalize();
+}
+
+void Context::finalize() {
+  require(N > 0 && M > 0, kErrState, "no genotypes uploaded");
+  DevBuf<uint32_t> cnt(4 * M);
+  launch_code_counts(xs_.get(), pitch_s_, M, N, cnt.get(), stream);
+  std::vector<uint32_t> h(4 * M);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), h.size() * 4, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  F.assign(M, 0.0);
+  nmiss = 0;
+  for (uint64_t j = 0; j < M; ++j) {
+    const uint64_t c00 = h[4 * j], c01 = h[4 * j + 1], c10 = h[4 * j + 2], c11 = h[4 * j + 3];
+    const uint64_t c = c00 + c10 + c11;
+    // the reference sums BED2GENO values (1 and 0.5, exact) then divides once
+    F[j] = c ? ((double)c00 + 0.5 * (double)c10) / (double)c : 0.0;
+    nmiss += c01;
+  }
+  perm.resize(M);
+  for (uint64_t j = 0; j < M; ++j) perm[j] = (int)j;
+  dF_.resize(M);
+  PCAONE_HIP(hipMemcpyAsync(dF_.get(), F.data(), M * 8, hipMemcpyHostToDevice, stream));
+  upload_col_state(F, std::vector<double>(M, 1.0));
+  clear_impute();
+  build_derived();
+}
+
+void Context::upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc) {
+  host_cf_ = cf;
+  host_sc_ = sc;
+  // padded to rows_s_ so window tiles may read past M
+  std::vector<double> pc(rows_s_, 0.0), ps(rows_s_, 0.0);
+  std::copy(cf.begin(), cf.end(), pc.begin());
+  std::copy(sc.begin(), sc.end(), ps.begin());
+  cF_.resize(rows_s_);
+  sc_.resize(rows_s_);
+  PCAONE_HIP(hipMemcpyAsync(cF_.get(), pc.data(), rows_s_ * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipMemcpyAsync(sc_.get(), ps.data(), rows_s_ * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::build_derived() {
+  xt_.resize(rows_t_ * pitch_t_);
+  launch_transpose_codes(xs_.get(), pitch_s_, M, N, xt_.get(), pitch_t_, rows_t_, stream);
+  if (nmiss == 0) {
+    rp_s_.resize(M + 1);
+    rp_t_.resize((uint64_t)N + 1);
+    PCAONE_HIP(hipMemsetAsync(rp_s_.get(), 0, (M + 1) * 8, stream));
+    PCAONE_HIP(hipMemsetAsync(rp_t_.get(), 0, ((uint64_t)N + 1) * 8, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  auto lists = [&](const uint32_t* x, uint64_t pitch, uint64_t nrows, uint64_t ncodes, DevBuf<uint64_t>& rp,
+                   DevBuf<uint32_t>& ix) {
+    DevBuf<uint32_t> cnt(4 * nrows);
+    launch_code_counts(x, pitch, nrows, ncodes, cnt.get(), stream);
+    std::vector<uint32_t> h(4 * nrows);
+    PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), h.size() * 4, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    std::vector<uint64_t> p(nrows + 1, 0);
+    for (uint64_t r = 0; r < nrows; ++r) p[r + 1] = p[r] + h[4 * r + 1];
+    rp.resize(nrows + 1);
+    PCAONE_HIP(hipMemcpyAsync(rp.get(), p.data(), (nrows + 1) * 8, hipMemcpyHostToDevice, stream));
+    ix.resize(std::max<uint64_t>(p[nrows], 1));
+    launch_missing_lists(x, pitch, nrows, ncodes, rp.get(), ix.get(), stream);
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    return p[nrows];
+  };
+  const uint64_t ns = lists(xs_.get(), pitch_s_, M, N, rp_s_, ix_s_);
+  const uint64_t nt = lists(xt_.get(), pitch_t_, N, M, rp_t_, ix_t_);
+  require(ns == nmiss && nt == nmiss, kErrInternal, "missing-entry lists disagree with the code counts");
+}
+
+void Context::filter_snps(const std::vector<uint32_t>& keep) {
+  require(!keep.empty(), kErrArgument, "no SNPs left after filtering!");
+  for (uint32_t j : keep) require(j < M, kErrArgument, "filter index out of range");
+  DevBuf<uint32_t> idx(keep.size());
+  PCAONE_HIP(hipMemcpyAsync(idx.get(), keep.data(), keep.size() * 4, hipMemcpyHostToDevice, stream));
+  const uint64_t newM = keep.size();
+  const uint64_t new_rows = round_up(newM, kRowPad) + kRowPad;
+  tmpx_.resize(new_rows * pitch_s_);
+  launch_fill_rows(tmpx_.get(), pitch_s_, 0, new_rows, stream);
+  launch_gather_rows(xs_.get(), pitch_s_, idx.get(), newM, tmpx_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  std::swap(xs_, tmpx_);
+  tmpx_.release();
+  std::vector<double> Fk(newM);
+  for (uint64_t i = 0; i < newM; ++i) Fk[i] = F[keep[i]];
+  M = newM;
+  kt_ = round_up(M, kKStep);
+  pitch_t_ = kt_ / 16;
+  rows_s_ = new_rows;
+  finalize();
+  F = Fk;  // identical to the recount, kept explicit
+}
+
+void Context::permute_columns(const std::vector<int>& p) {
+  require(p.size() == M, kErrArgument, "permutation size mismatch");
+  std::vector<uint32_t> idx(M);
+  for (uint64_t i = 0; i < M; ++i) {
+    require(p[i] >= 0 && (uint64_t)p[i] < M, kErrArgument, "permutation index out of range");
+    idx[i] = (uint32_t)p[i];
+  }
+  DevBuf<uint32_t> d(M);
+  PCAONE_HIP(hipMemcpyAsync(d.get(), idx.data(), M * 4, hipMemcpyHostToDevice, stream));
+  tmpx_.resize(rows_s_ * pitch_s_);
+  launch_fill_rows(tmpx_.get(), pitch_s_, M, rows_s_, stream);
+  launch_gather_rows(xs_.get(), pitch_s_, d.get(), M, tmpx_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  std::swap(xs_, tmpx_);
+  tmpx_.release();
+  std::vector<double> cf(M), sc(M);
+  std::vector<int> np(M);
+  for (uint64_t i = 0; i < M; ++i) {
+    cf[i] = host_cf_[idx[i]];
+    sc[i] = host_sc_[idx[i]];
+    np[i] = perm[idx[i]];
+  }
+  perm = np;
+  upload_col_state(cf, sc);
+  if (emu_k_ > 0) {  // imputation loadings follow their columns
+    std::vector<double> v(M * emu_k_), w(M * emu_k_);
+    PCAONE_HIP(hipMemcpyAsync(v.data(), vimp_.get(), v.size() * 8, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    for (uint64_t i = 0; i < M; ++i)
+      std::memcpy(&w[i * emu_k_], &v[(uint64_t)idx[i] * emu_k_], emu_k_ * 8);
+    PCAONE_HIP(hipMemcpyAsync(vimp_.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, stream));
+  }
+  build_derived();
+}
+
+void Context::standardize(int ploidy) {
+  require(!host_sc_.empty(), kErrState, "no data");
+  std::vector<double> sc = host_sc_;
+  for (uint64_t i = 0; i < M; ++i) {
+    const double f = F[i];  // reference indexes F by the current column (src/Data.cpp:356)
+    const double sd = std::sqrt(f * (1.0 - f));
+    if (sd > kVarTol) sc[i] *= std::sqrt((double)ploidy) / sd;
+  }
+  upload_col_state(host_cf_, sc);
+}
+
+void Context::set_impute(const double* U, const double* S, const double* V, int kk) {
+  require(kk > 0, kErrArgument, "impute rank must be positive");
+  std::vector<double> us((uint64_t)N * kk), v(M * kk);
+  for (uint64_t i = 0; i < N; ++i)
+    for (int t = 0; t < kk; ++t) us[i * kk + t] = U[(uint64_t)t * N + i] * S[t];
+  for (uint64_t j = 0; j < M; ++j)
+    for (int t = 0; t < kk; ++t) v[j * kk + t] = V[(uint64_t)t * M + j];
+  us_.resize(us.size());
+  vimp_.resize(v.size());
+  PCAONE_HIP(hipMemcpyAsync(us_.get(), us.data(), us.size() * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipMemcpyAsync(vimp_.get(), v.data(), v.size() * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  emu_k_ = kk;
+}
+
+void Context::clear_impute() { emu_k_ = 0; }
+
+void Context::reset_columns() {
+  require(!F.empty(), kErrState, "no data");
+  clear_impute();
+  bool identity = true;
+  for (uint64_t i = 0; i < M && identity; ++i) identity = perm[i] == (int)i;
+  if (!identity) {
+    std::vector<int> inv(M);
+    for (uint64_t i = 0; i < M; ++i) inv[perm[i]] = (int)i;
+    permute_columns(inv);
+  }
+  upload_col_state(F, std::vector<double>(M, 1.0));
+}
+
+// ------------------------------------------------------------------ rsvd
+void Context::configure(Form f, int Lr, const GemmTuning& t) {
+  require(N > 0 && M > 0 && !F.empty(), kErrState, "no genotype data (finalize first)");
+  require(Lr >= 1 && Lr <= 128, kErrArgument, "k + oversamples must be in [1, 128]");
+  require((uint64_t)Lr <= N && (uint64_t)Lr <= M, kErrArgument, "k + oversamples exceeds the matrix dimensions");
+  form = f;
+  L = Lr;
+  lp = (int)round_up((uint64_t)Lr, 16);
+  tuning = t;
+  const int P = form_pieces(f);
+  const uint64_t wb = bytes_of(f);
+  const uint64_t nrows_n = std::max<uint64_t>(rows_t_, ks_);
+  omega_.resize(nrows_n * lp);
+  omega_r_.resize(nrows_n * lp);
+  omega_c_.resize(nrows_n * lp);
+  colsum_.resize(lp);
+  bvec_.resize(lp);
+  wg_pitch_ = ks_;
+  wg_.resize((uint64_t)P * lp * wg_pitch_ * wb);
+  PCAONE_HIP(hipMemsetAsync(wg_.get(), 0, wg_.bytes(), stream));
+  wh_pitch_ = kt_ + kKStep;
+  wh_.resize((uint64_t)P * lp * wh_pitch_ * wb);
+  gt_.resize(rows_s_ * lp);
+  tmp_m_.resize(rows_s_ * lp);
+  vbuf_.resize(rows_s_ * lp);
+  H_.resize(nrows_n * lp);
+  H1_.resize(nrows_n * lp);
+  H2_.resize(nrows_n * lp);
+  tmp_n_.resize(nrows_n * lp);
+  ucur_.resize(nrows_n * lp);
+  upre_.resize(nrows_n * lp);
+  for (auto* b : {&H_, &H1_, &H2_, &gt_})
+    PCAONE_HIP(hipMemsetAsync(b->get(), 0, b->bytes(), stream));
+  small_.resize(2 * 128 * 128);
+  red_cap_ = std::max<uint64_t>(gram_partial_elems(std::max<uint64_t>(M, N), lp), 2048 * 256);
+  red_.resize(red_cap_ + 2 * 128 * 128);
+  // partial buffers: the largest G window is M rows, the H product has rows_t_ rows
+  const int wmg = tuning.wm_g ? tuning.wm_g : gemm_default_wm(f, lp);
+  const int wmh = tuning.wm_h ? tuning.wm_h : gemm_default_wm(f, lp);
+  tuning.wm_g = wmg;
+  tuning.wm_h = wmh;
+  const uint64_t tile_g = gemm_row_tile(f, wmg), tile_h = gemm_row_tile(f, wmh);
+  require(rows_t_ % tile_h == 0, kErrInternal, "row padding");
+  if (!tuning.kchunk_g) {
+    const uint64_t wgs = round_up(M, tile_g) / tile_g;
+    uint64_t nch = wgs >= 1024 ? 1 : (2048 + wgs - 1) / wgs;
+    nch = std::min<uint64_t>(nch, ks_ / kKStep);
+    tuning.kchunk_g = round_up((ks_ + nch - 1) / nch, kKStep);
+  }
+  if (!tuning.kchunk_h) tuning.kchunk_h = std::min<uint64_t>(16384, kt_);
+  const uint64_t nch_g = (ks_ + tuning.kchunk_g - 1) / tuning.kchunk_g;
+  const uint64_t nch_h = (wh_pitch_ + tuning.kchunk_h - 1) / tuning.kchunk_h;
+  const uint64_t pg = nch_g * (round_up(M, tile_g) + tile_g) * lp;
+  const uint64_t ph = nch_h * rows_t_ * lp;
+  part_.resize(std::max(pg, ph) * form_accbytes(f));
+  h_version_ = 0;
+  gram_version_ = ~0ull;
+  ktimes = KernelTimes{};
+}
+
+void Context::set_t(const la::Mat& T) {
+  std::vector<double> h((size_t)lp * lp, 0.0);
+  for (int i = 0; i < T.n; ++i)
+    for (int j = 0; j < T.n; ++j) h[(size_t)i * lp + j] = T(i, j);
+  PCAONE_HIP(hipMemcpyAsync(small_.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
+}
+
+la::Mat Context::gram_l(const double* X, uint64_t rows, const double* Y) {
+  double* out = red_.get() + red_cap_;
+  launch_gram(X, nullptr, Y, rows, lp, red_.get(), red_cap_, out, stream);
+  std::vector<double> h((size_t)lp * lp);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), out, h.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  la::Mat G(L);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < L; ++j) G(i, j) = h[(size_t)i * lp + j];
+  return G;
+}
+
+void Context::set_omega(const double* om) {
+  require(L > 0, kErrState, "configure() first");
+  std::vector<double> h(ks_ * (uint64_t)lp, 0.0);
+  for (uint64_t i = 0; i < N; ++i)
+    for (int l = 0; l < L; ++l) h[i * lp + l] = om[(uint64_t)l * N + i];
+  PCAONE_HIP(hipMemcpyAsync(omega_c_.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
+  launch_make_pieces(form, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
+  launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::run_gemm(bool g_product, const GemmArgs& a) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (time_kernels) {
+    PCAONE_HIP(hipEventCreate(&e0));
+    PCAONE_HIP(hipEventCreate(&e1));
+    PCAONE_HIP(hipEventRecord(e0, stream));
+  }
+  launch_packed_gemm(form, g_product ? tuning.wm_g : tuning.wm_h, a, stream);
+  if (time_kernels) {
+    PCAONE_HIP(hipEventRecord(e1, stream));
+    PCAONE_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    PCAONE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (g_product ? ktimes.g_ms : ktimes.h_ms) += ms;
+    (g_product ? ktimes.g_launches : ktimes.h_launches) += 1;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+}
+
+void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
+  require(L > 0, kErrState, "configure() first");
+  require(j0 < j1 && j1 <= M, kErrArgument, "sketch: bad SNP range");
+  const uint64_t nwin = j1 - j0;
+  const uint64_t wb = bytes_of(form);
+  const int P = form_pieces(form);
+  const uint64_t tile_g = gemm_row_tile(form, tuning.wm_g), tile_h = gemm_row_tile(form, tuning.wm_h);
+
+  // (1) G[j0:j1] = A Omega : SNP-major rows, K = samples
+  GemmArgs g{};
+  g.X = xs_.get() + j0 * pitch_s_;
+  g.xpitch = pitch_s_;
+  g.rows = round_up(nwin, tile_g);
+  g.Wt = wg_.get();
+  g.wpitch = wg_pitch_;
+  g.wpiece = (uint64_t)lp * wg_pitch_;
+  g.Y = part_.get();
+  g.ychunk = g.rows * lp;
+  g.k_begin = 0;
+  g.k_end = ks_;
+  g.kchunk = tuning.kchunk_g;
+  g.lp = lp;
+  require(j0 + g.rows <= rows_s_, kErrInternal, "G window exceeds padded rows");
+  run_gemm(true, g);
+  const uint64_t nch_g = (ks_ + g.kchunk - 1) / g.kchunk;
+  double* gwin = gt_.get() + j0 * lp;
+  launch_epi_g(form, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0, colsum_.get(),
+               gwin, stream);
+  if (nmiss) {
+    MissArgs m{};
+    m.rp = rp_s_.get() + j0;
+    m.ix = ix_s_.get();
+    m.nrows = nwin;
+    m.k_lo = 0;
+    m.k_hi = N;
+    m.by_snp = true;
+    m.sc = sc_.get() + j0;
+    m.cf = cF_.get() + j0;
+    m.Fclamp = emu_k_ ? dF_.get() + j0 : nullptr;
+    m.US = us_.get();
+    m.V = emu_k_ ? vimp_.get() + j0 * emu_k_ : nullptr;
+    m.kk = emu_k_;
+    m.src = omega_r_.get();
+    m.out = gwin;
+    m.lp = lp;
+    launch_missing_correction(m, stream);
+  }
+  // (2) operand pieces of diag(sc) G~ for the window, at K offset j0 % 128
+  const uint64_t off = j0 % kKStep;
+  const uint64_t klen = round_up(off + nwin, kKStep);
+  PCAONE_HIP(hipMemset2DAsync(wh_.get(), wh_pitch_ * wb, 0, klen * wb, (size_t)P * lp, stream));
+  launch_make_pieces(form, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
+                     (uint64_t)lp * wh_pitch_, gwin, nullptr, stream);
+  launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
+
+  // (3) H contribution: sample-major rows, K = the window's SNPs
+  GemmArgs h{};
+  h.X = xt_.get() + (j0 - off) / 16;
+  h.xpitch = pitch_t_;
+  h.rows = round_up(N, tile_h);
+  h.Wt = wh_.get();
+  h.wpitch = wh_pitch_;
+  h.wpiece = (uint64_t)lp * wh_pitch_;
+  h.Y = part_.get();
+  h.ychunk = h.rows * lp;
+  h.k_begin = 0;
+  h.k_end = klen;
+  h.kchunk = tuning.kchunk_h;
+  h.lp = lp;
+  require(h.rows <= rows_t_, kErrInternal, "H rows exceed padding");
+  run_gemm(false, h);
+  const uint64_t nch_h = (klen + h.kchunk - 1) / h.kchunk;
+  double* target = hmode == 0 ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
+  launch_epi_h(form, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream);
+  if (nmiss) {
+    MissArgs m{};
+    m.rp = rp_t_.get();
+    m.ix = ix_t_.get();
+    m.nrows = N;
+    m.k_lo = j0;
+    m.k_hi = j1;
+    m.by_snp = false;
+    m.sc = sc_.get();
+    m.cf = cF_.get();
+    m.Fclamp = emu_k_ ? dF_.get() : nullptr;
+    m.US = us_.get();
+    m.V = vimp_.get();
+    m.kk = emu_k_;
+    m.src = gt_.get();
+    m.out = target;
+    m.lp = lp;
+    launch_missing_correction(m, stream);
+  }
+  if (hmode == 0) ++h_version_;
+}
+
+void Context::zero_h(int which) {
+  DevBuf<double>& b = which == 1 ? H1_ : H2_;
+  PCAONE_HIP(hipMemsetAsync(b.get(), 0, b.bytes(), stream));
+}
+
+void Context::omega_from(int src) {
+  require(L > 0, kErrState, "configure() first");
+  if (src == 1) {
+    launch_add(H1_.get(), H2_.get(), H_.get(), H_.size(), stream);
+    ++h_version_;
+  }
+  const double* X = H_.get();
+  la::Mat G1;
+  if (gram_version_ == h_version_) {
+    G1 = gram_h_;
+  } else {
+    G1 = gram_l(X, N);
+    gram_h_ = G1;
+    gram_version_ = h_version_;
+  }
+  const la::Mat R1 = la::chol_upper(G1);
+  set_t(la::inv_upper(R1));
+  launch_apply(X, small_.get(), N, lp, tmp_n_.get(), stream);
+  const la::Mat R2 = la::chol_upper(gram_l(tmp_n_.get(), N));
+  const la::Mat Rinv = la::inv_upper(la::matmul(R2, R1));
+  // Householder signs from the top L x L block of Q = X R^-1
+  std::vector<double> top((size_t)L * lp);
+  PCAONE_HIP(hipMemcpyAsync(top.data(), X, top.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  la::Mat Xt(L);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < L; ++j) Xt(i, j) = top[(size_t)i * lp + j];
+  const std::vector<double> s = la::householder_signs(la::matmul(Xt, Rinv));
+  la::Mat T = Rinv;
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < L; ++j) T(i, j) *= s[j];
+  set_t(T);
+  launch_apply(X, small_.get(), N, lp, omega_c_.get(), stream);
+  // flipOmg (src/RSVD.hpp:80-89) against Omg2
+  double* norms = red_.get() + red_cap_;
+  launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
+  std::vector<double> ab(2 * (size_t)lp);
+  PCAONE_HIP(hipMemcpyAsync(ab.data(), norms, ab.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  std::vector<double> sign(lp, 1.0);
+  for (int l = 0; l < L; ++l)
+    if (ab[l] > 2.0 * ab[lp + l]) sign[l] = -1.0;
+  double* dsign = small_.get() + 128 * 128;
+  PCAONE_HIP(hipMemcpyAsync(dsign, sign.data(), lp * 8, hipMemcpyHostToDevice, stream));
+  launch_make_pieces(form, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
+  launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+EpochResult Context::finish_epoch(int k, bool have_prev) {
+  require(k >= 1 && k <= L, kErrArgument, "k out of range");
+  // QR of G twice (src/Halko.cpp:55-65): R = R2 R1
+  const la::Mat R1 = la::chol_upper(gram_l(gt_.get(), M));
+  set_t(la::inv_upper(R1));
+  launch_apply(gt_.get(), small_.get(), M, lp, tmp_m_.get(), stream);
+  const la::Mat R2 = la::chol_upper(gram_l(tmp_m_.get(), M));
+  R_ = la::matmul(R2, R1);
+  const la::Mat Rinv = la::inv_upper(R_);
+  // B B^T = R^-T (H^T H) R^-1; B = R^-T H^T (src/Halko.cpp:68)
+  la::Mat HH;
+  if (gram_version_ == h_version_) {
+    HH = gram_h_;
+  } else {
+    HH = gram_l(H_.get(), N);
+    gram_h_ = HH;
+    gram_version_ = h_version_;
+  }
+  const la::Mat C = la::matmul(la::transpose(Rinv), la::matmul(HH, Rinv));
+  std::vector<double> w;
+  la::sym_eig_desc(C, w, WB_);
+  sigma_.resize(L);
+  for (int i = 0; i < L; ++i) sigma_[i] = std::sqrt(std::max(w[i], 0.0));
+  // Ucur = right singular vectors of B = H R^-1 W_B Sigma^-1 (first k)
+  la::Mat T(L);
+  const la::Mat RW = la::matmul(Rinv, WB_);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < k; ++j) T(i, j) = sigma_[j] > 0 ? RW(i, j) / sigma_[j] : 0.0;
+  set_t(T);
+  launch_apply(H_.get(), small_.get(), N, lp, ucur_.get(), stream);
+  EpochResult r;
+  r.sigma = sigma_;
+  if (have_prev) {
+    // mev(Ucur, Upre) = (1/k) sum_i || Ucur^T upre_i ||   (src/Utils.cpp:194-200)
+    double* out = red_.get() + red_cap_;
+    launch_gram(ucur_.get(), nullptr, upre_.get(), N, lp, red_.get(), red_cap_, out, stream);
+    std::vector<double> h((size_t)lp * lp);
+    PCAONE_HIP(hipMemcpyAsync(h.data(), out, h.size() * 8, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    double m = 0.0;
+    for (int i = 0; i < k; ++i) {
+      double s2 = 0.0;
+      for (int a = 0; a < k; ++a) s2 += h[(size_t)a * lp + i] * h[(size_t)a * lp + i];
+      m += std::sqrt(s2);
+    }
+    r.diff = 1.0 - m / k;
+  }
+  return r;
+}
+
+void Context::save_prev() {
+  PCAONE_HIP(hipMemcpyAsync(upre_.get(), ucur_.get(), (uint64_t)N * lp * 8, hipMemcpyDeviceToDevice, stream));
+}
+
+void Context::download_u(int k, double* U) {
+  std::vector<double> h((uint64_t)N * lp);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), ucur_.get(), h.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  for (uint64_t i = 0; i < N; ++i)
+    for (int j = 0; j < k; ++j) U[(uint64_t)j * N + i] = h[i * lp + j];
+}
+
+void Context::compute_v(int k) {
+  // V = Q U_B[:, :k] = G~ R^-1 W_B[:, :k]   (src/Halko.cpp:84)
+  const la::Mat RW = la::matmul(la::inv_upper(R_), WB_);
+  la::Mat T(L);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < k; ++j) T(i, j) = RW(i, j);
+  set_t(T);
+  launch_apply(gt_.get(), small_.get(), M, lp, vbuf_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::download_v(int k, double* V) {
+  std::vector<double> h(M * (uint64_t)lp);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), vbuf_.get(), h.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  for (uint64_t i = 0; i < M; ++i)
+    for (int j = 0; j < k; ++j) V[(uint64_t)j * M + i] = h[i * lp + j];
+}
+
+}  // namespace pcaone

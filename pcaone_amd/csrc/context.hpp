@@ -1,0 +1,147 @@
+// Device context: the resident genotype matrix in two packed layouts, the
+// per-SNP centering/scaling state, and the randomized-SVD workspace.
+//
+// HBM layout (N samples, M SNPs, codes = 2 bits):
+//   xs  SNP-major interleaved words, rows_s x pitch_s   (product G = A * Omega)
+//   xt  sample-major interleaved words, rows_t x pitch_t (product H = A^T * G)
+//   cF, sc, F   per-column centering frequency, scale, and allele frequency
+//               in the reference's (unpermuted) SNP order
+//   miss_s/t    CSR lists of missing entries by SNP and by sample
+//   omega       Omega (FP64, N x lp, row-major) == Omg2 of the reference
+//   omega_r     the Omega actually multiplied (sum of operand pieces)
+//   wg          Omega operand pieces, transposed [piece][col][sample]
+//   gt          G~ (FP64, M x lp) = the G actually multiplied by A^T
+//   wh          window operand pieces of diag(sc) G~, transposed [piece][col][snp]
+//   H, H1, H2, Hsum   FP64 N x lp
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "gemm.hpp"
+#include "linalg_host.hpp"
+
+namespace pcaone {
+
+namespace synth {
+struct Spec;
+}
+
+struct GemmTuning {
+  int wm_g = 0;          // wave tile of the G product (0 = default)
+  int wm_h = 0;          // wave tile of the H product
+  uint64_t kchunk_g = 0; // split-K chunk of the G product in codes (0 = auto)
+  uint64_t kchunk_h = 0; // split-K chunk of the H product in codes (0 = auto)
+};
+
+struct EpochResult {
+  std::vector<double> sigma;  // L singular values of B, descending
+  double diff = 0.0;          // 1 - mev(Ucur, Upre) (valid if have_prev)
+};
+
+class Context {
+ public:
+  explicit Context(int device);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+
+  // ---------------------------------------------------------------- data
+  void reserve(uint32_t nsamples, uint64_t nsnps);
+  void upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n);  // raw .bed rows (bps bytes each)
+  void generate(const synth::Spec& spec);                          // synthetic data straight into HBM
+  // Counts codes, sets F (FilePlink.cpp:37-55), resets the column state
+  // (identity permutation, cF = F, sc = 1) and builds xt and the missing lists.
+  void finalize();
+  void filter_snps(const std::vector<uint32_t>& keep);  // keep SNPs (Data.cpp:88-105)
+  void permute_columns(const std::vector<int>& perm);   // column i <- column perm[i] (RSVD.hpp:61-78)
+  void standardize(int ploidy);                         // Data.cpp:351-362 semantics
+  void set_impute(const double* U, const double* S, const double* V, int kk);  // Data.cpp:318-348 (column order)
+  void clear_impute();
+  // Back to the state right after reading: original SNP order, centered and
+  // unscaled, no imputation (each run starts from freshly read data).
+  void reset_columns();
+
+  uint32_t N = 0;
+  uint64_t M = 0;
+  uint64_t nmiss = 0;
+  std::vector<double> F;       // unpermuted allele frequencies (after filtering)
+  std::vector<int> perm;       // device column i holds SNP perm[i]
+  double p_miss() const { return N && M ? (double)nmiss / ((double)N * (double)M) : 0.0; }
+
+  // ---------------------------------------------------------------- rsvd
+  void configure(Form f, int L, const GemmTuning& t = {});
+  Form form = Form::FP32;
+  int L = 0, lp = 0;
+  GemmTuning tuning;
+
+  void set_omega(const double* omega_colmajor);  // N x L, also Omg2 := Omega
+  // G rows [j0, j1) = A[j0:j1] Omega and the H contribution of those SNPs:
+  // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ...
+  void sketch(uint64_t j0, uint64_t j1, int hmode);
+  // Omega <- thin-Q(src) with Householder signs, then flipOmg against Omg2;
+  // src 0: H, 1: H1 + H2 (also stored as the current H).
+  void omega_from(int src);
+  void zero_h(int which);  // 1: H1, 2: H2
+  // QR of G twice, B = R^-T H^T, eigen-decomposition of B B^T, Ucur; mev
+  // against the previous Ucur if have_prev.
+  EpochResult finish_epoch(int k, bool have_prev);
+  void save_prev();  // Upre := Ucur
+  // Outputs of the last finish_epoch: U (N x k), V (M x k), column-major FP64.
+  void download_u(int k, double* U);
+  void compute_v(int k);  // V := Q U_B[:, :k] on device
+  void download_v(int k, double* V);
+  const double* device_v() const { return vbuf_.get(); }
+
+  // timing of the hot kernels (HIP events on the context stream)
+  struct KernelTimes {
+    double g_ms = 0, h_ms = 0;
+    uint64_t g_launches = 0, h_launches = 0;
+  } ktimes;
+  bool time_kernels = false;
+
+ private:
+  // data layout
+  uint64_t bps_ = 0;
+  uint64_t ks_ = 0, kt_ = 0;           // K extents (codes) of xs and xt
+  uint64_t pitch_s_ = 0, pitch_t_ = 0; // words per row
+  uint64_t rows_s_ = 0, rows_t_ = 0;   // allocated rows
+  DevBuf<uint32_t> xs_, xt_, tmpx_;
+  DevBuf<uint8_t> raw_;
+  DevBuf<double> dF_, cF_, sc_;
+  DevBuf<uint64_t> rp_s_, rp_t_;
+  DevBuf<uint32_t> ix_s_, ix_t_;
+  // EMU state
+  int emu_k_ = 0;
+  DevBuf<double> us_, vimp_;
+  // rsvd workspace
+  DevBuf<double> omega_, omega_r_, omega_c_, colsum_, bvec_;
+  DevBuf<unsigned char> wg_, wh_;
+  uint64_t wg_pitch_ = 0, wh_pitch_ = 0;
+  DevBuf<double> gt_, H_, H1_, H2_, tmp_n_, tmp_m_, ucur_, upre_, vbuf_;
+  DevBuf<unsigned char> part_;
+  DevBuf<double> small_, red_;
+  uint64_t red_cap_ = 0;
+  // host copies of the last epoch's small factors
+  la::Mat R_;         // R of QR(G), L x L
+  la::Mat WB_;        // eigenvectors of B B^T, L x L
+  std::vector<double> sigma_;
+  uint64_t h_version_ = 0, gram_version_ = ~0ull;
+  la::Mat gram_h_;    // Gram of H (L x L) for h_version_
+
+  void build_derived();  // xt + missing lists from xs
+  void upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc);
+  std::vector<double> host_cf_, host_sc_;
+  la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
+  void set_t(const la::Mat& T);  // uploads an L x L matrix into small_ as lp x lp (zero padded)
+  void run_gemm(bool g_product, const GemmArgs& a);
+};
+
+}  // namespace pcaone

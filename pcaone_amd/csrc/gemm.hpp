@@ -1,0 +1,51 @@
+// Packed-genotype MFMA GEMM: Y[chunk][r][:] = sum_{k in chunk} v(code[r][k]) * W[k][:]
+//
+// X holds 2-bit PLINK codes in the "interleaved" word layout (code e of a
+// 32-bit word sits at bit 8*(e%4) + 2*(e/4), so (w >> 2t) & 0x03030303 puts
+// codes 4t..4t+3 into the four bytes).  Rows are padded to a multiple of
+// kKStep codes with code 01, which decodes to 0.
+//
+// v(code) = 2*(copies of A1 - 1): 00 -> +2, 10 -> 0, 11 -> -2, 01 (missing)
+// -> 0.  These values and their products with BF16 pieces are exact, and the
+// affine offset (and the missing entries) are corrected by the callers'
+// epilogues, so the decode is two byte permutes per four codes.
+//
+// W is stored transposed per piece: Wt[piece][col][k], col < lp, pitch wpitch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pcaone {
+
+constexpr int kKStep = 128;       // codes consumed per K-step (and row padding unit)
+constexpr int kGemmWaves = 4;     // waves per workgroup
+
+enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3 };
+
+inline int form_pieces(Form f) { return f == Form::BF16 ? 2 : 1; }
+inline size_t form_wbytes(Form f) { return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : 2); }
+inline size_t form_accbytes(Form f) { return f == Form::FP64 ? 8 : 4; }
+
+struct GemmArgs {
+  const uint32_t* X;   // interleaved packed rows
+  uint64_t xpitch;     // 32-bit words per row
+  uint64_t rows;       // rows to compute (multiple of the row tile)
+  const void* Wt;      // pieces of W, transposed
+  uint64_t wpitch;     // elements between consecutive cols of one piece
+  uint64_t wpiece;     // elements between pieces
+  void* Y;             // partials [chunk][rows][lp] (float, or double for FP64)
+  uint64_t ychunk;     // elements between chunks of Y
+  uint64_t k_begin;    // first code (multiple of kKStep)
+  uint64_t k_end;      // one past the last code (multiple of kKStep)
+  uint64_t kchunk;     // codes per chunk (multiple of kKStep)
+  int lp;              // padded number of columns (multiple of 16, <= 128)
+};
+
+// Row-tile height (rows per workgroup) for a form and wave tile.
+int gemm_row_tile(Form f, int wm);
+// Default wave tile (row blocks of 16 per wave) for a form.
+int gemm_default_wm(Form f, int lp);
+void launch_packed_gemm(Form f, int wm, const GemmArgs& a, hipStream_t s);
+
+}  // namespace pcaone

@@ -1,0 +1,85 @@
+// Launchers of the non-GEMM kernels (kernels_data.hip, kernels_la.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gemm.hpp"
+
+namespace pcaone {
+
+// ---- data layout (kernels_data.hip) --------------------------------------
+// raw PLINK rows (bps bytes each) -> interleaved rows of `pitch` words; codes of
+// samples >= nsamples become 01.
+void launch_interleave(const uint8_t* raw, uint64_t nrows, uint64_t bps, uint32_t nsamples, uint32_t* out,
+                       uint64_t pitch, hipStream_t s);
+// Fills rows [r0, r1) of an interleaved matrix with code 01.
+void launch_fill_rows(uint32_t* x, uint64_t pitch, uint64_t r0, uint64_t r1, hipStream_t s);
+// Per-row counts of codes 00/01/10/11 over the first `ncodes` codes: cnt[4*r + c].
+void launch_code_counts(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint64_t ncodes, uint32_t* cnt,
+                        hipStream_t s);
+// SNP-major (rows = SNPs, codes = samples) -> sample-major (rows = samples,
+// codes = SNPs).  Rows nsnps.. of `xt` get code 01.
+void launch_transpose_codes(const uint32_t* xs, uint64_t pitch_s, uint64_t nsnps, uint64_t nsamples, uint32_t* xt,
+                            uint64_t pitch_t, uint64_t rows_t, hipStream_t s);
+// out row i = in row idx[i] (interleaved rows, whole words).
+void launch_gather_rows(const uint32_t* in, uint64_t pitch, const uint32_t* idx, uint64_t nrows, uint32_t* out,
+                        hipStream_t s);
+// Missing-entry lists: for row r, the positions k < ncodes whose code is 01, in
+// increasing order, written at ix[rp[r] ..].
+void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint64_t ncodes, const uint64_t* rp,
+                          uint32_t* ix, hipStream_t s);
+
+// ---- linear algebra on tall FP64 row-major matrices (kernels_la.hip) -----
+// Splits X (rows x lp, row-major FP64) into MFMA operand pieces, transposed:
+// Wt[p][l][r] for r < rows_pad (rows >= nreal written as 0).  Row r is first
+// multiplied by rowscale[r] (null = 1) and column l by colsign[l] (null = 1).
+// rounded (nullable, row-major rows x lp) receives sum(pieces) / rowscale.
+void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
+                        const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
+                        double* signed_out, hipStream_t s);
+// G epilogue: out[j][l] = alpha_j * sum_c P[c][j][l] + beta_j * colsum[l]  (j < nreal; 0 above)
+// with alpha_j = sc_j / 4, beta_j = sc_j * (1/2 - cf_j).
+void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
+                  int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s);
+// H epilogue: t = sum_c P[c][i][l] / 4 + b[l]; mode 0: out = t, 1: out += t (i < nreal; rows above = 0 in mode 0).
+void launch_epi_h(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
+                  int lp, const double* b, double* out, int mode, hipStream_t s);
+// Column sums out[l] = sum_{r in [r0,r1)} w_r * X[r][l] (w null = 1), two-stage, deterministic.
+void launch_colsum(const double* X, uint64_t r0, uint64_t r1, int lp, const double* w_sc, const double* w_cf,
+                   double* partial, double* out, hipStream_t s);
+// Gram-type reduction: out = (X T)^T (Y) over rows [0, nrows), where Y = Y0 if
+// given else X T; T null = identity.  out is lp x lp row-major.
+void launch_gram(const double* X, const double* T, const double* Y0, uint64_t nrows, int lp, double* partial,
+                 uint64_t partial_cap, double* out, hipStream_t s);
+uint64_t gram_partial_elems(uint64_t nrows, int lp);
+// Y = X T (rows x lp) for rows [0, nrows); T lp x lp row-major; rows above nrows untouched.
+void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, double* Y, hipStream_t s);
+// flipOmg norms: a[l] = sum_i |O2 - Q|, b[l] = sum_i |O2 + Q|.
+void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
+                       hipStream_t s);
+// Y = X + Z elementwise (n elements).
+void launch_add(const double* X, const double* Z, double* Y, uint64_t n, hipStream_t s);
+// Missing-entry corrections.  For row r (target rows) with list entries k:
+//   out[r][:] += sum_k wgt(r,k) * src[k][:]
+// wgt = -beta_c + (emu ? sc_c * clamp(sum_t US[i][t] V[c][t], -F_c, 1-F_c) : 0)
+// where c is the SNP index (r for by_snp, k otherwise) and i the sample index.
+struct MissArgs {
+  const uint64_t* rp;
+  const uint32_t* ix;
+  uint64_t nrows;      // target rows
+  uint64_t k_lo, k_hi; // only list entries with k in [k_lo, k_hi)
+  bool by_snp;         // target rows are SNPs (G correction) or samples (H correction)
+  const double* sc;    // per SNP column
+  const double* cf;    // per SNP column
+  const double* Fclamp;  // per SNP column (EMU clamp bound), null = no EMU
+  const double* US;    // N x kk row-major (U * S), EMU only
+  const double* V;     // M x kk row-major, EMU only
+  int kk;
+  const double* src;   // rows x lp (Omega~ for by_snp, G~ otherwise)
+  double* out;
+  int lp;
+};
+void launch_missing_correction(const MissArgs& a, hipStream_t s);
+
+}  // namespace pcaone

@@ -1,0 +1,269 @@
+// Packed-genotype GEMM on CDNA4 matrix cores (see gemm.hpp for the contract).
+//
+// Workgroup = 4 waves; wave w owns WM row blocks of 16 rows and all NC column
+// blocks of 16.  Per K-step of 128 codes the workgroup stages the W tile
+// (pieces x LP x 128, transposed) in LDS; every lane loads its 32 codes of
+// each of its rows (8 bytes) straight from HBM and decodes them in registers.
+// Lane group q = lane/16 owns codes [32q, 32q+32) of the step, so one 8-byte
+// load feeds all MFMAs of the step; the B fragment of the same K slots is a
+// contiguous run of the transposed W row.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "gemm.hpp"
+
+namespace pcaone {
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// code -> top byte of v in BF16/FP32/FP64: 00 -> +2 (0x40), 01 -> 0, 10 -> 0, 11 -> -2 (0xC0)
+constexpr uint32_t kLutTopByte = 0xC0000040u;
+
+__device__ __forceinline__ uint32_t spread(uint32_t w, int t) { return (w >> (2 * t)) & 0x03030303u; }
+__device__ __forceinline__ uint32_t top_bytes(uint32_t codes4) { return __builtin_amdgcn_perm(0u, kLutTopByte, codes4); }
+
+// Eight BF16 values (codes 8*half .. 8*half+7 of word w).
+__device__ __forceinline__ bf16x8_t decode_bf16x8(uint32_t w, int half) {
+  const uint32_t h0 = top_bytes(spread(w, 2 * half));
+  const uint32_t h1 = top_bytes(spread(w, 2 * half + 1));
+  u32x4_t d;
+  d[0] = __builtin_amdgcn_perm(h0, h0, 0x050C040Cu);
+  d[1] = __builtin_amdgcn_perm(h0, h0, 0x070C060Cu);
+  d[2] = __builtin_amdgcn_perm(h1, h1, 0x050C040Cu);
+  d[3] = __builtin_amdgcn_perm(h1, h1, 0x070C060Cu);
+  return __builtin_bit_cast(bf16x8_t, d);
+}
+
+// Top 32 bits of the FP32/FP64 value of byte e of `tb` (other bytes zero).
+__device__ __forceinline__ uint32_t top_word(uint32_t tb, int e) {
+  return __builtin_amdgcn_perm(tb, tb, ((uint32_t)(4 + e) << 24) | 0x0C0C0Cu);
+}
+
+template <Form F>
+struct FT;
+template <>
+struct FT<Form::BF16> {
+  using w_t = uint16_t;
+  using acc_t = float;
+  static constexpr int P = 2, PAD = 8;
+};
+template <>
+struct FT<Form::BF16X1> {
+  using w_t = uint16_t;
+  using acc_t = float;
+  static constexpr int P = 1, PAD = 8;
+};
+template <>
+struct FT<Form::FP32> {
+  using w_t = float;
+  using acc_t = float;
+  static constexpr int P = 1, PAD = 4;
+};
+template <>
+struct FT<Form::FP64> {
+  using w_t = double;
+  using acc_t = double;
+  static constexpr int P = 1, PAD = 2;
+};
+
+template <Form F, int NC>
+constexpr size_t lds_bytes() {
+  return (size_t)FT<F>::P * NC * 16 * (kKStep + FT<F>::PAD) * sizeof(typename FT<F>::w_t);
+}
+
+template <Form F, int WM, int NC>
+__global__ void __launch_bounds__(256) k_packed_gemm(GemmArgs a) {
+  using T = FT<F>;
+  using w_t = typename T::w_t;
+  using acc_t = typename T::acc_t;
+  constexpr int LP = NC * 16;
+  constexpr int LDSP = kKStep + T::PAD;
+  constexpr int VE = 16 / sizeof(w_t);
+  constexpr int VPR = kKStep / VE;
+  constexpr int NV = T::P * LP * VPR;
+  using accv_t = typename std::conditional<F == Form::FP64, f64x4_t, f32x4_t>::type;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  w_t* wl = reinterpret_cast<w_t*>(smem);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = lane >> 4, r16 = lane & 15;
+  const uint64_t rowbase = (uint64_t)blockIdx.x * (kGemmWaves * WM * 16) + (uint64_t)wave * WM * 16;
+  const uint64_t kb = a.k_begin + (uint64_t)blockIdx.y * a.kchunk;
+  const uint64_t ke = (kb + a.kchunk < a.k_end) ? kb + a.kchunk : a.k_end;
+
+  accv_t acc[WM][NC];
+#pragma unroll
+  for (int rb = 0; rb < WM; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[rb][cb][r] = 0;
+
+  const uint32_t* xr[WM];
+#pragma unroll
+  for (int rb = 0; rb < WM; ++rb) xr[rb] = a.X + (rowbase + rb * 16 + r16) * a.xpitch + 2 * q;
+
+  const w_t* wg = static_cast<const w_t*>(a.Wt);
+
+  for (uint64_t k0 = kb; k0 < ke; k0 += kKStep) {
+    __syncthreads();
+    for (int v = tid; v < NV; v += 256) {
+      const int row = v / VPR, part = v - row * VPR;
+      const int piece = row / LP, col = row - piece * LP;
+      const uint4 val = *reinterpret_cast<const uint4*>(wg + piece * a.wpiece + col * a.wpitch + k0 + part * VE);
+      *reinterpret_cast<uint4*>(wl + row * LDSP + part * VE) = val;
+    }
+    uint2 aw[WM];
+#pragma unroll
+    for (int rb = 0; rb < WM; ++rb) aw[rb] = *reinterpret_cast<const uint2*>(xr[rb] + (k0 >> 4));
+    __syncthreads();
+
+    if constexpr (F == Form::BF16 || F == Form::BF16X1) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        bf16x8_t af[WM];
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(s < 2 ? aw[rb].x : aw[rb].y, s & 1);
+#pragma unroll
+        for (int cb = 0; cb < NC; ++cb) {
+#pragma unroll
+          for (int p = 0; p < T::P; ++p) {
+            const u32x4_t bv =
+                *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP + 32 * q + 8 * s);
+            const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv);
+#pragma unroll
+            for (int rb = 0; rb < WM; ++rb)
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bf, acc[rb][cb], 0, 0, 0);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        uint32_t tb[WM];
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb) tb[rb] = top_bytes(spread(u < 4 ? aw[rb].x : aw[rb].y, u & 3));
+        acc_t av[WM][4];
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if constexpr (F == Form::FP32)
+              av[rb][e] = __uint_as_float(top_word(tb[rb], e));
+            else
+              av[rb][e] = __hiloint2double((int)top_word(tb[rb], e), 0);
+          }
+#pragma unroll
+        for (int cb = 0; cb < NC; ++cb) {
+          const w_t* bp = wl + (cb * 16 + r16) * LDSP + 32 * q + 4 * u;
+          acc_t bw[4];
+          if constexpr (F == Form::FP32) {
+            const float4 b4 = *reinterpret_cast<const float4*>(bp);
+            bw[0] = b4.x, bw[1] = b4.y, bw[2] = b4.z, bw[3] = b4.w;
+          } else {
+            const double2 b0 = *reinterpret_cast<const double2*>(bp);
+            const double2 b1 = *reinterpret_cast<const double2*>(bp + 2);
+            bw[0] = b0.x, bw[1] = b0.y, bw[2] = b1.x, bw[3] = b1.y;
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int rb = 0; rb < WM; ++rb) {
+              if constexpr (F == Form::FP32)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
+              else
+                acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
+            }
+        }
+      }
+    }
+  }
+
+  // Accumulator layout: lane group q, register r holds row 4q + r of a 16x16
+  // block for the FP32-accumulating MFMAs, and row q + 4r for the FP64 one.
+  acc_t* Y = static_cast<acc_t*>(a.Y) + (uint64_t)blockIdx.y * a.ychunk;
+#pragma unroll
+  for (int rb = 0; rb < WM; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = F == Form::FP64 ? q + 4 * r : 4 * q + r;
+        Y[(rowbase + rb * 16 + row) * LP + cb * 16 + r16] = acc[rb][cb][r];
+      }
+}
+
+template <Form F, int WM, int NC>
+void launch_t(const GemmArgs& a, hipStream_t s) {
+  constexpr size_t lds = lds_bytes<F, NC>();
+  static bool attr = false;
+  if (!attr) {
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_packed_gemm<F, WM, NC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  const uint64_t tile = (uint64_t)kGemmWaves * WM * 16;
+  const uint64_t nchunks = (a.k_end - a.k_begin + a.kchunk - 1) / a.kchunk;
+  dim3 grid((unsigned)(a.rows / tile), (unsigned)nchunks);
+  hipLaunchKernelGGL((k_packed_gemm<F, WM, NC>), grid, dim3(256), lds, s, a);
+  check_launch("packed_gemm");
+}
+
+template <Form F, int WM>
+void launch_nc(const GemmArgs& a, hipStream_t s) {
+  switch (a.lp / 16) {
+    case 1: return launch_t<F, WM, 1>(a, s);
+    case 2: return launch_t<F, WM, 2>(a, s);
+    case 3: return launch_t<F, WM, 3>(a, s);
+    case 4: return launch_t<F, WM, 4>(a, s);
+    case 5: return launch_t<F, WM, 5>(a, s);
+    case 6: return launch_t<F, WM, 6>(a, s);
+    case 7: return launch_t<F, WM, 7>(a, s);
+    case 8: return launch_t<F, WM, 8>(a, s);
+    default: fail(kErrArgument, "packed_gemm: lp must be 16..128 in steps of 16");
+  }
+}
+
+template <Form F>
+void launch_wm(int wm, const GemmArgs& a, hipStream_t s) {
+  switch (wm) {
+    case 1: return launch_nc<F, 1>(a, s);
+    case 2: return launch_nc<F, 2>(a, s);
+    case 4:
+      if constexpr (F != Form::FP64) return launch_nc<F, 4>(a, s);
+      [[fallthrough]];
+    default: fail(kErrArgument, "packed_gemm: unsupported wave tile");
+  }
+}
+
+}  // namespace
+
+int gemm_row_tile(Form, int wm) { return kGemmWaves * wm * 16; }
+
+int gemm_default_wm(Form f, int lp) {
+  if (f == Form::FP64) return 1;
+  return lp > 96 ? 2 : 4;
+}
+
+void launch_packed_gemm(Form f, int wm, const GemmArgs& a, hipStream_t s) {
+  require(a.lp >= 16 && a.lp <= 128 && a.lp % 16 == 0, kErrArgument, "packed_gemm: bad lp");
+  require(a.kchunk > 0 && a.kchunk % kKStep == 0 && a.k_begin % kKStep == 0 && a.k_end % kKStep == 0,
+          kErrArgument, "packed_gemm: K range must be multiples of 128");
+  require(a.rows % (uint64_t)gemm_row_tile(f, wm) == 0, kErrArgument, "packed_gemm: rows not a tile multiple");
+  require(a.xpitch % 8 == 0 && a.xpitch * 16 >= a.k_end, kErrArgument, "packed_gemm: X pitch too small");
+  require(a.wpitch % 8 == 0 && a.wpitch >= a.k_end, kErrArgument, "packed_gemm: W pitch too small");
+  if (a.rows == 0 || a.k_end <= a.k_begin) return;
+  switch (f) {
+    case Form::FP64: return launch_wm<Form::FP64>(wm, a, s);
+    case Form::FP32: return launch_wm<Form::FP32>(wm, a, s);
+    case Form::BF16: return launch_wm<Form::BF16>(wm, a, s);
+    case Form::BF16X1: return launch_wm<Form::BF16X1>(wm, a, s);
+  }
+}
+
+}  // namespace pcaone

@@ -1,0 +1,154 @@
+// PLINK input and PCAone output files.
+//   read_plink        FileBed::FileBed + read_all (src/FilePlink.hpp:10-30,
+//                     src/FilePlink.cpp:26-120): the .bed payload goes to HBM
+//                     undecoded; allele frequencies are counted on the device.
+//   MAF filter        Data::filter_snps_resize_F (src/Data.cpp:88-105)
+//   write_outputs     Data::write_eigs_files (src/Data.cpp:211-240),
+//                     save_snps_in_mbim (src/Data.cpp:110-170, in-core PLINK),
+//                     make_plink2_eigenvec_file (src/Utils.cpp:225-238)
+#include "plink_io.hpp"
+
+#include <fstream>
+#include <sstream>
+#include <vector>
+
+#include "common.hpp"
+
+namespace pcaone {
+
+namespace {
+
+uint64_t count_lines(const std::string& path) {
+  std::ifstream f(path);
+  require(f.is_open(), kErrIO, "can not open " + path);
+  uint64_t n = 0;
+  std::string line;
+  while (std::getline(f, line)) ++n;
+  return n;
+}
+
+// Eigen IOFormat(6, DontAlignCols, "\t", "\n"): stream precision 6.
+std::string fmt(double x) {
+  std::ostringstream o;
+  o.precision(6);
+  o << x;
+  return o.str();
+}
+
+void write_matrix(const std::string& path, const std::vector<double>& A, uint64_t rows, int cols) {
+  std::ofstream o(path);
+  require(o.is_open(), kErrIO, "can not open " + path);
+  std::string buf;
+  for (uint64_t r = 0; r < rows; ++r) {
+    buf.clear();
+    for (int c = 0; c < cols; ++c) {
+      if (c) buf += '\t';
+      buf += fmt(A[(uint64_t)c * rows + r]);
+    }
+    buf += '\n';
+    o << buf;
+  }
+}
+
+std::vector<std::string> split_ws(const std::string& s) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == ' ' || ch == '\t') {
+      out.push_back(cur);
+      cur.clear();
+    } else {
+      cur += ch;
+    }
+  }
+  out.push_back(cur);
+  return out;
+}
+
+}  // namespace
+
+PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& params) {
+  PlinkInfo info;
+  info.nsamples = count_lines(prefix + ".fam");
+  info.nsnps_file = count_lines(prefix + ".bim");
+  require(info.nsamples > 0 && info.nsnps_file > 0, kErrIO, "empty .fam or .bim");
+  require(info.nsamples < (1ull << 32), kErrIO, "too many samples");
+  std::ifstream bed(prefix + ".bed", std::ios::binary);
+  require(bed.is_open(), kErrIO, "Cannot open bed file.");
+  unsigned char header[3] = {0, 0, 0};
+  bed.read(reinterpret_cast<char*>(header), 3);
+  require(bed.gcount() == 3 && header[0] == 0x6c && header[1] == 0x1b && header[2] == 0x01, kErrIO,
+          "Incorrect magic number in plink bed file.");
+  const uint64_t bps = bytes_per_snp(info.nsamples);
+  ctx.reserve((uint32_t)info.nsamples, info.nsnps_file);
+  const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / bps);
+  std::vector<uint8_t> buf(std::min(chunk, info.nsnps_file) * bps);
+  for (uint64_t j0 = 0; j0 < info.nsnps_file; j0 += chunk) {
+    const uint64_t m = std::min(chunk, info.nsnps_file - j0);
+    bed.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)(m * bps));
+    require((uint64_t)bed.gcount() == m * bps, kErrIO, "truncated .bed file");
+    ctx.upload_rows(buf.data(), j0, m);
+  }
+  ctx.finalize();
+  if (params.filterSNP) {
+    std::vector<uint32_t> keep;
+    for (uint64_t j = 0; j < ctx.M; ++j) {
+      const double f = ctx.F[j];
+      const double maf = f > 0.5 ? 1 - f : f;  // MAF(a) (src/Common.hpp:26)
+      if (maf > params.maf) keep.push_back((uint32_t)j);
+    }
+    require(!keep.empty(), kErrArgument, "no SNPs left after filtering!");
+    ctx.filter_snps(keep);
+    info.keep = keep;
+  }
+  return info;
+}
+
+void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, const std::string& bfile,
+                   const std::string& out, const PlinkInfo& info) {
+  const uint64_t N = ctx.N, M = ctx.M;
+  const int k = r.k;
+  {
+    std::ofstream o(out + ".sigvals");
+    require(o.is_open(), kErrIO, "can not open " + out + ".sigvals");
+    o << '#' << N << ',' << (r.S.empty() ? 0 : M) << '\n';
+    for (double s : r.S) o << fmt(s) << '\n';
+  }
+  {
+    std::ofstream o(out + ".eigvals");
+    for (double e : r.E) o << fmt(e) << '\n';
+  }
+  write_matrix(out + ".eigvecs", r.U, N, k);
+  if (params.printv) {
+    // .mbim: .bim line + allele frequency, kept SNPs only
+    std::ifstream bim(bfile + ".bim");
+    std::ofstream mb(out + ".mbim");
+    if (bim.is_open() && mb.is_open()) {
+      std::string line;
+      uint64_t src = 0, kept = 0;
+      while (std::getline(bim, line) && kept < M) {
+        if (info.keep.empty() || info.keep[kept] == src) {
+          mb << line << '\t' << fmt(ctx.F[kept]) << '\n';
+          ++kept;
+        }
+        ++src;
+      }
+    }
+    write_matrix(out + ".loadings", r.V, M, k);
+  }
+  // .eigvecs2: "#FID IID PC1.." + the .eigvecs rows
+  std::ifstream fam(bfile + ".fam");
+  std::ifstream ev(out + ".eigvecs");
+  std::ofstream o2(out + ".eigvecs2");
+  o2 << "#FID\tIID";
+  for (int i = 0; i < k; ++i) o2 << "\tPC" << i + 1;
+  o2 << '\n';
+  std::string l1, l2;
+  while (std::getline(fam, l1)) {
+    auto tok = split_ws(l1);
+    std::getline(ev, l2);
+    o2 << tok[0] << '\t' << (tok.size() > 1 ? tok[1] : "") << '\t' << l2 << '\n';
+  }
+}
+
+}  // namespace pcaone

@@ -1,0 +1,230 @@
+// Randomized-SVD drivers on the device context (mirror of src/Halko.cpp).
+#include "rsvd.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <numeric>
+#include <random>
+
+#include "common.hpp"
+
+namespace pcaone {
+
+void Param::derive() {
+  oversamples = oversamples > k ? oversamples : k;  // src/Cmd.cpp:216
+  filterSNP = maf > 0;                              // src/Cmd.cpp:220
+  if (filterSNP) require(maf > 0 && maf < 0.5, kErrArgument, "--maf has to be between (0, 0.5)");
+  missme = emu;                                     // src/Cmd.cpp:228-232
+  if (!missme) maxiter = 0;
+  require(bands >= 4 && bands % 2 == 0, kErrArgument,
+          "the -w/--batches must be a power of 2 and the minimun is 4.");  // src/Cmd.cpp:235-236
+  perm = (svd == 2 && !noshuffle);                 // src/Cmd.cpp:238
+  require(svd == 1 || svd == 2, kErrArgument, "only --svd 1 and --svd 2 run on this path");
+}
+
+RsvdOpData::RsvdOpData(Context& ctx, const Param& p, int k, int os) : ctx_(ctx), params_(p), nk_(k), os_(os) {
+  ctx_.configure(p.precision, k + os, p.tuning);
+  initOmg();
+}
+
+// src/Halko.cpp:15-23 with src/RSVD.hpp:25-59: libstdc++ default_random_engine
+// seeded with --seed; one draw per element in column-major order.
+void RsvdOpData::initOmg() {
+  const uint64_t n = cols();
+  const int l = size();
+  std::vector<double> om(n * (uint64_t)l);
+  auto rng = std::default_random_engine{};
+  rng.seed(params_.seed);
+  if (params_.rand) {
+    std::normal_distribution<double> d{0, 1};
+    for (auto& x : om) x = d(rng);
+  } else {
+    std::uniform_real_distribution<double> d{-1, 1};
+    for (auto& x : om) x = d(rng);
+  }
+  ctx_.set_omega(om.data());
+}
+
+void RsvdOpData::fit_with_pi() {
+  // src/Data.cpp:318-348: missing entries <- clamp(U S V^T, -F, 1-F).  The
+  // EMU branch only runs with --emu; the device recomputes the imputed values
+  // inside every product from (U S, V).
+  if (!params_.emu) return;
+  require(!U.empty() && !S.empty() && !V.empty(), kErrState, "fit_with_pi before the first decomposition");
+  ctx_.set_impute(U.data(), S.data(), V.data(), (int)S.size());
+}
+
+void RsvdOpData::computeUSV(int p, double tol) {
+  const int nk = ranks();
+  for (int pi = 0; pi <= p; ++pi) {
+    computeGandH(pi);
+    const EpochResult r = ctx_.finish_epoch(nk, pi > 0);
+    if (info) info->epochs = pi + 1, info->total_epochs += 1;
+    if (pi > 0) {
+      const double diff = r.diff;
+      if (info) info->trace.push_back(diff);
+      if (params_.verbose && !params_.missme) std::printf("running of epoch = %d, diff = %f\n", pi, diff);
+      if (diff < tol || pi == p) {
+        if (params_.svd == 2 && std::pow(2, pi) < params_.bands) {
+          p = (int)std::log2(params_.bands);
+        } else {
+          S.assign(r.sigma.begin(), r.sigma.begin() + nk);
+          ctx_.compute_v(nk);
+          if (need_v_host) {
+            V.resize(rows() * (uint64_t)nk);
+            ctx_.download_v(nk, V.data());
+          }
+          break;
+        }
+      } else {
+        ctx_.save_prev();
+      }
+    } else {
+      ctx_.save_prev();
+    }
+  }
+  U.resize(cols() * (uint64_t)nk);
+  ctx_.download_u(nk, U.data());
+}
+
+void NormalRsvdOpData::computeGandH(int pi) {
+  // src/Halko.cpp:99-127 (in-core)
+  if (pi == 0) initOmg();
+  if (pi == 0) {
+    if (update) fit_with_pi();
+    if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
+  }
+  if (pi > 0) ctx_.omega_from(0);
+  ctx_.sketch(0, rows(), 0);
+}
+
+void FancyRsvdOpData::computeGandH(int pi) {
+  // src/Halko.cpp:155-224 (in-core).  Windows starting past the last SNP
+  // (possible when M < bands^2) are treated as empty.
+  const uint64_t bands = params_.bands;
+  if (pi == 0) initOmg();
+  if (std::pow(2, pi) >= (double)bands) {
+    ctx_.zero_h(1);
+    ctx_.zero_h(2);
+  }
+  if (pi == 0) {
+    if (update) fit_with_pi();
+    if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
+    bandsize_ = 1;
+    blocksize_ = (uint64_t)std::ceil((double)rows() / bands);
+    if (params_.perm && !(update && params_.emu)) {
+      std::vector<int> p(rows());
+      std::iota(p.begin(), p.end(), 0);
+      auto rng = std::default_random_engine{};
+      std::shuffle(p.data(), p.data() + p.size(), rng);  // src/RSVD.hpp:61-70
+      ctx_.permute_columns(p);
+    }
+  }
+  bandsize_ = std::min<uint64_t>(bandsize_ * 2, bands);
+  const uint64_t M = rows();
+  for (uint64_t b = 0, i = 1; b < bands; ++b, ++i) {
+    const uint64_t start = b * blocksize_;
+    const uint64_t stop = (b + 1) * blocksize_ >= M ? M - 1 : (b + 1) * blocksize_ - 1;
+    if (start <= stop) ctx_.sketch(start, stop + 1, i <= bandsize_ / 2 ? 1 : 2);
+    const bool adjacent = (pi > 0 && (double)(b + 1) == std::pow(2, pi - 1) && std::pow(2, pi) < (double)bands);
+    if ((b + 1) < bandsize_ && !adjacent) continue;
+    if (!((i == bandsize_) || (i == bandsize_ / 2) || adjacent)) continue;
+    ctx_.omega_from(1);
+    ++omega_updates;
+    if (i == bandsize_) {
+      ctx_.zero_h(1);
+      i = 0;
+    } else {
+      ctx_.zero_h(2);
+    }
+  }
+}
+
+void flip_uv_vbase(std::vector<double>& U, std::vector<double>& V, uint64_t n, uint64_t m, int k) {
+  for (int i = 0; i < k; ++i) {
+    double* v = V.data() + (uint64_t)i * m;
+    uint64_t x = 0;
+    double best = -1.0;
+    for (uint64_t r = 0; r < m; ++r)
+      if (std::fabs(v[r]) > best) best = std::fabs(v[r]), x = r;  // first maximum, as Eigen maxCoeff
+    if (v[x] < 0) {
+      for (uint64_t r = 0; r < m; ++r) v[r] = -v[r];
+      double* u = U.data() + (uint64_t)i * n;
+      for (uint64_t r = 0; r < n; ++r) u[r] = -u[r];
+    }
+  }
+}
+
+double mev_cols(const std::vector<double>& X, const std::vector<double>& Y, uint64_t n, int k) {
+  double res = 0.0;
+  for (int i = 0; i < k; ++i) {
+    double s2 = 0.0;
+    for (int a = 0; a < k; ++a) {
+      double d = 0.0;
+      const double* x = X.data() + (uint64_t)a * n;
+      const double* y = Y.data() + (uint64_t)i * n;
+      for (uint64_t r = 0; r < n; ++r) d += x[r] * y[r];
+      s2 += d * d;
+    }
+    res += std::sqrt(s2);
+  }
+  return res / k;
+}
+
+PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info) {
+  // src/Halko.cpp:271-345 (genetic input, in-core, no LD output)
+  ctx.reset_columns();  // restores the as-read state between runs; not part of the timed work
+  const auto t0 = std::chrono::steady_clock::now();
+  std::unique_ptr<RsvdOpData> rsvd;
+  if (params.svd == 2)
+    rsvd = std::make_unique<FancyRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
+  else
+    rsvd = std::make_unique<NormalRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
+  rsvd->info = info;
+  rsvd->need_v_host = params.printv || params.missme;
+  const int k = (int)params.k;
+  const uint64_t N = ctx.N, M = ctx.M;
+  if (!params.missme) {
+    rsvd->setFlags(false, true);
+    rsvd->computeUSV((int)params.maxp, params.tol);
+  } else {
+    if (ctx.nmiss == 0 && params.verbose) std::printf("warning: there is no missing values\n");
+    rsvd->setFlags(false, false);
+    rsvd->computeUSV((int)params.maxp, params.tol);
+    flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
+    for (uint32_t i = 0; i < params.maxiter; ++i) {
+      rsvd->setFlags(true, false);
+      const std::vector<double> Vpre = rsvd->V;
+      rsvd->computeUSV((int)params.maxp, params.tol);
+      flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
+      const double diff = 1.0 - mev_cols(rsvd->V, Vpre, M, k);
+      if (info) info->em_trace.push_back(diff), info->em_iters = (int)i + 1;
+      if (params.verbose) std::printf("individual allele frequencies estimated iter = %u, diff = %f\n", i + 1, diff);
+      if (diff < params.tolem) break;
+    }
+    if (params.emu) {
+      rsvd->setFlags(true, true);
+      rsvd->computeUSV((int)params.maxp, params.tol);
+      flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
+    }
+  }
+  PcaResult res;
+  res.k = k;
+  res.U = rsvd->U;
+  res.S = rsvd->S;
+  res.E.resize(res.S.size());
+  for (size_t i = 0; i < res.S.size(); ++i) res.E[i] = res.S[i] * res.S[i] / (double)M;  // src/Halko.cpp:339
+  if (rsvd->need_v_host && !rsvd->V.empty()) {
+    // loadings in the original SNP order (src/Data.cpp:226-233)
+    res.V.assign(rsvd->V.size(), 0.0);
+    for (int c = 0; c < k; ++c)
+      for (uint64_t i = 0; i < M; ++i) res.V[(uint64_t)c * M + ctx.perm[i]] = rsvd->V[(uint64_t)c * M + i];
+  }
+  if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+}  // namespace pcaone

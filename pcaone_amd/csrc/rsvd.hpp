@@ -1,0 +1,97 @@
+// Host-side mirror of PCAone's randomized-SVD operator interface, backed by
+// the device context.  Names, flags and control flow follow the reference:
+//   Param                          src/Cmd.hpp:16-98, derived rules src/Cmd.cpp:171-238
+//   RsvdOpData / Normal / Fancy    src/Halko.hpp:6-93, src/Halko.cpp:15-269
+//   run_pca_with_halko             src/Halko.cpp:271-345
+// The genotype matrix and the G/H sketches live in HBM; only U, S (and V
+// when needed) cross back to the host.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+namespace pcaone {
+
+struct Param {
+  int svd = 2;  // 1: single-pass RSVD (sSVD), 2: window-based RSVD (winSVD)
+  uint32_t k = 10, oversamples = 10, maxp = 20, bands = 64, maxiter = 100;
+  double tol = 1e-4, tolem = 1e-5, maf = 0.0;
+  int seed = 112, rand = 1, scale = -9, ploidy = 2, verbose = 0;
+  bool noshuffle = false, emu = false, printv = false;
+  Form precision = Form::FP32;
+  GemmTuning tuning;
+  // derived (src/Cmd.cpp:171-238)
+  bool perm = false, missme = false, filterSNP = false;
+  void derive();
+};
+
+struct RunInfo {
+  int epochs = 0;                 // epochs of the last computeUSV
+  int total_epochs = 0;           // over all computeUSV calls (EMU)
+  int em_iters = 0;
+  std::vector<double> trace;      // diff of every checked epoch (all computeUSV calls)
+  std::vector<double> em_trace;   // EM convergence diffs
+  double seconds = 0.0;           // wall time of run_pca_with_halko
+};
+
+class RsvdOpData {
+ public:
+  RsvdOpData(Context& ctx, const Param& p, int k, int os);
+  virtual ~RsvdOpData() = default;
+
+  uint64_t rows() const { return ctx_.M; }  // SNPs
+  uint64_t cols() const { return ctx_.N; }  // samples
+  int ranks() const { return nk_; }
+  int oversamples() const { return os_; }
+  int size() const { return nk_ + os_; }
+
+  virtual void computeGandH(int pi) = 0;
+  void setFlags(bool is_update, bool is_standardize) {
+    update = is_update;
+    standardize = is_standardize;
+  }
+  void computeUSV(int p, double tol);
+  void initOmg();
+
+  bool update = false, standardize = false;
+  std::vector<double> U, S, V;  // N x k, k, M x k (column-major; V in device column order)
+  bool need_v_host = false;
+  RunInfo* info = nullptr;
+
+ protected:
+  Context& ctx_;
+  const Param& params_;
+  int nk_, os_;
+  void fit_with_pi();  // src/Data.cpp:293-349 via the device impute state
+};
+
+class NormalRsvdOpData : public RsvdOpData {
+ public:
+  using RsvdOpData::RsvdOpData;
+  void computeGandH(int pi) override;
+};
+
+class FancyRsvdOpData : public RsvdOpData {
+ public:
+  using RsvdOpData::RsvdOpData;
+  void computeGandH(int pi) override;
+  uint64_t omega_updates = 0;
+
+ private:
+  uint64_t bandsize_ = 1, blocksize_ = 0;
+};
+
+struct PcaResult {
+  std::vector<double> U, S, E, V;  // V in the reference's SNP order (un-permuted)
+  int k = 0;
+};
+
+PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info);
+
+// flip_UV(U, V, ubase=false) and mev (src/Utils.cpp:118-154, 194-200), column-major.
+void flip_uv_vbase(std::vector<double>& U, std::vector<double>& V, uint64_t n, uint64_t m, int k);
+double mev_cols(const std::vector<double>& X, const std::vector<double>& Y, uint64_t n, int k);
+
+}  // namespace pcaone

@@ -1,0 +1,103 @@
+"""CPU-side checks of libpcaone_amd.so: exported symbols, host FP64 helpers,
+parameter defaults and the no-device error path (no GPU compute here)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pcaone_amd import _lib
+from pcaone_amd import api
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "pcaone_amd.h"
+
+
+def header_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pcaone_\w+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/pcaone_amd.h but not exported"
+
+
+def test_binding_covers_header():
+    assert set(header_functions()) == set(_lib._SIGS)
+
+
+def test_params_default_match_reference():
+    p = api.default_params()
+    # src/Cmd.hpp defaults and src/Cmd.cpp:41-45 (-d defaults to 2)
+    assert (p.svd, p.k, p.oversamples, p.maxp, p.bands, p.maxiter) == (2, 10, 10, 20, 64, 100)
+    assert (p.tol, p.tolem, p.seed, p.rand, p.ploidy) == (1e-4, 1e-5, 112, 1, 2)
+
+
+def test_open_without_device_reports_status():
+    if api.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(_lib.PcaoneError) as e:
+        api.Context(0)
+    assert e.value.status == _lib.ERR_NO_DEVICE
+    assert "no HIP device" in str(e.value)
+
+
+def _host(fn, *arrays, n):
+    outs = [np.zeros(a, dtype=np.float64) for a in arrays]
+    return outs
+
+
+def test_host_sym_eig_matches_lapack():
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 20, 80, 128):
+        A = rng.standard_normal((n, n))
+        A = A @ A.T + np.diag(rng.uniform(0, 1, n))
+        w = np.empty(n)
+        V = np.empty((n, n))
+        _lib.check(_lib.lib().pcaone_host_sym_eig(n, np.ascontiguousarray(A).ctypes.data, w.ctypes.data,
+                                                  V.ctypes.data))
+        wref = np.sort(np.linalg.eigvalsh(A))[::-1]
+        np.testing.assert_allclose(w, wref, rtol=1e-12, atol=1e-12 * wref[0])
+        np.testing.assert_allclose(V @ np.diag(w) @ V.T, A, atol=1e-11 * np.abs(A).max())
+        np.testing.assert_allclose(V.T @ V, np.eye(n), atol=1e-12)
+
+
+def test_host_householder_signs_reproduce_lapack_q():
+    rng = np.random.default_rng(6)
+    for trial in range(200):
+        n = int(rng.integers(30, 200))
+        L = int(rng.integers(1, 30))
+        A = rng.standard_normal((n, L)) * rng.uniform(0.1, 10, L)
+        Qh = np.linalg.qr(A)[0]  # Householder, beta = -sign(alpha) ||x||
+        R = np.linalg.cholesky(A.T @ A).T
+        Qc = A @ np.linalg.inv(R)
+        s = np.empty(L)
+        Y = np.ascontiguousarray(Qc[:L, :L])
+        _lib.check(_lib.lib().pcaone_host_householder_signs(L, Y.ctypes.data, s.ctypes.data))
+        np.testing.assert_allclose(Qc * s, Qh, atol=1e-9)
+
+
+def test_host_cholesky_and_breakdown():
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((12, 12))
+    A = A @ A.T + np.eye(12)
+    R = np.empty((12, 12))
+    _lib.check(_lib.lib().pcaone_host_chol_upper(12, A.ctypes.data, R.ctypes.data))
+    np.testing.assert_allclose(R.T @ R, A, rtol=1e-12, atol=1e-12)
+    assert np.allclose(np.tril(R, -1), 0) and np.all(np.diag(R) > 0)
+    B = -np.eye(3)
+    st = _lib.lib().pcaone_host_chol_upper(3, B.ctypes.data, R.ctypes.data)
+    assert st == 5  # PCAONE_ERR_NUMERIC
+    assert b"Cholesky" in _lib.lib().pcaone_last_error(None)
+
+
+def test_null_arguments_are_rejected():
+    L = _lib.lib()
+    assert L.pcaone_synth_packed(None, None, None) == 1
+    assert L.pcaone_close(None) == 0
+    assert L.pcaone_sketch(None, 0, 1, 0) == 1

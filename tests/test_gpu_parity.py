@@ -1,0 +1,183 @@
+"""GPU parity of the HIP path against the FP64 oracle and the reference's golden outputs.
+
+Tolerances: the FP64 form must reproduce the reference outputs to their print
+precision (6 significant digits); the FP32 and BF16 forms must meet the
+north-star bar against the FP64 oracle on the same Omega: eigenvalue relative
+error <= 1e-4 and every PC correlated >= 0.9999.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_cases, load_meta, params_from_args
+from oracle import pcaone_oracle as O
+from pcaone_amd import api
+
+pytestmark = pytest.mark.gpu
+
+DMAP = np.array([2.0, 0.0, 0.0, -2.0])
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = api.Context(0)
+    yield c
+    c.close()
+
+
+def pack(codes: np.ndarray) -> np.ndarray:
+    m, n = codes.shape
+    bps = (n + 3) // 4
+    out = np.zeros((m, bps), dtype=np.uint8)
+    for e in range(4):
+        sub = codes[:, e::4].astype(np.uint8)
+        out[:, : sub.shape[1]] |= sub << (2 * e)
+    return out
+
+
+# ---------------------------------------------------------------- kernel level
+@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16", "bf16x1"])
+@pytest.mark.parametrize("nrows,ncodes,L", [(1, 1, 1), (37, 127, 20), (300, 129, 80), (513, 1000, 128),
+                                            (64, 4096, 16), (257, 333, 40)])
+def test_packed_gemm_exact(ctx, form, nrows, ncodes, L):
+    say(f"packed_gemm form={form} nrows={nrows} ncodes={ncodes} L={L}")
+    rng = np.random.default_rng(nrows * 7 + ncodes + L)
+    codes = rng.integers(0, 4, (nrows, ncodes)).astype(np.uint8)
+    W = rng.integers(-64, 64, (ncodes, L)) / 8.0  # exact in every operand type
+    Y = ctx.packed_gemm(form, pack(codes), ncodes, W)
+    np.testing.assert_array_equal(Y, DMAP[codes] @ W)
+
+
+def test_packed_gemm_rounding_fp32(ctx):
+    say("packed_gemm fp32 rounding")
+    rng = np.random.default_rng(11)
+    codes = rng.integers(0, 4, (200, 3000)).astype(np.uint8)
+    W = rng.standard_normal((3000, 24))
+    ref = DMAP[codes] @ W
+    for form, tol in (("fp64", 1e-12), ("fp32", 2e-5), ("bf16", 3e-5), ("bf16x1", 2e-2)):
+        Y = ctx.packed_gemm(form, pack(codes), 3000, W)
+        err = np.abs(Y - ref).max() / np.abs(ref).max()
+        say(f"  {form}: max rel err {err:.2e}")
+        assert err < tol
+
+
+# ---------------------------------------------------------------- data level
+def test_allele_freq_bit_exact(ctx, golden_inputs):
+    for name in ("f1", "f4"):
+        say(f"allele_freq {name}")
+        packed, n, m = O.read_plink(golden_inputs[name])
+        ctx.read_bed(golden_inputs[name])
+        F = ctx.allele_freq()
+        Fo = O.allele_freq(O.decode_codes(packed, n))
+        assert np.array_equal(F, Fo)
+        _, _, miss = ctx.dims()
+        assert miss == int((O.decode_codes(packed, n) == 1).sum())
+
+
+def test_allele_freq_ragged_and_edge_rows(ctx):
+    say("allele_freq ragged")
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 5, 129, 1000):
+        codes = rng.integers(0, 4, (70, n)).astype(np.uint8)
+        codes[0] = 1  # all missing -> F = 0
+        codes[1] = 3  # monomorphic
+        ctx.upload_packed(pack(codes), n)
+        assert np.array_equal(ctx.allele_freq(), O.allele_freq(codes))
+
+
+# ---------------------------------------------------------------- operator steps
+def test_operator_epoch_matches_oracle(ctx, golden_inputs):
+    say("operator steps fp64 f1")
+    prefix = golden_inputs["f1"]
+    packed, n, m = O.read_plink(prefix)
+    k, L = 5, 15
+    ctx.read_bed(prefix)
+    ctx.configure("fp64", L)
+    ctx.standardize(2)
+    om = O.omega(112, True, n, L)
+    ctx.set_omega(om)
+    ctx.sketch(0, m, 0)
+    sig, _ = ctx.finish_epoch(k, L, False)
+    d = O.Data.from_packed(packed, n, O.Params(svd=1, k=k))
+    d.standardize_E()
+    G = d.G.T @ om
+    H = d.G @ G
+    Q1, R1 = np.linalg.qr(G)
+    Q2, R2 = np.linalg.qr(Q1)
+    B = np.linalg.solve((R2 @ R1).T, H.T)
+    _, s, vt = np.linalg.svd(B, full_matrices=False)
+    np.testing.assert_allclose(sig, s, rtol=1e-11)
+    U = ctx.get_u(k)
+    np.testing.assert_allclose(O.sign_align(vt.T[:, :k], U), vt.T[:, :k], atol=1e-10)
+    ctx.save_prev()
+    # second epoch: Omega <- thin-Q(H) with flipOmg, then the products again
+    ctx.update_omega(0)
+    ctx.sketch(0, m, 0)
+    sig2, diff = ctx.finish_epoch(k, L, True)
+    om2 = O.flip_omg(om, O.thin_q(H))
+    G = d.G.T @ om2
+    H = d.G @ G
+    Q1, R1 = np.linalg.qr(G)
+    Q2, R2 = np.linalg.qr(Q1)
+    B = np.linalg.solve((R2 @ R1).T, H.T)
+    _, s2, vt2 = np.linalg.svd(B, full_matrices=False)
+    np.testing.assert_allclose(sig2, s2, rtol=1e-11)
+    assert abs(diff - (1 - O.mev(vt2.T[:, :k], vt.T[:, :k]))) < 1e-11
+
+
+# ---------------------------------------------------------------- whole runs vs golden
+def run_case(ctx, golden_inputs, case, form):
+    meta = load_meta(case)
+    params, printv = params_from_args(meta["args"])
+    say(f"golden {case} form={form} args='{meta['args']}'")
+    ctx.read_bed(golden_inputs[meta["input"]], maf=params.maf)
+    p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
+                           tol=params.tol, seed=params.seed, rand=params.rand, bands=params.bands,
+                           noshuffle=int(params.noshuffle), emu=int(params.emu), maxiter=params.maxiter,
+                           tolem=params.tolem, precision=form)
+    return meta, params, printv, ctx.run(p, want_v=printv)
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_golden_fp64(ctx, golden_inputs, case):
+    meta, params, printv, r = run_case(ctx, golden_inputs, case, "fp64")
+    d = GOLDEN / case
+    if meta["trace"]:
+        np.testing.assert_allclose(r.trace, meta["trace"], atol=1.5e-6)
+    if meta["stop_epochs"]:
+        assert r.info["epochs"] == meta["stop_epochs"][-1]
+    if meta["em_trace"]:
+        np.testing.assert_allclose(r.em_trace, meta["em_trace"], atol=1.5e-6)
+    if (d / "eigvals").stat().st_size > 1:
+        np.testing.assert_allclose(r.E, O.read_matrix(d / "eigvals").ravel(), rtol=1e-5)
+        np.testing.assert_allclose(r.S, O.read_matrix(d / "sigvals").ravel(), rtol=1e-5)
+    Uref = O.read_matrix(d / "eigvecs")
+    np.testing.assert_allclose(O.sign_align(Uref, r.U), Uref, atol=2e-6)
+    if printv and (d / "loadings").exists():
+        Vref = O.read_matrix(d / "loadings")
+        np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=2e-6)
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16"])
+@pytest.mark.parametrize("case", ["f1_d1", "f1_d2", "f4_d1_miss", "f4_emu_d1", "cfg1_d1", "cfg1_d2"])
+def test_golden_reduced_precision(ctx, golden_inputs, case, form):
+    meta, params, printv, r = run_case(ctx, golden_inputs, case, form)
+    ref = O.run_plink(golden_inputs[meta["input"]], params)
+    rel = np.abs(r.E - ref.E) / ref.E
+    corr = O.pc_correlations(ref.U, r.U)
+    say(f"  eig rel err max {rel.max():.2e}  min PC corr {corr.min():.8f}")
+    assert rel.max() <= 1e-4
+    assert corr.min() >= 0.9999
+
+
+def test_bad_shapes_are_errors(ctx, golden_inputs):
+    say("error paths")
+    ctx.read_bed(golden_inputs["f1"])
+    with pytest.raises(api._lib.PcaoneError) as e:
+        ctx.run(api.default_params(svd=1, k=150, oversamples=150, maxp=1))
+    assert e.value.status == 1
+    with pytest.raises(api._lib.PcaoneError):
+        ctx.sketch(10, 5, 0)
