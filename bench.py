@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: time to the top-k PCs of PCAone's single-pass randomized SVD
+(--svd 1) on one MI355X, BF16 form against the plain FP32 form at equal error.
+
+Workload (the precision form): 50,000 samples x 1,000,000 SNPs of synthetic
+Balding-Nichols genotypes (8 populations, Fst 0.1, p ~ U(0.05, 0.5)),
+generated straight into HBM as packed 2-bit PLINK codes; k = 40 with
+oversampling 20 raised to 40 by the reference rule (L = 80); 7 power
+iterations forced with --tol-rsvd 0 (8 passes over the data).  One step is
+one complete randomized SVD (top 40 PCs) with the genotypes already resident.
+
+Printed: one JSON line (rank 0).  Errors of the BF16 and FP32 forms are
+measured against the FP64 form of the same run (same Omega).  The CPU
+baseline is the reference's own binary (oracle/_ref/pcaone_ref, built from
+the reference sources) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "time-to-k PCs (s) + HBM/MFMA % of roofline, FP8/BF16 vs FP32 at equal error"
+PEAK_TFLOPS = {"bf16": 2500.0, "bf16x1": 2500.0, "fp32": 157.3, "fp64": 78.6}
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def errors(ref, x):
+    from oracle import pcaone_oracle as O  # checker only
+    rel = np.abs(x.E - ref.E) / ref.E
+    corr = O.pc_correlations(ref.U, x.U)
+    ang = O.principal_angles(ref.U, x.U)
+    return {"eig_rel_err_max": float(rel.max()), "pc_corr_min": float(corr.min()),
+            "principal_angle_max_rad": float(ang.max())}
+
+
+def cpu_baseline(args, spec_full, k, os_, maxp):
+    """Reference binary on a bounded sample (same N, fewer SNPs), scaled to M."""
+    from pcaone_amd import synth
+    ref = ROOT / "oracle" / "_ref" / "pcaone_ref"
+    cores = min(16, os.cpu_count() or 1)
+    m_s = args.cpu_snps
+    spec = synth.Spec(**{**spec_full.__dict__, "nsnps": m_s})
+    with tempfile.TemporaryDirectory() as td:
+        prefix = os.path.join(td, "sample")
+        t0 = time.perf_counter()
+        synth.write_plink(spec, prefix)
+        log(f"cpu sample written ({spec.nsamples} x {m_s}) in {time.perf_counter() - t0:.1f}s")
+        if ref.exists():
+            cmd = [str(ref), "-b", prefix, "-d", "1", "-k", str(k), "--oversamples", str(os_), "--maxp", str(maxp),
+                   "--tol-rsvd", "0", "-n", str(cores), "-o", os.path.join(td, "out"), "-v", "1"]
+            env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+            t0 = time.perf_counter()
+            res = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
+            wall = time.perf_counter() - t0
+            if res.returncode != 0:
+                raise RuntimeError(f"reference binary failed: {res.stderr[-500:]}")
+            m_wall = re.search(r"total elapsed wall time:\s*([0-9.]+)", res.stdout)
+            m_read = re.search(r"total elapsed reading time:\s*([0-9.]+)", res.stdout)
+            t_wall = float(m_wall.group(1)) if m_wall else wall
+            t_read = float(m_read.group(1)) if m_read else 0.0
+            t_compute = t_wall - t_read
+            kind = "reference"
+            what = "oracle/_ref/pcaone_ref (PCAone v0.7.2 path built from its sources, Eigen FP64, OpenMP)"
+        else:
+            from oracle import pcaone_oracle as O
+            packed = synth.packed(spec)
+            t0 = time.perf_counter()
+            data = O.Data.from_packed(packed, spec.nsamples, O.Params(svd=1, k=k, oversamples=os_, maxp=maxp, tol=0))
+            t_read = time.perf_counter() - t0
+            O.run_pca_with_halko(data)
+            t_compute = time.perf_counter() - t0 - t_read
+            t_wall = t_compute + t_read
+            kind = "port"
+            what = "numpy FP64 restatement (oracle/pcaone_oracle.py)"
+    scale = spec_full.nsnps / m_s
+    return {"value": round(t_compute * scale, 3), "unit": "s", "cores": cores, "kind": kind,
+            "sample": (f"{spec.nsamples} samples x {m_s} SNPs, same generator and flags (-d 1 -k {k} "
+                       f"--oversamples {os_} --maxp {maxp} --tol-rsvd 0); measured {t_compute:.2f}s compute "
+                       f"(wall {t_wall:.2f}s minus {t_read:.2f}s reading) scaled x{scale:.0f} linearly in SNPs; {what}")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsamples", type=int, default=50_000)
+    ap.add_argument("--nsnps", type=int, default=1_000_000)
+    ap.add_argument("--k", type=int, default=40)
+    ap.add_argument("--oversamples", type=int, default=20)
+    ap.add_argument("--maxp", type=int, default=7)
+    ap.add_argument("--cpu-snps", type=int, default=10_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of forms to run (profiling)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU only: the randomized-SVD hot path runs on a "
+                                   "single MI355X (multi-GPU sharding is described in DESIGN.md, not run)",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from pcaone_amd import api, synth
+
+    spec = synth.config(3, nsamples=args.nsamples, nsnps=args.nsnps)
+    k, os_, maxp = args.k, args.oversamples, args.maxp
+    L = k + max(os_, k)  # src/Cmd.cpp:216
+    ctx = api.Context(0)
+    t0 = time.perf_counter()
+    ctx.generate(spec)
+    setup_s = time.perf_counter() - t0
+    log(f"generated {spec.nsamples} x {spec.nsnps} genotypes in HBM in {setup_s:.1f}s")
+
+    def params(form):
+        return api.default_params(svd=1, k=k, oversamples=os_, maxp=maxp, tol=0.0, precision=form)
+
+    only = set(args.only.split(",")) if args.only else None
+    results = {}
+
+    def timed(form, steps, warmup):
+        for i in range(warmup):
+            r = ctx.run(params(form))
+            log(f"{form} warmup {i}: {r.info['seconds']:.3f}s")
+        ts = []
+        r = None
+        for i in range(steps):
+            t0 = time.perf_counter()
+            r = ctx.run(params(form))
+            ts.append(time.perf_counter() - t0)
+            log(f"{form} step {i}: {ts[-1]:.3f}s ({r.info['epochs']} epochs)")
+        return r, ts
+
+    head = "bf16"
+    r_bf16, ts_bf16 = (None, [])
+    if only is None or head in only:
+        r_bf16, ts_bf16 = timed(head, args.steps, args.warmup)
+        results[head] = r_bf16
+        # live kernel timing (HIP events on the context stream), separate run
+        ctx.time_kernels(True)
+        rk = ctx.run(params(head))
+        ctx.time_kernels(False)
+        kt = rk.info
+    if only is None or "fp32" in only:
+        r_fp32, ts_fp32 = timed("fp32", 1, 1 if args.warmup else 0)
+        results["fp32"] = r_fp32
+    if (only is None and not args.no_fp64) or (only and "fp64" in only):
+        r64, ts64 = timed("fp64", 1, 0)
+        results["fp64"] = r64
+    if only is not None and head not in only:
+        print(json.dumps({f: {"seconds": results[f].info["seconds"]} for f in results}))
+        return 0
+
+    N, M = spec.nsamples, spec.nsnps
+    flops_launch = 2.0 * N * M * L  # one product over the whole matrix, algorithmic
+    g_avg = kt["g_ms"] / max(kt["g_launches"], 1)
+    h_avg = kt["h_ms"] / max(kt["h_launches"], 1)
+    dom, dom_ms = ("H = A^T G", h_avg) if h_avg >= g_avg else ("G = A Omega", g_avg)
+    achieved = flops_launch / (dom_ms * 1e-3) / 1e12
+    value = float(np.mean(ts_bf16))
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(value * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic Balding-Nichols genotypes generated in HBM (no dataset download)",
+        "config": {"workload": f"precision form: {N} samples x {M} SNPs, 8 populations, --svd 1, k={k}, "
+                               f"oversamples {os_}->{L - k}, L={L}, {maxp} power iterations ({maxp + 1} passes)",
+                   "nsamples": N, "nsnps": M, "k": k, "L": L, "passes": maxp + 1,
+                   "form": "BF16 MFMA, exact genotypes, hi/lo BF16 split of Omega and G, FP32 accumulation"},
+        "roofline": {"bound": "mfma", "kernel": f"packed_gemm ({dom})", "achieved": round(achieved, 2),
+                     "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
+                     "traffic": None, "avg_launch_ms": round(dom_ms, 3),
+                     "algorithmic_flop_per_launch": flops_launch,
+                     "other_product_avg_ms": round(g_avg if dom.startswith("H") else h_avg, 3)},
+        "setup_s": round(setup_s, 2),
+    }
+    if "fp32" in results:
+        t32 = float(np.mean(ts_fp32))
+        out["fp32_form"] = {"seconds": round(t32, 4), "speedup_bf16_over_fp32": round(t32 / value, 2)}
+    if "fp64" in results:
+        ref = results["fp64"]
+        out["fp64_form_seconds"] = round(ts64[0], 3)
+        out["error_vs_fp64"] = {f: errors(ref, results[f]) for f in ("bf16", "fp32") if f in results}
+    if not args.no_cpu:
+        try:
+            log("cpu baseline ...")
+            out["cpu_baseline"] = cpu_baseline(args, spec, k, os_, maxp)
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

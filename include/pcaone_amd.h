@@ -112,8 +112,11 @@ int pcaone_bed_read(pcaone_ctx* ctx, const char* prefix, double maf, uint32_t* n
 int pcaone_bed_reserve(pcaone_ctx* ctx, uint32_t nsamples, uint64_t nsnps);
 int pcaone_bed_upload(pcaone_ctx* ctx, const uint8_t* rows, uint64_t snp_begin, uint64_t nsnps);
 int pcaone_bed_finalize(pcaone_ctx* ctx);
-/* Synthetic data generated into the context (reserve + upload + finalize). */
+/* Synthetic data generated on the device, byte-identical to
+ * pcaone_synth_packed (reserve + upload + finalize). */
 int pcaone_bed_generate(pcaone_ctx* ctx, const pcaone_synth_spec* spec);
+/* The resident genotypes back as .bed rows (nsnps x ceil(N/4) bytes). */
+int pcaone_bed_download(pcaone_ctx* ctx, uint8_t* rows);
 int pcaone_dims(const pcaone_ctx* ctx, uint32_t* nsamples, uint64_t* nsnps, uint64_t* nmissing);
 /* Data::F (src/FilePlink.cpp:37-55), unpermuted SNP order, nsnps values. */
 int pcaone_allele_freq(const pcaone_ctx* ctx, double* F);

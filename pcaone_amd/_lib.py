@@ -56,6 +56,7 @@ _SIGS = {
     "pcaone_bed_upload": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
     "pcaone_bed_finalize": (C.c_int, [_P]),
     "pcaone_bed_generate": (C.c_int, [_P, _P]),
+    "pcaone_bed_download": (C.c_int, [_P, _P]),
     "pcaone_dims": (C.c_int, [_P, _P, _P, _P]),
     "pcaone_allele_freq": (C.c_int, [_P, _P]),
     "pcaone_standardize": (C.c_int, [_P, C.c_int]),

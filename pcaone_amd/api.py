@@ -96,6 +96,12 @@ class Context:
         cs = spec.c()
         self._check(lib().pcaone_bed_generate(self._h, C.byref(cs)))
 
+    def download_packed(self) -> np.ndarray:
+        n, m, _ = self.dims()
+        out = np.empty((m, (n + 3) // 4), dtype=np.uint8)
+        self._check(lib().pcaone_bed_download(self._h, _ptr(out)))
+        return out
+
     def dims(self):
         n, m, miss = C.c_uint32(), C.c_uint64(), C.c_uint64()
         self._check(lib().pcaone_dims(self._h, C.byref(n), C.byref(m), C.byref(miss)))

@@ -208,6 +208,13 @@ int pcaone_bed_generate(pcaone_ctx* c, const pcaone_synth_spec* spec) {
   return guard(c, [&](Context& x) { x.generate(to_spec(spec)); });
 }
 
+int pcaone_bed_download(pcaone_ctx* c, uint8_t* rows) {
+  return guard(c, [&](Context& x) {
+    require(rows != nullptr, kErrArgument, "null output");
+    x.download_rows(rows);
+  });
+}
+
 int pcaone_dims(const pcaone_ctx* c, uint32_t* nsamples, uint64_t* nsnps, uint64_t* nmissing) {
   return guard(const_cast<pcaone_ctx*>(c), [&](Context& x) {
     if (nsamples) *nsamples = x.N;

@@ -69,37 +69,61 @@ void Context::upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n) {
 }
 
 void Context::generate(const synth::Spec& spec) {
-  // Host writes the packed rows in bounded chunks (same generator as the
-  // .bed writer), then uploads them.
+  // Per-SNP population frequencies (Beta draws) on the host, genotypes on the
+  // device with the shared integer generator; SNPs that come out
+  // monomorphic are redrawn with the next attempt, as the host writer does.
   synth::validate(spec);
   reserve(spec.nsamples, spec.nsnps);
+  const uint32_t P = spec.npops;
   const auto fst = synth::pop_fst(spec);
   const auto flags = synth::sample_flags(spec);
   const synth::MissSpec ms = synth::miss_spec(spec);
-  const uint64_t chunk = std::max<uint64_t>(1, (512ull << 20) / bps_);
-  std::vector<uint8_t> buf(std::min<uint64_t>(chunk, M) * bps_);
-  for (uint64_t j0 = 0; j0 < M; j0 += chunk) {
-    const uint64_t m = std::min(chunk, M - j0);
-    bool bad = false;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t jj = 0; jj < (int64_t)m; ++jj) {
-      const uint64_t j = j0 + (uint64_t)jj;
-      std::vector<uint64_t> thr(spec.npops);
-      uint32_t t = 0;
-      for (;; ++t) {
-        synth::snp_thresholds(spec, fst, j, t, thr.data());
-        if (synth::snp_row(spec, ms, flags, j, t, thr.data(), buf.data() + (uint64_t)jj * bps_)) break;
-        if (t == 63) {
-#pragma omp atomic write
-          bad = true;
-          break;
-        }
-      }
+  std::vector<uint64_t> thr(M * P);
+  std::vector<uint32_t> att(M, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t j = 0; j < (int64_t)M; ++j) synth::snp_thresholds(spec, fst, (uint64_t)j, 0, &thr[(uint64_t)j * P]);
+  DevBuf<uint64_t> dthr(M * P);
+  DevBuf<uint32_t> datt(M), drows(M), cnt(4 * M);
+  DevBuf<uint8_t> dflags(N);
+  PCAONE_HIP(hipMemcpyAsync(dthr.get(), thr.data(), thr.size() * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipMemcpyAsync(datt.get(), att.data(), M * 4, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipMemcpyAsync(dflags.get(), flags.data(), N, hipMemcpyHostToDevice, stream));
+  launch_synth(spec.seed, N, P, dthr.get(), datt.get(), dflags.get(), ms, nullptr, M, xs_.get(), pitch_s_, stream);
+  std::vector<uint32_t> h(4 * M);
+  for (uint32_t round = 0;; ++round) {
+    launch_code_counts(xs_.get(), pitch_s_, M, N, cnt.get(), stream);
+    PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), h.size() * 4, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    std::vector<uint32_t> redo;
+    for (uint64_t j = 0; j < M; ++j) {
+      const uint32_t c00 = h[4 * j], c10 = h[4 * j + 2], c11 = h[4 * j + 3];
+      if (!(c10 > 0 || (c00 > 0 && c11 > 0))) redo.push_back((uint32_t)j);
     }
-    require(!bad, kErrNumeric, "synth: a SNP stayed monomorphic after 64 attempts");
-    upload_rows(buf.data(), j0, m);
+    if (redo.empty()) break;
+    require(round < 63, kErrNumeric, "synth: a SNP stayed monomorphic after 64 attempts");
+    for (uint32_t j : redo) {
+      ++att[j];
+      synth::snp_thresholds(spec, fst, j, att[j], &thr[(uint64_t)j * P]);
+    }
+    PCAONE_HIP(hipMemcpyAsync(dthr.get(), thr.data(), thr.size() * 8, hipMemcpyHostToDevice, stream));
+    PCAONE_HIP(hipMemcpyAsync(datt.get(), att.data(), M * 4, hipMemcpyHostToDevice, stream));
+    PCAONE_HIP(hipMemcpyAsync(drows.get(), redo.data(), redo.size() * 4, hipMemcpyHostToDevice, stream));
+    launch_synth(spec.seed, N, P, dthr.get(), datt.get(), dflags.get(), ms, drows.get(), redo.size(), xs_.get(),
+                 pitch_s_, stream);
   }
   finalize();
+}
+
+void Context::download_rows(uint8_t* out) {
+  require(!F.empty(), kErrState, "no data");
+  const uint64_t chunk = std::max<uint64_t>(1, (256ull << 20) / bps_);
+  raw_.resize(std::min(chunk, M) * bps_);
+  for (uint64_t j0 = 0; j0 < M; j0 += chunk) {
+    const uint64_t m = std::min(chunk, M - j0);
+    launch_deinterleave(xs_.get() + j0 * pitch_s_, pitch_s_, m, N, raw_.get(), bps_, stream);
+    PCAONE_HIP(hipMemcpyAsync(out + j0 * bps_, raw_.get(), m * bps_, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+  }
 }
 
 void Context::finalize() {

@@ -57,6 +57,7 @@ class Context {
   void reserve(uint32_t nsamples, uint64_t nsnps);
   void upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n);  // raw .bed rows (bps bytes each)
   void generate(const synth::Spec& spec);                          // synthetic data straight into HBM
+  void download_rows(uint8_t* out);  // current column order, PLINK row format
   // Counts codes, sets F (FilePlink.cpp:37-55), resets the column state
   // (identity permutation, cF = F, sc = 1) and builds xt and the missing lists.
   void finalize();

@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "gemm.hpp"
+#include "synth.hpp"
 
 namespace pcaone {
 
@@ -25,6 +26,14 @@ void launch_transpose_codes(const uint32_t* xs, uint64_t pitch_s, uint64_t nsnps
 // out row i = in row idx[i] (interleaved rows, whole words).
 void launch_gather_rows(const uint32_t* in, uint64_t pitch, const uint32_t* idx, uint64_t nrows, uint32_t* out,
                         hipStream_t s);
+// Synthetic genotype rows (rows[r] or r for r < nrows) from the shared
+// generator; thr is nsnps x npops, attempts per SNP, flags per sample.
+void launch_synth(uint64_t seed, uint32_t nsamples, uint32_t npops, const uint64_t* thr, const uint32_t* attempts,
+                  const uint8_t* flags, const synth::MissSpec& ms, const uint32_t* rows, uint64_t nrows, uint32_t* out,
+                  uint64_t pitch, hipStream_t s);
+// Interleaved rows -> PLINK rows of bps bytes.
+void launch_deinterleave(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint32_t nsamples, uint8_t* raw,
+                         uint64_t bps, hipStream_t s);
 // Missing-entry lists: for row r, the positions k < ncodes whose code is 01, in
 // increasing order, written at ix[rp[r] ..].
 void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint64_t ncodes, const uint64_t* rp,

@@ -6,6 +6,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "synth.hpp"
 
 namespace pcaone {
 namespace {
@@ -143,6 +144,48 @@ __global__ void k_missing_lists(const uint32_t* __restrict__ x, uint64_t pitch, 
   }
 }
 
+// Synthetic genotypes straight into interleaved rows: one thread per output
+// word of row rows[r] (or r when rows is null).
+__global__ void k_synth(uint64_t seed, uint32_t nsamples, uint32_t npops, const uint64_t* __restrict__ thr,
+                        const uint32_t* __restrict__ attempts, const uint8_t* __restrict__ flags,
+                        synth::MissSpec ms, const uint32_t* __restrict__ rows, uint64_t nrows,
+                        uint32_t* __restrict__ out, uint64_t pitch) {
+  const uint64_t total = nrows * pitch;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = t / pitch, w = t - r * pitch;
+    const uint64_t j = rows ? rows[r] : r;
+    const uint32_t att = attempts[j];
+    uint32_t word = 0;
+    for (int e = 0; e < 16; ++e) {
+      const uint64_t i = w * 16 + e;
+      uint32_t code = 1u;
+      if (i < nsamples) {
+        const uint32_t pop = (uint32_t)(i * npops / nsamples);
+        code = synth::genotype_code(seed, att, j, i, thr[j * npops + pop], ms, flags[i] != 0);
+      }
+      word |= code << ipos(e);
+    }
+    out[j * pitch + w] = word;
+  }
+}
+
+// Inverse of k_interleave: interleaved rows -> PLINK rows (padding bits zero).
+__global__ void k_deinterleave(const uint32_t* __restrict__ x, uint64_t pitch, uint64_t nrows, uint32_t nsamples,
+                               uint8_t* __restrict__ raw, uint64_t bps) {
+  const uint64_t total = nrows * bps;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = t / bps, b = t - r * bps;
+    uint8_t v = 0;
+    for (int s = 0; s < 4; ++s) {
+      const uint64_t i = b * 4 + s;
+      if (i >= nsamples) break;
+      const uint32_t word = x[r * pitch + i / 16];
+      v |= (uint8_t)(((word >> ipos((int)(i & 15))) & 3u) << (2 * s));
+    }
+    raw[t] = v;
+  }
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 65535u * 4) {
   uint64_t g = (n + block - 1) / block;
   if (g > cap) g = cap;
@@ -192,6 +235,23 @@ void launch_gather_rows(const uint32_t* in, uint64_t pitch, const uint32_t* idx,
   if (!nrows) return;
   hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(nrows * pitch)), dim3(256), 0, s, in, pitch, idx, nrows, out);
   check_launch("gather_rows");
+}
+
+void launch_synth(uint64_t seed, uint32_t nsamples, uint32_t npops, const uint64_t* thr, const uint32_t* attempts,
+                  const uint8_t* flags, const synth::MissSpec& ms, const uint32_t* rows, uint64_t nrows, uint32_t* out,
+                  uint64_t pitch, hipStream_t s) {
+  if (!nrows) return;
+  hipLaunchKernelGGL(k_synth, dim3(grid_for(nrows * pitch)), dim3(256), 0, s, seed, nsamples, npops, thr, attempts,
+                     flags, ms, rows, nrows, out, pitch);
+  check_launch("synth");
+}
+
+void launch_deinterleave(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint32_t nsamples, uint8_t* raw,
+                         uint64_t bps, hipStream_t s) {
+  if (!nrows) return;
+  hipLaunchKernelGGL(k_deinterleave, dim3(grid_for(nrows * bps)), dim3(256), 0, s, x, pitch, nrows, nsamples, raw,
+                     bps);
+  check_launch("deinterleave");
 }
 
 void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint64_t ncodes, const uint64_t* rp,

@@ -181,3 +181,20 @@ def test_bad_shapes_are_errors(ctx, golden_inputs):
     assert e.value.status == 1
     with pytest.raises(api._lib.PcaoneError):
         ctx.sketch(10, 5, 0)
+
+
+@pytest.mark.parametrize("spec_kw", [dict(seed=9, nsamples=301, nsnps=777, npops=3),
+                                     dict(seed=10, nsamples=5, nsnps=300, npops=2, freq_lo=0.01, freq_hi=0.05,
+                                          fst_lo=0.05, fst_hi=0.15),
+                                     dict(seed=11, nsamples=130, nsnps=400, npops=3, miss_rate=0.1,
+                                          high_miss_frac=0.2, high_miss_rate=0.3)])
+def test_device_generator_matches_host(ctx, spec_kw):
+    from pcaone_amd import synth
+    spec = synth.Spec(**spec_kw)
+    say(f"device generator {spec_kw}")
+    host, att = synth.packed(spec, with_attempts=True)
+    ctx.generate(spec)
+    dev = ctx.download_packed()
+    assert np.array_equal(dev, host)
+    if spec.nsamples == 5:
+        assert att.max() > 0  # the redraw path was exercised
