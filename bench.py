@@ -109,6 +109,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only", default="", help="comma list of forms to run (profiling)")
+    ap.add_argument("--wm-g", type=int, default=0, help="GEMM wave tile of the G product (0 = tuned default)")
+    ap.add_argument("--wm-h", type=int, default=0, help="GEMM wave tile of the H product (0 = tuned default)")
+    ap.add_argument("--waves-g", type=int, default=0, help="waves per workgroup of the G product (0 = default)")
+    ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
+    ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo) or bf16mix")
+    ap.add_argument("--kchunk-g", type=int, default=0, help="split-K chunk of the G product in codes (0 = auto)")
+    ap.add_argument("--kchunk-h", type=int, default=0, help="split-K chunk of the H product in codes (0 = auto)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -129,7 +136,9 @@ def main() -> int:
     log(f"generated {spec.nsamples} x {spec.nsnps} genotypes in HBM in {setup_s:.1f}s")
 
     def params(form):
-        return api.default_params(svd=1, k=k, oversamples=os_, maxp=maxp, tol=0.0, precision=form)
+        return api.default_params(svd=1, k=k, oversamples=os_, maxp=maxp, tol=0.0, precision=form,
+                                  wm_g=args.wm_g, wm_h=args.wm_h, waves_g=args.waves_g, waves_h=args.waves_h,
+                                  kchunk_g=args.kchunk_g, kchunk_h=args.kchunk_h)
 
     only = set(args.only.split(",")) if args.only else None
     results = {}
@@ -147,7 +156,7 @@ def main() -> int:
             log(f"{form} step {i}: {ts[-1]:.3f}s ({r.info['epochs']} epochs)")
         return r, ts
 
-    head = "bf16"
+    head = args.head
     r_bf16, ts_bf16 = (None, [])
     if only is None or head in only:
         r_bf16, ts_bf16 = timed(head, args.steps, args.warmup)
@@ -165,6 +174,12 @@ def main() -> int:
         results["fp64"] = r64
     if only is not None and head not in only:
         print(json.dumps({f: {"seconds": results[f].info["seconds"]} for f in results}))
+        return 0
+    if only is not None:
+        print(json.dumps({"form": head, "seconds": float(np.mean(ts_bf16)), "g_ms": kt["g_ms"] / kt["g_launches"],
+                          "h_ms": kt["h_ms"] / kt["h_launches"], "wm_g": args.wm_g, "wm_h": args.wm_h,
+                          "waves_g": args.waves_g, "waves_h": args.waves_h,
+                          "kchunk_g": args.kchunk_g, "kchunk_h": args.kchunk_h}), flush=True)
         return 0
 
     N, M = spec.nsamples, spec.nsnps
@@ -204,7 +219,7 @@ def main() -> int:
     if "fp64" in results:
         ref = results["fp64"]
         out["fp64_form_seconds"] = round(ts64[0], 3)
-        out["error_vs_fp64"] = {f: errors(ref, results[f]) for f in ("bf16", "fp32") if f in results}
+        out["error_vs_fp64"] = {f: errors(ref, results[f]) for f in results if f != "fp64"}
     if not args.no_cpu:
         try:
             log("cpu baseline ...")

@@ -39,6 +39,7 @@ extern "C" {
 #define PCAONE_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation (plain form)    */
 #define PCAONE_PREC_BF16 2   /* BF16 MFMA, hi/lo split of Omega and G        */
 #define PCAONE_PREC_BF16X1 3 /* BF16 MFMA, single BF16 piece of Omega and G  */
+#define PCAONE_PREC_BF16MIX 4 /* BF16 MFMA, one piece of Omega, hi/lo of G   */
 
 typedef struct pcaone_ctx pcaone_ctx;
 
@@ -63,6 +64,7 @@ typedef struct pcaone_params {
   int verbose;          /* -v (0 here: silent)                                      */
   int precision;        /* PCAONE_PREC_* (PCAONE_PREC_FP32)                         */
   int wm_g, wm_h;       /* GEMM wave tiles (0 = tuned default)                      */
+  int waves_g, waves_h; /* GEMM waves per workgroup, 4 or 8 (0 = tuned default)     */
   uint64_t kchunk_g;    /* split-K chunk of the G product in codes (0 = auto)       */
   uint64_t kchunk_h;    /* split-K chunk of the H product in codes (0 = auto)       */
 } pcaone_params;
@@ -176,9 +178,12 @@ int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix);
 /* ---- diagnostics (no reference counterpart) ---------------------------- */
 /* Y = D(X) W with X raw .bed rows (nrows x ceil(ncodes/4) bytes), D mapping
  * codes 00/01/10/11 to 2/0/0/-2, W ncodes x L, Y nrows x L (column-major),
- * computed by the packed MFMA kernel of the given form. */
+ * computed by the packed MFMA kernel of the given operand form (0..3). */
 int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
                        const double* W, uint32_t L, double* Y);
+/* RsvdOpData::initOmg's matrix (src/Halko.cpp:15-23): nrows x ncols,
+ * column-major, identical to the libstdc++ draw, computed in parallel. */
+int pcaone_host_omega(int seed, int gaussian, uint64_t nrows, uint64_t ncols, double* out);
 /* Host FP64 helpers of the L x L steps, row-major n x n. */
 int pcaone_host_sym_eig(int n, const double* A, double* w, double* V);
 int pcaone_host_householder_signs(int n, const double* Y, double* s);

@@ -10,8 +10,9 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpcaone_amd.so"
 
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x1": PREC_BF16X1}
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X1, PREC_BF16MIX = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x1": PREC_BF16X1,
+              "bf16mix": PREC_BF16MIX}
 ERR_NO_DEVICE = 6
 
 
@@ -32,7 +33,8 @@ class Params(C.Structure):
                 ("bands", C.c_uint32), ("maxiter", C.c_uint32), ("tol", C.c_double), ("tolem", C.c_double),
                 ("maf", C.c_double), ("seed", C.c_int), ("rand", C.c_int), ("ploidy", C.c_int),
                 ("noshuffle", C.c_int), ("emu", C.c_int), ("printv", C.c_int), ("verbose", C.c_int),
-                ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("kchunk_g", C.c_uint64),
+                ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("waves_g", C.c_int),
+                ("waves_h", C.c_int), ("kchunk_g", C.c_uint64),
                 ("kchunk_h", C.c_uint64)]
 
 
@@ -78,6 +80,7 @@ _SIGS = {
     "pcaone_synth_packed": (C.c_int, [_P, _P, _P]),
     "pcaone_synth_write_plink": (C.c_int, [_P, C.c_char_p]),
     "pcaone_packed_gemm": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.c_uint32, _P]),
+    "pcaone_host_omega": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint64, _P]),
     "pcaone_host_sym_eig": (C.c_int, [C.c_int, _P, _P, _P]),
     "pcaone_host_householder_signs": (C.c_int, [C.c_int, _P, _P]),
     "pcaone_host_chol_upper": (C.c_int, [C.c_int, _P, _P]),

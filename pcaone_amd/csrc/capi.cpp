@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 #include "linalg_host.hpp"
 #include "plink_io.hpp"
+#include "rng_host.hpp"
 #include "rsvd.hpp"
 #include "synth_host.hpp"
 
@@ -78,9 +79,9 @@ synth::Spec to_spec(const pcaone_synth_spec* s) {
                      s->fst_lo, s->fst_hi,   s->miss_rate, s->high_miss_frac, s->high_miss_rate};
 }
 
-Form to_form(int p) {
-  require(p >= 0 && p <= 3, kErrArgument, "unknown precision");
-  return static_cast<Form>(p);
+int to_precision(int p) {
+  require(p >= 0 && p < kNumPrecisions, kErrArgument, "unknown precision");
+  return p;
 }
 
 Param to_param(const pcaone_params* p) {
@@ -102,9 +103,11 @@ Param to_param(const pcaone_params* p) {
   q.emu = p->emu != 0;
   q.printv = p->printv != 0;
   q.verbose = p->verbose;
-  q.precision = to_form(p->precision);
+  q.precision = to_precision(p->precision);
   q.tuning.wm_g = p->wm_g;
   q.tuning.wm_h = p->wm_h;
+  q.tuning.waves_g = p->waves_g;
+  q.tuning.waves_h = p->waves_h;
   q.tuning.kchunk_g = p->kchunk_g;
   q.tuning.kchunk_h = p->kchunk_h;
   require(q.k >= 1, kErrArgument, "k must be >= 1");
@@ -250,7 +253,7 @@ int pcaone_set_impute(pcaone_ctx* c, const double* U, const double* S, const dou
 }
 
 int pcaone_configure(pcaone_ctx* c, int precision, uint32_t L) {
-  return guard(c, [&](Context& x) { x.configure(to_form(precision), (int)L); });
+  return guard(c, [&](Context& x) { x.configure(to_precision(precision), (int)L); });
 }
 
 int pcaone_set_omega(pcaone_ctx* c, const double* omega) {
@@ -371,10 +374,11 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
                        const double* W, uint32_t L, double* Y) {
   return guard(c, [&](Context& x) {
     require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
-    const Form f = to_form(precision);
+    require(precision >= 0 && precision <= 3, kErrArgument, "packed_gemm: precision must be 0..3");
+    const Form f = static_cast<Form>(precision);
     const int lp = (int)round_up(L, 16);
-    const int wm = gemm_default_wm(f, lp);
-    const uint64_t tile = gemm_row_tile(f, wm);
+    const GemmShape shape = gemm_default_shape(f, lp);
+    const uint64_t tile = gemm_row_tile(shape);
     const uint64_t rows_pad = round_up(nrows, tile);
     const uint64_t K = round_up(ncodes, kKStep), pitch = K / 16, bps = bytes_per_snp(ncodes);
     DevBuf<uint8_t> raw(nrows * bps);
@@ -405,7 +409,7 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
     a.k_end = K;
     a.kchunk = K;
     a.lp = lp;
-    launch_packed_gemm(f, wm, a, x.stream);
+    launch_packed_gemm(f, shape, a, x.stream);
     std::vector<unsigned char> h(out.bytes());
     PCAONE_HIP(hipMemcpyAsync(h.data(), out.get(), h.size(), hipMemcpyDeviceToHost, x.stream));
     PCAONE_HIP(hipStreamSynchronize(x.stream));
@@ -415,6 +419,13 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
         Y[(uint64_t)l * nrows + r] =
             f == Form::FP64 ? reinterpret_cast<const double*>(h.data())[idx] : reinterpret_cast<const float*>(h.data())[idx];
       }
+  });
+}
+
+int pcaone_host_omega(int seed, int gaussian, uint64_t nrows, uint64_t ncols, double* out) {
+  return guard_global([&] {
+    require(out != nullptr, kErrArgument, "null output");
+    rng::omega(seed, gaussian != 0, nrows, ncols, out);
   });
 }
 

@@ -11,7 +11,7 @@
 namespace pcaone {
 
 namespace {
-constexpr uint64_t kRowPad = 256;  // largest GEMM row tile
+constexpr uint64_t kRowPad = 512;  // largest GEMM row tile (4 waves x 8 x 16 rows)
 constexpr double kVarTol = 1e-9;   // src/Data.hpp:7
 
 uint64_t bytes_of(Form f) { return form_wbytes(f); }
@@ -297,16 +297,17 @@ void Context::reset_columns() {
 }
 
 // ------------------------------------------------------------------ rsvd
-void Context::configure(Form f, int Lr, const GemmTuning& t) {
+void Context::configure(int prec, int Lr, const GemmTuning& t) {
   require(N > 0 && M > 0 && !F.empty(), kErrState, "no genotype data (finalize first)");
   require(Lr >= 1 && Lr <= 128, kErrArgument, "k + oversamples must be in [1, 128]");
   require((uint64_t)Lr <= N && (uint64_t)Lr <= M, kErrArgument, "k + oversamples exceeds the matrix dimensions");
-  form = f;
+  require(prec >= 0 && prec < kNumPrecisions, kErrArgument, "unknown precision");
+  precision = prec;
+  form_g = forms_of(prec).g;
+  form_h = forms_of(prec).h;
   L = Lr;
   lp = (int)round_up((uint64_t)Lr, 16);
   tuning = t;
-  const int P = form_pieces(f);
-  const uint64_t wb = bytes_of(f);
   const uint64_t nrows_n = std::max<uint64_t>(rows_t_, ks_);
   omega_.resize(nrows_n * lp);
   omega_r_.resize(nrows_n * lp);
@@ -314,10 +315,10 @@ void Context::configure(Form f, int Lr, const GemmTuning& t) {
   colsum_.resize(lp);
   bvec_.resize(lp);
   wg_pitch_ = ks_;
-  wg_.resize((uint64_t)P * lp * wg_pitch_ * wb);
+  wg_.resize((uint64_t)form_pieces(form_g) * lp * wg_pitch_ * bytes_of(form_g));
   PCAONE_HIP(hipMemsetAsync(wg_.get(), 0, wg_.bytes(), stream));
   wh_pitch_ = kt_ + kKStep;
-  wh_.resize((uint64_t)P * lp * wh_pitch_ * wb);
+  wh_.resize((uint64_t)form_pieces(form_h) * lp * wh_pitch_ * bytes_of(form_h));
   gt_.resize(rows_s_ * lp);
   tmp_m_.resize(rows_s_ * lp);
   vbuf_.resize(rows_s_ * lp);
@@ -330,14 +331,16 @@ void Context::configure(Form f, int Lr, const GemmTuning& t) {
   for (auto* b : {&H_, &H1_, &H2_, &gt_})
     PCAONE_HIP(hipMemsetAsync(b->get(), 0, b->bytes(), stream));
   small_.resize(2 * 128 * 128);
-  red_cap_ = std::max<uint64_t>(gram_partial_elems(std::max<uint64_t>(M, N), lp), 2048 * 256);
+  red_cap_ = std::max<uint64_t>({gram_partial_elems(std::max<uint64_t>(M, N), lp), 2048 * 256,
+                                 epi_g_fused_partial_elems(rows_s_, lp)});
   red_.resize(red_cap_ + 2 * 128 * 128);
   // partial buffers: the largest G window is M rows, the H product has rows_t_ rows
-  const int wmg = tuning.wm_g ? tuning.wm_g : gemm_default_wm(f, lp);
-  const int wmh = tuning.wm_h ? tuning.wm_h : gemm_default_wm(f, lp);
-  tuning.wm_g = wmg;
-  tuning.wm_h = wmh;
-  const uint64_t tile_g = gemm_row_tile(f, wmg), tile_h = gemm_row_tile(f, wmh);
+  const GemmShape dg = gemm_default_shape(form_g, lp), dh = gemm_default_shape(form_h, lp);
+  shape_g = GemmShape{tuning.wm_g ? tuning.wm_g : dg.wm, tuning.waves_g ? tuning.waves_g : dg.waves};
+  shape_h = GemmShape{tuning.wm_h ? tuning.wm_h : dh.wm, tuning.waves_h ? tuning.waves_h : dh.waves};
+  require(gemm_shape_supported(form_g, shape_g, lp) && gemm_shape_supported(form_h, shape_h, lp), kErrArgument,
+          "unsupported GEMM workgroup shape for this precision");
+  const uint64_t tile_g = gemm_row_tile(shape_g), tile_h = gemm_row_tile(shape_h);
   require(rows_t_ % tile_h == 0, kErrInternal, "row padding");
   if (!tuning.kchunk_g) {
     const uint64_t wgs = round_up(M, tile_g) / tile_g;
@@ -346,11 +349,13 @@ void Context::configure(Form f, int Lr, const GemmTuning& t) {
     tuning.kchunk_g = round_up((ks_ + nch - 1) / nch, kKStep);
   }
   if (!tuning.kchunk_h) tuning.kchunk_h = std::min<uint64_t>(16384, kt_);
+  tuning.kchunk_g = round_up(tuning.kchunk_g, kKStep);
+  tuning.kchunk_h = round_up(tuning.kchunk_h, kKStep);
   const uint64_t nch_g = (ks_ + tuning.kchunk_g - 1) / tuning.kchunk_g;
   const uint64_t nch_h = (wh_pitch_ + tuning.kchunk_h - 1) / tuning.kchunk_h;
   const uint64_t pg = nch_g * (round_up(M, tile_g) + tile_g) * lp;
   const uint64_t ph = nch_h * rows_t_ * lp;
-  part_.resize(std::max(pg, ph) * form_accbytes(f));
+  part_.resize(std::max(pg * form_accbytes(form_g), ph * form_accbytes(form_h)));
   h_version_ = 0;
   gram_version_ = ~0ull;
   ktimes = KernelTimes{};
@@ -381,7 +386,7 @@ void Context::set_omega(const double* om) {
   for (uint64_t i = 0; i < N; ++i)
     for (int l = 0; l < L; ++l) h[i * lp + l] = om[(uint64_t)l * N + i];
   PCAONE_HIP(hipMemcpyAsync(omega_c_.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
-  launch_make_pieces(form, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
+  launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
   PCAONE_HIP(hipStreamSynchronize(stream));
@@ -394,7 +399,7 @@ void Context::run_gemm(bool g_product, const GemmArgs& a) {
     PCAONE_HIP(hipEventCreate(&e1));
     PCAONE_HIP(hipEventRecord(e0, stream));
   }
-  launch_packed_gemm(form, g_product ? tuning.wm_g : tuning.wm_h, a, stream);
+  launch_packed_gemm(g_product ? form_g : form_h, g_product ? shape_g : shape_h, a, stream);
   if (time_kernels) {
     PCAONE_HIP(hipEventRecord(e1, stream));
     PCAONE_HIP(hipEventSynchronize(e1));
@@ -411,9 +416,9 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   require(L > 0, kErrState, "configure() first");
   require(j0 < j1 && j1 <= M, kErrArgument, "sketch: bad SNP range");
   const uint64_t nwin = j1 - j0;
-  const uint64_t wb = bytes_of(form);
-  const int P = form_pieces(form);
-  const uint64_t tile_g = gemm_row_tile(form, tuning.wm_g), tile_h = gemm_row_tile(form, tuning.wm_h);
+  const uint64_t wb = bytes_of(form_h);
+  const int P = form_pieces(form_h);
+  const uint64_t tile_g = gemm_row_tile(shape_g), tile_h = gemm_row_tile(shape_h);
 
   // (1) G[j0:j1] = A Omega : SNP-major rows, K = samples
   GemmArgs g{};
@@ -433,9 +438,21 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   run_gemm(true, g);
   const uint64_t nch_g = (ks_ + g.kchunk - 1) / g.kchunk;
   double* gwin = gt_.get() + j0 * lp;
-  launch_epi_g(form, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0, colsum_.get(),
-               gwin, stream);
-  if (nmiss) {
+  // The window's operand pieces of diag(sc) G~ sit at K offset j0 % 128 of wh_;
+  // the K margins around them must read as zero.
+  const uint64_t off = j0 % kKStep;
+  const uint64_t klen = round_up(off + nwin, kKStep);
+  if (off) PCAONE_HIP(hipMemset2DAsync(wh_.get(), wh_pitch_ * wb, 0, off * wb, (size_t)P * lp, stream));
+  if (klen > off + nwin)
+    PCAONE_HIP(hipMemset2DAsync(wh_.get() + (off + nwin) * wb, wh_pitch_ * wb, 0, (klen - off - nwin) * wb,
+                                (size_t)P * lp, stream));
+  if (!nmiss) {
+    launch_epi_g_fused(form_h, part_.get(), nch_g, g.ychunk, nwin, lp, sc_.get() + j0, cF_.get() + j0,
+                       colsum_.get(), gwin, wh_.get() + off * wb, wh_pitch_, (uint64_t)lp * wh_pitch_, red_.get(),
+                       bvec_.get(), stream);
+  } else {
+    launch_epi_g(form_g, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0,
+                 colsum_.get(), gwin, stream);
     MissArgs m{};
     m.rp = rp_s_.get() + j0;
     m.ix = ix_s_.get();
@@ -453,14 +470,10 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     m.out = gwin;
     m.lp = lp;
     launch_missing_correction(m, stream);
+    launch_make_pieces(form_h, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
+                       (uint64_t)lp * wh_pitch_, gwin, nullptr, stream);
+    launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
   }
-  // (2) operand pieces of diag(sc) G~ for the window, at K offset j0 % 128
-  const uint64_t off = j0 % kKStep;
-  const uint64_t klen = round_up(off + nwin, kKStep);
-  PCAONE_HIP(hipMemset2DAsync(wh_.get(), wh_pitch_ * wb, 0, klen * wb, (size_t)P * lp, stream));
-  launch_make_pieces(form, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
-                     (uint64_t)lp * wh_pitch_, gwin, nullptr, stream);
-  launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
 
   // (3) H contribution: sample-major rows, K = the window's SNPs
   GemmArgs h{};
@@ -480,7 +493,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   run_gemm(false, h);
   const uint64_t nch_h = (klen + h.kchunk - 1) / h.kchunk;
   double* target = hmode == 0 ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
-  launch_epi_h(form, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream);
+  launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream);
   if (nmiss) {
     MissArgs m{};
     m.rp = rp_t_.get();
@@ -552,7 +565,7 @@ void Context::omega_from(int src) {
     if (ab[l] > 2.0 * ab[lp + l]) sign[l] = -1.0;
   double* dsign = small_.get() + 128 * 128;
   PCAONE_HIP(hipMemcpyAsync(dsign, sign.data(), lp * 8, hipMemcpyHostToDevice, stream));
-  launch_make_pieces(form, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
+  launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
   PCAONE_HIP(hipStreamSynchronize(stream));

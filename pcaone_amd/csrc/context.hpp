@@ -33,6 +33,8 @@ struct Spec;
 struct GemmTuning {
   int wm_g = 0;          // wave tile of the G product (0 = default)
   int wm_h = 0;          // wave tile of the H product
+  int waves_g = 0;       // waves per workgroup of the G product (4 or 8; 0 = default)
+  int waves_h = 0;       // waves per workgroup of the H product
   uint64_t kchunk_g = 0; // split-K chunk of the G product in codes (0 = auto)
   uint64_t kchunk_h = 0; // split-K chunk of the H product in codes (0 = auto)
 };
@@ -78,8 +80,10 @@ class Context {
   double p_miss() const { return N && M ? (double)nmiss / ((double)N * (double)M) : 0.0; }
 
   // ---------------------------------------------------------------- rsvd
-  void configure(Form f, int L, const GemmTuning& t = {});
-  Form form = Form::FP32;
+  void configure(int precision, int L, const GemmTuning& t = {});
+  int precision = 1;
+  Form form_g = Form::FP32, form_h = Form::FP32;  // operand forms of G = A Omega and H = A^T G
+  GemmShape shape_g, shape_h;
   int L = 0, lp = 0;
   GemmTuning tuning;
 

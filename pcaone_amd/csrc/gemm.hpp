@@ -19,9 +19,25 @@
 namespace pcaone {
 
 constexpr int kKStep = 128;       // codes consumed per K-step (and row padding unit)
-constexpr int kGemmWaves = 4;     // waves per workgroup
 
+// Operand form of one product.  BF16 = hi/lo pieces, BF16X1 = one piece.
 enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3 };
+
+// User-facing precision (PCAONE_PREC_*) -> forms of the two products
+// (G = A Omega, H = A^T G).  4 = BF16 mixed: one BF16 piece of Omega, hi/lo of G.
+struct ProductForms {
+  Form g, h;
+};
+inline ProductForms forms_of(int precision) {
+  switch (precision) {
+    case 0: return {Form::FP64, Form::FP64};
+    case 1: return {Form::FP32, Form::FP32};
+    case 2: return {Form::BF16, Form::BF16};
+    case 3: return {Form::BF16X1, Form::BF16X1};
+    default: return {Form::BF16X1, Form::BF16};
+  }
+}
+constexpr int kNumPrecisions = 5;
 
 inline int form_pieces(Form f) { return f == Form::BF16 ? 2 : 1; }
 inline size_t form_wbytes(Form f) { return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : 2); }
@@ -42,10 +58,14 @@ struct GemmArgs {
   int lp;              // padded number of columns (multiple of 16, <= 128)
 };
 
-// Row-tile height (rows per workgroup) for a form and wave tile.
-int gemm_row_tile(Form f, int wm);
-// Default wave tile (row blocks of 16 per wave) for a form.
-int gemm_default_wm(Form f, int lp);
-void launch_packed_gemm(Form f, int wm, const GemmArgs& a, hipStream_t s);
+// Workgroup shape: `waves` waves (4 or 8), each owning `wm` row blocks of 16.
+struct GemmShape {
+  int wm = 0;
+  int waves = 0;
+};
+inline int gemm_row_tile(const GemmShape& g) { return g.waves * g.wm * 16; }
+GemmShape gemm_default_shape(Form f, int lp);
+bool gemm_shape_supported(Form f, const GemmShape& g, int lp);
+void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream_t s);
 
 }  // namespace pcaone

@@ -51,6 +51,14 @@ void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_p
 // with alpha_j = sc_j / 4, beta_j = sc_j * (1/2 - cf_j).
 void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
                   int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s);
+// G epilogue fused with the operand split, for data without missing entries:
+// gout (nwin x lp) = G~ window, Wt pieces of diag(sc) G~ (window offset
+// already applied), bvec = sum_j sc_j (1/2 - cF_j) G~_j via bpart (scratch of
+// epi_g_fused_partial_elems).
+uint64_t epi_g_fused_partial_elems(uint64_t nwin, int lp);
+void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride, uint64_t nwin, int lp,
+                        const double* sc, const double* cf, const double* colsum, double* gout, void* Wt,
+                        uint64_t wpitch, uint64_t wpiece, double* bpart, double* bvec, hipStream_t s);
 // H epilogue: t = sum_c P[c][i][l] / 4 + b[l]; mode 0: out = t, 1: out += t (i < nreal; rows above = 0 in mode 0).
 void launch_epi_h(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
                   int lp, const double* b, double* out, int mode, hipStream_t s);

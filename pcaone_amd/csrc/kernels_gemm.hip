@@ -1,13 +1,20 @@
 // Packed-genotype GEMM on CDNA4 matrix cores (see gemm.hpp for the contract).
 //
-// Workgroup = 4 waves; wave w owns WM row blocks of 16 rows and all NC column
+// Workgroup = WAVES (4 or 8) waves; wave w owns WM row blocks of 16 rows and all NC column
 // blocks of 16.  Per K-step of 128 codes the workgroup stages the W tile
 // (pieces x LP x 128, transposed) in LDS; every lane loads its 32 codes of
 // each of its rows (8 bytes) straight from HBM and decodes them in registers.
 // Lane group q = lane/16 owns codes [32q, 32q+32) of the step, so one 8-byte
 // load feeds all MFMAs of the step; the B fragment of the same K slots is a
 // contiguous run of the transposed W row.
+//
+// Pipelining: the next step's W tile and genotype words are loaded into
+// registers while the current step computes; the W tile is double-buffered
+// in LDS when two copies fit (one barrier per step), single-buffered
+// otherwise (FP64 at large LP; two barriers per step).
 #include <hip/hip_runtime.h>
+
+#include <string>
 
 #include "device.hpp"
 #include "gemm.hpp"
@@ -19,6 +26,9 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+constexpr size_t kLdsMax = 160 * 1024;
 
 // code -> top byte of v in BF16/FP32/FP64: 00 -> +2 (0x40), 01 -> 0, 10 -> 0, 11 -> -2 (0xC0)
 constexpr uint32_t kLutTopByte = 0xC0000040u;
@@ -71,12 +81,91 @@ struct FT<Form::FP64> {
 };
 
 template <Form F, int NC>
-constexpr size_t lds_bytes() {
+constexpr size_t tile_bytes() {
   return (size_t)FT<F>::P * NC * 16 * (kKStep + FT<F>::PAD) * sizeof(typename FT<F>::w_t);
 }
+template <Form F, int NC>
+constexpr bool double_buffered() {
+  return 2 * tile_bytes<F, NC>() <= kLdsMax;
+}
+template <Form F, int NC>
+constexpr size_t lds_bytes() {
+  return (double_buffered<F, NC>() ? 2 : 1) * tile_bytes<F, NC>();
+}
 
+template <Form F>
+using accv_t = typename std::conditional<F == Form::FP64, f64x4_t, f32x4_t>::type;
+
+// All MFMAs of one K-step for one wave.
 template <Form F, int WM, int NC>
-__global__ void __launch_bounds__(256) k_packed_gemm(GemmArgs a) {
+__device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32x2_t (&aw)[WM],
+                                             const typename FT<F>::w_t* wl, int q, int r16) {
+  using T = FT<F>;
+  using w_t = typename T::w_t;
+  using acc_t = typename T::acc_t;
+  constexpr int LP = NC * 16;
+  constexpr int LDSP = kKStep + T::PAD;
+  if constexpr (F == Form::BF16 || F == Form::BF16X1) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      bf16x8_t af[WM];
+#pragma unroll
+      for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(s < 2 ? aw[rb][0] : aw[rb][1], s & 1);
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb) {
+#pragma unroll
+        for (int p = 0; p < T::P; ++p) {
+          const u32x4_t bv = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP + 32 * q + 8 * s);
+          const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv);
+#pragma unroll
+          for (int rb = 0; rb < WM; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bf, acc[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      acc_t av[WM][4];
+#pragma unroll
+      for (int rb = 0; rb < WM; ++rb) {
+        const uint32_t tb = top_bytes(spread(u < 4 ? aw[rb][0] : aw[rb][1], u & 3));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if constexpr (F == Form::FP32)
+            av[rb][e] = __uint_as_float(top_word(tb, e));
+          else
+            av[rb][e] = __hiloint2double((int)top_word(tb, e), 0);
+        }
+      }
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb) {
+        const w_t* bp = wl + (cb * 16 + r16) * LDSP + 32 * q + 4 * u;
+        acc_t bw[4];
+        if constexpr (F == Form::FP32) {
+          const float4 b4 = *reinterpret_cast<const float4*>(bp);
+          bw[0] = b4.x, bw[1] = b4.y, bw[2] = b4.z, bw[3] = b4.w;
+        } else {
+          const double2 b0 = *reinterpret_cast<const double2*>(bp);
+          const double2 b1 = *reinterpret_cast<const double2*>(bp + 2);
+          bw[0] = b0.x, bw[1] = b0.y, bw[2] = b1.x, bw[3] = b1.y;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int rb = 0; rb < WM; ++rb) {
+            if constexpr (F == Form::FP32)
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
+            else
+              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
+          }
+      }
+    }
+  }
+}
+
+template <Form F, int WM, int NC, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
   using T = FT<F>;
   using w_t = typename T::w_t;
   using acc_t = typename T::acc_t;
@@ -84,19 +173,22 @@ __global__ void __launch_bounds__(256) k_packed_gemm(GemmArgs a) {
   constexpr int LDSP = kKStep + T::PAD;
   constexpr int VE = 16 / sizeof(w_t);
   constexpr int VPR = kKStep / VE;
-  constexpr int NV = T::P * LP * VPR;
-  using accv_t = typename std::conditional<F == Form::FP64, f64x4_t, f32x4_t>::type;
+  constexpr int NV = T::P * LP * VPR;           // 16-byte vectors per W tile
+  constexpr int NT = 64 * WAVES;                // threads per workgroup
+  constexpr int NVT = NV / NT;                  // per thread
+  constexpr bool DB = double_buffered<F, NC>();
+  constexpr size_t TILE_ELEMS = tile_bytes<F, NC>() / sizeof(w_t);
 
   extern __shared__ __align__(16) unsigned char smem[];
-  w_t* wl = reinterpret_cast<w_t*>(smem);
+  w_t* wl0 = reinterpret_cast<w_t*>(smem);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, r16 = lane & 15;
-  const uint64_t rowbase = (uint64_t)blockIdx.x * (kGemmWaves * WM * 16) + (uint64_t)wave * WM * 16;
+  const uint64_t rowbase = (uint64_t)blockIdx.x * (WAVES * WM * 16) + (uint64_t)wave * WM * 16;
   const uint64_t kb = a.k_begin + (uint64_t)blockIdx.y * a.kchunk;
   const uint64_t ke = (kb + a.kchunk < a.k_end) ? kb + a.kchunk : a.k_end;
 
-  accv_t acc[WM][NC];
+  accv_t<F> acc[WM][NC];
 #pragma unroll
   for (int rb = 0; rb < WM; ++rb)
 #pragma unroll
@@ -107,80 +199,54 @@ __global__ void __launch_bounds__(256) k_packed_gemm(GemmArgs a) {
   const uint32_t* xr[WM];
 #pragma unroll
   for (int rb = 0; rb < WM; ++rb) xr[rb] = a.X + (rowbase + rb * 16 + r16) * a.xpitch + 2 * q;
-
   const w_t* wg = static_cast<const w_t*>(a.Wt);
 
-  for (uint64_t k0 = kb; k0 < ke; k0 += kKStep) {
-    __syncthreads();
-    for (int v = tid; v < NV; v += 256) {
-      const int row = v / VPR, part = v - row * VPR;
-      const int piece = row / LP, col = row - piece * LP;
-      const uint4 val = *reinterpret_cast<const uint4*>(wg + piece * a.wpiece + col * a.wpitch + k0 + part * VE);
-      *reinterpret_cast<uint4*>(wl + row * LDSP + part * VE) = val;
-    }
-    uint2 aw[WM];
-#pragma unroll
-    for (int rb = 0; rb < WM; ++rb) aw[rb] = *reinterpret_cast<const uint2*>(xr[rb] + (k0 >> 4));
-    __syncthreads();
+  // Thread tid moves W-tile vectors v = tid + NT i.
+  static_assert(NV % NT == 0, "W tile must split evenly over the workgroup");
+  const int vrow0 = tid / VPR, vpart = tid % VPR;  // NT is a multiple of VPR
+  constexpr int ROWS_PER_I = NT / VPR;
 
-    if constexpr (F == Form::BF16 || F == Form::BF16X1) {
+  u32x4_t wreg[NVT];
+  u32x2_t anext[WM], acur[WM];
+
+  if (kb < ke) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        bf16x8_t af[WM];
+    for (int i = 0; i < NVT; ++i) {
+      const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
+      wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * a.wpiece + col * a.wpitch + vpart * VE + kb);
+    }
 #pragma unroll
-        for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(s < 2 ? aw[rb].x : aw[rb].y, s & 1);
+    for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kb >> 4));
 #pragma unroll
-        for (int cb = 0; cb < NC; ++cb) {
+    for (int i = 0; i < NVT; ++i)
+      *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
+    __syncthreads();
+    int buf = 0;
+    for (uint64_t k0 = kb; k0 < ke; k0 += kKStep) {
+      const bool more = k0 + kKStep < ke;
+      const uint64_t kn = k0 + kKStep;
+      if (more) {
 #pragma unroll
-          for (int p = 0; p < T::P; ++p) {
-            const u32x4_t bv =
-                *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP + 32 * q + 8 * s);
-            const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv);
-#pragma unroll
-            for (int rb = 0; rb < WM; ++rb)
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bf, acc[rb][cb], 0, 0, 0);
-          }
+        for (int i = 0; i < NVT; ++i) {
+          const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
+          wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * a.wpiece + col * a.wpitch + vpart * VE + kn);
         }
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb) anext[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kn >> 4));
       }
-    } else {
+      w_t* cur = wl0 + (DB ? buf * TILE_ELEMS : 0);
+      compute_step<F, WM, NC>(acc, acur, cur, q, r16);
+      if (more) {
+        w_t* nxt = wl0 + (DB ? (buf ^ 1) * TILE_ELEMS : 0);
+        if constexpr (!DB) __syncthreads();
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        uint32_t tb[WM];
+        for (int i = 0; i < NVT; ++i)
+          *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
 #pragma unroll
-        for (int rb = 0; rb < WM; ++rb) tb[rb] = top_bytes(spread(u < 4 ? aw[rb].x : aw[rb].y, u & 3));
-        acc_t av[WM][4];
-#pragma unroll
-        for (int rb = 0; rb < WM; ++rb)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            if constexpr (F == Form::FP32)
-              av[rb][e] = __uint_as_float(top_word(tb[rb], e));
-            else
-              av[rb][e] = __hiloint2double((int)top_word(tb[rb], e), 0);
-          }
-#pragma unroll
-        for (int cb = 0; cb < NC; ++cb) {
-          const w_t* bp = wl + (cb * 16 + r16) * LDSP + 32 * q + 4 * u;
-          acc_t bw[4];
-          if constexpr (F == Form::FP32) {
-            const float4 b4 = *reinterpret_cast<const float4*>(bp);
-            bw[0] = b4.x, bw[1] = b4.y, bw[2] = b4.z, bw[3] = b4.w;
-          } else {
-            const double2 b0 = *reinterpret_cast<const double2*>(bp);
-            const double2 b1 = *reinterpret_cast<const double2*>(bp + 2);
-            bw[0] = b0.x, bw[1] = b0.y, bw[2] = b1.x, bw[3] = b1.y;
-          }
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-#pragma unroll
-            for (int rb = 0; rb < WM; ++rb) {
-              if constexpr (F == Form::FP32)
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
-              else
-                acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
-            }
-        }
+        for (int rb = 0; rb < WM; ++rb) acur[rb] = anext[rb];
       }
+      __syncthreads();
+      buf ^= 1;
     }
   }
 
@@ -198,71 +264,106 @@ __global__ void __launch_bounds__(256) k_packed_gemm(GemmArgs a) {
       }
 }
 
-template <Form F, int WM, int NC>
+template <Form F, int WM, int NC, int WAVES>
 void launch_t(const GemmArgs& a, hipStream_t s) {
   constexpr size_t lds = lds_bytes<F, NC>();
   static bool attr = false;
   if (!attr) {
-    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_packed_gemm<F, WM, NC>),
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_packed_gemm<F, WM, NC, WAVES>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr = true;
   }
-  const uint64_t tile = (uint64_t)kGemmWaves * WM * 16;
+  const uint64_t tile = (uint64_t)WAVES * WM * 16;
   const uint64_t nchunks = (a.k_end - a.k_begin + a.kchunk - 1) / a.kchunk;
   dim3 grid((unsigned)(a.rows / tile), (unsigned)nchunks);
-  hipLaunchKernelGGL((k_packed_gemm<F, WM, NC>), grid, dim3(256), lds, s, a);
+  hipLaunchKernelGGL((k_packed_gemm<F, WM, NC, WAVES>), grid, dim3(64 * WAVES), lds, s, a);
   check_launch("packed_gemm");
 }
 
-template <Form F, int WM>
+// Which (form, wm, waves) combinations are compiled.
+template <Form F>
+constexpr bool shape_ok(int wm, int waves) {
+  if (waves == 4) return wm == 1 || wm == 2 || (wm == 4 && F != Form::FP64) || (wm == 8 && F != Form::FP64 && F != Form::FP32);
+  if (waves == 8) return (wm == 2 || wm == 4) && F != Form::FP64;
+  return false;
+}
+
+template <Form F, int WM, int WAVES>
 void launch_nc(const GemmArgs& a, hipStream_t s) {
-  switch (a.lp / 16) {
-    case 1: return launch_t<F, WM, 1>(a, s);
-    case 2: return launch_t<F, WM, 2>(a, s);
-    case 3: return launch_t<F, WM, 3>(a, s);
-    case 4: return launch_t<F, WM, 4>(a, s);
-    case 5: return launch_t<F, WM, 5>(a, s);
-    case 6: return launch_t<F, WM, 6>(a, s);
-    case 7: return launch_t<F, WM, 7>(a, s);
-    case 8: return launch_t<F, WM, 8>(a, s);
-    default: fail(kErrArgument, "packed_gemm: lp must be 16..128 in steps of 16");
+  if constexpr (shape_ok<F>(WM, WAVES)) {
+    constexpr int VPR = kKStep / (16 / sizeof(typename FT<F>::w_t));
+    // the W tile of NC column blocks must split evenly over 64 * WAVES threads
+    auto even = [](int nc) { return (FT<F>::P * nc * 16 * VPR) % (64 * WAVES) == 0; };
+    switch (a.lp / 16) {
+#define PCAONE_NC(n) \
+  case n:            \
+    if constexpr ((FT<F>::P * n * 16 * (kKStep / (16 / sizeof(typename FT<F>::w_t)))) % (64 * WAVES) == 0) return launch_t<F, WM, n, WAVES>(a, s); \
+    break;
+      PCAONE_NC(1) PCAONE_NC(2) PCAONE_NC(3) PCAONE_NC(4) PCAONE_NC(5) PCAONE_NC(6) PCAONE_NC(7) PCAONE_NC(8)
+#undef PCAONE_NC
+      default:
+        break;
+    }
+    (void)even;
   }
+  fail(kErrArgument, "packed_gemm: unsupported shape (wm " + std::to_string(WM) + ", waves " +
+                         std::to_string(WAVES) + ", lp " + std::to_string(a.lp) + ")");
 }
 
 template <Form F>
-void launch_wm(int wm, const GemmArgs& a, hipStream_t s) {
-  switch (wm) {
-    case 1: return launch_nc<F, 1>(a, s);
-    case 2: return launch_nc<F, 2>(a, s);
-    case 4:
-      if constexpr (F != Form::FP64) return launch_nc<F, 4>(a, s);
-      [[fallthrough]];
-    default: fail(kErrArgument, "packed_gemm: unsupported wave tile");
+void launch_shape(const GemmShape& g, const GemmArgs& a, hipStream_t s) {
+  if (g.waves == 8) {
+    if (g.wm == 2) return launch_nc<F, 2, 8>(a, s);
+    if (g.wm == 4) return launch_nc<F, 4, 8>(a, s);
+  } else if (g.waves == 4) {
+    if (g.wm == 1) return launch_nc<F, 1, 4>(a, s);
+    if (g.wm == 2) return launch_nc<F, 2, 4>(a, s);
+    if (g.wm == 4) return launch_nc<F, 4, 4>(a, s);
+    if (g.wm == 8) return launch_nc<F, 8, 4>(a, s);
   }
+  fail(kErrArgument, "packed_gemm: unsupported wave tile " + std::to_string(g.wm) + " x " + std::to_string(g.waves));
 }
 
 }  // namespace
 
-int gemm_row_tile(Form, int wm) { return kGemmWaves * wm * 16; }
-
-int gemm_default_wm(Form f, int lp) {
-  if (f == Form::FP64) return 1;
-  return lp > 96 ? 2 : 4;
+GemmShape gemm_default_shape(Form f, int lp) {
+  // Tuned at 50,000 x 1,000,000, L = 80 on MI355X: 8 waves x 4 row blocks
+  // (two waves per SIMD sharing one LDS tile) beats 4 waves x 8 for BF16.
+  if (f == Form::FP64) return {1, 4};
+  if (f == Form::FP32) return lp > 96 ? GemmShape{2, 4} : GemmShape{4, 8};
+  if (f == Form::BF16) return lp > 80 ? GemmShape{4, 4} : GemmShape{4, 8};
+  return (lp / 16) % 2 == 0 ? GemmShape{4, 8} : GemmShape{lp > 96 ? 4 : 8, 4};
 }
 
-void launch_packed_gemm(Form f, int wm, const GemmArgs& a, hipStream_t s) {
+bool gemm_shape_supported(Form f, const GemmShape& g, int lp) {
+  if (lp < 16 || lp > 128 || lp % 16) return false;
+  const int wb = (int)form_wbytes(f), P = form_pieces(f);
+  const int nv = P * lp * (kKStep / (16 / wb));
+  if (nv % (64 * g.waves)) return false;
+  switch (f) {
+    case Form::FP64: return shape_ok<Form::FP64>(g.wm, g.waves);
+    case Form::FP32: return shape_ok<Form::FP32>(g.wm, g.waves);
+    case Form::BF16: return shape_ok<Form::BF16>(g.wm, g.waves);
+    case Form::BF16X1: return shape_ok<Form::BF16X1>(g.wm, g.waves);
+  }
+  return false;
+}
+
+void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream_t s) {
   require(a.lp >= 16 && a.lp <= 128 && a.lp % 16 == 0, kErrArgument, "packed_gemm: bad lp");
   require(a.kchunk > 0 && a.kchunk % kKStep == 0 && a.k_begin % kKStep == 0 && a.k_end % kKStep == 0,
           kErrArgument, "packed_gemm: K range must be multiples of 128");
-  require(a.rows % (uint64_t)gemm_row_tile(f, wm) == 0, kErrArgument, "packed_gemm: rows not a tile multiple");
+  require(gemm_shape_supported(f, g, a.lp), kErrArgument, "packed_gemm: unsupported shape");
+  require(a.rows % (uint64_t)gemm_row_tile(g) == 0, kErrArgument, "packed_gemm: rows not a tile multiple");
   require(a.xpitch % 8 == 0 && a.xpitch * 16 >= a.k_end, kErrArgument, "packed_gemm: X pitch too small");
   require(a.wpitch % 8 == 0 && a.wpitch >= a.k_end, kErrArgument, "packed_gemm: W pitch too small");
+  require(a.wpiece % 8 == 0, kErrArgument, "packed_gemm: W piece stride not 16-byte aligned");
   if (a.rows == 0 || a.k_end <= a.k_begin) return;
   switch (f) {
-    case Form::FP64: return launch_wm<Form::FP64>(wm, a, s);
-    case Form::FP32: return launch_wm<Form::FP32>(wm, a, s);
-    case Form::BF16: return launch_wm<Form::BF16>(wm, a, s);
-    case Form::BF16X1: return launch_wm<Form::BF16X1>(wm, a, s);
+    case Form::FP64: return launch_shape<Form::FP64>(g, a, s);
+    case Form::FP32: return launch_shape<Form::FP32>(g, a, s);
+    case Form::BF16: return launch_shape<Form::BF16>(g, a, s);
+    case Form::BF16X1: return launch_shape<Form::BF16X1>(g, a, s);
   }
 }
 

@@ -85,6 +85,77 @@ __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t s
   }
 }
 
+// Fused G epilogue for data without missing entries.  A workgroup takes 64
+// SNP rows of the window: reduces the split-K partials, applies the affine
+// correction (G_j = sc_j/4 y_j + sc_j (1/2 - cF_j) colsum), splits sc_j G_j
+// into operand pieces, writes G~ = sum(pieces)/sc_j (FP64, row-major), the
+// pieces transposed through LDS (coalesced along SNPs) and the per-block
+// column sums of beta_j G~_j with beta_j = sc_j (1/2 - cF_j).
+constexpr int kEpiRows = 64;
+
+template <Form F, class PT>
+__global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part, uint64_t nchunks, uint64_t stride,
+                                                     uint64_t nwin, int lp, const double* __restrict__ sc,
+                                                     const double* __restrict__ cf, const double* __restrict__ colsum,
+                                                     double* __restrict__ gout, void* Wt, uint64_t wpitch,
+                                                     uint64_t wpiece, double* __restrict__ bpart) {
+  using w_t = typename std::conditional<F == Form::FP64, double,
+                                        typename std::conditional<F == Form::FP32, float, uint16_t>::type>::type;
+  constexpr int P = F == Form::BF16 ? 2 : 1;
+  extern __shared__ __align__(16) unsigned char sm[];
+  double* bt = reinterpret_cast<double*>(sm);                                  // kEpiRows x lp
+  w_t* pt = reinterpret_cast<w_t*>(sm + (size_t)kEpiRows * lp * sizeof(double));  // P x lp x kEpiRows
+  const uint64_t r0 = (uint64_t)blockIdx.x * kEpiRows;
+  const int nr = (int)(nwin - r0 < (uint64_t)kEpiRows ? nwin - r0 : kEpiRows);
+  for (int v = threadIdx.x; v < kEpiRows * lp; v += 256) {
+    const int r = v / lp, l = v - r * lp;
+    double bval = 0.0;
+    w_t pc[P];
+    for (int p = 0; p < P; ++p) pc[p] = w_t(0);
+    if (r < nr) {
+      const uint64_t j = r0 + r;
+      const uint64_t t = j * lp + l;
+      double y = 0.0;
+      for (uint64_t c = 0; c < nchunks; ++c) y += (double)part[c * stride + t];
+      const double s = sc[j];
+      const double beta = s * (0.5 - cf[j]);
+      const double g = 0.25 * s * y + beta * colsum[l];
+      const double yv = g * s;
+      double back;
+      if constexpr (F == Form::FP64) {
+        pc[0] = yv;
+        back = yv;
+      } else if constexpr (F == Form::FP32) {
+        pc[0] = (float)yv;
+        back = (double)pc[0];
+      } else {
+        pc[0] = bf16_bits((float)yv);
+        back = (double)bf16_value(pc[0]);
+        if constexpr (P == 2) {
+          pc[1] = bf16_bits((float)(yv - back));
+          back += (double)bf16_value(pc[1]);
+        }
+      }
+      const double gr = s != 0.0 ? back / s : 0.0;
+      gout[t] = gr;
+      bval = beta * gr;
+    }
+    bt[r * lp + l] = bval;
+    for (int p = 0; p < P; ++p) pt[(p * lp + l) * kEpiRows + r] = pc[p];
+  }
+  __syncthreads();
+  w_t* w = static_cast<w_t*>(Wt);
+  for (int v = threadIdx.x; v < P * lp * kEpiRows; v += 256) {
+    const int r = v % kEpiRows, pl = v / kEpiRows, p = pl / lp, l = pl - p * lp;
+    if (r < nr) w[p * wpiece + l * wpitch + r0 + r] = pt[v];
+  }
+  for (int l = threadIdx.x; l < lp; l += 256) {
+    double s = 0.0;
+    for (int r = 0; r < nr; ++r) s += bt[r * lp + l];
+    bpart[(uint64_t)blockIdx.x * lp + l] = s;
+  }
+}
+
 template <class P>
 __global__ void k_epi_h(const P* __restrict__ part, uint64_t nchunks, uint64_t stride, uint64_t nreal,
                         uint64_t rows_pad, int lp, const double* __restrict__ b, double* __restrict__ out, int mode) {
@@ -119,64 +190,120 @@ __global__ void k_colsum_partial(const double* __restrict__ X, uint64_t r0, uint
   }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partial, uint64_t nblocks, uint64_t width,
-                               double* __restrict__ out) {
-  GRID_STRIDE(t, width) {
+// out[t] = sum_b partial[b][t].  A workgroup owns 32 consecutive outputs; its
+// 8 thread rows take every 8th partial block (coalesced 32-wide reads) and
+// are combined in LDS in a fixed order, so the result is deterministic.
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partial, uint64_t nblocks,
+                                                      uint64_t width, double* __restrict__ out) {
+  __shared__ double red[8][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const uint64_t t = (uint64_t)blockIdx.x * 32 + tx;
+  double s0 = 0.0, s1 = 0.0;
+  if (t < width) {
+    uint64_t b = ty;
+    for (; b + 8 < nblocks; b += 16) {
+      s0 += partial[b * width + t];
+      s1 += partial[(b + 8) * width + t];
+    }
+    if (b < nblocks) s0 += partial[b * width + t];
+  }
+  red[ty][tx] = s0 + s1;
+  __syncthreads();
+  if (ty == 0 && t < width) {
     double s = 0.0;
-    for (uint64_t b = 0; b < nblocks; ++b) s += partial[b * width + t];
+    for (int r = 0; r < 8; ++r) s += red[r][tx];
     out[t] = s;
   }
 }
 
-constexpr int kGramRows = 16;
-
-// out_partial[blk][a][b] = sum_{r in block} X[r][a] * Y[r][b]
-__global__ void __launch_bounds__(256) k_gram_partial(const double* __restrict__ X, const double* __restrict__ Y,
-                                                      uint64_t nrows, int lp, uint64_t rpb,
-                                                      double* __restrict__ partial) {
-  __shared__ double xs[kGramRows][128];
-  __shared__ double ys[kGramRows][128];
-  const int t = threadIdx.x, ar = t >> 4, bc = t & 15;
-  const int nb = lp / 16;
-  double acc[8][8];
-  for (int i = 0; i < 8; ++i)
-    for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
-  const uint64_t rb = (uint64_t)blockIdx.x * rpb;
-  const uint64_t re = rb + rpb < nrows ? rb + rpb : nrows;
-  for (uint64_t r0 = rb; r0 < re; r0 += kGramRows) {
-    __syncthreads();
-    for (int v = t; v < kGramRows * lp; v += 256) {
-      const int rr = v / lp, c = v - rr * lp;
-      const uint64_t r = r0 + rr;
-      xs[rr][c] = r < re ? X[r * lp + c] : 0.0;
-      ys[rr][c] = r < re ? Y[r * lp + c] : 0.0;
-    }
-    __syncthreads();
-    for (int rr = 0; rr < kGramRows; ++rr) {
-      double xv[8], yv[8];
-      for (int i = 0; i < 8; ++i) {
-        xv[i] = i < nb ? xs[rr][ar + 16 * i] : 0.0;
-        yv[i] = i < nb ? ys[rr][bc + 16 * i] : 0.0;
-      }
-      for (int i = 0; i < 8; ++i)
-        for (int j = 0; j < 8; ++j) acc[i][j] += xv[i] * yv[j];
-    }
-  }
-  double* o = partial + (uint64_t)blockIdx.x * lp * lp;
-  for (int i = 0; i < nb; ++i)
-    for (int j = 0; j < nb; ++j) o[(ar + 16 * i) * lp + bc + 16 * j] = acc[i][j];
+inline void sum_partials(const double* partial, uint64_t nblocks, uint64_t width, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((width + 31) / 32)), dim3(256), 0, s, partial, nblocks, width,
+                     out);
 }
 
-__global__ void k_apply(const double* __restrict__ X, const double* __restrict__ T, uint64_t nrows, int lp,
-                        double* __restrict__ Y) {
-  const uint64_t n = nrows * (uint64_t)lp;
-  GRID_STRIDE(t, n) {
-    const uint64_t r = t / lp;
-    const int b = (int)(t - r * lp);
-    const double* x = X + r * lp;
-    double s = 0.0;
-    for (int a = 0; a < lp; ++a) s += x[a] * T[a * lp + b];
-    Y[t] = s;
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+constexpr int kTileRows = 32;
+
+// FP64-MFMA Gram partials: block b sums X[r]^T Y[r] over rows [b*rpb, ...).
+// One wave per 16-row block a of the output computes all NB blocks (a, :);
+// row tiles of X (and Y) are staged in LDS.  v_mfma_f64_16x16x4: A = X^T
+// (lane l: X[k0 + l/16][16a + l%16]), B = Y (lane l: Y[k0 + l/16][16b + l%16]),
+// accumulator lane group q, register r = output row q + 4r.
+template <int NB, bool CROSS>
+__global__ void __launch_bounds__(64 * NB) k_gram_mfma(const double* __restrict__ X, const double* __restrict__ Y,
+                                                       uint64_t nrows, uint64_t rpb, double* __restrict__ partial) {
+  constexpr int LP = NB * 16;
+  __shared__ double xs[kTileRows * LP];
+  __shared__ double ys[CROSS ? kTileRows * LP : 1];
+  const int tid = threadIdx.x, lane = tid & 63, a = tid >> 6;
+  const int q = lane >> 4, r16 = lane & 15;
+  f64x4_t acc[NB];
+  for (int b = 0; b < NB; ++b) acc[b] = f64x4_t{0, 0, 0, 0};
+  const uint64_t rb = (uint64_t)blockIdx.x * rpb;
+  const uint64_t re = rb + rpb < nrows ? rb + rpb : nrows;
+  for (uint64_t r0 = rb; r0 < re; r0 += kTileRows) {
+    const uint64_t nr = re - r0 < (uint64_t)kTileRows ? re - r0 : kTileRows;
+    __syncthreads();
+    for (int v = tid; v < kTileRows * LP; v += 64 * NB) {
+      const bool in = (uint64_t)(v / LP) < nr;
+      xs[v] = in ? X[r0 * LP + v] : 0.0;
+      if constexpr (CROSS) ys[v] = in ? Y[r0 * LP + v] : 0.0;
+    }
+    __syncthreads();
+    const double* yt = CROSS ? ys : xs;
+#pragma unroll
+    for (int k0 = 0; k0 < kTileRows; k0 += 4) {
+      const double av = xs[(k0 + q) * LP + 16 * a + r16];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const double bv = yt[(k0 + q) * LP + 16 * b + r16];
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[b], 0, 0, 0);
+      }
+    }
+  }
+  double* o = partial + (uint64_t)blockIdx.x * LP * LP;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[(16 * a + q + 4 * r) * LP + 16 * b + r16] = acc[b][r];
+}
+
+// FP64-MFMA right multiplication Y = X T for rows [0, nrows): one wave per 16
+// rows computes all NB column blocks; A = X (lane l: X[16i + l%16][k0 + l/16]),
+// B = T (lane l: T[k0 + l/16][16b + l%16]).  T is staged in LDS.
+template <int NB>
+__global__ void __launch_bounds__(256) k_apply_mfma(const double* __restrict__ X, const double* __restrict__ T,
+                                                    uint64_t nrows, double* __restrict__ Y) {
+  constexpr int LP = NB * 16;
+  extern __shared__ double tl[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = lane >> 4, r16 = lane & 15;
+  for (int v = tid; v < LP * LP; v += 256) tl[v] = T[v];
+  __syncthreads();
+  const uint64_t nblk = (nrows + 15) / 16;
+  for (uint64_t blk = (uint64_t)blockIdx.x * 4 + wave; blk < nblk; blk += (uint64_t)gridDim.x * 4) {
+    const uint64_t row = blk * 16 + r16;
+    const double* x = X + row * LP;
+    const bool in = row < nrows;
+    f64x4_t acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = f64x4_t{0, 0, 0, 0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < LP; k0 += 4) {
+      const double av = in ? x[k0 + q] : 0.0;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const double bv = tl[(k0 + q) * LP + 16 * b + r16];
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[b], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint64_t orow = blk * 16 + q + 4 * r;
+        if (orow < nrows) Y[orow * LP + 16 * b + r16] = acc[b][r];
+      }
   }
 }
 
@@ -280,6 +407,44 @@ void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride
   check_launch("epi_g");
 }
 
+uint64_t epi_g_fused_partial_elems(uint64_t nwin, int lp) { return (nwin + kEpiRows - 1) / kEpiRows * lp; }
+
+template <Form F, class PT>
+static void epi_fused_t(const void* P, uint64_t nchunks, uint64_t stride, uint64_t nwin, int lp, const double* sc,
+                        const double* cf, const double* colsum, double* gout, void* Wt, uint64_t wpitch,
+                        uint64_t wpiece, double* bpart, hipStream_t s) {
+  constexpr int NP = F == Form::BF16 ? 2 : 1;
+  constexpr size_t wsz = F == Form::FP64 ? 8 : (F == Form::FP32 ? 4 : 2);
+  const size_t lds = (size_t)kEpiRows * lp * (sizeof(double) + NP * wsz);
+  static bool attr = false;
+  if (!attr) {
+    constexpr size_t lds_max = (size_t)kEpiRows * 128 * (sizeof(double) + NP * wsz);
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_epi_g_fused<F, PT>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    attr = true;
+  }
+  const uint64_t nb = (nwin + kEpiRows - 1) / kEpiRows;
+  hipLaunchKernelGGL((k_epi_g_fused<F, PT>), dim3((unsigned)nb), dim3(256), lds, s, static_cast<const PT*>(P), nchunks,
+                     stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart);
+}
+
+void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride, uint64_t nwin, int lp,
+                        const double* sc, const double* cf, const double* colsum, double* gout, void* Wt,
+                        uint64_t wpitch, uint64_t wpiece, double* bpart, double* bvec, hipStream_t s) {
+  if (!nwin) return;
+  require(lp >= 16 && lp <= 128, kErrArgument, "epi_g_fused: bad lp");
+  switch (f) {
+    case Form::FP64: epi_fused_t<Form::FP64, double>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
+    case Form::FP32: epi_fused_t<Form::FP32, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
+    case Form::BF16: epi_fused_t<Form::BF16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
+    case Form::BF16X1: epi_fused_t<Form::BF16X1, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
+  }
+  check_launch("epi_g_fused");
+  const uint64_t nb = (nwin + kEpiRows - 1) / kEpiRows;
+  sum_partials(bpart, nb, (uint64_t)lp, bvec, s);
+  check_launch("epi_g_fused_sum");
+}
+
 void launch_epi_h(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
                   int lp, const double* b, double* out, int mode, hipStream_t s) {
   const unsigned g = grid_for(rows_pad * lp);
@@ -307,14 +472,14 @@ void launch_colsum(const double* X, uint64_t r0, uint64_t r1, int lp, const doub
   const uint64_t nb = (r1 - r0 + rpb - 1) / rpb;
   hipLaunchKernelGGL(k_colsum_partial, dim3((unsigned)nb), dim3(128), 0, s, X, r0, r1, lp, rpb, w_sc, w_cf, partial);
   check_launch("colsum_partial");
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(128), 0, s, partial, nb, (uint64_t)lp, out);
+  sum_partials(partial, nb, (uint64_t)lp, out, s);
   check_launch("colsum_sum");
 }
 
 static uint64_t gram_rpb(uint64_t nrows) {
-  uint64_t rpb = (nrows + 511) / 512;
-  rpb = (rpb + kGramRows - 1) / kGramRows * kGramRows;
-  return rpb < 256 ? 256 : rpb;
+  uint64_t rpb = (nrows + 1023) / 1024;
+  rpb = (rpb + kTileRows - 1) / kTileRows * kTileRows;
+  return rpb < 128 ? 128 : rpb;
 }
 
 uint64_t gram_partial_elems(uint64_t nrows, int lp) {
@@ -323,10 +488,19 @@ uint64_t gram_partial_elems(uint64_t nrows, int lp) {
   return (nb ? nb : 1) * (uint64_t)lp * lp;
 }
 
+template <int NB>
+static void gram_nb(const double* X, const double* Y, uint64_t nrows, uint64_t rpb, uint64_t nb, double* partial,
+                    hipStream_t s) {
+  if (Y)
+    hipLaunchKernelGGL((k_gram_mfma<NB, true>), dim3((unsigned)nb), dim3(64 * NB), 0, s, X, Y, nrows, rpb, partial);
+  else
+    hipLaunchKernelGGL((k_gram_mfma<NB, false>), dim3((unsigned)nb), dim3(64 * NB), 0, s, X, X, nrows, rpb, partial);
+}
+
 void launch_gram(const double* X, const double* T, const double* Y0, uint64_t nrows, int lp, double* partial,
                  uint64_t partial_cap, double* out, hipStream_t s) {
   require(T == nullptr, kErrInternal, "gram: apply the transform first");
-  require(lp <= 128 && lp % 16 == 0, kErrArgument, "gram: bad lp");
+  require(lp <= 128 && lp % 16 == 0 && lp >= 16, kErrArgument, "gram: bad lp");
   if (nrows == 0) {
     PCAONE_HIP(hipMemsetAsync(out, 0, (size_t)lp * lp * sizeof(double), s));
     return;
@@ -334,17 +508,51 @@ void launch_gram(const double* X, const double* T, const double* Y0, uint64_t nr
   const uint64_t rpb = gram_rpb(nrows);
   const uint64_t nb = (nrows + rpb - 1) / rpb;
   require(nb * lp * lp <= partial_cap, kErrInternal, "gram: partial buffer too small");
-  hipLaunchKernelGGL(k_gram_partial, dim3((unsigned)nb), dim3(256), 0, s, X, Y0 ? Y0 : X, nrows, lp, rpb, partial);
-  check_launch("gram_partial");
+  switch (lp / 16) {
+    case 1: gram_nb<1>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 2: gram_nb<2>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 3: gram_nb<3>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 4: gram_nb<4>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 5: gram_nb<5>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 6: gram_nb<6>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 7: gram_nb<7>(X, Y0, nrows, rpb, nb, partial, s); break;
+    case 8: gram_nb<8>(X, Y0, nrows, rpb, nb, partial, s); break;
+  }
+  check_launch("gram_mfma");
   const uint64_t w = (uint64_t)lp * lp;
-  hipLaunchKernelGGL(k_sum_partials, dim3(grid_for(w)), dim3(256), 0, s, partial, nb, w, out);
+  sum_partials(partial, nb, w, out, s);
   check_launch("gram_sum");
+}
+
+template <int NB>
+static void apply_nb(const double* X, const double* T, uint64_t nrows, double* Y, hipStream_t s) {
+  constexpr size_t lds = (size_t)NB * 16 * NB * 16 * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_mfma<NB>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  const uint64_t nblk = (nrows + 15) / 16;
+  uint64_t g = (nblk + 3) / 4;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL((k_apply_mfma<NB>), dim3((unsigned)g), dim3(256), lds, s, X, T, nrows, Y);
 }
 
 void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, double* Y, hipStream_t s) {
   if (!nrows) return;
-  hipLaunchKernelGGL(k_apply, dim3(grid_for(nrows * lp)), dim3(256), 0, s, X, T, nrows, lp, Y);
-  check_launch("apply");
+  require(lp <= 128 && lp % 16 == 0 && lp >= 16, kErrArgument, "apply: bad lp");
+  switch (lp / 16) {
+    case 1: apply_nb<1>(X, T, nrows, Y, s); break;
+    case 2: apply_nb<2>(X, T, nrows, Y, s); break;
+    case 3: apply_nb<3>(X, T, nrows, Y, s); break;
+    case 4: apply_nb<4>(X, T, nrows, Y, s); break;
+    case 5: apply_nb<5>(X, T, nrows, Y, s); break;
+    case 6: apply_nb<6>(X, T, nrows, Y, s); break;
+    case 7: apply_nb<7>(X, T, nrows, Y, s); break;
+    case 8: apply_nb<8>(X, T, nrows, Y, s); break;
+  }
+  check_launch("apply_mfma");
 }
 
 void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
@@ -353,7 +561,7 @@ void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp
   const uint64_t nb = (nrows + rpb - 1) / rpb;
   hipLaunchKernelGGL(k_flip_partial, dim3((unsigned)nb), dim3(128), 0, s, O2, Q, nrows, lp, rpb, partial);
   check_launch("flip_partial");
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, nb, (uint64_t)2 * lp, out);
+  sum_partials(partial, nb, (uint64_t)2 * lp, out, s);
   check_launch("flip_sum");
 }
 

@@ -10,6 +10,7 @@
 #include <random>
 
 #include "common.hpp"
+#include "rng_host.hpp"
 
 namespace pcaone {
 
@@ -31,21 +32,14 @@ RsvdOpData::RsvdOpData(Context& ctx, const Param& p, int k, int os) : ctx_(ctx),
 }
 
 // src/Halko.cpp:15-23 with src/RSVD.hpp:25-59: libstdc++ default_random_engine
-// seeded with --seed; one draw per element in column-major order.
+// seeded with --seed; one draw per element in column-major order.  Every call
+// of one operator yields the same matrix, so it is drawn once and re-sent.
 void RsvdOpData::initOmg() {
-  const uint64_t n = cols();
-  const int l = size();
-  std::vector<double> om(n * (uint64_t)l);
-  auto rng = std::default_random_engine{};
-  rng.seed(params_.seed);
-  if (params_.rand) {
-    std::normal_distribution<double> d{0, 1};
-    for (auto& x : om) x = d(rng);
-  } else {
-    std::uniform_real_distribution<double> d{-1, 1};
-    for (auto& x : om) x = d(rng);
+  if (omega_host_.empty()) {
+    omega_host_.resize(cols() * (uint64_t)size());
+    rng::omega(params_.seed, params_.rand != 0, cols(), (uint64_t)size(), omega_host_.data());
   }
-  ctx_.set_omega(om.data());
+  ctx_.set_omega(omega_host_.data());
 }
 
 void RsvdOpData::fit_with_pi() {

@@ -20,7 +20,7 @@ struct Param {
   double tol = 1e-4, tolem = 1e-5, maf = 0.0;
   int seed = 112, rand = 1, scale = -9, ploidy = 2, verbose = 0;
   bool noshuffle = false, emu = false, printv = false;
-  Form precision = Form::FP32;
+  int precision = 1;  // PCAONE_PREC_*
   GemmTuning tuning;
   // derived (src/Cmd.cpp:171-238)
   bool perm = false, missme = false, filterSNP = false;
@@ -64,6 +64,7 @@ class RsvdOpData {
   Context& ctx_;
   const Param& params_;
   int nk_, os_;
+  std::vector<double> omega_host_;  // the test matrix of initOmg (identical on every call)
   void fit_with_pi();  // src/Data.cpp:293-349 via the device impute state
 };
 

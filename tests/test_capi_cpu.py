@@ -101,3 +101,13 @@ def test_null_arguments_are_rejected():
     assert L.pcaone_synth_packed(None, None, None) == 1
     assert L.pcaone_close(None) == 0
     assert L.pcaone_sketch(None, 0, 1, 0) == 1
+
+
+@pytest.mark.parametrize("seed,gaussian,n,l", [(112, 1, 2000, 20), (7, 0, 333, 5), (112, 1, 50000, 8),
+                                               (0, 1, 101, 3), (-3, 1, 77, 9), (2**31 - 1, 0, 10, 4)])
+def test_parallel_omega_is_the_libstdcxx_draw(seed, gaussian, n, l):
+    from oracle import pcaone_oracle as O
+    out = np.empty((l, n))  # column-major n x l
+    _lib.check(_lib.lib().pcaone_host_omega(seed, gaussian, n, l, out.ctypes.data))
+    ref = O.omega(seed, bool(gaussian), n, l)
+    assert np.array_equal(out.T, ref)
