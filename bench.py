@@ -49,6 +49,20 @@ def errors(ref, x):
             "principal_angle_max_rad": float(ang.max())}
 
 
+def recorded_traffic(N, M, L, form, product):
+    """HBM bytes per launch of the GEMM from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None when the
+    summary was not taken on this workload."""
+    for f in sorted((ROOT / "profiles").glob("packed_gemm_*_traffic_*.md")):
+        for line in f.read_text().splitlines():
+            if not line.startswith("traffic_bytes_per_launch "):
+                continue
+            kv = dict(t.split("=", 1) for t in line.split()[1:])
+            if (int(kv["nsamples"]), int(kv["nsnps"]), int(kv["L"]), kv["form"]) == (N, M, L, form):
+                return float(kv[product])
+    return None
+
+
 def cpu_baseline(args, spec_full, k, os_, maxp):
     """Reference binary on a bounded sample (same N, fewer SNPs), scaled to M."""
     from pcaone_amd import synth
@@ -114,6 +128,7 @@ def main() -> int:
     ap.add_argument("--waves-g", type=int, default=0, help="waves per workgroup of the G product (0 = default)")
     ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
     ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo) or bf16mix")
+    ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--kchunk-g", type=int, default=0, help="split-K chunk of the G product in codes (0 = auto)")
     ap.add_argument("--kchunk-h", type=int, default=0, help="split-K chunk of the H product in codes (0 = auto)")
     args = ap.parse_args()
@@ -140,6 +155,9 @@ def main() -> int:
                                   wm_g=args.wm_g, wm_h=args.wm_h, waves_g=args.waves_g, waves_h=args.waves_h,
                                   kchunk_g=args.kchunk_g, kchunk_h=args.kchunk_h)
 
+    if args.copy_probe:
+        ctx.copy_probe(1 << 30)
+        log("copy probe: 1 GiB")
     only = set(args.only.split(",")) if args.only else None
     results = {}
 
@@ -188,6 +206,7 @@ def main() -> int:
     h_avg = kt["h_ms"] / max(kt["h_launches"], 1)
     dom, dom_ms = ("H = A^T G", h_avg) if h_avg >= g_avg else ("G = A Omega", g_avg)
     achieved = flops_launch / (dom_ms * 1e-3) / 1e12
+    traffic = recorded_traffic(N, M, L, head, "H" if dom.startswith("H") else "G")
     value = float(np.mean(ts_bf16))
     out = {
         "metric": METRIC,
@@ -208,7 +227,8 @@ def main() -> int:
                    "form": "BF16 MFMA, exact genotypes, hi/lo BF16 split of Omega and G, FP32 accumulation"},
         "roofline": {"bound": "mfma", "kernel": f"packed_gemm ({dom})", "achieved": round(achieved, 2),
                      "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
-                     "traffic": None, "avg_launch_ms": round(dom_ms, 3),
+                     "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+                     "avg_launch_ms": round(dom_ms, 3),
                      "algorithmic_flop_per_launch": flops_launch,
                      "other_product_avg_ms": round(g_avg if dom.startswith("H") else h_avg, 3)},
         "setup_s": round(setup_s, 2),

@@ -184,6 +184,9 @@ int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint
 /* RsvdOpData::initOmg's matrix (src/Halko.cpp:15-23): nrows x ncols,
  * column-major, identical to the libstdc++ draw, computed in parallel. */
 int pcaone_host_omega(int seed, int gaussian, uint64_t nrows, uint64_t ncols, double* out);
+/* Copies `bytes` (rounded down to 16) between two scratch device buffers
+ * with a plain streaming kernel: a known-volume reference for HBM counters. */
+int pcaone_copy_probe(pcaone_ctx* ctx, uint64_t bytes);
 /* Host FP64 helpers of the L x L steps, row-major n x n. */
 int pcaone_host_sym_eig(int n, const double* A, double* w, double* V);
 int pcaone_host_householder_signs(int n, const double* Y, double* s);

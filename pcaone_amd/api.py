@@ -164,6 +164,9 @@ class Context:
         self._check(lib().pcaone_get_v(self._h, k, _ptr(V)))
         return V
 
+    def copy_probe(self, nbytes: int):
+        self._check(lib().pcaone_copy_probe(self._h, nbytes))
+
     def time_kernels(self, on: bool = True):
         self._check(lib().pcaone_time_kernels(self._h, int(on)))
 

@@ -422,6 +422,16 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
   });
 }
 
+int pcaone_copy_probe(pcaone_ctx* c, uint64_t bytes) {
+  return guard(c, [&](Context& x) {
+    const uint64_t n = bytes / 16 * 16;
+    DevBuf<unsigned char> a(n), b(n);
+    PCAONE_HIP(hipMemsetAsync(a.get(), 1, n, x.stream));
+    launch_copy_probe(a.get(), b.get(), n, x.stream);
+    PCAONE_HIP(hipStreamSynchronize(x.stream));
+  });
+}
+
 int pcaone_host_omega(int seed, int gaussian, uint64_t nrows, uint64_t ncols, double* out) {
   return guard_global([&] {
     require(out != nullptr, kErrArgument, "null output");

@@ -34,6 +34,8 @@ void launch_synth(uint64_t seed, uint32_t nsamples, uint32_t npops, const uint64
 // Interleaved rows -> PLINK rows of bps bytes.
 void launch_deinterleave(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint32_t nsamples, uint8_t* raw,
                          uint64_t bps, hipStream_t s);
+// Plain device-to-device copy of `bytes` (multiple of 16), for counter calibration.
+void launch_copy_probe(const void* in, void* out, uint64_t bytes, hipStream_t s);
 // Missing-entry lists: for row r, the positions k < ncodes whose code is 01, in
 // increasing order, written at ix[rp[r] ..].
 void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uint64_t ncodes, const uint64_t* rp,

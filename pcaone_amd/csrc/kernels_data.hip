@@ -263,3 +263,22 @@ void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uin
 }
 
 }  // namespace pcaone
+
+namespace pcaone {
+namespace {
+// Plain streaming copy (16 bytes per thread per iteration), used only to
+// calibrate HBM byte counters against a known traffic volume.
+__global__ void k_copy_probe(const uint4* __restrict__ in, uint4* __restrict__ out, uint64_t n) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x)
+    out[t] = in[t];
+}
+}  // namespace
+
+void launch_copy_probe(const void* in, void* out, uint64_t bytes, hipStream_t s) {
+  const uint64_t n = bytes / 16;
+  if (!n) return;
+  hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, s, static_cast<const uint4*>(in),
+                     static_cast<uint4*>(out), n);
+  check_launch("copy_probe");
+}
+}  // namespace pcaone

@@ -83,7 +83,7 @@ void omega(int seed, bool gaussian, uint64_t nrows, uint64_t ncols, double* out)
     return;
   }
   // A round covers the attempts the remaining values need on average (the
-  // polar method accepts pi/4 of them) plus 2%, split into 64 blocks.
+  // polar method accepts pi/4 of them) plus 2%, split into 256 blocks.
   // Two passes per round: count the accepted attempts of every block (no
   // log/sqrt), then redraw each block and write its values at its prefix
   // offset directly into `out`.
