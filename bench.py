@@ -129,6 +129,8 @@ def main() -> int:
     ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
     ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo) or bf16mix")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB plain copy (counter calibration)")
+    ap.add_argument("--pcie", action="store_true",
+                    help="also time the host-buffer boundary: .bed rows from host memory into HBM (upload + finalize)")
     ap.add_argument("--kchunk-g", type=int, default=0, help="split-K chunk of the G product in codes (0 = auto)")
     ap.add_argument("--kchunk-h", type=int, default=0, help="split-K chunk of the H product in codes (0 = auto)")
     args = ap.parse_args()
@@ -240,6 +242,15 @@ def main() -> int:
         ref = results["fp64"]
         out["fp64_form_seconds"] = round(ts64[0], 3)
         out["error_vs_fp64"] = {f: errors(ref, results[f]) for f in results if f != "fp64"}
+    if args.pcie:
+        packed = ctx.download_packed()
+        t0 = time.perf_counter()
+        ctx.upload_packed(packed, N)
+        t_up = time.perf_counter() - t0
+        out["pcie_inclusive"] = {"upload_finalize_s": round(t_up, 3),
+                                 "host_GB_per_s": round(packed.nbytes / t_up / 1e9, 2),
+                                 "time_to_k_pcs_from_host_s": round(t_up + value, 3)}
+        del packed
     if not args.no_cpu:
         try:
             log("cpu baseline ...")

@@ -153,6 +153,15 @@ int pcaone_save_prev(pcaone_ctx* ctx); /* Upre = Ucur */
 int pcaone_get_u(pcaone_ctx* ctx, uint32_t k, double* U); /* Ucur, N x k */
 int pcaone_get_v(pcaone_ctx* ctx, uint32_t k, double* V); /* G U_B, M x k (src/Halko.cpp:84) */
 
+/* The G of computeGandH (src/Halko.cpp:125, 195-197) for SNP rows
+ * [snp_begin, snp_end) in the current column order, as last written by
+ * pcaone_sketch and as multiplied by A^T (rounded to the form's operand):
+ * (snp_end - snp_begin) x L.  Lets a caller keep the reference's host
+ * computeUSV and replace only computeGandH. */
+int pcaone_get_g(pcaone_ctx* ctx, uint64_t snp_begin, uint64_t snp_end, double* G);
+/* H (which 0, src/Halko.cpp:126), H1 (1) or H2 (2) (src/Halko.cpp:198-203): N x L. */
+int pcaone_get_h(pcaone_ctx* ctx, int which, double* H);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* run_pca_with_halko (src/Halko.cpp:271-345) on the resident data.  U: N x k,
  * S and E (= S^2/M): k, V (nullable; filled with printv or emu): M x k in the

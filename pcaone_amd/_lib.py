@@ -81,6 +81,8 @@ _SIGS = {
     "pcaone_synth_write_plink": (C.c_int, [_P, C.c_char_p]),
     "pcaone_packed_gemm": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.c_uint32, _P]),
     "pcaone_copy_probe": (C.c_int, [_P, C.c_uint64]),
+    "pcaone_get_g": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
+    "pcaone_get_h": (C.c_int, [_P, C.c_int, _P]),
     "pcaone_host_omega": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint64, _P]),
     "pcaone_host_sym_eig": (C.c_int, [C.c_int, _P, _P, _P]),
     "pcaone_host_householder_signs": (C.c_int, [C.c_int, _P, _P]),

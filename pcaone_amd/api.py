@@ -164,6 +164,17 @@ class Context:
         self._check(lib().pcaone_get_v(self._h, k, _ptr(V)))
         return V
 
+    def get_g(self, j0: int, j1: int, L: int) -> np.ndarray:
+        G = np.empty((j1 - j0, L), order="F")
+        self._check(lib().pcaone_get_g(self._h, j0, j1, _ptr(G)))
+        return G
+
+    def get_h(self, which: int, L: int) -> np.ndarray:
+        n, _, _ = self.dims()
+        H = np.empty((n, L), order="F")
+        self._check(lib().pcaone_get_h(self._h, which, _ptr(H)))
+        return H
+
     def copy_probe(self, nbytes: int):
         self._check(lib().pcaone_copy_probe(self._h, nbytes))
 

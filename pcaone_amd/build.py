@@ -34,11 +34,9 @@ def _sources():
 
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
-    cmd = [HIPCC] + COMMON
-    if src.suffix == ".hip":
-        cmd += DEVICE + ["-x", "hip"]
-    else:
-        cmd += HOST
+    # hipcc compiles .cpp as HIP too: every source gets the gfx950 device flags.
+    cmd = [HIPCC] + COMMON + DEVICE
+    cmd += ["-x", "hip"] if src.suffix == ".hip" else HOST
     cmd += ["-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(line for line in (res.stdout + res.stderr).splitlines()

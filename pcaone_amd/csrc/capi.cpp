@@ -303,6 +303,20 @@ int pcaone_get_u(pcaone_ctx* c, uint32_t k, double* U) {
   });
 }
 
+int pcaone_get_g(pcaone_ctx* c, uint64_t snp_begin, uint64_t snp_end, double* G) {
+  return guard(c, [&](Context& x) {
+    require(G != nullptr, kErrArgument, "null G");
+    x.download_g(snp_begin, snp_end, G);
+  });
+}
+
+int pcaone_get_h(pcaone_ctx* c, int which, double* H) {
+  return guard(c, [&](Context& x) {
+    require(H != nullptr, kErrArgument, "null H");
+    x.download_h(which, H);
+  });
+}
+
 int pcaone_get_v(pcaone_ctx* c, uint32_t k, double* V) {
   return guard(c, [&](Context& x) {
     require(V != nullptr && k >= 1 && (int)k <= x.L, kErrArgument, "bad V request");

@@ -58,14 +58,31 @@ void Context::reserve(uint32_t nsamples, uint64_t nsnps) {
 void Context::upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n) {
   require(N > 0, kErrState, "reserve() first");
   require(j0 + n <= M, kErrArgument, "upload beyond the reserved SNP count");
-  const uint64_t chunk = std::max<uint64_t>(1, (256ull << 20) / std::max<uint64_t>(bps_, 1));
-  raw_.resize(std::min(n, chunk) * bps_);
-  for (uint64_t done = 0; done < n; done += chunk) {
-    const uint64_t m = std::min(chunk, n - done);
-    PCAONE_HIP(hipMemcpyAsync(raw_.get(), rows + done * bps_, m * bps_, hipMemcpyHostToDevice, stream));
-    launch_interleave(raw_.get(), m, bps_, N, xs_.get() + (j0 + done) * pitch_s_, pitch_s_, stream);
-    PCAONE_HIP(hipStreamSynchronize(stream));
+  if (!n) return;
+  // Two staging slots (pinned host + raw device rows): the host copies chunk
+  // c+1 into its pinned slot while the DMA and the interleave of chunk c run.
+  const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / std::max<uint64_t>(bps_, 1));
+  const uint64_t slot = std::min(n, chunk) * bps_;
+  raw_.resize(2 * slot);
+  pinned_.resize(2 * slot);
+  Event done[2];
+  for (uint64_t c = 0, r = 0; r < n; ++c, r += chunk) {
+    const int s = (int)(c & 1);
+    const uint64_t m = std::min(chunk, n - r), bytes = m * bps_;
+    if (c >= 2) PCAONE_HIP(hipEventSynchronize(done[s].get()));
+    unsigned char* host = pinned_.get() + s * slot;
+    const uint8_t* src = rows + r * bps_;
+    const int64_t pieces = 16;
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < pieces; ++q) {
+      const uint64_t b0 = bytes * q / pieces, b1 = bytes * (q + 1) / pieces;
+      std::memcpy(host + b0, src + b0, b1 - b0);
+    }
+    PCAONE_HIP(hipMemcpyAsync(raw_.get() + s * slot, host, bytes, hipMemcpyHostToDevice, stream));
+    launch_interleave(raw_.get() + s * slot, m, bps_, N, xs_.get() + (j0 + r) * pitch_s_, pitch_s_, stream);
+    PCAONE_HIP(hipEventRecord(done[s].get(), stream));
   }
+  PCAONE_HIP(hipStreamSynchronize(stream));
 }
 
 void Context::generate(const synth::Spec& spec) {
@@ -631,6 +648,26 @@ void Context::download_u(int k, double* U) {
   PCAONE_HIP(hipStreamSynchronize(stream));
   for (uint64_t i = 0; i < N; ++i)
     for (int j = 0; j < k; ++j) U[(uint64_t)j * N + i] = h[i * lp + j];
+}
+
+void Context::download_rowmajor(const double* src, uint64_t rows, double* out) {
+  std::vector<double> h(rows * (uint64_t)lp);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  for (int j = 0; j < L; ++j)
+    for (uint64_t i = 0; i < rows; ++i) out[(uint64_t)j * rows + i] = h[i * lp + j];
+}
+
+void Context::download_g(uint64_t j0, uint64_t j1, double* G) {
+  require(L > 0, kErrState, "configure() first");
+  require(j0 < j1 && j1 <= M, kErrArgument, "get_g: bad SNP range");
+  download_rowmajor(gt_.get() + j0 * lp, j1 - j0, G);
+}
+
+void Context::download_h(int which, double* H) {
+  require(L > 0, kErrState, "configure() first");
+  require(which >= 0 && which <= 2, kErrArgument, "get_h: which must be 0 (H), 1 (H1) or 2 (H2)");
+  download_rowmajor(which == 0 ? H_.get() : (which == 1 ? H1_.get() : H2_.get()), N, H);
 }
 
 void Context::compute_v(int k) {

@@ -103,6 +103,10 @@ class Context {
   void download_u(int k, double* U);
   void compute_v(int k);  // V := Q U_B[:, :k] on device
   void download_v(int k, double* V);
+  // G of the last sketches, rows [j0, j1) (current column order), and H / H1 / H2:
+  // column-major FP64 with L columns.
+  void download_g(uint64_t j0, uint64_t j1, double* G);
+  void download_h(int which, double* H);
   const double* device_v() const { return vbuf_.get(); }
 
   // timing of the hot kernels (HIP events on the context stream)
@@ -120,6 +124,7 @@ class Context {
   uint64_t rows_s_ = 0, rows_t_ = 0;   // allocated rows
   DevBuf<uint32_t> xs_, xt_, tmpx_;
   DevBuf<uint8_t> raw_;
+  PinnedBuf pinned_;
   DevBuf<double> dF_, cF_, sc_;
   DevBuf<uint64_t> rp_s_, rp_t_;
   DevBuf<uint32_t> ix_s_, ix_t_;
@@ -145,7 +150,8 @@ class Context {
   void upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc);
   std::vector<double> host_cf_, host_sc_;
   la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
-  void set_t(const la::Mat& T);  // uploads an L x L matrix into small_ as lp x lp (zero padded)
+  void set_t(const la::Mat& T);
+  void download_rowmajor(const double* src, uint64_t rows, double* out);  // rows x lp -> column-major rows x L  // uploads an L x L matrix into small_ as lp x lp (zero padded)
   void run_gemm(bool g_product, const GemmArgs& a);
 };
 

@@ -69,4 +69,47 @@ class DevBuf {
   size_t n_ = 0, cap_ = 0;
 };
 
+// Page-locked host memory (hipHostMalloc), for staging host <-> device copies.
+class PinnedBuf {
+ public:
+  PinnedBuf() = default;
+  ~PinnedBuf() { release(); }
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  // Reallocates only when growing; contents are not preserved.
+  void resize(size_t bytes) {
+    if (bytes <= cap_) return;
+    release();
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess)
+      fail(kErrDevice, "hipHostMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+    p_ = static_cast<unsigned char*>(p);
+    cap_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  unsigned char* get() const { return p_; }
+
+ private:
+  unsigned char* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
+// Owning HIP event (timing disabled: used for ordering only).
+class Event {
+ public:
+  Event() { PCAONE_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming)); }
+  ~Event() { (void)hipEventDestroy(ev_); }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  hipEvent_t get() const { return ev_; }
+
+ private:
+  hipEvent_t ev_ = nullptr;
+};
+
 }  // namespace pcaone

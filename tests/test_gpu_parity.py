@@ -100,11 +100,15 @@ def test_operator_epoch_matches_oracle(ctx, golden_inputs):
     om = O.omega(112, True, n, L)
     ctx.set_omega(om)
     ctx.sketch(0, m, 0)
-    sig, _ = ctx.finish_epoch(k, L, False)
     d = O.Data.from_packed(packed, n, O.Params(svd=1, k=k))
     d.standardize_E()
     G = d.G.T @ om
     H = d.G @ G
+    # the products themselves, as a caller keeping a host computeUSV reads them
+    np.testing.assert_allclose(ctx.get_g(0, m, L), G, rtol=1e-11, atol=1e-11 * np.abs(G).max())
+    np.testing.assert_allclose(ctx.get_g(100, 357, L), G[100:357], rtol=1e-11, atol=1e-11 * np.abs(G).max())
+    np.testing.assert_allclose(ctx.get_h(0, L), H, rtol=1e-11, atol=1e-11 * np.abs(H).max())
+    sig, _ = ctx.finish_epoch(k, L, False)
     Q1, R1 = np.linalg.qr(G)
     Q2, R2 = np.linalg.qr(Q1)
     B = np.linalg.solve((R2 @ R1).T, H.T)

@@ -144,14 +144,60 @@ def test_golden_mbim_allele_frequencies(golden_inputs):
     np.testing.assert_allclose(F, mb, rtol=5e-6)
 
 
-def test_planted_mistake_is_caught(golden_inputs, monkeypatch):
-    """Disabling flipOmg in the oracle must break the winSVD golden trace."""
-    meta = load_meta("f1_d2")
-    params, _ = params_from_args(meta["args"])
-    packed, n, _ = O.read_plink(golden_inputs["f1"])
+def _plant_no_flip(monkeypatch):
     monkeypatch.setattr(O, "flip_omg", lambda a, b: -b)
-    res = O.run_pca_with_halko(O.Data.from_packed(packed, n, params))
-    assert not np.allclose(res.trace, meta["trace"], atol=1.5e-6)
+
+
+def _plant_omega_row_major(monkeypatch):
+    orig = O.omega
+    # the draw sequence laid out row by row instead of column by column
+    monkeypatch.setattr(O, "omega", lambda s, g, n, l: orig(s, g, n, l).T.ravel().reshape(n, l).copy())
+
+
+def _plant_no_scaling(monkeypatch):
+    monkeypatch.setattr(O.Data, "standardize_E", lambda self: None)
+
+
+def _plant_no_centering(monkeypatch):
+    orig = O.Data.from_packed.__func__
+
+    def bad(cls, packed, n, params):
+        d = orig(cls, packed, n, params)
+        d.G = np.where(d.G != 0.0, d.G + d.F[None, :], 0.0)
+        return d
+    monkeypatch.setattr(O.Data, "from_packed", classmethod(bad))
+
+
+def _plant_swapped_clamp(monkeypatch):
+    def bad(self, U, S, VT):
+        pred = (U * S[None, :]) @ VT
+        mask = self.C[self.perm].T if self.params.perm else self.C.T
+        imp = np.minimum(np.maximum(pred, -(1.0 - self.F[None, :])), self.F[None, :])
+        self.G = np.where(mask, imp, self.G)
+    monkeypatch.setattr(O.Data, "fit_with_pi", bad)
+
+
+@pytest.mark.parametrize("case,plant", [
+    ("f1_d2", _plant_no_flip),
+    ("f1_d1_p0", _plant_omega_row_major),
+    ("f1_d1", _plant_no_scaling),
+    ("f1_d1", _plant_no_centering),
+    ("f4_emu_d1", _plant_swapped_clamp),
+])
+def test_planted_mistake_is_caught(case, plant, golden_inputs, monkeypatch):
+    """A plausible mistake planted in the oracle must fail the golden comparison."""
+    meta = load_meta(case)
+    params, printv = params_from_args(meta["args"])
+    packed, n, _ = O.read_plink(golden_inputs[meta["input"]])
+    plant(monkeypatch)
+    data = O.Data.from_packed(packed, n, params)
+    res = O.run_pca_with_halko(data)
+    with pytest.raises(AssertionError):
+        if meta["trace"]:
+            np.testing.assert_allclose(res.trace, meta["trace"], atol=1.5e-6)
+        if meta["em_trace"]:
+            np.testing.assert_allclose(res.em_trace, meta["em_trace"], atol=1.5e-6)
+        _outputs_match(case, res, data, printv)
 
 
 def test_dense_oracle_agrees_at_convergence(golden_inputs):
