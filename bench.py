@@ -109,6 +109,118 @@ def cpu_baseline(args, spec_full, k, os_, maxp):
                        f"(wall {t_wall:.2f}s minus {t_read:.2f}s reading) scaled x{scale:.0f} linearly in SNPs; {what}")}
 
 
+CONFIG_RUNS = {
+    # winSVD, 64 windows, 10 epochs (--maxp 9 --tol-rsvd 0), k = 20 (L = 40)
+    2: dict(params=dict(svd=2, k=20, oversamples=10, bands=64, maxp=9, tol=0.0), forms=("fp32", "bf16", "fp64"),
+            what="winSVD (--svd 2 -w 64 --maxp 9 --tol-rsvd 0), k=20, L=40"),
+    # EMU, 20 EM iterations (--emu --maxiter 20 --tol-em 0), --svd 1, k = 10 (L = 20)
+    5: dict(params=dict(svd=1, k=10, oversamples=10, emu=1, maxiter=20, tolem=0.0), forms=("bf16", "fp32", "fp64"),
+            what="EMU (--emu -d 1 --maxiter 20 --tol-em 0), k=10, L=20"),
+}
+
+
+def batch_line(args) -> int:
+    """Batched form: 256 cohorts of uneven size (N log-uniform in [500, 20000],
+    M log-uniform in [10,000, 200,000], 2-6 populations), --svd 1, k = 10,
+    oversamples 10, 5 power iterations (6 passes), all cohorts in one
+    persistent launch per step inside one HIP graph; against the same cohorts
+    run one after another through the single-matrix path."""
+    from oracle import pcaone_oracle as O  # checker only
+    from pcaone_amd import api, synth
+    specs = synth.cohorts(args.cohorts)
+    ctx = api.Context(0)
+    t0 = time.perf_counter()
+    ctx.batch_generate(specs)
+    log(f"batch: generated {len(specs)} cohorts in {time.perf_counter() - t0:.1f}s")
+    kw = dict(svd=1, k=10, oversamples=10, maxp=5, tol=0.0)
+    L, passes = 20, 6
+    flops = sum(4.0 * sp.nsamples * sp.nsnps * L * passes for sp in specs)
+    out = {"config": 4, "workload": f"{len(specs)} cohorts, N 500-20000, M 10k-200k (log-uniform), --svd 1, k=10, "
+                                    f"L=20, {passes} passes, one persistent launch per step, one HIP graph",
+           "cohorts": len(specs), "total_samples": sum(sp.nsamples for sp in specs),
+           "total_snps": sum(sp.nsnps for sp in specs), "algorithmic_flop": flops}
+    res, secs = {}, {}
+    forms = [f for f in (args.forms.split(",") if args.forms else ["bf16", "fp32"]) if f != "fp64"] + ["fp64"]
+    for form in forms:
+        p = api.default_params(precision=form, **kw)
+        r = ctx.batch_run(p, use_graph=True)  # first run: tuning + graph capture
+        log(f"batch {form}: first run {r['info']['seconds']:.3f}s, tuned G {r['info']['wm_g']}x{r['info']['waves_g']}/"
+            f"{r['info']['kchunk_g']} H {r['info']['wm_h']}x{r['info']['waves_h']}/{r['info']['kchunk_h']}")
+        ts = []
+        for _ in range(max(args.steps, 1) if form != "fp64" else 1):
+            t0 = time.perf_counter()
+            r = ctx.batch_run(p, use_graph=True)
+            ts.append(time.perf_counter() - t0)
+        assert int(r["info"]["failed"]) == 0
+        secs[form] = float(np.mean(ts))
+        res[form] = (r, [ctx.batch_get_u(c) for c in range(len(specs))])
+        out[f"{form}_info"] = r["info"]
+        log(f"batch {form}: {secs[form]:.4f}s per run ({flops / secs[form] / 1e12:.1f} TFLOP/s algorithmic)")
+    out["seconds"] = {f: round(v, 4) for f, v in secs.items()}
+    out["algorithmic_tflops"] = {f: round(flops / v / 1e12, 1) for f, v in secs.items()}
+    r64, u64 = res["fp64"]
+    errs = {}
+    for f in res:
+        if f == "fp64":
+            continue
+        r, us = res[f]
+        rel = max(float(np.max(np.abs(r["E"][c] - r64["E"][c]) / r64["E"][c])) for c in range(len(specs)))
+        corr = min(float(O.pc_correlations(u64[c], us[c]).min()) for c in range(len(specs)))
+        errs[f] = {"eig_rel_err_max": rel, "pc_corr_min": corr}
+    out["error_vs_fp64"] = errs
+    if args.sequential:
+        # the same cohorts one after another through the single-matrix path (generation untimed)
+        t_seq = 0.0
+        p = api.default_params(precision=forms[0], **kw)
+        for sp in specs[: args.sequential]:
+            ctx.generate(sp)
+            t0 = time.perf_counter()
+            ctx.run(p)
+            t_seq += time.perf_counter() - t0
+        scale = len(specs) / min(args.sequential, len(specs))
+        out["sequential_single_path"] = {"form": forms[0], "cohorts_timed": min(args.sequential, len(specs)),
+                                         "seconds_scaled_to_all": round(t_seq * scale, 3)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def config_line(args) -> int:
+    """One line of timings and errors (against the FP64 form) for another configuration."""
+    from pcaone_amd import api, synth
+    if args.config == 4:
+        return batch_line(args)
+    run = CONFIG_RUNS[args.config]
+    spec = synth.config(args.config)
+    ctx = api.Context(0)
+    t0 = time.perf_counter()
+    ctx.generate(spec)
+    log(f"config {args.config}: generated {spec.nsamples} x {spec.nsnps} in {time.perf_counter() - t0:.1f}s")
+    res, secs = {}, {}
+    forms = [f for f in (args.forms.split(",") if args.forms else run["forms"]) if f != "fp64"] + ["fp64"]
+    for form in forms:
+        p = api.default_params(precision=form, verbose=int(args.verbose), **run["params"])
+        if form != "fp64" and args.warmup:
+            ctx.run(p)  # warm-up
+        t0 = time.perf_counter()
+        res[form] = ctx.run(p)
+        secs[form] = time.perf_counter() - t0
+        log(f"config {args.config} {form}: {secs[form]:.3f}s, epochs {res[form].info['total_epochs']}, "
+            f"em {res[form].info['em_iters']}")
+    ref = res["fp64"]
+    out = {"config": args.config, "workload": f"{spec.nsamples} samples x {spec.nsnps} SNPs, {run['what']}",
+           "seconds": {f: round(v, 4) for f, v in secs.items()},
+           "epochs": {f: r.info["total_epochs"] for f, r in res.items()},
+           "error_vs_fp64": {f: errors(ref, r) for f, r in res.items() if f != "fp64"},
+           "g_ms_h_ms_per_run": None}
+    ctx.time_kernels(True)
+    rk = ctx.run(api.default_params(precision=forms[0], **run["params"]))
+    ctx.time_kernels(False)
+    out["g_ms_h_ms_per_run"] = [round(rk.info["g_ms"], 2), round(rk.info["h_ms"], 2),
+                                rk.info["g_launches"], rk.info["h_launches"]]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +240,14 @@ def main() -> int:
     ap.add_argument("--waves-g", type=int, default=0, help="waves per workgroup of the G product (0 = default)")
     ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
     ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo) or bf16mix")
+    ap.add_argument("--config", type=int, default=3,
+                    help="3: the headline workload; 2 (winSVD), 4 (batched cohorts) or 5 (EMU): a timing/error "
+                         "line for that configuration at full size instead")
+    ap.add_argument("--forms", default="", help="with --config: comma list of forms (default: all, fp64 last)")
+    ap.add_argument("--cohorts", type=int, default=256, help="with --config 4: number of cohorts")
+    ap.add_argument("--sequential", type=int, default=256,
+                    help="with --config 4: cohorts also run one by one through the single-matrix path (0 = none)")
+    ap.add_argument("--verbose", action="store_true", help="with --config: the library prints epochs and EM steps")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--pcie", action="store_true",
                     help="also time the host-buffer boundary: .bed rows from host memory into HBM (upload + finalize)")
@@ -143,6 +263,8 @@ def main() -> int:
 
     from pcaone_amd import api, synth
 
+    if args.config != 3:
+        return config_line(args)
     spec = synth.config(3, nsamples=args.nsamples, nsnps=args.nsnps)
     k, os_, maxp = args.k, args.oversamples, args.maxp
     L = k + max(os_, k)  # src/Cmd.cpp:216

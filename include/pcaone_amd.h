@@ -178,6 +178,42 @@ int pcaone_run_bfile(pcaone_ctx* ctx, const char* bfile, const char* out, const 
 /* Kernel timing with HIP events around the two products (on/off). */
 int pcaone_time_kernels(pcaone_ctx* ctx, int on);
 
+/* ---- batched form -------------------------------------------------------- */
+/* Many independent cohorts (each its own genotype matrix, complete data) run
+ * the --svd 1 randomized SVD together: equivalent to one run_pca_with_halko
+ * (src/Halko.cpp:271-345, NormalRsvdOpData src/Halko.cpp:99-127) per cohort
+ * with shared flags.  Every product of one step of all cohorts is one
+ * persistent grouped launch; the L x L steps run on the device (L = k +
+ * oversamples <= 32); the whole run is one HIP graph.  GEMM tile shapes and
+ * split-K chunks are picked by timing candidates on the first run (unless
+ * the wm_* / waves_* / kchunk_* fields of pcaone_params fix them). */
+typedef struct pcaone_batch_info {
+  double seconds;               /* device time of the run (launch to completion)   */
+  int wm_g, waves_g, wm_h, waves_h; /* GEMM shapes used for G = A Omega, H = A^T G */
+  uint64_t kchunk_g, kchunk_h;  /* split-K chunks used (codes)                      */
+  double tuned_g_ms, tuned_h_ms; /* timed grouped launches of the chosen shapes    */
+  int tuned_candidates;         /* configurations the tuner timed (0 = fixed)      */
+  int failed;                   /* cohorts with a non-zero status                  */
+} pcaone_batch_info;
+int pcaone_batch_reserve(pcaone_ctx* ctx, uint32_t ncohorts, const uint32_t* nsamples, const uint64_t* nsnps);
+/* Raw .bed rows of one cohort (ceil(N_c/4) bytes per SNP), then finalize. */
+int pcaone_batch_upload(pcaone_ctx* ctx, uint32_t cohort, const uint8_t* rows, uint64_t snp_begin,
+                        uint64_t nsnps);
+int pcaone_batch_finalize(pcaone_ctx* ctx);
+/* reserve + device generation + finalize, one spec per cohort. */
+int pcaone_batch_generate(pcaone_ctx* ctx, uint32_t ncohorts, const pcaone_synth_spec* specs);
+int pcaone_batch_allele_freq(pcaone_ctx* ctx, uint32_t cohort, double* F);
+/* S, E: ncohorts x k (E = S^2 / M_c, src/Halko.cpp:339; zero past a cohort's
+ * singular values); epochs, status: per cohort (status 0 or PCAONE_ERR_*).
+ * use_graph: capture the run once and replay it; want_v: also loadings. */
+int pcaone_batch_run(pcaone_ctx* ctx, const pcaone_params* p, int use_graph, int want_v, double* S, double* E,
+                     int* epochs, int* status, pcaone_batch_info* info);
+/* Results of the last batched run: U (N_c x k) and V (M_c x k), column-major;
+ * the diff trace of the checked epochs (computeUSV's 1 - mev). */
+int pcaone_batch_get_u(pcaone_ctx* ctx, uint32_t cohort, double* U);
+int pcaone_batch_get_v(pcaone_ctx* ctx, uint32_t cohort, double* V);
+int pcaone_batch_trace(const pcaone_ctx* ctx, uint32_t cohort, double* out, int cap, int* n);
+
 /* ---- synthetic data ----------------------------------------------------- */
 /* packed_out: nsnps x ceil(nsamples/4) bytes; attempts_out (nullable): per-SNP
  * redraw counter. */

@@ -47,6 +47,13 @@ class RunInfo(C.Structure):
 _lib = None
 
 _P = C.c_void_p
+class BatchInfo(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("wm_g", C.c_int), ("waves_g", C.c_int), ("wm_h", C.c_int),
+                ("waves_h", C.c_int), ("kchunk_g", C.c_uint64), ("kchunk_h", C.c_uint64),
+                ("tuned_g_ms", C.c_double), ("tuned_h_ms", C.c_double), ("tuned_candidates", C.c_int),
+                ("failed", C.c_int)]
+
+
 _SIGS = {
     "pcaone_last_error": (C.c_char_p, [_P]),
     "pcaone_device_count": (C.c_int, [_P]),
@@ -75,6 +82,15 @@ _SIGS = {
     "pcaone_get_v": (C.c_int, [_P, C.c_uint32, _P]),
     "pcaone_run_halko": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "pcaone_run_trace": (C.c_int, [_P, C.c_int, _P, C.c_int, _P]),
+    "pcaone_batch_reserve": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "pcaone_batch_upload": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint64]),
+    "pcaone_batch_finalize": (C.c_int, [_P]),
+    "pcaone_batch_generate": (C.c_int, [_P, C.c_uint32, _P]),
+    "pcaone_batch_allele_freq": (C.c_int, [_P, C.c_uint32, _P]),
+    "pcaone_batch_run": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "pcaone_batch_get_u": (C.c_int, [_P, C.c_uint32, _P]),
+    "pcaone_batch_get_v": (C.c_int, [_P, C.c_uint32, _P]),
+    "pcaone_batch_trace": (C.c_int, [_P, C.c_uint32, _P, C.c_int, _P]),
     "pcaone_run_bfile": (C.c_int, [_P, C.c_char_p, C.c_char_p, _P, _P]),
     "pcaone_time_kernels": (C.c_int, [_P, C.c_int]),
     "pcaone_synth_packed": (C.c_int, [_P, _P, _P]),

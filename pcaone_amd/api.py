@@ -175,6 +175,59 @@ class Context:
         self._check(lib().pcaone_get_h(self._h, which, _ptr(H)))
         return H
 
+    # ------------------------------------------------------------ batched form
+    def batch_generate(self, specs):
+        arr = (_lib.SynthSpec * len(specs))(*[sp.c() for sp in specs])
+        self._check(lib().pcaone_batch_generate(self._h, len(specs), arr))
+        self._batch_dims = [(sp.nsamples, sp.nsnps) for sp in specs]
+
+    def batch_upload(self, cohorts):
+        """cohorts: list of (packed rows (M x ceil(N/4) uint8), N)."""
+        n = np.array([c[1] for c in cohorts], dtype=np.uint32)
+        m = np.array([c[0].shape[0] for c in cohorts], dtype=np.uint64)
+        self._check(lib().pcaone_batch_reserve(self._h, len(cohorts), _ptr(n), _ptr(m)))
+        for i, (packed, nsamp) in enumerate(cohorts):
+            packed = np.ascontiguousarray(packed, dtype=np.uint8)
+            assert packed.shape[1] == (nsamp + 3) // 4
+            self._check(lib().pcaone_batch_upload(self._h, i, _ptr(packed), 0, packed.shape[0]))
+        self._check(lib().pcaone_batch_finalize(self._h))
+        self._batch_dims = [(int(a), int(b)) for a, b in zip(n, m)]
+
+    def batch_allele_freq(self, c: int) -> np.ndarray:
+        F = np.empty(self._batch_dims[c][1])
+        self._check(lib().pcaone_batch_allele_freq(self._h, c, _ptr(F)))
+        return F
+
+    def batch_run(self, params: Params | None = None, use_graph: bool = True, want_v: bool = False, **kw):
+        p = params if params is not None else default_params(**kw)
+        nc, k = len(self._batch_dims), p.k
+        S = np.zeros((nc, k))
+        E = np.zeros((nc, k))
+        ep = np.zeros(nc, dtype=np.int32)
+        stt = np.zeros(nc, dtype=np.int32)
+        info = _lib.BatchInfo()
+        self._check(lib().pcaone_batch_run(self._h, C.byref(p), int(use_graph), int(want_v), _ptr(S), _ptr(E),
+                                           _ptr(ep), _ptr(stt), C.byref(info)))
+        self._batch_k = k
+        d = {f: getattr(info, f) for f, _ in _lib.BatchInfo._fields_}
+        return {"S": S, "E": E, "epochs": ep, "status": stt, "info": d}
+
+    def batch_get_u(self, c: int) -> np.ndarray:
+        U = np.empty((self._batch_dims[c][0], self._batch_k), order="F")
+        self._check(lib().pcaone_batch_get_u(self._h, c, _ptr(U)))
+        return U
+
+    def batch_get_v(self, c: int) -> np.ndarray:
+        V = np.empty((self._batch_dims[c][1], self._batch_k), order="F")
+        self._check(lib().pcaone_batch_get_v(self._h, c, _ptr(V)))
+        return V
+
+    def batch_trace(self, c: int):
+        buf = np.empty(64)
+        n = C.c_int(0)
+        self._check(lib().pcaone_batch_trace(self._h, c, _ptr(buf), 64, C.byref(n)))
+        return buf[: n.value].tolist()
+
     def copy_probe(self, nbytes: int):
         self._check(lib().pcaone_copy_probe(self._h, nbytes))
 

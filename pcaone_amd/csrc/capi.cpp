@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/pcaone_amd.h"
+#include "batch.hpp"
 #include "common.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
@@ -25,7 +26,9 @@ using namespace pcaone;
 
 struct pcaone_ctx {
   std::unique_ptr<Context> ctx;
+  std::unique_ptr<Batch> batch;
   RunInfo last;
+  BatchResult last_batch;
   std::string err;
 };
 
@@ -340,6 +343,116 @@ int pcaone_run_halko(pcaone_ctx* c, const pcaone_params* p, double* U, double* S
     if (E) std::memcpy(E, r.E.data(), r.E.size() * sizeof(double));
     if (V && !r.V.empty()) std::memcpy(V, r.V.data(), M * q.k * sizeof(double));
     fill_info(info, c->last, x, (int)r.S.size());
+  });
+}
+
+// ---------------------------------------------------------------- batched form
+int pcaone_batch_reserve(pcaone_ctx* c, uint32_t ncohorts, const uint32_t* nsamples, const uint64_t* nsnps) {
+  return guard(c, [&](Context& x) {
+    require(ncohorts > 0 && nsamples && nsnps, kErrArgument, "batch: empty cohort list");
+    if (!c->batch) c->batch = std::make_unique<Batch>(x);
+    c->batch->reserve(std::vector<uint32_t>(nsamples, nsamples + ncohorts),
+                      std::vector<uint64_t>(nsnps, nsnps + ncohorts));
+  });
+}
+
+int pcaone_batch_upload(pcaone_ctx* c, uint32_t cohort, const uint8_t* rows, uint64_t snp_begin, uint64_t nsnps) {
+  return guard(c, [&](Context&) {
+    require(c->batch != nullptr, kErrState, "batch: reserve first");
+    require(rows != nullptr || nsnps == 0, kErrArgument, "null rows");
+    c->batch->upload_rows(cohort, rows, snp_begin, nsnps);
+  });
+}
+
+int pcaone_batch_generate(pcaone_ctx* c, uint32_t ncohorts, const pcaone_synth_spec* specs) {
+  return guard(c, [&](Context& x) {
+    require(ncohorts > 0 && specs, kErrArgument, "batch: empty spec list");
+    std::vector<synth::Spec> v;
+    std::vector<uint32_t> n;
+    std::vector<uint64_t> m;
+    for (uint32_t i = 0; i < ncohorts; ++i) {
+      v.push_back(to_spec(&specs[i]));
+      n.push_back(specs[i].nsamples);
+      m.push_back(specs[i].nsnps);
+    }
+    if (!c->batch) c->batch = std::make_unique<Batch>(x);
+    c->batch->reserve(n, m);
+    c->batch->generate(v);
+    c->batch->finalize();
+  });
+}
+
+int pcaone_batch_finalize(pcaone_ctx* c) {
+  return guard(c, [&](Context&) {
+    require(c->batch != nullptr, kErrState, "batch: reserve first");
+    c->batch->finalize();
+  });
+}
+
+int pcaone_batch_allele_freq(pcaone_ctx* c, uint32_t cohort, double* F) {
+  return guard(c, [&](Context&) {
+    require(c->batch != nullptr && cohort < c->batch->size() && F, kErrArgument, "batch: bad cohort");
+    const auto& f = c->batch->allele_freq(cohort);
+    require(!f.empty(), kErrState, "batch: finalize first");
+    std::memcpy(F, f.data(), f.size() * sizeof(double));
+  });
+}
+
+int pcaone_batch_run(pcaone_ctx* c, const pcaone_params* p, int use_graph, int want_v, double* S, double* E,
+                     int* epochs, int* status, pcaone_batch_info* info) {
+  return guard(c, [&](Context&) {
+    require(c->batch != nullptr, kErrState, "batch: no cohorts");
+    Param q = to_param(p);
+    Batch& b = *c->batch;
+    b.run(q, want_v != 0, use_graph != 0, c->last_batch);
+    const BatchResult& r = c->last_batch;
+    const uint32_t n = b.size(), k = r.k;
+    for (uint32_t i = 0; i < n; ++i) {
+      for (uint32_t j = 0; j < k; ++j) {
+        const double sv = (int)j < r.nsv[i] ? r.S[(uint64_t)i * k + j] : 0.0;
+        if (S) S[(uint64_t)i * k + j] = sv;
+        if (E) E[(uint64_t)i * k + j] = sv * sv / (double)b.nsnps(i);  // src/Halko.cpp:339
+      }
+      if (epochs) epochs[i] = r.epochs[i];
+      if (status) status[i] = r.status[i];
+    }
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      const BatchTuning& t = b.tuning();
+      info->seconds = b.last_seconds;
+      info->wm_g = t.g.wm, info->waves_g = t.g.waves, info->wm_h = t.h.wm, info->waves_h = t.h.waves;
+      info->kchunk_g = t.kchunk_g, info->kchunk_h = t.kchunk_h;
+      info->tuned_g_ms = t.g_ms, info->tuned_h_ms = t.h_ms;
+      info->tuned_candidates = t.candidates;
+      int failed = 0;
+      for (uint32_t i = 0; i < n; ++i) failed += r.status[i] != 0;
+      info->failed = failed;
+    }
+  });
+}
+
+int pcaone_batch_get_u(pcaone_ctx* c, uint32_t cohort, double* U) {
+  return guard(c, [&](Context&) {
+    require(c->batch != nullptr && U, kErrArgument, "batch: bad request");
+    c->batch->get_u(cohort, U);
+  });
+}
+
+int pcaone_batch_get_v(pcaone_ctx* c, uint32_t cohort, double* V) {
+  return guard(c, [&](Context&) {
+    require(c->batch != nullptr && V, kErrArgument, "batch: bad request");
+    c->batch->get_v(cohort, V);
+  });
+}
+
+int pcaone_batch_trace(const pcaone_ctx* c, uint32_t cohort, double* out, int cap, int* n) {
+  return guard(const_cast<pcaone_ctx*>(c), [&](Context&) {
+    const BatchResult& r = c->last_batch;
+    require(cohort < r.epochs.size(), kErrArgument, "batch: bad cohort");
+    const int cnt = r.epochs[cohort] > 1 ? r.epochs[cohort] - 1 : 0;  // checked epochs 1 ..
+    if (n) *n = cnt;
+    if (out)
+      for (int i = 0; i < cap && i < cnt; ++i) out[i] = r.trace[(uint64_t)cohort * kBatchMaxEpochs + 1 + i];
   });
 }
 

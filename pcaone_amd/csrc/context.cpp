@@ -13,6 +13,9 @@ namespace pcaone {
 namespace {
 constexpr uint64_t kRowPad = 512;  // largest GEMM row tile (4 waves x 8 x 16 rows)
 constexpr double kVarTol = 1e-9;   // src/Data.hpp:7
+constexpr uint64_t kChunkCapH = 16384;  // largest split-K chunk of the H product (codes)
+constexpr uint64_t kL2Bytes = 2ull << 20;  // working set per chunk of gathered rows (half an XCD's L2)
+constexpr uint64_t kMissPartialElems = 32ull << 20;  // FP64 partials of chunked corrections (256 MB)
 
 uint64_t bytes_of(Form f) { return form_wbytes(f); }
 }  // namespace
@@ -85,12 +88,12 @@ void Context::upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n) {
   PCAONE_HIP(hipStreamSynchronize(stream));
 }
 
-void Context::generate(const synth::Spec& spec) {
+void synth_rows(const synth::Spec& spec, uint32_t* xs, uint64_t pitch, hipStream_t stream) {
   // Per-SNP population frequencies (Beta draws) on the host, genotypes on the
   // device with the shared integer generator; SNPs that come out
   // monomorphic are redrawn with the next attempt, as the host writer does.
-  synth::validate(spec);
-  reserve(spec.nsamples, spec.nsnps);
+  const uint64_t M = spec.nsnps;
+  const uint32_t N = spec.nsamples;
   const uint32_t P = spec.npops;
   const auto fst = synth::pop_fst(spec);
   const auto flags = synth::sample_flags(spec);
@@ -105,10 +108,10 @@ void Context::generate(const synth::Spec& spec) {
   PCAONE_HIP(hipMemcpyAsync(dthr.get(), thr.data(), thr.size() * 8, hipMemcpyHostToDevice, stream));
   PCAONE_HIP(hipMemcpyAsync(datt.get(), att.data(), M * 4, hipMemcpyHostToDevice, stream));
   PCAONE_HIP(hipMemcpyAsync(dflags.get(), flags.data(), N, hipMemcpyHostToDevice, stream));
-  launch_synth(spec.seed, N, P, dthr.get(), datt.get(), dflags.get(), ms, nullptr, M, xs_.get(), pitch_s_, stream);
+  launch_synth(spec.seed, N, P, dthr.get(), datt.get(), dflags.get(), ms, nullptr, M, xs, pitch, stream);
   std::vector<uint32_t> h(4 * M);
   for (uint32_t round = 0;; ++round) {
-    launch_code_counts(xs_.get(), pitch_s_, M, N, cnt.get(), stream);
+    launch_code_counts(xs, pitch, M, N, cnt.get(), stream);
     PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), h.size() * 4, hipMemcpyDeviceToHost, stream));
     PCAONE_HIP(hipStreamSynchronize(stream));
     std::vector<uint32_t> redo;
@@ -125,9 +128,15 @@ void Context::generate(const synth::Spec& spec) {
     PCAONE_HIP(hipMemcpyAsync(dthr.get(), thr.data(), thr.size() * 8, hipMemcpyHostToDevice, stream));
     PCAONE_HIP(hipMemcpyAsync(datt.get(), att.data(), M * 4, hipMemcpyHostToDevice, stream));
     PCAONE_HIP(hipMemcpyAsync(drows.get(), redo.data(), redo.size() * 4, hipMemcpyHostToDevice, stream));
-    launch_synth(spec.seed, N, P, dthr.get(), datt.get(), dflags.get(), ms, drows.get(), redo.size(), xs_.get(),
-                 pitch_s_, stream);
+    launch_synth(spec.seed, N, P, dthr.get(), datt.get(), dflags.get(), ms, drows.get(), redo.size(), xs, pitch,
+                 stream);
   }
+}
+
+void Context::generate(const synth::Spec& spec) {
+  synth::validate(spec);
+  reserve(spec.nsamples, spec.nsnps);
+  synth_rows(spec, xs_.get(), pitch_s_, stream);
   finalize();
 }
 
@@ -169,6 +178,7 @@ void Context::finalize() {
 }
 
 void Context::upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc) {
+  weights_dirty_ = true;
   host_cf_ = cf;
   host_sc_ = sc;
   // padded to rows_s_ so window tiles may read past M
@@ -183,6 +193,7 @@ void Context::upload_col_state(const std::vector<double>& cf, const std::vector<
 }
 
 void Context::build_derived() {
+  weights_dirty_ = true;
   xt_.resize(rows_t_ * pitch_t_);
   launch_transpose_codes(xs_.get(), pitch_s_, M, N, xt_.get(), pitch_t_, rows_t_, stream);
   if (nmiss == 0) {
@@ -296,9 +307,13 @@ void Context::set_impute(const double* U, const double* S, const double* V, int 
   PCAONE_HIP(hipMemcpyAsync(vimp_.get(), v.data(), v.size() * 8, hipMemcpyHostToDevice, stream));
   PCAONE_HIP(hipStreamSynchronize(stream));
   emu_k_ = kk;
+  weights_dirty_ = true;
 }
 
-void Context::clear_impute() { emu_k_ = 0; }
+void Context::clear_impute() {
+  if (emu_k_) weights_dirty_ = true;
+  emu_k_ = 0;
+}
 
 void Context::reset_columns() {
   require(!F.empty(), kErrState, "no data");
@@ -357,22 +372,21 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   shape_h = GemmShape{tuning.wm_h ? tuning.wm_h : dh.wm, tuning.waves_h ? tuning.waves_h : dh.waves};
   require(gemm_shape_supported(form_g, shape_g, lp) && gemm_shape_supported(form_h, shape_h, lp), kErrArgument,
           "unsupported GEMM workgroup shape for this precision");
-  const uint64_t tile_g = gemm_row_tile(shape_g), tile_h = gemm_row_tile(shape_h);
-  require(rows_t_ % tile_h == 0, kErrInternal, "row padding");
-  if (!tuning.kchunk_g) {
-    const uint64_t wgs = round_up(M, tile_g) / tile_g;
-    uint64_t nch = wgs >= 1024 ? 1 : (2048 + wgs - 1) / wgs;
-    nch = std::min<uint64_t>(nch, ks_ / kKStep);
-    tuning.kchunk_g = round_up((ks_ + nch - 1) / nch, kKStep);
+  miss_f32_ = precision != 0;  // FP64 form: FP64 sources
+  cp_ = (int)round_up((uint64_t)L, miss_f32_ ? 4 : 2);
+  if (nmiss) {
+    const uint64_t esz = miss_f32_ ? 4 : 8;
+    csg_.resize(nrows_n * cp_ * esz);
+    csh_.resize(rows_s_ * cp_ * esz);
+    mpart_.resize(kMissPartialElems);
   }
-  if (!tuning.kchunk_h) tuning.kchunk_h = std::min<uint64_t>(16384, kt_);
-  tuning.kchunk_g = round_up(tuning.kchunk_g, kKStep);
+  fixed_g_ = tuning.wm_g || tuning.waves_g;
+  fixed_h_ = tuning.wm_h || tuning.waves_h;
+  tuning.kchunk_g = round_up(tuning.kchunk_g, kKStep);  // 0 stays 0 (planned per launch)
   tuning.kchunk_h = round_up(tuning.kchunk_h, kKStep);
-  const uint64_t nch_g = (ks_ + tuning.kchunk_g - 1) / tuning.kchunk_g;
-  const uint64_t nch_h = (wh_pitch_ + tuning.kchunk_h - 1) / tuning.kchunk_h;
-  const uint64_t pg = nch_g * (round_up(M, tile_g) + tile_g) * lp;
-  const uint64_t ph = nch_h * rows_t_ * lp;
-  part_.resize(std::max(pg * form_accbytes(form_g), ph * form_accbytes(form_h)));
+  const uint64_t pg = plan_partial_rows(rows_s_, ks_, tuning.kchunk_g, ks_) * lp * form_accbytes(form_g);
+  const uint64_t ph = plan_partial_rows(rows_t_, wh_pitch_, tuning.kchunk_h, kChunkCapH) * lp * form_accbytes(form_h);
+  part_.resize(std::max(pg, ph));
   h_version_ = 0;
   gram_version_ = ~0ull;
   ktimes = KernelTimes{};
@@ -409,14 +423,16 @@ void Context::set_omega(const double* om) {
   PCAONE_HIP(hipStreamSynchronize(stream));
 }
 
-void Context::run_gemm(bool g_product, const GemmArgs& a) {
+void Context::run_gemm(bool g_product, const GemmShape& shape, uint64_t nchunks, const GemmArgs& a) {
+  require(nchunks * a.rows * (uint64_t)lp * form_accbytes(g_product ? form_g : form_h) <= part_.bytes(),
+          kErrInternal, "partial products exceed their buffer");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (time_kernels) {
     PCAONE_HIP(hipEventCreate(&e0));
     PCAONE_HIP(hipEventCreate(&e1));
     PCAONE_HIP(hipEventRecord(e0, stream));
   }
-  launch_packed_gemm(g_product ? form_g : form_h, g_product ? shape_g : shape_h, a, stream);
+  launch_packed_gemm(g_product ? form_g : form_h, shape, a, stream);
   if (time_kernels) {
     PCAONE_HIP(hipEventRecord(e1, stream));
     PCAONE_HIP(hipEventSynchronize(e1));
@@ -435,13 +451,13 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   const uint64_t nwin = j1 - j0;
   const uint64_t wb = bytes_of(form_h);
   const int P = form_pieces(form_h);
-  const uint64_t tile_g = gemm_row_tile(shape_g), tile_h = gemm_row_tile(shape_h);
 
   // (1) G[j0:j1] = A Omega : SNP-major rows, K = samples
+  const GemmPlan pg = plan_gemm(form_g, shape_g, fixed_g_, lp, nwin, ks_, tuning.kchunk_g, ks_);
   GemmArgs g{};
   g.X = xs_.get() + j0 * pitch_s_;
   g.xpitch = pitch_s_;
-  g.rows = round_up(nwin, tile_g);
+  g.rows = pg.rows;
   g.Wt = wg_.get();
   g.wpitch = wg_pitch_;
   g.wpiece = (uint64_t)lp * wg_pitch_;
@@ -449,11 +465,11 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   g.ychunk = g.rows * lp;
   g.k_begin = 0;
   g.k_end = ks_;
-  g.kchunk = tuning.kchunk_g;
+  g.kchunk = pg.kchunk;
   g.lp = lp;
   require(j0 + g.rows <= rows_s_, kErrInternal, "G window exceeds padded rows");
-  run_gemm(true, g);
-  const uint64_t nch_g = (ks_ + g.kchunk - 1) / g.kchunk;
+  run_gemm(true, pg.shape, pg.nchunks, g);
+  const uint64_t nch_g = pg.nchunks;
   double* gwin = gt_.get() + j0 * lp;
   // The window's operand pieces of diag(sc) G~ sit at K offset j0 % 128 of wh_;
   // the K margins around them must read as zero.
@@ -470,33 +486,38 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   } else {
     launch_epi_g(form_g, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                  colsum_.get(), gwin, stream);
-    MissArgs m{};
+    update_missing_weights();
+    launch_compact_rows(omega_r_.get(), N, lp, L, cp_, miss_f32_, csg_.get(), stream);
+    MissSpmmArgs m{};
     m.rp = rp_s_.get() + j0;
     m.ix = ix_s_.get();
+    m.w = w_s_.get();
     m.nrows = nwin;
     m.k_lo = 0;
     m.k_hi = N;
-    m.by_snp = true;
-    m.sc = sc_.get() + j0;
-    m.cf = cF_.get() + j0;
-    m.Fclamp = emu_k_ ? dF_.get() + j0 : nullptr;
-    m.US = us_.get();
-    m.V = emu_k_ ? vimp_.get() + j0 * emu_k_ : nullptr;
-    m.kk = emu_k_;
-    m.src = omega_r_.get();
+    m.src = csg_.get();
+    m.f32 = miss_f32_;
+    m.src_k0 = 0;
+    m.cp = cp_;
+    m.ncols = L;
+    m.kspan = miss_span(nwin, N);
+    m.partial = mpart_.get();
     m.out = gwin;
     m.lp = lp;
-    launch_missing_correction(m, stream);
+    require(missing_spmm_chunks(m) == 1 || missing_spmm_chunks(m) * nwin * L <= mpart_.size(), kErrInternal,
+            "missing-entry partials exceed their buffer");
+    launch_missing_spmm(m, stream);
     launch_make_pieces(form_h, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
                        (uint64_t)lp * wh_pitch_, gwin, nullptr, stream);
     launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
   }
 
   // (3) H contribution: sample-major rows, K = the window's SNPs
+  const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h, kChunkCapH);
   GemmArgs h{};
   h.X = xt_.get() + (j0 - off) / 16;
   h.xpitch = pitch_t_;
-  h.rows = round_up(N, tile_h);
+  h.rows = ph.rows;
   h.Wt = wh_.get();
   h.wpitch = wh_pitch_;
   h.wpiece = (uint64_t)lp * wh_pitch_;
@@ -504,33 +525,81 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   h.ychunk = h.rows * lp;
   h.k_begin = 0;
   h.k_end = klen;
-  h.kchunk = tuning.kchunk_h;
+  h.kchunk = ph.kchunk;
   h.lp = lp;
   require(h.rows <= rows_t_, kErrInternal, "H rows exceed padding");
-  run_gemm(false, h);
-  const uint64_t nch_h = (klen + h.kchunk - 1) / h.kchunk;
+  run_gemm(false, ph.shape, ph.nchunks, h);
+  const uint64_t nch_h = ph.nchunks;
   double* target = hmode == 0 ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
   launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream);
   if (nmiss) {
-    MissArgs m{};
+    launch_compact_rows(gwin, nwin, lp, L, cp_, miss_f32_, csh_.get(), stream);
+    MissSpmmArgs m{};
     m.rp = rp_t_.get();
     m.ix = ix_t_.get();
+    m.w = w_t_.get();
     m.nrows = N;
     m.k_lo = j0;
     m.k_hi = j1;
-    m.by_snp = false;
-    m.sc = sc_.get();
-    m.cf = cF_.get();
-    m.Fclamp = emu_k_ ? dF_.get() : nullptr;
-    m.US = us_.get();
-    m.V = vimp_.get();
-    m.kk = emu_k_;
-    m.src = gt_.get();
+    m.src = csh_.get();
+    m.f32 = miss_f32_;
+    m.src_k0 = j0;
+    m.cp = cp_;
+    m.ncols = L;
+    m.kspan = miss_span(N, nwin);
+    m.partial = mpart_.get();
     m.out = target;
     m.lp = lp;
-    launch_missing_correction(m, stream);
+    require(missing_spmm_chunks(m) == 1 || missing_spmm_chunks(m) * N * L <= mpart_.size(), kErrInternal,
+            "missing-entry partials exceed their buffer");
+    launch_missing_spmm(m, stream);
   }
   if (hmode == 0) ++h_version_;
+}
+
+uint64_t Context::miss_span(uint64_t nrows, uint64_t kext) const {
+  // chunks of source rows that fit in L2, as many as the partial buffer holds
+  const uint64_t esz = miss_f32_ ? 4 : 8;
+  uint64_t sp = std::max<uint64_t>(1024, kL2Bytes / ((uint64_t)cp_ * esz));
+  const uint64_t zmax = std::max<uint64_t>(1, mpart_.size() / std::max<uint64_t>(nrows * (uint64_t)L, 1));
+  if ((kext + sp - 1) / sp > zmax) sp = (kext + zmax - 1) / zmax;
+  return std::max(sp, (kext + 65534) / 65535);
+}
+
+void Context::update_missing_weights() {
+  if (!nmiss || !weights_dirty_) return;
+  w_s_.resize(nmiss);
+  w_t_.resize(nmiss);
+  MissWeightArgs a{};
+  a.sc = sc_.get();
+  a.cf = cF_.get();
+  a.Fclamp = emu_k_ ? dF_.get() : nullptr;
+  a.US = us_.get();
+  a.V = vimp_.get();
+  a.kk = emu_k_;
+  // index chunks keep the gathered rows of US (by SNP) or V (by sample) in L2
+  const auto span = [&](uint64_t kext) {
+    if (!emu_k_) return std::max<uint64_t>(kext, 1);
+    uint64_t sp = std::max<uint64_t>(1024, kL2Bytes / ((uint64_t)emu_k_ * 8));
+    return std::max(sp, (kext + 65534) / 65535);
+  };
+  a.rp = rp_s_.get();
+  a.ix = ix_s_.get();
+  a.nrows = M;
+  a.by_snp = true;
+  a.w = w_s_.get();
+  a.kext = N;
+  a.kspan = span(N);
+  launch_missing_weights(a, stream);
+  a.rp = rp_t_.get();
+  a.ix = ix_t_.get();
+  a.nrows = N;
+  a.by_snp = false;
+  a.w = w_t_.get();
+  a.kext = M;
+  a.kspan = span(M);
+  launch_missing_weights(a, stream);
+  weights_dirty_ = false;
 }
 
 void Context::zero_h(int which) {

@@ -30,6 +30,10 @@ namespace synth {
 struct Spec;
 }
 
+// Synthetic genotypes of `spec` straight into interleaved rows (row j = SNP j,
+// `pitch` words per row) on `stream`, byte-identical to the host generator.
+void synth_rows(const synth::Spec& spec, uint32_t* xs, uint64_t pitch, hipStream_t stream);
+
 struct GemmTuning {
   int wm_g = 0;          // wave tile of the G product (0 = default)
   int wm_h = 0;          // wave tile of the H product
@@ -128,6 +132,15 @@ class Context {
   DevBuf<double> dF_, cF_, sc_;
   DevBuf<uint64_t> rp_s_, rp_t_;
   DevBuf<uint32_t> ix_s_, ix_t_;
+  DevBuf<double> w_s_, w_t_;   // per-entry weights of the missing-entry corrections
+  bool weights_dirty_ = true;
+  void update_missing_weights();
+  // compact sources of the sparse corrections (Omega~ rows, G~ window rows)
+  DevBuf<unsigned char> csg_, csh_;
+  DevBuf<double> mpart_;
+  int cp_ = 0;
+  bool miss_f32_ = true;
+  uint64_t miss_span(uint64_t nrows, uint64_t kext) const;
   // EMU state
   int emu_k_ = 0;
   DevBuf<double> us_, vimp_;
@@ -152,7 +165,8 @@ class Context {
   la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
   void set_t(const la::Mat& T);
   void download_rowmajor(const double* src, uint64_t rows, double* out);  // rows x lp -> column-major rows x L  // uploads an L x L matrix into small_ as lp x lp (zero padded)
-  void run_gemm(bool g_product, const GemmArgs& a);
+  void run_gemm(bool g_product, const GemmShape& shape, uint64_t nchunks, const GemmArgs& a);
+  bool fixed_g_ = false, fixed_h_ = false;  // shapes set by the caller (no per-launch change)
 };
 
 }  // namespace pcaone

@@ -58,6 +58,28 @@ struct GemmArgs {
   int lp;              // padded number of columns (multiple of 16, <= 128)
 };
 
+// Grouped (batched) form: many independent products in one persistent launch.
+struct GemmCohort {
+  const uint32_t* X;   // packed rows of this product
+  uint64_t xpitch;
+  const void* Wt;      // operand pieces, transposed
+  uint64_t wpitch, wpiece;
+  void* Y;             // partials [chunk][rows][lp]
+  uint64_t ychunk;
+};
+struct GemmItem {
+  uint32_t cohort;
+  uint32_t chunk;      // index of the partial block written
+  uint64_t row0;       // first row of the tile
+  uint64_t k0, k1;     // code range
+};
+struct GroupedGemmArgs {
+  const GemmItem* items;
+  uint32_t nitems;
+  const GemmCohort* cohorts;
+  const int* active;   // per cohort, null = all active
+};
+
 // Workgroup shape: `waves` waves (4 or 8), each owning `wm` row blocks of 16.
 struct GemmShape {
   int wm = 0;
@@ -67,5 +89,29 @@ inline int gemm_row_tile(const GemmShape& g) { return g.waves * g.wm * 16; }
 GemmShape gemm_default_shape(Form f, int lp);
 bool gemm_shape_supported(Form f, const GemmShape& g, int lp);
 void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream_t s);
+// `grid` workgroups walk the item list (rows of every item are a whole tile of g).
+void launch_grouped_gemm(Form f, const GemmShape& g, int lp, const GroupedGemmArgs& a, unsigned grid,
+                         hipStream_t s);
+// Workgroups of the grouped kernel of this shape resident on one CU.
+int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp);
+
+// Launch plan of one product: shape, padded rows and split-K chunk.
+struct GemmPlan {
+  GemmShape shape;
+  uint64_t rows = 0;    // rows rounded up to the shape's tile
+  uint64_t kchunk = 0;  // codes per chunk
+  uint64_t nchunks = 0;
+  uint64_t workgroups() const { return rows / (uint64_t)gemm_row_tile(shape) * nchunks; }
+};
+// Picks the split-K chunk (and, when the preferred shape leaves the chip
+// under-filled and `fixed_shape` is false, a smaller row tile) for a product
+// of `nrows` rows over `kext` codes: at least 2,048 workgroups where the
+// matrix allows it, at least 4 K-steps per workgroup, chunks of at most
+// `kcap` codes.  kchunk_user > 0 fixes the chunk.
+GemmPlan plan_gemm(Form f, const GemmShape& preferred, bool fixed_shape, int lp, uint64_t nrows, uint64_t kext,
+                   uint64_t kchunk_user, uint64_t kcap);
+// Upper bound of nchunks * rows over every plan_gemm result for these
+// arguments' extremes (partial-product buffer sizing).
+uint64_t plan_partial_rows(uint64_t nrows_max, uint64_t kext_max, uint64_t kchunk_user, uint64_t kcap);
 
 }  // namespace pcaone

@@ -79,26 +79,54 @@ void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp
                        hipStream_t s);
 // Y = X + Z elementwise (n elements).
 void launch_add(const double* X, const double* Z, double* Y, uint64_t n, hipStream_t s);
-// Missing-entry corrections.  For row r (target rows) with list entries k:
-//   out[r][:] += sum_k wgt(r,k) * src[k][:]
-// wgt = -beta_c + (emu ? sc_c * clamp(sum_t US[i][t] V[c][t], -F_c, 1-F_c) : 0)
-// where c is the SNP index (r for by_snp, k otherwise) and i the sample index.
-struct MissArgs {
+// Missing-entry corrections, in two steps.
+// (1) Weights, one per list entry e = (row r, index k), recomputed whenever the
+//     column state or the imputation changes:
+//     w[e] = -sc_c (1/2 - cF_c) + (emu ? sc_c * clamp(sum_t US[i][t] V[c][t], -F_c, 1-F_c) : 0)
+//     where c is the SNP (r for by_snp lists, k otherwise) and i the sample.
+//     The index range [0, kext) is processed in chunks of kspan.
+struct MissWeightArgs {
   const uint64_t* rp;
   const uint32_t* ix;
-  uint64_t nrows;      // target rows
-  uint64_t k_lo, k_hi; // only list entries with k in [k_lo, k_hi)
-  bool by_snp;         // target rows are SNPs (G correction) or samples (H correction)
-  const double* sc;    // per SNP column
-  const double* cf;    // per SNP column
+  uint64_t nrows;
+  bool by_snp;
+  const double* sc;      // per SNP column
+  const double* cf;      // per SNP column
   const double* Fclamp;  // per SNP column (EMU clamp bound), null = no EMU
-  const double* US;    // N x kk row-major (U * S), EMU only
-  const double* V;     // M x kk row-major, EMU only
+  const double* US;      // N x kk row-major (U * S), EMU only
+  const double* V;       // M x kk row-major, EMU only
   int kk;
-  const double* src;   // rows x lp (Omega~ for by_snp, G~ otherwise)
-  double* out;
+  double* w;             // one weight per list entry
+  uint64_t kext, kspan;
+};
+void launch_missing_weights(const MissWeightArgs& a, hipStream_t s);
+// Compact copy of the first L columns of rows x lp FP64 rows into rows x cp
+// (FP32 if f32, else FP64), padding columns zero.
+void launch_compact_rows(const double* src, uint64_t rows, int lp, int L, int cp, bool f32, void* out,
+                         hipStream_t s);
+// (2) out[r][:ncols] += sum_e w[e] * src[ix[e] - src_k0][:ncols] over the
+//     entries of row r with index in [k_lo, k_hi).  src is a compact copy
+//     (pitch cp, a multiple of 4 for FP32 / 2 for FP64).  The index range is
+//     split into chunks of kspan (0 = one chunk); with several chunks the
+//     per-chunk sums go to `partial` (chunks x nrows x ncols FP64) and are
+//     added to out in chunk order.
+struct MissSpmmArgs {
+  const uint64_t* rp;  // row pointers (rows may start at an offset into the lists)
+  const uint32_t* ix;
+  const double* w;
+  uint64_t nrows;
+  uint64_t k_lo, k_hi;
+  const void* src;
+  bool f32;
+  uint64_t src_k0;     // index of the first source row
+  int cp;
+  int ncols;
+  uint64_t kspan;
+  double* partial;
+  double* out;         // nrows x lp
   int lp;
 };
-void launch_missing_correction(const MissArgs& a, hipStream_t s);
+uint64_t missing_spmm_chunks(const MissSpmmArgs& a);
+void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s);
 
 }  // namespace pcaone

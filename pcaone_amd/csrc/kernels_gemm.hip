@@ -14,6 +14,7 @@
 // otherwise (FP64 at large LP; two barriers per step).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "device.hpp"
@@ -164,8 +165,12 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
   }
 }
 
+// One row tile (WAVES x WM x 16 rows) over codes [kb, ke): Y receives the
+// tile's partial products (row-major, LP columns) at row `rowtile`.
 template <Form F, int WM, int NC, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
+__device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64_t xpitch, const void* Wt,
+                                          uint64_t wpitch, uint64_t wpiece, void* Yv, uint64_t rowtile, uint64_t kb,
+                                          uint64_t ke, unsigned char* smem) {
   using T = FT<F>;
   using w_t = typename T::w_t;
   using acc_t = typename T::acc_t;
@@ -179,14 +184,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
   constexpr bool DB = double_buffered<F, NC>();
   constexpr size_t TILE_ELEMS = tile_bytes<F, NC>() / sizeof(w_t);
 
-  extern __shared__ __align__(16) unsigned char smem[];
   w_t* wl0 = reinterpret_cast<w_t*>(smem);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, r16 = lane & 15;
-  const uint64_t rowbase = (uint64_t)blockIdx.x * (WAVES * WM * 16) + (uint64_t)wave * WM * 16;
-  const uint64_t kb = a.k_begin + (uint64_t)blockIdx.y * a.kchunk;
-  const uint64_t ke = (kb + a.kchunk < a.k_end) ? kb + a.kchunk : a.k_end;
+  const uint64_t rowbase = rowtile + (uint64_t)wave * WM * 16;
 
   accv_t<F> acc[WM][NC];
 #pragma unroll
@@ -198,8 +200,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
 
   const uint32_t* xr[WM];
 #pragma unroll
-  for (int rb = 0; rb < WM; ++rb) xr[rb] = a.X + (rowbase + rb * 16 + r16) * a.xpitch + 2 * q;
-  const w_t* wg = static_cast<const w_t*>(a.Wt);
+  for (int rb = 0; rb < WM; ++rb) xr[rb] = X + (rowbase + rb * 16 + r16) * xpitch + 2 * q;
+  const w_t* wg = static_cast<const w_t*>(Wt);
 
   // Thread tid moves W-tile vectors v = tid + NT i.
   static_assert(NV % NT == 0, "W tile must split evenly over the workgroup");
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
 #pragma unroll
     for (int i = 0; i < NVT; ++i) {
       const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-      wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * a.wpiece + col * a.wpitch + vpart * VE + kb);
+      wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kb);
     }
 #pragma unroll
     for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kb >> 4));
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
 #pragma unroll
         for (int i = 0; i < NVT; ++i) {
           const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-          wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * a.wpiece + col * a.wpitch + vpart * VE + kn);
+          wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kn);
         }
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) anext[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kn >> 4));
@@ -252,7 +254,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
 
   // Accumulator layout: lane group q, register r holds row 4q + r of a 16x16
   // block for the FP32-accumulating MFMAs, and row q + 4r for the FP64 one.
-  acc_t* Y = static_cast<acc_t*>(a.Y) + (uint64_t)blockIdx.y * a.ychunk;
+  acc_t* Y = static_cast<acc_t*>(Yv);
 #pragma unroll
   for (int rb = 0; rb < WM; ++rb)
 #pragma unroll
@@ -262,6 +264,53 @@ __global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
         const int row = F == Form::FP64 ? q + 4 * r : 4 * q + r;
         Y[(rowbase + rb * 16 + row) * LP + cb * 16 + r16] = acc[rb][cb][r];
       }
+}
+
+template <Form F, int WM, int NC, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_packed_gemm(GemmArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using acc_t = typename FT<F>::acc_t;
+  const uint64_t kb = a.k_begin + (uint64_t)blockIdx.y * a.kchunk;
+  const uint64_t ke = (kb + a.kchunk < a.k_end) ? kb + a.kchunk : a.k_end;
+  gemm_tile<F, WM, NC, WAVES>(a.X, a.xpitch, a.Wt, a.wpitch, a.wpiece,
+                              static_cast<acc_t*>(a.Y) + (uint64_t)blockIdx.y * a.ychunk,
+                              (uint64_t)blockIdx.x * (WAVES * WM * 16), kb, ke, smem);
+}
+
+// Persistent grouped form: the workgroups walk a list of (cohort, row tile,
+// chunk) items of many independent products; items of inactive cohorts are
+// skipped.  Every workgroup reaches the end of the list.
+template <Form F, int WM, int NC, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_grouped_gemm(GroupedGemmArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using acc_t = typename FT<F>::acc_t;
+  for (uint32_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const GemmItem item = a.items[it];
+    if (a.active && !a.active[item.cohort]) continue;
+    const GemmCohort c = a.cohorts[item.cohort];
+    gemm_tile<F, WM, NC, WAVES>(c.X, c.xpitch, c.Wt, c.wpitch, c.wpiece,
+                                static_cast<acc_t*>(c.Y) + (uint64_t)item.chunk * c.ychunk, item.row0, item.k0,
+                                item.k1, smem);
+  }
+}
+
+// Launches the grouped kernel, or (occ != null) reports how many of its
+// workgroups fit on one CU.
+template <Form F, int WM, int NC, int WAVES>
+void launch_grouped_t(const GroupedGemmArgs& a, unsigned grid, hipStream_t s, int* occ) {
+  constexpr size_t lds = lds_bytes<F, NC>();
+  const void* fn = reinterpret_cast<const void*>(&k_grouped_gemm<F, WM, NC, WAVES>);
+  static bool attr = false;
+  if (!attr) {
+    PCAONE_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  if (occ) {
+    PCAONE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, fn, 64 * WAVES, lds));
+    return;
+  }
+  hipLaunchKernelGGL((k_grouped_gemm<F, WM, NC, WAVES>), dim3(grid), dim3(64 * WAVES), lds, s, a);
+  check_launch("grouped_gemm");
 }
 
 template <Form F, int WM, int NC, int WAVES>
@@ -288,26 +337,57 @@ constexpr bool shape_ok(int wm, int waves) {
   return false;
 }
 
+// Calls fn.template operator()<NC>() for lp = 16 NC when the W tile of NC
+// column blocks splits evenly over 64 * WAVES threads.
+template <Form F, int WAVES, class Fn>
+bool with_nc(int lp, Fn&& fn) {
+  switch (lp / 16) {
+#define PCAONE_NC(n) \
+  case n:            \
+    if constexpr ((FT<F>::P * n * 16 * (kKStep / (16 / sizeof(typename FT<F>::w_t)))) % (64 * WAVES) == 0) { \
+      fn.template operator()<n>();                                                                           \
+      return true;                                                                                            \
+    }                                                                                                         \
+    break;
+    PCAONE_NC(1) PCAONE_NC(2) PCAONE_NC(3) PCAONE_NC(4) PCAONE_NC(5) PCAONE_NC(6) PCAONE_NC(7) PCAONE_NC(8)
+#undef PCAONE_NC
+    default:
+      break;
+  }
+  return false;
+}
+
 template <Form F, int WM, int WAVES>
 void launch_nc(const GemmArgs& a, hipStream_t s) {
   if constexpr (shape_ok<F>(WM, WAVES)) {
-    constexpr int VPR = kKStep / (16 / sizeof(typename FT<F>::w_t));
-    // the W tile of NC column blocks must split evenly over 64 * WAVES threads
-    auto even = [](int nc) { return (FT<F>::P * nc * 16 * VPR) % (64 * WAVES) == 0; };
-    switch (a.lp / 16) {
-#define PCAONE_NC(n) \
-  case n:            \
-    if constexpr ((FT<F>::P * n * 16 * (kKStep / (16 / sizeof(typename FT<F>::w_t)))) % (64 * WAVES) == 0) return launch_t<F, WM, n, WAVES>(a, s); \
-    break;
-      PCAONE_NC(1) PCAONE_NC(2) PCAONE_NC(3) PCAONE_NC(4) PCAONE_NC(5) PCAONE_NC(6) PCAONE_NC(7) PCAONE_NC(8)
-#undef PCAONE_NC
-      default:
-        break;
-    }
-    (void)even;
+    if (with_nc<F, WAVES>(a.lp, [&]<int NC>() { launch_t<F, WM, NC, WAVES>(a, s); })) return;
   }
   fail(kErrArgument, "packed_gemm: unsupported shape (wm " + std::to_string(WM) + ", waves " +
                          std::to_string(WAVES) + ", lp " + std::to_string(a.lp) + ")");
+}
+
+template <Form F, int WM, int WAVES>
+void launch_grouped_nc(int lp, const GroupedGemmArgs& a, unsigned grid, hipStream_t s, int* occ) {
+  if constexpr (shape_ok<F>(WM, WAVES)) {
+    if (with_nc<F, WAVES>(lp, [&]<int NC>() { launch_grouped_t<F, WM, NC, WAVES>(a, grid, s, occ); })) return;
+  }
+  fail(kErrArgument, "grouped_gemm: unsupported shape (wm " + std::to_string(WM) + ", waves " +
+                         std::to_string(WAVES) + ", lp " + std::to_string(lp) + ")");
+}
+
+template <Form F>
+void launch_grouped_shape(const GemmShape& g, int lp, const GroupedGemmArgs& a, unsigned grid, hipStream_t s,
+                          int* occ) {
+  if (g.waves == 8) {
+    if (g.wm == 2) return launch_grouped_nc<F, 2, 8>(lp, a, grid, s, occ);
+    if (g.wm == 4) return launch_grouped_nc<F, 4, 8>(lp, a, grid, s, occ);
+  } else if (g.waves == 4) {
+    if (g.wm == 1) return launch_grouped_nc<F, 1, 4>(lp, a, grid, s, occ);
+    if (g.wm == 2) return launch_grouped_nc<F, 2, 4>(lp, a, grid, s, occ);
+    if (g.wm == 4) return launch_grouped_nc<F, 4, 4>(lp, a, grid, s, occ);
+    if (g.wm == 8) return launch_grouped_nc<F, 8, 4>(lp, a, grid, s, occ);
+  }
+  fail(kErrArgument, "grouped_gemm: unsupported wave tile " + std::to_string(g.wm) + " x " + std::to_string(g.waves));
 }
 
 template <Form F>
@@ -347,6 +427,77 @@ bool gemm_shape_supported(Form f, const GemmShape& g, int lp) {
     case Form::BF16X1: return shape_ok<Form::BF16X1>(g.wm, g.waves);
   }
   return false;
+}
+
+void launch_grouped_gemm(Form f, const GemmShape& g, int lp, const GroupedGemmArgs& a, unsigned grid,
+                         hipStream_t s) {
+  require(lp >= 16 && lp <= 128 && lp % 16 == 0, kErrArgument, "grouped_gemm: bad lp");
+  require(gemm_shape_supported(f, g, lp), kErrArgument, "grouped_gemm: unsupported shape");
+  if (!a.nitems || !grid) return;
+  switch (f) {
+    case Form::FP64: return launch_grouped_shape<Form::FP64>(g, lp, a, grid, s, nullptr);
+    case Form::FP32: return launch_grouped_shape<Form::FP32>(g, lp, a, grid, s, nullptr);
+    case Form::BF16: return launch_grouped_shape<Form::BF16>(g, lp, a, grid, s, nullptr);
+    case Form::BF16X1: return launch_grouped_shape<Form::BF16X1>(g, lp, a, grid, s, nullptr);
+  }
+}
+
+int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
+  require(gemm_shape_supported(f, g, lp), kErrArgument, "grouped_gemm: unsupported shape");
+  int occ = 0;
+  const GroupedGemmArgs none{};
+  switch (f) {
+    case Form::FP64: launch_grouped_shape<Form::FP64>(g, lp, none, 0, nullptr, &occ); break;
+    case Form::FP32: launch_grouped_shape<Form::FP32>(g, lp, none, 0, nullptr, &occ); break;
+    case Form::BF16: launch_grouped_shape<Form::BF16>(g, lp, none, 0, nullptr, &occ); break;
+    case Form::BF16X1: launch_grouped_shape<Form::BF16X1>(g, lp, none, 0, nullptr, &occ); break;
+  }
+  return occ > 0 ? occ : 1;
+}
+
+namespace {
+constexpr uint64_t kTargetWorkgroups = 2048;  // 8 per CU
+constexpr uint64_t kMinStepsPerChunk = 4;
+
+GemmPlan plan_for(const GemmShape& sh, uint64_t nrows, uint64_t kext, uint64_t kchunk_user, uint64_t kcap) {
+  GemmPlan p;
+  p.shape = sh;
+  const uint64_t tile = (uint64_t)gemm_row_tile(sh);
+  p.rows = (nrows + tile - 1) / tile * tile;
+  const uint64_t nrt = p.rows / tile;
+  uint64_t kc = kchunk_user;
+  if (!kc) {
+    uint64_t nch = nrt >= kTargetWorkgroups / 2 ? 1 : (kTargetWorkgroups + nrt - 1) / nrt;
+    nch = std::max<uint64_t>(nch, (kext + kcap - 1) / kcap);
+    nch = std::min<uint64_t>(nch, std::max<uint64_t>(1, kext / (kMinStepsPerChunk * kKStep)));
+    kc = (kext + nch - 1) / nch;
+  }
+  p.kchunk = std::max<uint64_t>(kKStep, (kc + kKStep - 1) / kKStep * kKStep);
+  p.nchunks = std::max<uint64_t>(1, (kext + p.kchunk - 1) / p.kchunk);
+  return p;
+}
+}  // namespace
+
+GemmPlan plan_gemm(Form f, const GemmShape& preferred, bool fixed_shape, int lp, uint64_t nrows, uint64_t kext,
+                   uint64_t kchunk_user, uint64_t kcap) {
+  GemmPlan best = plan_for(preferred, nrows, kext, kchunk_user, kcap);
+  if (fixed_shape || best.workgroups() >= kTargetWorkgroups / 2) return best;
+  static const GemmShape smaller[] = {{4, 8}, {4, 4}, {2, 8}, {2, 4}, {1, 4}};
+  for (const GemmShape& sh : smaller) {
+    if (gemm_row_tile(sh) >= gemm_row_tile(preferred) || !gemm_shape_supported(f, sh, lp)) continue;
+    const GemmPlan p = plan_for(sh, nrows, kext, kchunk_user, kcap);
+    if (p.workgroups() > best.workgroups()) best = p;
+    if (best.workgroups() >= kTargetWorkgroups / 2) break;
+  }
+  return best;
+}
+
+uint64_t plan_partial_rows(uint64_t nrows_max, uint64_t kext_max, uint64_t kchunk_user, uint64_t kcap) {
+  // nchunks <= max(target / row tiles + 1, kext / kcap + 1) when automatic, so
+  // nchunks * rows <= target * 512 + rows or (kext / kcap + 1) * rows.
+  const uint64_t rows = nrows_max + 512;
+  if (kchunk_user) return ((kext_max + kchunk_user - 1) / kchunk_user + 1) * rows;
+  return std::max(kTargetWorkgroups * 512 + rows, ((kext_max + kcap - 1) / kcap + 1) * rows);
 }
 
 void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream_t s) {

@@ -4,6 +4,9 @@
 // missing-entry (and EMU imputation) corrections.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
+#include "bf16.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -19,13 +22,6 @@ inline unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 65535u
 #define GRID_STRIDE(t, n) \
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (n); t += (uint64_t)gridDim.x * blockDim.x)
 
-// Round-to-nearest-even FP32 -> BF16 bits (inputs are finite).
-__device__ __forceinline__ uint16_t bf16_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 template <Form F>
 __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint64_t rows_pad, int lp,
@@ -327,27 +323,36 @@ __global__ void k_add(const double* __restrict__ X, const double* __restrict__ Z
   GRID_STRIDE(t, n) Y[t] = X[t] + Z[t];
 }
 
-// One wave per target row.
-__global__ void k_missing_correction(MissArgs a) {
+// First list position in [lo, hi) whose index is >= key (lists are sorted).
+__device__ __forceinline__ uint64_t lower_bound_ix(const uint32_t* ix, uint64_t lo, uint64_t hi, uint64_t key) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (ix[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Entry range of row r restricted to indices [klo, khi).
+__device__ __forceinline__ void entry_range(const uint64_t* rp, const uint32_t* ix, uint64_t r, uint64_t klo,
+                                            uint64_t khi, uint64_t& e0, uint64_t& e1) {
+  e0 = rp[r];
+  e1 = rp[r + 1];
+  if (klo > 0) e0 = lower_bound_ix(ix, e0, e1, klo);
+  e1 = lower_bound_ix(ix, e0, e1, khi);
+}
+
+// Per-entry weights of the missing-entry corrections.  One wave per row of
+// the lists and index chunk (grid.y), lanes over the row's entries; chunking
+// the index range keeps the gathered rows of US (or V) within L2.
+__global__ void k_missing_weights(MissWeightArgs a) {
   const uint64_t r = blockIdx.x * (uint64_t)(blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (r >= a.nrows) return;
-  uint64_t e = a.rp[r];
-  const uint64_t e1 = a.rp[r + 1];
-  // entries are sorted by k: skip those below k_lo
-  {
-    uint64_t lo = e, hi = e1;
-    while (lo < hi) {
-      const uint64_t mid = (lo + hi) / 2;
-      if (a.ix[mid] < a.k_lo) lo = mid + 1; else hi = mid;
-    }
-    e = lo;
-  }
-  double acc0 = 0.0, acc1 = 0.0;
-  const int l0 = lane, l1 = lane + 64;
-  for (; e < e1; ++e) {
+  const uint64_t klo = (uint64_t)blockIdx.y * a.kspan, khi = klo + a.kspan;
+  uint64_t e0, e1;
+  entry_range(a.rp, a.ix, r, klo, khi, e0, e1);
+  for (uint64_t e = e0 + lane; e < e1; e += 64) {
     const uint64_t k = a.ix[e];
-    if (k >= a.k_hi) break;
     const uint64_t c = a.by_snp ? r : k;  // SNP column
     const uint64_t i = a.by_snp ? k : r;  // sample
     const double s = a.sc[c];
@@ -359,13 +364,94 @@ __global__ void k_missing_correction(MissArgs a) {
       p = fmin(fmax(p, -f), 1.0 - f);
       w += s * p;
     }
-    const double* src = a.src + k * a.lp;
-    if (l0 < a.lp) acc0 += w * src[l0];
-    if (l1 < a.lp) acc1 += w * src[l1];
+    a.w[e] = w;
   }
-  double* o = a.out + r * a.lp;
-  if (l0 < a.lp) o[l0] += acc0;
-  if (l1 < a.lp) o[l1] += acc1;
+}
+
+// out (rows x cp, T) = the first L columns of src (rows x lp, FP64), zero padded.
+template <class T>
+__global__ void k_compact_rows(const double* __restrict__ src, uint64_t rows, int lp, int L, int cp,
+                               T* __restrict__ out) {
+  GRID_STRIDE(t, rows * (uint64_t)cp) {
+    const uint64_t r = t / cp;
+    const int c = (int)(t - r * cp);
+    out[t] = c < L ? (T)src[r * lp + c] : (T)0;
+  }
+}
+
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+
+// Sparse correction, one wave per (row, column chunk of CW = grid.y, index
+// chunk = grid.z): lane l takes the row's entries l, l+64, ... within the
+// index chunk and reads CW consecutive columns of each entry's compact source
+// row (16-byte loads), so 64 gathers are in flight per wave; the lanes' sums
+// are combined by butterfly shuffles.  With several index chunks each writes
+// its own partial row (summed later in chunk order), else it adds to out.
+template <class T, int CW>
+__global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
+  using V = typename Vec16<T>::type;
+  constexpr int NV = Vec16<T>::n;
+  const uint64_t r = blockIdx.x * 4ull + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  const int c0 = (int)blockIdx.y * CW;
+  const int ncol = min(CW, a.ncols - c0);
+  if (r >= a.nrows) return;
+  const uint64_t klo = a.k_lo + (uint64_t)blockIdx.z * a.kspan;
+  const uint64_t khi = min(a.k_hi, klo + a.kspan);
+  uint64_t e0, e1;
+  entry_range(a.rp, a.ix, r, klo, khi, e0, e1);
+  double acc[CW];
+#pragma unroll
+  for (int c = 0; c < CW; ++c) acc[c] = 0.0;
+  const T* src = static_cast<const T*>(a.src);
+#pragma unroll 2
+  for (uint64_t e = e0 + lane; e < e1; e += 64) {
+    const double w = a.w[e];
+    const V* row = reinterpret_cast<const V*>(src + ((uint64_t)a.ix[e] - a.src_k0) * a.cp + c0);
+#pragma unroll
+    for (int p = 0; p < CW / NV; ++p) {
+      if (p * NV < ncol) {
+        const V v = row[p];
+        const T* x = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[p * NV + q] += w * (double)x[q];
+      }
+    }
+  }
+  const bool partial = gridDim.z > 1;
+  double* o = partial ? a.partial + ((uint64_t)blockIdx.z * a.nrows + r) * a.ncols + c0 : a.out + r * a.lp + c0;
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    if (c < ncol) {
+      double v = acc[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) o[c] = partial ? v : o[c] + v;
+    }
+  }
+}
+
+// out[r][l] += sum_z partial[z][r][l] (z in order), l < ncols.
+__global__ void k_sum_chunks(const double* __restrict__ partial, uint64_t nz, uint64_t nrows, int ncols,
+                             double* __restrict__ out, int lp) {
+  GRID_STRIDE(t, nrows * (uint64_t)ncols) {
+    const uint64_t r = t / ncols;
+    const int l = (int)(t - r * ncols);
+    double v = 0.0;
+    for (uint64_t z = 0; z < nz; ++z) v += partial[z * nrows * ncols + t];
+    out[r * lp + l] += v;
+  }
 }
 
 }  // namespace
@@ -570,11 +656,57 @@ void launch_add(const double* X, const double* Z, double* Y, uint64_t n, hipStre
   check_launch("add");
 }
 
-void launch_missing_correction(const MissArgs& a, hipStream_t s) {
+void launch_missing_weights(const MissWeightArgs& a, hipStream_t s) {
   if (!a.nrows) return;
-  require(a.lp <= 128, kErrArgument, "missing_correction: lp > 128");
-  hipLaunchKernelGGL(k_missing_correction, dim3((unsigned)((a.nrows + 3) / 4)), dim3(256), 0, s, a);
-  check_launch("missing_correction");
+  require(a.kspan > 0, kErrArgument, "missing_weights: zero index span");
+  const uint64_t nz = (a.kext + a.kspan - 1) / a.kspan;
+  require(nz < 65536, kErrArgument, "missing_weights: too many index chunks");
+  hipLaunchKernelGGL(k_missing_weights, dim3((unsigned)((a.nrows + 3) / 4), (unsigned)std::max<uint64_t>(nz, 1)),
+                     dim3(256), 0, s, a);
+  check_launch("missing_weights");
+}
+
+void launch_compact_rows(const double* src, uint64_t rows, int lp, int L, int cp, bool f32, void* out,
+                         hipStream_t s) {
+  if (!rows) return;
+  const unsigned g = grid_for(rows * (uint64_t)cp);
+  if (f32)
+    hipLaunchKernelGGL(k_compact_rows<float>, dim3(g), dim3(256), 0, s, src, rows, lp, L, cp,
+                       static_cast<float*>(out));
+  else
+    hipLaunchKernelGGL(k_compact_rows<double>, dim3(g), dim3(256), 0, s, src, rows, lp, L, cp,
+                       static_cast<double*>(out));
+  check_launch("compact_rows");
+}
+
+uint64_t missing_spmm_chunks(const MissSpmmArgs& a) {
+  const uint64_t span = a.k_hi > a.k_lo ? a.k_hi - a.k_lo : 0;
+  return a.kspan && span ? (span + a.kspan - 1) / a.kspan : 1;
+}
+
+void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
+  if (!a.nrows || a.k_hi <= a.k_lo) return;
+  const int vec = a.f32 ? 4 : 2;
+  require(a.ncols >= 1 && a.ncols <= a.lp && a.cp >= a.ncols && a.cp % vec == 0, kErrArgument,
+          "missing_spmm: bad column counts");
+  const uint64_t nz = missing_spmm_chunks(a);
+  require(nz < 65536 && (nz == 1 || a.partial), kErrArgument, "missing_spmm: bad index chunking");
+  const dim3 block(256);
+  const unsigned gx = (unsigned)((a.nrows + 3) / 4), gz = (unsigned)nz;
+  if (a.ncols <= 16) {
+    if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float, 16>), dim3(gx, 1, gz), block, 0, s, a);
+    else hipLaunchKernelGGL((k_missing_spmm<double, 16>), dim3(gx, 1, gz), block, 0, s, a);
+  } else {
+    const unsigned gy = (unsigned)((a.ncols + 31) / 32);
+    if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float, 32>), dim3(gx, gy, gz), block, 0, s, a);
+    else hipLaunchKernelGGL((k_missing_spmm<double, 32>), dim3(gx, gy, gz), block, 0, s, a);
+  }
+  check_launch("missing_spmm");
+  if (nz > 1) {
+    hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(a.nrows * (uint64_t)a.ncols)), dim3(256), 0, s, a.partial, nz,
+                       a.nrows, a.ncols, a.out, a.lp);
+    check_launch("sum_chunks");
+  }
 }
 
 }  // namespace pcaone

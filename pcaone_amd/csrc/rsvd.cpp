@@ -60,7 +60,10 @@ void RsvdOpData::computeUSV(int p, double tol) {
     if (pi > 0) {
       const double diff = r.diff;
       if (info) info->trace.push_back(diff);
-      if (params_.verbose && !params_.missme) std::printf("running of epoch = %d, diff = %f\n", pi, diff);
+      if (params_.verbose && !params_.missme) {
+        std::printf("running of epoch = %d, diff = %f\n", pi, diff);
+        std::fflush(stdout);
+      }
       if (diff < tol || pi == p) {
         if (params_.svd == 2 && std::pow(2, pi) < params_.bands) {
           p = (int)std::log2(params_.bands);
@@ -196,7 +199,10 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info) {
       flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
       const double diff = 1.0 - mev_cols(rsvd->V, Vpre, M, k);
       if (info) info->em_trace.push_back(diff), info->em_iters = (int)i + 1;
-      if (params.verbose) std::printf("individual allele frequencies estimated iter = %u, diff = %f\n", i + 1, diff);
+      if (params.verbose) {
+        std::printf("individual allele frequencies estimated iter = %u, diff = %f\n", i + 1, diff);
+        std::fflush(stdout);
+      }
       if (diff < params.tolem) break;
     }
     if (params.emu) {

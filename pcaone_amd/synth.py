@@ -47,6 +47,20 @@ def write_plink(spec: Spec, prefix: str) -> None:
     _lib.check(_lib.lib().pcaone_synth_write_plink(C.byref(cs), str(prefix).encode()))
 
 
+def cohorts(n: int = 256, seed0: int = 4000, n_range=(500, 20000), m_range=(10000, 200000), pops=(2, 6)):
+    """Batched workload: per cohort c, N and M log-uniform in the ranges and
+    2..6 populations, drawn from PCG64(seed0 + c); genotype seed seed0 + c."""
+    out = []
+    for c in range(n):
+        g = np.random.Generator(np.random.PCG64(seed0 + c))
+        ns = int(round(np.exp(g.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        ms = int(round(np.exp(g.uniform(np.log(m_range[0]), np.log(m_range[1])))))
+        npops = int(g.integers(pops[0], pops[1] + 1))
+        out.append(Spec(seed=seed0 + c, nsamples=ns, nsnps=ms, npops=npops, freq_lo=0.05, freq_hi=0.5,
+                        fst_lo=0.1, fst_hi=0.1))
+    return out
+
+
 # Named workloads (shapes and generator settings of the benchmark configurations).
 def config(cfg: int, **over) -> Spec:
     base = {
