@@ -107,8 +107,17 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
   constexpr int LP = NC * 16;
   constexpr int LDSP = kKStep + T::PAD;
   if constexpr (F == Form::BF16 || F == Form::BF16X1) {
+    // All B fragments of a substep are read before its MFMAs, so the LDS
+    // latency is paid once per substep instead of once per fragment.
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
+      u32x4_t bv[NC * T::P];
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+        for (int p = 0; p < T::P; ++p)
+          bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP + 32 * q + 8 * s);
+      __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
       bf16x8_t af[WM];
 #pragma unroll
       for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(s < 2 ? aw[rb][0] : aw[rb][1], s & 1);
@@ -116,8 +125,7 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
       for (int cb = 0; cb < NC; ++cb) {
 #pragma unroll
         for (int p = 0; p < T::P; ++p) {
-          const u32x4_t bv = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP + 32 * q + 8 * s);
-          const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv);
+          const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]);
 #pragma unroll
           for (int rb = 0; rb < WM; ++rb)
             acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bf, acc[rb][cb], 0, 0, 0);
@@ -139,26 +147,31 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
             av[rb][e] = __hiloint2double((int)top_word(tb, e), 0);
         }
       }
+      // all B values of this u-step are read before its MFMAs (one LDS wait)
+      acc_t bw[NC][4];
 #pragma unroll
       for (int cb = 0; cb < NC; ++cb) {
         const w_t* bp = wl + (cb * 16 + r16) * LDSP + 32 * q + 4 * u;
-        acc_t bw[4];
         if constexpr (F == Form::FP32) {
           const float4 b4 = *reinterpret_cast<const float4*>(bp);
-          bw[0] = b4.x, bw[1] = b4.y, bw[2] = b4.z, bw[3] = b4.w;
+          bw[cb][0] = b4.x, bw[cb][1] = b4.y, bw[cb][2] = b4.z, bw[cb][3] = b4.w;
         } else {
           const double2 b0 = *reinterpret_cast<const double2*>(bp);
           const double2 b1 = *reinterpret_cast<const double2*>(bp + 2);
-          bw[0] = b0.x, bw[1] = b0.y, bw[2] = b1.x, bw[3] = b1.y;
+          bw[cb][0] = b0.x, bw[cb][1] = b0.y, bw[cb][2] = b1.x, bw[cb][3] = b1.y;
         }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
           for (int rb = 0; rb < WM; ++rb) {
             if constexpr (F == Form::FP32)
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[cb][e], acc[rb][cb], 0, 0, 0);
             else
-              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[e], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[cb][e], acc[rb][cb], 0, 0, 0);
           }
       }
     }
