@@ -226,15 +226,19 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 
   if (kb < ke) {
 #pragma unroll
+    for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kb >> 4));
+#pragma unroll
     for (int i = 0; i < NVT; ++i) {
       const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
       wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kb);
     }
 #pragma unroll
-    for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kb >> 4));
-#pragma unroll
     for (int i = 0; i < NVT; ++i)
       *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
+    // Nothing may stay pending into the loop: the wait inserted for a load
+    // still in flight at the loop head would also cover the loop's own
+    // prefetch and expose its latency every step.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int buf = 0;
     for (uint64_t k0 = kb; k0 < ke; k0 += kKStep) {
