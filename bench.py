@@ -32,8 +32,15 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "time-to-k PCs (s) + HBM/MFMA % of roofline, FP8/BF16 vs FP32 at equal error"
-PEAK_TFLOPS = {"bf16": 2500.0, "bf16x1": 2500.0, "fp32": 157.3, "fp64": 78.6}
+PEAK_TFLOPS = {"bf16": 2500.0, "bf16x1": 2500.0, "bf16mix": 2500.0, "fp16": 2500.0, "fp16mix": 2500.0,
+               "fp32": 157.3, "fp64": 78.6}
 PEAK_HBM_GBS = 8000.0
+FORM_TEXT = {
+    "bf16": "BF16 MFMA, exact genotypes, hi/lo BF16 split of Omega and G, FP32 accumulation",
+    "fp16": "FP16 MFMA, exact genotypes, one FP16 piece of Omega and G, FP32 accumulation",
+    "bf16mix": "BF16 MFMA, exact genotypes, one BF16 piece of Omega, hi/lo of G, FP32 accumulation",
+    "fp16mix": "FP16 (G = A Omega) and BF16 hi/lo (H = A^T G) MFMA, exact genotypes, FP32 accumulation",
+}
 
 
 def log(msg: str) -> None:
@@ -239,7 +246,7 @@ def main() -> int:
     ap.add_argument("--wm-h", type=int, default=0, help="GEMM wave tile of the H product (0 = tuned default)")
     ap.add_argument("--waves-g", type=int, default=0, help="waves per workgroup of the G product (0 = default)")
     ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
-    ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo) or bf16mix")
+    ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo), fp16 or bf16mix")
     ap.add_argument("--config", type=int, default=3,
                     help="3: the headline workload; 2 (winSVD), 4 (batched cohorts) or 5 (EMU): a timing/error "
                          "line for that configuration at full size instead")
@@ -343,12 +350,12 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": "fp16" if head == "fp16" else "bf16",
         "data": "synthetic Balding-Nichols genotypes generated in HBM (no dataset download)",
         "config": {"workload": f"precision form: {N} samples x {M} SNPs, 8 populations, --svd 1, k={k}, "
                                f"oversamples {os_}->{L - k}, L={L}, {maxp} power iterations ({maxp + 1} passes)",
                    "nsamples": N, "nsnps": M, "k": k, "L": L, "passes": maxp + 1,
-                   "form": "BF16 MFMA, exact genotypes, hi/lo BF16 split of Omega and G, FP32 accumulation"},
+                   "form": FORM_TEXT.get(head, head)},
         "roofline": {"bound": "mfma", "kernel": f"packed_gemm ({dom})", "achieved": round(achieved, 2),
                      "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
                      "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)",

@@ -40,6 +40,8 @@ extern "C" {
 #define PCAONE_PREC_BF16 2   /* BF16 MFMA, hi/lo split of Omega and G        */
 #define PCAONE_PREC_BF16X1 3 /* BF16 MFMA, single BF16 piece of Omega and G  */
 #define PCAONE_PREC_BF16MIX 4 /* BF16 MFMA, one piece of Omega, hi/lo of G   */
+#define PCAONE_PREC_FP16 5   /* FP16 MFMA, single FP16 piece of Omega and G  */
+#define PCAONE_PREC_FP16MIX 6 /* one FP16 piece of Omega, BF16 hi/lo of G    */
 
 typedef struct pcaone_ctx pcaone_ctx;
 
@@ -223,7 +225,8 @@ int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix);
 /* ---- diagnostics (no reference counterpart) ---------------------------- */
 /* Y = D(X) W with X raw .bed rows (nrows x ceil(ncodes/4) bytes), D mapping
  * codes 00/01/10/11 to 2/0/0/-2, W ncodes x L, Y nrows x L (column-major),
- * computed by the packed MFMA kernel of the given operand form (0..3). */
+ * computed by the packed MFMA kernel of the given precision's form (any
+ * PCAONE_PREC_* but the mixed BF16MIX and FP16MIX). */
 int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
                        const double* W, uint32_t L, double* Y);
 /* RsvdOpData::initOmg's matrix (src/Halko.cpp:15-23): nrows x ncols,

@@ -25,7 +25,11 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{IN
 # Device code: gfx950 only; packed FP32 instructions are disabled.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+# Host code: OpenMP, and AVX2/FMA for the small L x L host steps (the same
+# x86-64-v3 baseline the reference's own build targets); no FMA contraction, so
+# the bit-exact host arithmetic (the Omega draw) stays the library's.
 HOST = ["-fopenmp"]
+HOST_ARCH = ["-Xarch_host", "-march=x86-64-v3", "-Xarch_host", "-ffp-contract=off"]
 
 
 def _sources():
@@ -35,7 +39,7 @@ def _sources():
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
     # hipcc compiles .cpp as HIP too: every source gets the gfx950 device flags.
-    cmd = [HIPCC] + COMMON + DEVICE
+    cmd = [HIPCC] + COMMON + DEVICE + HOST_ARCH
     cmd += ["-x", "hip"] if src.suffix == ".hip" else HOST
     cmd += ["-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,4 +1,4 @@
-// BF16 rounding helpers shared by the operand-split kernels.
+// BF16 / FP16 rounding helpers shared by the operand-split kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -13,5 +13,9 @@ __device__ __forceinline__ uint16_t bf16_bits(float x) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Round-to-nearest-even FP64 -> FP16 bits, and back.
+__device__ __forceinline__ uint16_t fp16_bits(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+__device__ __forceinline__ double fp16_value(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
 
 }  // namespace pcaone

@@ -501,8 +501,9 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
                        const double* W, uint32_t L, double* Y) {
   return guard(c, [&](Context& x) {
     require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
-    require(precision >= 0 && precision <= 3, kErrArgument, "packed_gemm: precision must be 0..3");
-    const Form f = static_cast<Form>(precision);
+    require(precision >= 0 && precision < kNumPrecisions && forms_of(precision).g == forms_of(precision).h,
+            kErrArgument, "packed_gemm: precision must be a single-form precision (0..3 or 5)");
+    const Form f = forms_of(precision).g;
     const int lp = (int)round_up(L, 16);
     const GemmShape shape = gemm_default_shape(f, lp);
     const uint64_t tile = gemm_row_tile(shape);

@@ -659,11 +659,12 @@ void Context::omega_from(int src) {
 
 EpochResult Context::finish_epoch(int k, bool have_prev) {
   require(k >= 1 && k <= L, kErrArgument, "k out of range");
-  // QR of G twice (src/Halko.cpp:55-65): R = R2 R1
-  const la::Mat R1 = la::chol_upper(gram_l(gt_.get(), M));
-  set_t(la::inv_upper(R1));
-  launch_apply(gt_.get(), small_.get(), M, lp, tmp_m_.get(), stream);
-  const la::Mat R2 = la::chol_upper(gram_l(tmp_m_.get(), M));
+  // QR of G twice (src/Halko.cpp:55-65): R = R2 R1, CholeskyQR2 with the
+  // second Gram formed on the L x L side: Q1^T Q1 = R1^-T (G^T G) R1^-1.
+  const la::Mat G1 = gram_l(gt_.get(), M);
+  const la::Mat R1 = la::chol_upper(G1);
+  const la::Mat R1i = la::inv_upper(R1);
+  const la::Mat R2 = la::chol_upper(la::matmul(la::transpose(R1i), la::matmul(G1, R1i)));
   R_ = la::matmul(R2, R1);
   const la::Mat Rinv = la::inv_upper(R_);
   // B B^T = R^-T (H^T H) R^-1; B = R^-T H^T (src/Halko.cpp:68)

@@ -20,11 +20,14 @@ namespace pcaone {
 
 constexpr int kKStep = 128;       // codes consumed per K-step (and row padding unit)
 
-// Operand form of one product.  BF16 = hi/lo pieces, BF16X1 = one piece.
-enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3 };
+// Operand form of one product.  BF16 = hi/lo pieces, BF16X1 = one piece,
+// FP16 = one FP16 piece (11 significand bits).
+enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3, FP16 = 4 };
 
 // User-facing precision (PCAONE_PREC_*) -> forms of the two products
-// (G = A Omega, H = A^T G).  4 = BF16 mixed: one BF16 piece of Omega, hi/lo of G.
+// (G = A Omega, H = A^T G).  4 = BF16 mixed: one BF16 piece of Omega, hi/lo of
+// G; 5 = FP16, one piece of each; 6 = FP16 mixed: one FP16 piece of Omega,
+// BF16 hi/lo of G.
 struct ProductForms {
   Form g, h;
 };
@@ -34,10 +37,12 @@ inline ProductForms forms_of(int precision) {
     case 1: return {Form::FP32, Form::FP32};
     case 2: return {Form::BF16, Form::BF16};
     case 3: return {Form::BF16X1, Form::BF16X1};
-    default: return {Form::BF16X1, Form::BF16};
+    case 4: return {Form::BF16X1, Form::BF16};
+    case 5: return {Form::FP16, Form::FP16};
+    default: return {Form::FP16, Form::BF16};
   }
 }
-constexpr int kNumPrecisions = 5;
+constexpr int kNumPrecisions = 7;
 
 inline int form_pieces(Form f) { return f == Form::BF16 ? 2 : 1; }
 inline size_t form_wbytes(Form f) { return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : 2); }

@@ -33,6 +33,9 @@ __device__ __forceinline__ double split(double x, piece_t<F> (&pc)[2]) {
   } else if constexpr (F == Form::FP32) {
     pc[0] = (float)x;
     return (double)pc[0];
+  } else if constexpr (F == Form::FP16) {
+    pc[0] = fp16_bits(x);
+    return fp16_value(pc[0]);
   } else {
     pc[0] = bf16_bits((float)x);
     double back = (double)bf16_value(pc[0]);
@@ -636,6 +639,7 @@ void launch_batch_make_pieces(const BatchArgs& a, const RowItem* items_n, uint32
     case Form::FP32: hipLaunchKernelGGL(k_b_make_pieces<Form::FP32>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::BF16: hipLaunchKernelGGL(k_b_make_pieces<Form::BF16>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::BF16X1: hipLaunchKernelGGL(k_b_make_pieces<Form::BF16X1>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
+    case Form::FP16: hipLaunchKernelGGL(k_b_make_pieces<Form::FP16>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
   }
   check_launch("batch_make_pieces");
 }
@@ -666,6 +670,7 @@ void launch_batch_epi_g(const BatchArgs& a, const RowItem* items_e, uint32_t nit
     case Form::FP32: hipLaunchKernelGGL((k_b_epi_g<Form::FP32, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
     case Form::BF16: hipLaunchKernelGGL((k_b_epi_g<Form::BF16, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
     case Form::BF16X1: hipLaunchKernelGGL((k_b_epi_g<Form::BF16X1, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
+    case Form::FP16: hipLaunchKernelGGL((k_b_epi_g<Form::FP16, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
   }
   hipLaunchKernelGGL(k_b_sum_items, dim3(nblocks(a.ncoh)), dim3(256), 0, s, a, partial, LP, (int)kListE,
                      (int)kDestBvec, 1);

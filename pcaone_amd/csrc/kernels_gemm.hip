@@ -24,6 +24,7 @@ namespace pcaone {
 namespace {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
@@ -69,6 +70,12 @@ struct FT<Form::BF16X1> {
   static constexpr int P = 1, PAD = 8;
 };
 template <>
+struct FT<Form::FP16> {
+  using w_t = uint16_t;
+  using acc_t = float;
+  static constexpr int P = 1, PAD = 8;
+};
+template <>
 struct FT<Form::FP32> {
   using w_t = float;
   using acc_t = float;
@@ -106,7 +113,7 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
   using acc_t = typename T::acc_t;
   constexpr int LP = NC * 16;
   constexpr int LDSP = kKStep + T::PAD;
-  if constexpr (F == Form::BF16 || F == Form::BF16X1) {
+  if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
     // All B fragments of a substep are read before its MFMAs, so the LDS
     // latency is paid once per substep instead of once per fragment.
 #pragma unroll
@@ -125,10 +132,16 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
       for (int cb = 0; cb < NC; ++cb) {
 #pragma unroll
         for (int p = 0; p < T::P; ++p) {
-          const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]);
 #pragma unroll
-          for (int rb = 0; rb < WM; ++rb)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bf, acc[rb][cb], 0, 0, 0);
+          for (int rb = 0; rb < WM; ++rb) {
+            if constexpr (F == Form::FP16)  // +-2 and 0 have the same top byte in FP16 as in BF16
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, af[rb]),
+                                                                   __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
+                                                                   acc[rb][cb], 0, 0, 0);
+            else
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]),
+                                                                    acc[rb][cb], 0, 0, 0);
+          }
         }
       }
     }
@@ -193,7 +206,8 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   constexpr int VPR = kKStep / VE;
   constexpr int NV = T::P * LP * VPR;           // 16-byte vectors per W tile
   constexpr int NT = 64 * WAVES;                // threads per workgroup
-  constexpr int NVT = NV / NT;                  // per thread
+  constexpr int NVT = (NV + NT - 1) / NT;       // per thread (the last round may be partial)
+  constexpr bool EVEN = NV % NT == 0;
   constexpr bool DB = double_buffered<F, NC>();
   constexpr size_t TILE_ELEMS = tile_bytes<F, NC>() / sizeof(w_t);
 
@@ -217,7 +231,7 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   const w_t* wg = static_cast<const w_t*>(Wt);
 
   // Thread tid moves W-tile vectors v = tid + NT i.
-  static_assert(NV % NT == 0, "W tile must split evenly over the workgroup");
+  static_assert(NT % VPR == 0, "a W tile row must split evenly over the workgroup");
   const int vrow0 = tid / VPR, vpart = tid % VPR;  // NT is a multiple of VPR
   constexpr int ROWS_PER_I = NT / VPR;
 
@@ -230,11 +244,13 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 #pragma unroll
     for (int i = 0; i < NVT; ++i) {
       const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-      wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kb);
+      if (EVEN || row < T::P * LP)
+        wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kb);
     }
 #pragma unroll
     for (int i = 0; i < NVT; ++i)
-      *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
+      if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
+        *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
     // Nothing may stay pending into the loop: the wait inserted for a load
     // still in flight at the loop head would also cover the loop's own
     // prefetch and expose its latency every step.
@@ -248,7 +264,8 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 #pragma unroll
         for (int i = 0; i < NVT; ++i) {
           const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-          wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kn);
+          if (EVEN || row < T::P * LP)
+            wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kn);
         }
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) anext[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kn >> 4));
@@ -260,7 +277,8 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
         if constexpr (!DB) __syncthreads();
 #pragma unroll
         for (int i = 0; i < NVT; ++i)
-          *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
+          if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
+            *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) acur[rb] = anext[rb];
       }
@@ -354,18 +372,14 @@ constexpr bool shape_ok(int wm, int waves) {
   return false;
 }
 
-// Calls fn.template operator()<NC>() for lp = 16 NC when the W tile of NC
-// column blocks splits evenly over 64 * WAVES threads.
+// Calls fn.template operator()<NC>() for lp = 16 NC.
 template <Form F, int WAVES, class Fn>
 bool with_nc(int lp, Fn&& fn) {
   switch (lp / 16) {
 #define PCAONE_NC(n) \
   case n:            \
-    if constexpr ((FT<F>::P * n * 16 * (kKStep / (16 / sizeof(typename FT<F>::w_t)))) % (64 * WAVES) == 0) { \
-      fn.template operator()<n>();                                                                           \
-      return true;                                                                                            \
-    }                                                                                                         \
-    break;
+    fn.template operator()<n>();    \
+    return true;
     PCAONE_NC(1) PCAONE_NC(2) PCAONE_NC(3) PCAONE_NC(4) PCAONE_NC(5) PCAONE_NC(6) PCAONE_NC(7) PCAONE_NC(8)
 #undef PCAONE_NC
     default:
@@ -429,19 +443,17 @@ GemmShape gemm_default_shape(Form f, int lp) {
   if (f == Form::FP64) return {1, 4};
   if (f == Form::FP32) return lp > 96 ? GemmShape{2, 4} : GemmShape{4, 8};
   if (f == Form::BF16) return lp > 80 ? GemmShape{4, 4} : GemmShape{4, 8};
-  return (lp / 16) % 2 == 0 ? GemmShape{4, 8} : GemmShape{lp > 96 ? 4 : 8, 4};
+  return lp > 96 ? GemmShape{4, 4} : GemmShape{4, 8};
 }
 
 bool gemm_shape_supported(Form f, const GemmShape& g, int lp) {
   if (lp < 16 || lp > 128 || lp % 16) return false;
-  const int wb = (int)form_wbytes(f), P = form_pieces(f);
-  const int nv = P * lp * (kKStep / (16 / wb));
-  if (nv % (64 * g.waves)) return false;
   switch (f) {
     case Form::FP64: return shape_ok<Form::FP64>(g.wm, g.waves);
     case Form::FP32: return shape_ok<Form::FP32>(g.wm, g.waves);
     case Form::BF16: return shape_ok<Form::BF16>(g.wm, g.waves);
     case Form::BF16X1: return shape_ok<Form::BF16X1>(g.wm, g.waves);
+    case Form::FP16: return shape_ok<Form::FP16>(g.wm, g.waves);
   }
   return false;
 }
@@ -456,6 +468,7 @@ void launch_grouped_gemm(Form f, const GemmShape& g, int lp, const GroupedGemmAr
     case Form::FP32: return launch_grouped_shape<Form::FP32>(g, lp, a, grid, s, nullptr);
     case Form::BF16: return launch_grouped_shape<Form::BF16>(g, lp, a, grid, s, nullptr);
     case Form::BF16X1: return launch_grouped_shape<Form::BF16X1>(g, lp, a, grid, s, nullptr);
+    case Form::FP16: return launch_grouped_shape<Form::FP16>(g, lp, a, grid, s, nullptr);
   }
 }
 
@@ -468,6 +481,7 @@ int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
     case Form::FP32: launch_grouped_shape<Form::FP32>(g, lp, none, 0, nullptr, &occ); break;
     case Form::BF16: launch_grouped_shape<Form::BF16>(g, lp, none, 0, nullptr, &occ); break;
     case Form::BF16X1: launch_grouped_shape<Form::BF16X1>(g, lp, none, 0, nullptr, &occ); break;
+    case Form::FP16: launch_grouped_shape<Form::FP16>(g, lp, none, 0, nullptr, &occ); break;
   }
   return occ > 0 ? occ : 1;
 }
@@ -532,6 +546,7 @@ void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream
     case Form::FP32: return launch_shape<Form::FP32>(g, a, s);
     case Form::BF16: return launch_shape<Form::BF16>(g, a, s);
     case Form::BF16X1: return launch_shape<Form::BF16X1>(g, a, s);
+    case Form::FP16: return launch_shape<Form::FP16>(g, a, s);
   }
 }
 

@@ -47,6 +47,10 @@ __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint
       const float yf = (float)y;
       static_cast<float*>(Wt)[l * wpitch + r] = yf;
       back = (double)yf;
+    } else if constexpr (F == Form::FP16) {
+      const uint16_t h = fp16_bits(y);
+      static_cast<uint16_t*>(Wt)[l * wpitch + r] = h;
+      back = fp16_value(h);
     } else {
       const uint16_t hi = bf16_bits((float)y);
       uint16_t* w = static_cast<uint16_t*>(Wt);
@@ -124,6 +128,9 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
       } else if constexpr (F == Form::FP32) {
         pc[0] = (float)yv;
         back = (double)pc[0];
+      } else if constexpr (F == Form::FP16) {
+        pc[0] = fp16_bits(yv);
+        back = fp16_value(pc[0]);
       } else {
         pc[0] = bf16_bits((float)yv);
         back = (double)bf16_value(pc[0]);
@@ -477,6 +484,10 @@ void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_p
       hipLaunchKernelGGL(k_make_pieces<Form::BF16X1>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
                          colsign, Wt, wpitch, wpiece, rounded, signed_out);
       break;
+    case Form::FP16:
+      hipLaunchKernelGGL(k_make_pieces<Form::FP16>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out);
+      break;
   }
   check_launch("make_pieces");
 }
@@ -524,6 +535,7 @@ void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride
     case Form::FP32: epi_fused_t<Form::FP32, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
     case Form::BF16: epi_fused_t<Form::BF16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
     case Form::BF16X1: epi_fused_t<Form::BF16X1, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
+    case Form::FP16: epi_fused_t<Form::FP16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
   }
   check_launch("epi_g_fused");
   const uint64_t nb = (nwin + kEpiRows - 1) / kEpiRows;

@@ -39,7 +39,7 @@ def pack(codes: np.ndarray) -> np.ndarray:
 
 
 # ---------------------------------------------------------------- kernel level
-@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16", "bf16x1"])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16", "bf16x1", "fp16"])
 @pytest.mark.parametrize("nrows,ncodes,L", [(1, 1, 1), (37, 127, 20), (300, 129, 80), (513, 1000, 128),
                                             (64, 4096, 16), (257, 333, 40)])
 def test_packed_gemm_exact(ctx, form, nrows, ncodes, L):
@@ -57,7 +57,7 @@ def test_packed_gemm_rounding_fp32(ctx):
     codes = rng.integers(0, 4, (200, 3000)).astype(np.uint8)
     W = rng.standard_normal((3000, 24))
     ref = DMAP[codes] @ W
-    for form, tol in (("fp64", 1e-12), ("fp32", 2e-5), ("bf16", 3e-5), ("bf16x1", 2e-2)):
+    for form, tol in (("fp64", 1e-12), ("fp32", 2e-5), ("bf16", 3e-5), ("bf16x1", 2e-2), ("fp16", 3e-3)):
         Y = ctx.packed_gemm(form, pack(codes), 3000, W)
         err = np.abs(Y - ref).max() / np.abs(ref).max()
         say(f"  {form}: max rel err {err:.2e}")
@@ -165,10 +165,11 @@ def test_golden_fp64(ctx, golden_inputs, case):
         np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=2e-6)
 
 
-REDUCED = [(c, f) for f in ("fp32", "bf16") for c in ("f1_d1", "f1_d2", "f4_d1_miss", "f4_emu_d1", "cfg1_d1", "cfg1_d2")]
+REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16")
+           for c in ("f1_d1", "f1_d2", "f4_d1_miss", "f4_emu_d1", "cfg1_d1", "cfg1_d2")]
 # The mixed form (one BF16 piece of Omega) meets the bar on --svd 1 only; on
 # winSVD the rounded per-window Omega moves the unconverged noise PCs.
-REDUCED += [(c, "bf16mix") for c in ("f1_d1", "f4_d1_miss", "cfg1_d1")]
+REDUCED += [(c, f) for f in ("bf16mix", "fp16mix") for c in ("f1_d1", "f4_d1_miss", "cfg1_d1")]
 
 
 @pytest.mark.parametrize("case,form", REDUCED)
@@ -210,7 +211,8 @@ def test_device_generator_matches_host(ctx, spec_kw):
 
 
 @pytest.mark.parametrize("form,wm,waves", [("bf16", 4, 8), ("bf16", 2, 8), ("bf16", 8, 4), ("fp32", 4, 8),
-                                           ("bf16mix", 4, 8), ("bf16x1", 8, 4)])
+                                           ("bf16mix", 4, 8), ("bf16x1", 8, 4), ("bf16x1", 4, 8), ("fp16", 4, 8),
+                                           ("fp16", 2, 8), ("fp16", 8, 4)])
 def test_workgroup_shapes_agree(ctx, golden_inputs, form, wm, waves):
     """Every compiled workgroup shape gives the same decomposition (cfg1, 8 passes)."""
     say(f"shape form={form} wm={wm} waves={waves}")
