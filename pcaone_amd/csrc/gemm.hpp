@@ -18,7 +18,7 @@
 
 namespace pcaone {
 
-constexpr int kKStep = 128;       // codes consumed per K-step (and row padding unit)
+constexpr int kKStep = 256;       // codes consumed per K-step (and row padding unit)
 
 // Operand form of one product.  BF16 = hi/lo pieces, BF16X1 = one piece,
 // FP16 = one FP16 piece (11 significand bits).

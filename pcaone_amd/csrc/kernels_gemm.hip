@@ -1,12 +1,15 @@
 // Packed-genotype GEMM on CDNA4 matrix cores (see gemm.hpp for the contract).
 //
 // Workgroup = WAVES (4 or 8) waves; wave w owns WM row blocks of 16 rows and all NC column
-// blocks of 16.  Per K-step of 128 codes the workgroup stages the W tile
-// (pieces x LP x 128, transposed) in LDS; every lane loads its 32 codes of
-// each of its rows (8 bytes) straight from HBM and decodes them in registers.
-// Lane group q = lane/16 owns codes [32q, 32q+32) of the step, so one 8-byte
-// load feeds all MFMAs of the step; the B fragment of the same K slots is a
-// contiguous run of the transposed W row.
+// blocks of 16.  Per K-step of KS = 256 codes (128 for FP64) the workgroup
+// stages the W tile (pieces x LP x KS, transposed) in LDS; every lane loads
+// its KS/4 codes of each of its rows (16 bytes) straight from HBM and decodes
+// them in registers.  Lane group q = lane/16 owns codes [q KS/4, (q+1) KS/4)
+// of the step, so one load feeds all MFMAs of the step; the B fragment of the same K
+// slots is a contiguous run of the transposed W row.  W rows are stored
+// unpadded with their 16-byte vectors XOR-swizzled by the row index, so the
+// reads are conflict-free and a double-buffered BF16 hi/lo tile at L = 80
+// fits the 160 KB of LDS.
 //
 // Pipelining: the next step's W tile and genotype words are loaded into
 // registers while the current step computes; the W tile is double-buffered
@@ -61,36 +64,36 @@ template <>
 struct FT<Form::BF16> {
   using w_t = uint16_t;
   using acc_t = float;
-  static constexpr int P = 2, PAD = 8;
+  static constexpr int P = 2, KS = 256;
 };
 template <>
 struct FT<Form::BF16X1> {
   using w_t = uint16_t;
   using acc_t = float;
-  static constexpr int P = 1, PAD = 8;
+  static constexpr int P = 1, KS = 256;
 };
 template <>
 struct FT<Form::FP16> {
   using w_t = uint16_t;
   using acc_t = float;
-  static constexpr int P = 1, PAD = 8;
+  static constexpr int P = 1, KS = 256;
 };
 template <>
 struct FT<Form::FP32> {
   using w_t = float;
   using acc_t = float;
-  static constexpr int P = 1, PAD = 4;
+  static constexpr int P = 1, KS = 256;
 };
 template <>
 struct FT<Form::FP64> {
   using w_t = double;
   using acc_t = double;
-  static constexpr int P = 1, PAD = 2;
+  static constexpr int P = 1, KS = 128;  // a 256-code FP64 tile would not fit in LDS at L = 128
 };
 
 template <Form F, int NC>
 constexpr size_t tile_bytes() {
-  return (size_t)FT<F>::P * NC * 16 * (kKStep + FT<F>::PAD) * sizeof(typename FT<F>::w_t);
+  return (size_t)FT<F>::P * NC * 16 * FT<F>::KS * sizeof(typename FT<F>::w_t);
 }
 template <Form F, int NC>
 constexpr bool double_buffered() {
@@ -104,30 +107,40 @@ constexpr size_t lds_bytes() {
 template <Form F>
 using accv_t = typename std::conditional<F == Form::FP64, f64x4_t, f32x4_t>::type;
 
+// Code words per lane and row block for one K-step (each lane group of 16
+// lanes owns a contiguous quarter of the step).
+template <Form F>
+using aword_t = typename std::conditional<FT<F>::KS == 256, u32x4_t, u32x2_t>::type;
+
+// Position of the 16-byte vector v of LDS W-tile row `row`.  Sixteen
+// consecutive rows read at one v land in sixteen different bank groups.
+__device__ __forceinline__ int swz(int row, int v) { return v ^ (row & 15); }
+
 // All MFMAs of one K-step for one wave.
 template <Form F, int WM, int NC>
-__device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32x2_t (&aw)[WM],
+__device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const aword_t<F> (&aw)[WM],
                                              const typename FT<F>::w_t* wl, int q, int r16) {
   using T = FT<F>;
   using w_t = typename T::w_t;
   using acc_t = typename T::acc_t;
   constexpr int LP = NC * 16;
-  constexpr int LDSP = kKStep + T::PAD;
+  constexpr int LDSP = T::KS;
   if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
     // All B fragments of a substep are read before its MFMAs, so the LDS
     // latency is paid once per substep instead of once per fragment.
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < T::KS / 32; ++s) {
       u32x4_t bv[NC * T::P];
 #pragma unroll
       for (int cb = 0; cb < NC; ++cb)
 #pragma unroll
         for (int p = 0; p < T::P; ++p)
-          bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP + 32 * q + 8 * s);
+          bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
+                                                                 swz(r16, (T::KS / 32) * q + s) * 8);
       __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
       bf16x8_t af[WM];
 #pragma unroll
-      for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(s < 2 ? aw[rb][0] : aw[rb][1], s & 1);
+      for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(aw[rb][s >> 1], s & 1);
 #pragma unroll
       for (int cb = 0; cb < NC; ++cb) {
 #pragma unroll
@@ -147,11 +160,11 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
     }
   } else {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < T::KS / 16; ++u) {
       acc_t av[WM][4];
 #pragma unroll
       for (int rb = 0; rb < WM; ++rb) {
-        const uint32_t tb = top_bytes(spread(u < 4 ? aw[rb][0] : aw[rb][1], u & 3));
+        const uint32_t tb = top_bytes(spread(aw[rb][u >> 2], u & 3));
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           if constexpr (F == Form::FP32)
@@ -164,13 +177,13 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const u32
       acc_t bw[NC][4];
 #pragma unroll
       for (int cb = 0; cb < NC; ++cb) {
-        const w_t* bp = wl + (cb * 16 + r16) * LDSP + 32 * q + 4 * u;
+        const w_t* row = wl + (cb * 16 + r16) * LDSP;
         if constexpr (F == Form::FP32) {
-          const float4 b4 = *reinterpret_cast<const float4*>(bp);
+          const float4 b4 = *reinterpret_cast<const float4*>(row + swz(r16, (T::KS / 16) * q + u) * 4);
           bw[cb][0] = b4.x, bw[cb][1] = b4.y, bw[cb][2] = b4.z, bw[cb][3] = b4.w;
         } else {
-          const double2 b0 = *reinterpret_cast<const double2*>(bp);
-          const double2 b1 = *reinterpret_cast<const double2*>(bp + 2);
+          const double2 b0 = *reinterpret_cast<const double2*>(row + swz(r16, (T::KS / 8) * q + 2 * u) * 2);
+          const double2 b1 = *reinterpret_cast<const double2*>(row + swz(r16, (T::KS / 8) * q + 2 * u + 1) * 2);
           bw[cb][0] = b0.x, bw[cb][1] = b0.y, bw[cb][2] = b1.x, bw[cb][3] = b1.y;
         }
       }
@@ -201,9 +214,9 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   using w_t = typename T::w_t;
   using acc_t = typename T::acc_t;
   constexpr int LP = NC * 16;
-  constexpr int LDSP = kKStep + T::PAD;
+  constexpr int LDSP = T::KS;
   constexpr int VE = 16 / sizeof(w_t);
-  constexpr int VPR = kKStep / VE;
+  constexpr int VPR = T::KS / VE;
   constexpr int NV = T::P * LP * VPR;           // 16-byte vectors per W tile
   constexpr int NT = 64 * WAVES;                // threads per workgroup
   constexpr int NVT = (NV + NT - 1) / NT;       // per thread (the last round may be partial)
@@ -227,7 +240,7 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 
   const uint32_t* xr[WM];
 #pragma unroll
-  for (int rb = 0; rb < WM; ++rb) xr[rb] = X + (rowbase + rb * 16 + r16) * xpitch + 2 * q;
+  for (int rb = 0; rb < WM; ++rb) xr[rb] = X + (rowbase + rb * 16 + r16) * xpitch + (T::KS / 64) * q;
   const w_t* wg = static_cast<const w_t*>(Wt);
 
   // Thread tid moves W-tile vectors v = tid + NT i.
@@ -236,11 +249,11 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   constexpr int ROWS_PER_I = NT / VPR;
 
   u32x4_t wreg[NVT];
-  u32x2_t anext[WM], acur[WM];
+  aword_t<F> anext[WM], acur[WM];
 
   if (kb < ke) {
 #pragma unroll
-    for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kb >> 4));
+    for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const aword_t<F>*>(xr[rb] + (kb >> 4));
 #pragma unroll
     for (int i = 0; i < NVT; ++i) {
       const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
@@ -250,16 +263,17 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 #pragma unroll
     for (int i = 0; i < NVT; ++i)
       if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
-        *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
+        *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + swz(vrow0 + ROWS_PER_I * i, vpart) * VE) =
+            wreg[i];
     // Nothing may stay pending into the loop: the wait inserted for a load
     // still in flight at the loop head would also cover the loop's own
     // prefetch and expose its latency every step.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int buf = 0;
-    for (uint64_t k0 = kb; k0 < ke; k0 += kKStep) {
-      const bool more = k0 + kKStep < ke;
-      const uint64_t kn = k0 + kKStep;
+    for (uint64_t k0 = kb; k0 < ke; k0 += T::KS) {
+      const bool more = k0 + T::KS < ke;
+      const uint64_t kn = k0 + T::KS;
       if (more) {
 #pragma unroll
         for (int i = 0; i < NVT; ++i) {
@@ -268,7 +282,7 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
             wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kn);
         }
 #pragma unroll
-        for (int rb = 0; rb < WM; ++rb) anext[rb] = *reinterpret_cast<const u32x2_t*>(xr[rb] + (kn >> 4));
+        for (int rb = 0; rb < WM; ++rb) anext[rb] = *reinterpret_cast<const aword_t<F>*>(xr[rb] + (kn >> 4));
       }
       w_t* cur = wl0 + (DB ? buf * TILE_ELEMS : 0);
       compute_step<F, WM, NC>(acc, acur, cur, q, r16);
@@ -278,7 +292,8 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 #pragma unroll
         for (int i = 0; i < NVT; ++i)
           if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
-            *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP + vpart * VE) = wreg[i];
+            *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP + swz(vrow0 + ROWS_PER_I * i, vpart) * VE) =
+                wreg[i];
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) acur[rb] = anext[rb];
       }
