@@ -179,6 +179,7 @@ void Context::finalize() {
 
 void Context::upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc) {
   weights_dirty_ = true;
+  cols_pristine_ = false;
   host_cf_ = cf;
   host_sc_ = sc;
   // padded to rows_s_ so window tiles may read past M
@@ -317,6 +318,7 @@ void Context::clear_impute() {
 
 void Context::reset_columns() {
   require(!F.empty(), kErrState, "no data");
+  if (cols_pristine_ && !emu_k_) return;  // nothing changed since the last reset
   clear_impute();
   bool identity = true;
   for (uint64_t i = 0; i < M && identity; ++i) identity = perm[i] == (int)i;
@@ -326,6 +328,7 @@ void Context::reset_columns() {
     permute_columns(inv);
   }
   upload_col_state(F, std::vector<double>(M, 1.0));
+  cols_pristine_ = true;
 }
 
 // ------------------------------------------------------------------ rsvd
@@ -411,12 +414,37 @@ la::Mat Context::gram_l(const double* X, uint64_t rows, const double* Y) {
   return G;
 }
 
+void Context::stage_in(const double* host, uint64_t count, double* dev) {
+  // host -> pinned (parallel copy) -> device
+  pinned_.resize(count * 8);
+  unsigned char* st = pinned_.get();
+  const int64_t pieces = 16;
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < pieces; ++q) {
+    const uint64_t b0 = count * q / pieces, b1 = count * (q + 1) / pieces;
+    std::memcpy(st + b0 * 8, host + b0, (b1 - b0) * 8);
+  }
+  PCAONE_HIP(hipMemcpyAsync(dev, st, count * 8, hipMemcpyHostToDevice, stream));
+}
+
+void Context::stage_out(const double* dev, uint64_t count, double* host) {
+  pinned_.resize(count * 8);
+  unsigned char* st = pinned_.get();
+  PCAONE_HIP(hipMemcpyAsync(st, dev, count * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  const int64_t pieces = 16;
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < pieces; ++q) {
+    const uint64_t b0 = count * q / pieces, b1 = count * (q + 1) / pieces;
+    std::memcpy(host + b0, st + b0 * 8, (b1 - b0) * 8);
+  }
+}
+
 void Context::set_omega(const double* om) {
   require(L > 0, kErrState, "configure() first");
-  std::vector<double> h(ks_ * (uint64_t)lp, 0.0);
-  for (uint64_t i = 0; i < N; ++i)
-    for (int l = 0; l < L; ++l) h[i * lp + l] = om[(uint64_t)l * N + i];
-  PCAONE_HIP(hipMemcpyAsync(omega_c_.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
+  // column-major N x L as given, laid out row-major (ks_ x lp) on the device
+  stage_in(om, (uint64_t)N * L, tmp_n_.get());
+  launch_colmajor_to_rows(tmp_n_.get(), N, L, lp, ks_, omega_c_.get(), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
@@ -713,11 +741,8 @@ void Context::save_prev() {
 }
 
 void Context::download_u(int k, double* U) {
-  std::vector<double> h((uint64_t)N * lp);
-  PCAONE_HIP(hipMemcpyAsync(h.data(), ucur_.get(), h.size() * 8, hipMemcpyDeviceToHost, stream));
-  PCAONE_HIP(hipStreamSynchronize(stream));
-  for (uint64_t i = 0; i < N; ++i)
-    for (int j = 0; j < k; ++j) U[(uint64_t)j * N + i] = h[i * lp + j];
+  launch_rows_to_colmajor(ucur_.get(), N, lp, k, tmp_n_.get(), stream);
+  stage_out(tmp_n_.get(), (uint64_t)N * k, U);
 }
 
 void Context::download_rowmajor(const double* src, uint64_t rows, double* out) {

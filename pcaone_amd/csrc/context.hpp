@@ -164,7 +164,11 @@ class Context {
   std::vector<double> host_cf_, host_sc_;
   la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
   void set_t(const la::Mat& T);
-  void download_rowmajor(const double* src, uint64_t rows, double* out);  // rows x lp -> column-major rows x L  // uploads an L x L matrix into small_ as lp x lp (zero padded)
+  void download_rowmajor(const double* src, uint64_t rows, double* out);  // rows x lp -> column-major rows x L
+  // host <-> device copies of count doubles through the pinned staging buffer
+  void stage_in(const double* host, uint64_t count, double* dev);
+  void stage_out(const double* dev, uint64_t count, double* host);
+  bool cols_pristine_ = false;  // column state equals the as-read state (identity order, cF = F, sc = 1)  // uploads an L x L matrix into small_ as lp x lp (zero padded)
   void run_gemm(bool g_product, const GemmShape& shape, uint64_t nchunks, const GemmArgs& a);
   bool fixed_g_ = false, fixed_h_ = false;  // shapes set by the caller (no per-launch change)
 };

@@ -77,6 +77,11 @@ void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, doub
 // flipOmg norms: a[l] = sum_i |O2 - Q|, b[l] = sum_i |O2 + Q|.
 void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
                        hipStream_t s);
+// Column-major n x L (leading dimension n) -> row-major rows_pad x lp (zero padded).
+void launch_colmajor_to_rows(const double* src, uint64_t n, int L, int lp, uint64_t rows_pad, double* dst,
+                             hipStream_t s);
+// Row-major n x lp -> column-major n x k.
+void launch_rows_to_colmajor(const double* src, uint64_t n, int lp, int k, double* dst, hipStream_t s);
 // Y = X + Z elementwise (n elements).
 void launch_add(const double* X, const double* Z, double* Y, uint64_t n, hipStream_t s);
 // Missing-entry corrections, in two steps.

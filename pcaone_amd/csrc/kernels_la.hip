@@ -326,6 +326,25 @@ __global__ void k_flip_partial(const double* __restrict__ O2, const double* __re
   }
 }
 
+// Column-major n x L (leading dimension n) -> row-major rows_pad x lp, zero
+// outside the n x L block.
+__global__ void k_colmajor_to_rows(const double* __restrict__ src, uint64_t n, int L, int lp, uint64_t rows_pad,
+                                   double* __restrict__ dst) {
+  GRID_STRIDE(t, rows_pad * (uint64_t)lp) {
+    const uint64_t r = t / lp;
+    const int l = (int)(t - r * lp);
+    dst[t] = (r < n && l < L) ? src[(uint64_t)l * n + r] : 0.0;
+  }
+}
+
+// Row-major n x lp -> column-major n x k.
+__global__ void k_rows_to_colmajor(const double* __restrict__ src, uint64_t n, int lp, int k, double* __restrict__ dst) {
+  GRID_STRIDE(t, n * (uint64_t)k) {
+    const uint64_t j = t / n, r = t - j * n;
+    dst[t] = src[r * lp + j];
+  }
+}
+
 __global__ void k_add(const double* __restrict__ X, const double* __restrict__ Z, double* __restrict__ Y, uint64_t n) {
   GRID_STRIDE(t, n) Y[t] = X[t] + Z[t];
 }
@@ -462,6 +481,17 @@ __global__ void k_sum_chunks(const double* __restrict__ partial, uint64_t nz, ui
 }
 
 }  // namespace
+
+void launch_colmajor_to_rows(const double* src, uint64_t n, int L, int lp, uint64_t rows_pad, double* dst,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_colmajor_to_rows, dim3(grid_for(rows_pad * lp)), dim3(256), 0, s, src, n, L, lp, rows_pad, dst);
+  check_launch("colmajor_to_rows");
+}
+
+void launch_rows_to_colmajor(const double* src, uint64_t n, int lp, int k, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_rows_to_colmajor, dim3(grid_for(n * k)), dim3(256), 0, s, src, n, lp, k, dst);
+  check_launch("rows_to_colmajor");
+}
 
 void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
