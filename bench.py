@@ -33,14 +33,17 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "time-to-k PCs (s) + HBM/MFMA % of roofline, FP8/BF16 vs FP32 at equal error"
 PEAK_TFLOPS = {"bf16": 2500.0, "bf16x1": 2500.0, "bf16mix": 2500.0, "fp16": 2500.0, "fp16mix": 2500.0,
-               "fp32": 157.3, "fp64": 78.6}
+               "int8": 5000.0, "fp32": 157.3, "fp64": 78.6}  # dense MFMA (int8: TOP/s)
 PEAK_HBM_GBS = 8000.0
 FORM_TEXT = {
     "bf16": "BF16 MFMA, exact genotypes, hi/lo BF16 split of Omega and G, FP32 accumulation",
     "fp16": "FP16 MFMA, exact genotypes, one FP16 piece of Omega and G, FP32 accumulation",
     "bf16mix": "BF16 MFMA, exact genotypes, one BF16 piece of Omega, hi/lo of G, FP32 accumulation",
     "fp16mix": "FP16 (G = A Omega) and BF16 hi/lo (H = A^T G) MFMA, exact genotypes, FP32 accumulation",
+    "int8": "int8 MFMA, exact genotypes (v/2), Omega and G as per-column 16-bit fixed point split into two int8 "
+            "pieces, exact int32 accumulation",
 }
+DTYPE = {"bf16": "bf16", "bf16x1": "bf16", "bf16mix": "bf16", "fp16": "fp16", "fp16mix": "fp16", "int8": "int8"}
 
 
 def log(msg: str) -> None:
@@ -246,7 +249,7 @@ def main() -> int:
     ap.add_argument("--wm-h", type=int, default=0, help="GEMM wave tile of the H product (0 = tuned default)")
     ap.add_argument("--waves-g", type=int, default=0, help="waves per workgroup of the G product (0 = default)")
     ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
-    ap.add_argument("--head", default="bf16", help="headline form: bf16 (hi/lo), fp16 or bf16mix")
+    ap.add_argument("--head", default="bf16", choices=sorted(DTYPE), help="headline form (default: bf16 hi/lo)")
     ap.add_argument("--config", type=int, default=3,
                     help="3: the headline workload; 2 (winSVD), 4 (batched cohorts) or 5 (EMU): a timing/error "
                          "line for that configuration at full size instead")
@@ -318,6 +321,10 @@ def main() -> int:
     if only is None or "fp32" in only:
         r_fp32, ts_fp32 = timed("fp32", 1, 1 if args.warmup else 0)
         results["fp32"] = r_fp32
+    if only is None and head != "int8":
+        # the int8 form, reported beside the headline with its errors
+        r_i8, ts_i8 = timed("int8", 1, 1 if args.warmup else 0)
+        results["int8"] = r_i8
     if (only is None and not args.no_fp64) or (only and "fp64" in only):
         r64, ts64 = timed("fp64", 1, 0)
         results["fp64"] = r64
@@ -350,14 +357,15 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16" if head == "fp16" else "bf16",
+        "dtype": DTYPE[head],
         "data": "synthetic Balding-Nichols genotypes generated in HBM (no dataset download)",
         "config": {"workload": f"precision form: {N} samples x {M} SNPs, 8 populations, --svd 1, k={k}, "
                                f"oversamples {os_}->{L - k}, L={L}, {maxp} power iterations ({maxp + 1} passes)",
                    "nsamples": N, "nsnps": M, "k": k, "L": L, "passes": maxp + 1,
                    "form": FORM_TEXT.get(head, head)},
         "roofline": {"bound": "mfma", "kernel": f"packed_gemm ({dom})", "achieved": round(achieved, 2),
-                     "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
+                     "peak": PEAK_TFLOPS[head], "unit": "TOP/s" if head == "int8" else "TFLOP/s",
+                     "frac": round(achieved / PEAK_TFLOPS[head], 4),
                      "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
                      "avg_launch_ms": round(dom_ms, 3),
                      "algorithmic_flop_per_launch": flops_launch,
@@ -366,7 +374,11 @@ def main() -> int:
     }
     if "fp32" in results:
         t32 = float(np.mean(ts_fp32))
-        out["fp32_form"] = {"seconds": round(t32, 4), "speedup_bf16_over_fp32": round(t32 / value, 2)}
+        out["fp32_form"] = {"seconds": round(t32, 4), f"speedup_{head}_over_fp32": round(t32 / value, 2)}
+        if "int8" in results and head != "int8":
+            out["int8_form"] = {"seconds": round(float(np.mean(ts_i8)), 4),
+                                "speedup_int8_over_fp32": round(t32 / float(np.mean(ts_i8)), 2),
+                                "form": FORM_TEXT["int8"]}
     if "fp64" in results:
         ref = results["fp64"]
         out["fp64_form_seconds"] = round(ts64[0], 3)

@@ -34,7 +34,8 @@ extern "C" {
 
 /* Arithmetic form of the genotype products G = A*Omega and H = A^T*G.
  * Genotypes enter every form exactly; Omega and G are rounded to the form's
- * operand type (split into BF16 hi/lo pieces for PCAONE_PREC_BF16). */
+ * operand type (split into BF16 hi/lo pieces for PCAONE_PREC_BF16, into two
+ * int8 fixed-point pieces for PCAONE_PREC_INT8). */
 #define PCAONE_PREC_FP64 0   /* FP64 MFMA, FP64 accumulation                 */
 #define PCAONE_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation (plain form)    */
 #define PCAONE_PREC_BF16 2   /* BF16 MFMA, hi/lo split of Omega and G        */
@@ -42,6 +43,11 @@ extern "C" {
 #define PCAONE_PREC_BF16MIX 4 /* BF16 MFMA, one piece of Omega, hi/lo of G   */
 #define PCAONE_PREC_FP16 5   /* FP16 MFMA, single FP16 piece of Omega and G  */
 #define PCAONE_PREC_FP16MIX 6 /* one FP16 piece of Omega, BF16 hi/lo of G    */
+/* int8 MFMA: genotypes as v/2 in {+1, 0, -1}; each column of Omega and of
+ * diag(sc) G as a 16-bit fixed-point value (column scale = max |value| /
+ * 32639) split into two int8 pieces; exact int32 accumulation.  Single-matrix
+ * path only (not the batched form). */
+#define PCAONE_PREC_INT8 7
 
 typedef struct pcaone_ctx pcaone_ctx;
 
@@ -226,7 +232,8 @@ int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix);
 /* Y = D(X) W with X raw .bed rows (nrows x ceil(ncodes/4) bytes), D mapping
  * codes 00/01/10/11 to 2/0/0/-2, W ncodes x L, Y nrows x L (column-major),
  * computed by the packed MFMA kernel of the given precision's form (any
- * PCAONE_PREC_* but the mixed BF16MIX and FP16MIX). */
+ * PCAONE_PREC_* but the mixed BF16MIX and FP16MIX; INT8 takes at most 65536
+ * codes). */
 int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
                        const double* W, uint32_t L, double* Y);
 /* RsvdOpData::initOmg's matrix (src/Halko.cpp:15-23): nrows x ncols,

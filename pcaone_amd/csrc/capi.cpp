@@ -502,8 +502,10 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
   return guard(c, [&](Context& x) {
     require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
     require(precision >= 0 && precision < kNumPrecisions && forms_of(precision).g == forms_of(precision).h,
-            kErrArgument, "packed_gemm: precision must be a single-form precision (0..3 or 5)");
+            kErrArgument, "packed_gemm: precision must be a single-form precision (0..3, 5 or 7)");
     const Form f = forms_of(precision).g;
+    require(f != Form::I8 || round_up(ncodes, kKStep) <= kI8ChunkCap, kErrArgument,
+            "packed_gemm: the int8 form takes at most 65536 codes per call");
     const int lp = (int)round_up(L, 16);
     const GemmShape shape = gemm_default_shape(f, lp);
     const uint64_t tile = gemm_row_tile(shape);
@@ -521,8 +523,11 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
     PCAONE_HIP(hipMemcpyAsync(dw.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, x.stream));
     const int P = form_pieces(f);
     DevBuf<unsigned char> wt((uint64_t)P * lp * K * form_wbytes(f));
+    DevBuf<double> i8s(2 * 128);
+    DevBuf<unsigned long long> i8max(128);
+    if (f == Form::I8) launch_i8_scales(dw.get(), ncodes, lp, nullptr, i8max.get(), i8s.get(), i8s.get() + 128, x.stream);
     launch_make_pieces(f, dw.get(), ncodes, K, lp, nullptr, nullptr, wt.get(), K, (uint64_t)lp * K, nullptr,
-                       nullptr, x.stream);
+                       nullptr, x.stream, i8s.get());
     DevBuf<unsigned char> out(rows_pad * lp * form_accbytes(f));
     GemmArgs a{};
     a.X = X.get();
@@ -539,13 +544,22 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
     a.lp = lp;
     launch_packed_gemm(f, shape, a, x.stream);
     std::vector<unsigned char> h(out.bytes());
+    std::vector<double> mul(128, 1.0);
     PCAONE_HIP(hipMemcpyAsync(h.data(), out.get(), h.size(), hipMemcpyDeviceToHost, x.stream));
+    if (f == Form::I8)
+      PCAONE_HIP(hipMemcpyAsync(mul.data(), i8s.get() + 128, 128 * 8, hipMemcpyDeviceToHost, x.stream));
     PCAONE_HIP(hipStreamSynchronize(x.stream));
     for (uint64_t r = 0; r < nrows; ++r)
       for (uint32_t l = 0; l < L; ++l) {
         const uint64_t idx = r * lp + l;
-        Y[(uint64_t)l * nrows + r] =
-            f == Form::FP64 ? reinterpret_cast<const double*>(h.data())[idx] : reinterpret_cast<const float*>(h.data())[idx];
+        double v;
+        if (f == Form::FP64)
+          v = reinterpret_cast<const double*>(h.data())[idx];
+        else if (f == Form::I8)  // int32 sum of (v/2) q, times 2 s_l
+          v = (double)reinterpret_cast<const int32_t*>(h.data())[idx] * mul[l];
+        else
+          v = reinterpret_cast<const float*>(h.data())[idx];
+        Y[(uint64_t)l * nrows + r] = v;
       }
   });
 }

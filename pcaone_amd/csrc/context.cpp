@@ -349,6 +349,10 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   omega_c_.resize(nrows_n * lp);
   colsum_.resize(lp);
   bvec_.resize(lp);
+  if (form_g == Form::I8 || form_h == Form::I8) {
+    i8s_.resize(4 * 128);
+    i8max_.resize(128);
+  }
   wg_pitch_ = ks_;
   wg_.resize((uint64_t)form_pieces(form_g) * lp * wg_pitch_ * bytes_of(form_g));
   PCAONE_HIP(hipMemsetAsync(wg_.get(), 0, wg_.bytes(), stream));
@@ -387,8 +391,10 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   fixed_h_ = tuning.wm_h || tuning.waves_h;
   tuning.kchunk_g = round_up(tuning.kchunk_g, kKStep);  // 0 stays 0 (planned per launch)
   tuning.kchunk_h = round_up(tuning.kchunk_h, kKStep);
-  const uint64_t pg = plan_partial_rows(rows_s_, ks_, tuning.kchunk_g, ks_) * lp * form_accbytes(form_g);
-  const uint64_t ph = plan_partial_rows(rows_t_, wh_pitch_, tuning.kchunk_h, kChunkCapH) * lp * form_accbytes(form_h);
+  const uint64_t pg =
+      plan_partial_rows(rows_s_, ks_, tuning.kchunk_g, form_kcap(form_g, ks_)) * lp * form_accbytes(form_g);
+  const uint64_t ph = plan_partial_rows(rows_t_, wh_pitch_, tuning.kchunk_h, form_kcap(form_h, kChunkCapH)) * lp *
+                      form_accbytes(form_h);
   part_.resize(std::max(pg, ph));
   h_version_ = 0;
   gram_version_ = ~0ull;
@@ -445,8 +451,9 @@ void Context::set_omega(const double* om) {
   // column-major N x L as given, laid out row-major (ks_ x lp) on the device
   stage_in(om, (uint64_t)N * L, tmp_n_.get());
   launch_colmajor_to_rows(tmp_n_.get(), N, L, lp, ks_, omega_c_.get(), stream);
+  if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
-                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
   PCAONE_HIP(hipStreamSynchronize(stream));
 }
@@ -481,7 +488,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   const int P = form_pieces(form_h);
 
   // (1) G[j0:j1] = A Omega : SNP-major rows, K = samples
-  const GemmPlan pg = plan_gemm(form_g, shape_g, fixed_g_, lp, nwin, ks_, tuning.kchunk_g, ks_);
+  const GemmPlan pg = plan_gemm(form_g, shape_g, fixed_g_, lp, nwin, ks_, tuning.kchunk_g, form_kcap(form_g, ks_));
   GemmArgs g{};
   g.X = xs_.get() + j0 * pitch_s_;
   g.xpitch = pitch_s_;
@@ -507,13 +514,15 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   if (klen > off + nwin)
     PCAONE_HIP(hipMemset2DAsync(wh_.get() + (off + nwin) * wb, wh_pitch_ * wb, 0, (klen - off - nwin) * wb,
                                 (size_t)P * lp, stream));
-  if (!nmiss) {
+  if (!nmiss && form_g != Form::I8 && form_h != Form::I8) {
     launch_epi_g_fused(form_h, part_.get(), nch_g, g.ychunk, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                        colsum_.get(), gwin, wh_.get() + off * wb, wh_pitch_, (uint64_t)lp * wh_pitch_, red_.get(),
                        bvec_.get(), stream);
   } else {
     launch_epi_g(form_g, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0,
-                 colsum_.get(), gwin, stream);
+                 colsum_.get(), gwin, stream, i8_mul(0));
+  }
+  if (nmiss) {
     update_missing_weights();
     launch_compact_rows(omega_r_.get(), N, lp, L, cp_, miss_f32_, csg_.get(), stream);
     MissSpmmArgs m{};
@@ -535,13 +544,17 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     require(missing_spmm_chunks(m) == 1 || missing_spmm_chunks(m) * nwin * L <= mpart_.size(), kErrInternal,
             "missing-entry partials exceed their buffer");
     launch_missing_spmm(m, stream);
+  }
+  if (nmiss || form_g == Form::I8 || form_h == Form::I8) {
+    if (form_h == Form::I8)
+      launch_i8_scales(gwin, nwin, lp, sc_.get() + j0, i8max_.get(), i8_scale(1), i8_mul(1), stream);
     launch_make_pieces(form_h, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
-                       (uint64_t)lp * wh_pitch_, gwin, nullptr, stream);
+                       (uint64_t)lp * wh_pitch_, gwin, nullptr, stream, i8_scale(1));
     launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
   }
 
   // (3) H contribution: sample-major rows, K = the window's SNPs
-  const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h, kChunkCapH);
+  const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h, form_kcap(form_h, kChunkCapH));
   GemmArgs h{};
   h.X = xt_.get() + (j0 - off) / 16;
   h.xpitch = pitch_t_;
@@ -559,7 +572,8 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   run_gemm(false, ph.shape, ph.nchunks, h);
   const uint64_t nch_h = ph.nchunks;
   double* target = hmode == 0 ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
-  launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream);
+  launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream,
+               i8_mul(1));
   if (nmiss) {
     launch_compact_rows(gwin, nwin, lp, L, cp_, miss_f32_, csh_.get(), stream);
     MissSpmmArgs m{};
@@ -679,8 +693,9 @@ void Context::omega_from(int src) {
     if (ab[l] > 2.0 * ab[lp + l]) sign[l] = -1.0;
   double* dsign = small_.get() + 128 * 128;
   PCAONE_HIP(hipMemcpyAsync(dsign, sign.data(), lp * 8, hipMemcpyHostToDevice, stream));
+  if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
-                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream);
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
   PCAONE_HIP(hipStreamSynchronize(stream));
 }

@@ -146,6 +146,11 @@ class Context {
   DevBuf<double> us_, vimp_;
   // rsvd workspace
   DevBuf<double> omega_, omega_r_, omega_c_, colsum_, bvec_;
+  // I8 form: column scales and epilogue factors of Omega (G product) and of
+  // diag(sc) G~ (H product), [scale_g | mul_g | scale_h | mul_h] x 128, and
+  // the column-maximum scratch.
+  DevBuf<double> i8s_;
+  DevBuf<unsigned long long> i8max_;
   DevBuf<unsigned char> wg_, wh_;
   uint64_t wg_pitch_ = 0, wh_pitch_ = 0;
   DevBuf<double> gt_, H_, H1_, H2_, tmp_n_, tmp_m_, ucur_, upre_, vbuf_;
@@ -163,6 +168,10 @@ class Context {
   void upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc);
   std::vector<double> host_cf_, host_sc_;
   la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
+  // I8 column scales / epilogue factors of the G (h = 0) or H (h = 1)
+  // product's operand; null when that product is not in the I8 form.
+  double* i8_scale(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h : nullptr; }
+  double* i8_mul(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h + 128 : nullptr; }
   void set_t(const la::Mat& T);
   void download_rowmajor(const double* src, uint64_t rows, double* out);  // rows x lp -> column-major rows x L
   // host <-> device copies of count doubles through the pinned staging buffer

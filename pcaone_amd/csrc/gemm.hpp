@@ -21,13 +21,22 @@ namespace pcaone {
 constexpr int kKStep = 256;       // codes consumed per K-step (and row padding unit)
 
 // Operand form of one product.  BF16 = hi/lo pieces, BF16X1 = one piece,
-// FP16 = one FP16 piece (11 significand bits).
-enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3, FP16 = 4 };
+// FP16 = one FP16 piece (11 significand bits).  I8 = 16-bit fixed point per
+// column, q = rint(w / s_col) with |q| <= kI8Max, split into two int8 pieces
+// q = 256 q_hi + q_lo on the int8 matrix cores; the genotype operand is
+// v/2 in {+1, 0, -1} and the products accumulate exactly in int32.
+enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3, FP16 = 4, I8 = 5 };
+
+// Largest |q| of the I8 form: q_hi = (q + 128) >> 8 stays within int8.
+constexpr int kI8Max = 32639;
+// K codes per I8 chunk: |256 sum a q_hi + sum a q_lo| <= K (127 * 256 + 128)
+// must stay below 2^31.
+constexpr uint64_t kI8ChunkCap = 65536;
 
 // User-facing precision (PCAONE_PREC_*) -> forms of the two products
 // (G = A Omega, H = A^T G).  4 = BF16 mixed: one BF16 piece of Omega, hi/lo of
 // G; 5 = FP16, one piece of each; 6 = FP16 mixed: one FP16 piece of Omega,
-// BF16 hi/lo of G.
+// BF16 hi/lo of G; 7 = I8 for both.
 struct ProductForms {
   Form g, h;
 };
@@ -39,14 +48,20 @@ inline ProductForms forms_of(int precision) {
     case 3: return {Form::BF16X1, Form::BF16X1};
     case 4: return {Form::BF16X1, Form::BF16};
     case 5: return {Form::FP16, Form::FP16};
-    default: return {Form::FP16, Form::BF16};
+    case 6: return {Form::FP16, Form::BF16};
+    default: return {Form::I8, Form::I8};
   }
 }
-constexpr int kNumPrecisions = 7;
+constexpr int kNumPrecisions = 8;
 
-inline int form_pieces(Form f) { return f == Form::BF16 ? 2 : 1; }
-inline size_t form_wbytes(Form f) { return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : 2); }
+inline int form_pieces(Form f) { return f == Form::BF16 || f == Form::I8 ? 2 : 1; }
+inline size_t form_wbytes(Form f) {
+  return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : (f == Form::I8 ? 1 : 2));
+}
+// Partial products: FP64, int32 (I8) or FP32.
 inline size_t form_accbytes(Form f) { return f == Form::FP64 ? 8 : 4; }
+// Largest split-K chunk a form accepts.
+inline uint64_t form_kcap(Form f, uint64_t cap) { return f == Form::I8 && cap > kI8ChunkCap ? kI8ChunkCap : cap; }
 
 struct GemmArgs {
   const uint32_t* X;   // interleaved packed rows
@@ -55,7 +70,7 @@ struct GemmArgs {
   const void* Wt;      // pieces of W, transposed
   uint64_t wpitch;     // elements between consecutive cols of one piece
   uint64_t wpiece;     // elements between pieces
-  void* Y;             // partials [chunk][rows][lp] (float, or double for FP64)
+  void* Y;             // partials [chunk][rows][lp] (float; double for FP64, int32 for I8)
   uint64_t ychunk;     // elements between chunks of Y
   uint64_t k_begin;    // first code (multiple of kKStep)
   uint64_t k_end;      // one past the last code (multiple of kKStep)

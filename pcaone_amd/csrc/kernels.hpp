@@ -46,14 +46,23 @@ void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uin
 // Wt[p][l][r] for r < rows_pad (rows >= nreal written as 0).  Row r is first
 // multiplied by rowscale[r] (null = 1) and column l by colsign[l] (null = 1).
 // rounded (nullable, row-major rows x lp) receives sum(pieces) / rowscale.
+// The I8 form takes its column scales from colscale (launch_i8_scales).
 void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
-                        double* signed_out, hipStream_t s);
+                        double* signed_out, hipStream_t s, const double* colscale = nullptr);
+// I8 column scales of X (rows [0, nreal), each row times rowscale[r], null =
+// 1): scale[l] = max_r |X[r][l] rowscale[r]| / kI8Max (1 if the column is
+// zero), mul[l] = 2 scale[l].  maxbits: lp words of scratch.
+void launch_i8_scales(const double* X, uint64_t nreal, int lp, const double* rowscale, unsigned long long* maxbits,
+                      double* scale, double* mul, hipStream_t s);
 // G epilogue: out[j][l] = alpha_j * sum_c P[c][j][l] + beta_j * colsum[l]  (j < nreal; 0 above)
-// with alpha_j = sc_j / 4, beta_j = sc_j * (1/2 - cf_j).
+// with alpha_j = sc_j / 4, beta_j = sc_j * (1/2 - cf_j).  colmul (required for
+// I8, whose int32 partials are sum (v/2) q): per-column factor of the sum.
 void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
-                  int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s);
-// G epilogue fused with the operand split, for data without missing entries:
+                  int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s,
+                  const double* colmul = nullptr);
+// G epilogue fused with the operand split, for data without missing entries
+// (not for I8, whose column scales need the whole window first):
 // gout (nwin x lp) = G~ window, Wt pieces of diag(sc) G~ (window offset
 // already applied), bvec = sum_j sc_j (1/2 - cF_j) G~_j via bpart (scratch of
 // epi_g_fused_partial_elems).
@@ -63,7 +72,7 @@ void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride
                         uint64_t wpitch, uint64_t wpiece, double* bpart, double* bvec, hipStream_t s);
 // H epilogue: t = sum_c P[c][i][l] / 4 + b[l]; mode 0: out = t, 1: out += t (i < nreal; rows above = 0 in mode 0).
 void launch_epi_h(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
-                  int lp, const double* b, double* out, int mode, hipStream_t s);
+                  int lp, const double* b, double* out, int mode, hipStream_t s, const double* colmul = nullptr);
 // Column sums out[l] = sum_{r in [r0,r1)} w_r * X[r][l] (w null = 1), two-stage, deterministic.
 void launch_colsum(const double* X, uint64_t r0, uint64_t r1, int lp, const double* w_sc, const double* w_cf,
                    double* partial, double* out, hipStream_t s);

@@ -30,6 +30,7 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 
@@ -38,8 +39,19 @@ constexpr size_t kLdsMax = 160 * 1024;
 // code -> top byte of v in BF16/FP32/FP64: 00 -> +2 (0x40), 01 -> 0, 10 -> 0, 11 -> -2 (0xC0)
 constexpr uint32_t kLutTopByte = 0xC0000040u;
 
+// code -> int8 v/2: 00 -> +1, 01 -> 0, 10 -> 0, 11 -> -1
+constexpr uint32_t kLutI8 = 0xFF000001u;
+
 __device__ __forceinline__ uint32_t spread(uint32_t w, int t) { return (w >> (2 * t)) & 0x03030303u; }
 __device__ __forceinline__ uint32_t top_bytes(uint32_t codes4) { return __builtin_amdgcn_perm(0u, kLutTopByte, codes4); }
+
+// Sixteen int8 values v/2 (codes 0..15 of word w, in order).
+__device__ __forceinline__ i32x4_t decode_i8x16(uint32_t w) {
+  i32x4_t d;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) d[t] = (int)__builtin_amdgcn_perm(0u, kLutI8, spread(w, t));
+  return d;
+}
 
 // Eight BF16 values (codes 8*half .. 8*half+7 of word w).
 __device__ __forceinline__ bf16x8_t decode_bf16x8(uint32_t w, int half) {
@@ -60,35 +72,43 @@ __device__ __forceinline__ uint32_t top_word(uint32_t tb, int e) {
 
 template <Form F>
 struct FT;
+// P: operand pieces; KS: codes per K-step; NACC: accumulator sets (the I8
+// pieces carry different weights, 256 and 1, so each has its own).
 template <>
 struct FT<Form::BF16> {
   using w_t = uint16_t;
   using acc_t = float;
-  static constexpr int P = 2, KS = 256;
+  static constexpr int P = 2, KS = 256, NACC = 1;
 };
 template <>
 struct FT<Form::BF16X1> {
   using w_t = uint16_t;
   using acc_t = float;
-  static constexpr int P = 1, KS = 256;
+  static constexpr int P = 1, KS = 256, NACC = 1;
 };
 template <>
 struct FT<Form::FP16> {
   using w_t = uint16_t;
   using acc_t = float;
-  static constexpr int P = 1, KS = 256;
+  static constexpr int P = 1, KS = 256, NACC = 1;
+};
+template <>
+struct FT<Form::I8> {
+  using w_t = uint8_t;
+  using acc_t = int32_t;
+  static constexpr int P = 2, KS = 256, NACC = 2;
 };
 template <>
 struct FT<Form::FP32> {
   using w_t = float;
   using acc_t = float;
-  static constexpr int P = 1, KS = 256;
+  static constexpr int P = 1, KS = 256, NACC = 1;
 };
 template <>
 struct FT<Form::FP64> {
   using w_t = double;
   using acc_t = double;
-  static constexpr int P = 1, KS = 128;  // a 256-code FP64 tile would not fit in LDS at L = 128
+  static constexpr int P = 1, KS = 128, NACC = 1;  // a 256-code FP64 tile would not fit in LDS at L = 128
 };
 
 template <Form F, int NC>
@@ -105,7 +125,8 @@ constexpr size_t lds_bytes() {
 }
 
 template <Form F>
-using accv_t = typename std::conditional<F == Form::FP64, f64x4_t, f32x4_t>::type;
+using accv_t = typename std::conditional<F == Form::FP64, f64x4_t,
+                                         typename std::conditional<F == Form::I8, i32x4_t, f32x4_t>::type>::type;
 
 // Code words per lane and row block for one K-step (each lane group of 16
 // lanes owns a contiguous quarter of the step).
@@ -118,14 +139,39 @@ __device__ __forceinline__ int swz(int row, int v) { return v ^ (row & 15); }
 
 // All MFMAs of one K-step for one wave.
 template <Form F, int WM, int NC>
-__device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const aword_t<F> (&aw)[WM],
+__device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][NC], const aword_t<F> (&aw)[WM],
                                              const typename FT<F>::w_t* wl, int q, int r16) {
   using T = FT<F>;
   using w_t = typename T::w_t;
   using acc_t = typename T::acc_t;
   constexpr int LP = NC * 16;
   constexpr int LDSP = T::KS;
-  if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
+  if constexpr (F == Form::I8) {
+    // 16x16x64 int8 MFMAs: lane group q's word s holds the 16 codes of
+    // substep s, and B vector 4q + s of each W row the same 16 K slots.
+#pragma unroll
+    for (int s = 0; s < T::KS / 64; ++s) {
+      u32x4_t bv[NC * T::P];
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+        for (int p = 0; p < T::P; ++p)
+          bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
+                                                                 swz(r16, (T::KS / 64) * q + s) * 16);
+      __builtin_amdgcn_sched_barrier(0);
+      i32x4_t af[WM];
+#pragma unroll
+      for (int rb = 0; rb < WM; ++rb) af[rb] = decode_i8x16(aw[rb][s]);
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+        for (int p = 0; p < T::P; ++p)
+#pragma unroll
+          for (int rb = 0; rb < WM; ++rb)
+            acc[p][rb][cb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                af[rb], __builtin_bit_cast(i32x4_t, bv[cb * T::P + p]), acc[p][rb][cb], 0, 0, 0);
+    }
+  } else if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
     // All B fragments of a substep are read before its MFMAs, so the LDS
     // latency is paid once per substep instead of once per fragment.
 #pragma unroll
@@ -148,12 +194,12 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const awo
 #pragma unroll
           for (int rb = 0; rb < WM; ++rb) {
             if constexpr (F == Form::FP16)  // +-2 and 0 have the same top byte in FP16 as in BF16
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, af[rb]),
-                                                                   __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
-                                                                   acc[rb][cb], 0, 0, 0);
+              acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, af[rb]),
+                                                                      __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
+                                                                      acc[0][rb][cb], 0, 0, 0);
             else
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]),
-                                                                    acc[rb][cb], 0, 0, 0);
+              acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  af[rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]), acc[0][rb][cb], 0, 0, 0);
           }
         }
       }
@@ -195,9 +241,9 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[WM][NC], const awo
 #pragma unroll
           for (int rb = 0; rb < WM; ++rb) {
             if constexpr (F == Form::FP32)
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[cb][e], acc[rb][cb], 0, 0, 0);
+              acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb][e], bw[cb][e], acc[0][rb][cb], 0, 0, 0);
             else
-              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[cb][e], acc[rb][cb], 0, 0, 0);
+              acc[0][rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][e], bw[cb][e], acc[0][rb][cb], 0, 0, 0);
           }
       }
     }
@@ -230,13 +276,15 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   const int q = lane >> 4, r16 = lane & 15;
   const uint64_t rowbase = rowtile + (uint64_t)wave * WM * 16;
 
-  accv_t<F> acc[WM][NC];
+  accv_t<F> acc[T::NACC][WM][NC];
 #pragma unroll
-  for (int rb = 0; rb < WM; ++rb)
+  for (int a = 0; a < T::NACC; ++a)
 #pragma unroll
-    for (int cb = 0; cb < NC; ++cb)
+    for (int rb = 0; rb < WM; ++rb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[rb][cb][r] = 0;
+      for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a][rb][cb][r] = 0;
 
   const uint32_t* xr[WM];
 #pragma unroll
@@ -303,7 +351,9 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   }
 
   // Accumulator layout: lane group q, register r holds row 4q + r of a 16x16
-  // block for the FP32-accumulating MFMAs, and row q + 4r for the FP64 one.
+  // block for the FP32- and int32-accumulating MFMAs, and row q + 4r for the
+  // FP64 one.  I8: the chunk's product is 256 (A q_hi) + A q_lo, exact in
+  // int32 for chunks of at most kI8ChunkCap codes.
   acc_t* Y = static_cast<acc_t*>(Yv);
 #pragma unroll
   for (int rb = 0; rb < WM; ++rb)
@@ -312,7 +362,12 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = F == Form::FP64 ? q + 4 * r : 4 * q + r;
-        Y[(rowbase + rb * 16 + row) * LP + cb * 16 + r16] = acc[rb][cb][r];
+        acc_t v;
+        if constexpr (F == Form::I8)
+          v = (int32_t)((uint32_t)acc[0][rb][cb][r] << 8) + acc[1][rb][cb][r];
+        else
+          v = acc[0][rb][cb][r];
+        Y[(rowbase + rb * 16 + row) * LP + cb * 16 + r16] = v;
       }
 }
 
@@ -458,6 +513,7 @@ GemmShape gemm_default_shape(Form f, int lp) {
   if (f == Form::FP64) return {1, 4};
   if (f == Form::FP32) return lp > 96 ? GemmShape{2, 4} : GemmShape{4, 8};
   if (f == Form::BF16) return lp > 80 ? GemmShape{4, 4} : GemmShape{4, 8};
+  if (f == Form::I8) return lp > 64 ? GemmShape{2, 8} : GemmShape{4, 8};  // two accumulator sets
   return lp > 96 ? GemmShape{4, 4} : GemmShape{4, 8};
 }
 
@@ -469,6 +525,7 @@ bool gemm_shape_supported(Form f, const GemmShape& g, int lp) {
     case Form::BF16: return shape_ok<Form::BF16>(g.wm, g.waves);
     case Form::BF16X1: return shape_ok<Form::BF16X1>(g.wm, g.waves);
     case Form::FP16: return shape_ok<Form::FP16>(g.wm, g.waves);
+    case Form::I8: return shape_ok<Form::I8>(g.wm, g.waves);
   }
   return false;
 }
@@ -484,6 +541,7 @@ void launch_grouped_gemm(Form f, const GemmShape& g, int lp, const GroupedGemmAr
     case Form::BF16: return launch_grouped_shape<Form::BF16>(g, lp, a, grid, s, nullptr);
     case Form::BF16X1: return launch_grouped_shape<Form::BF16X1>(g, lp, a, grid, s, nullptr);
     case Form::FP16: return launch_grouped_shape<Form::FP16>(g, lp, a, grid, s, nullptr);
+    case Form::I8: return launch_grouped_shape<Form::I8>(g, lp, a, grid, s, nullptr);
   }
 }
 
@@ -497,6 +555,7 @@ int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
     case Form::BF16: launch_grouped_shape<Form::BF16>(g, lp, none, 0, nullptr, &occ); break;
     case Form::BF16X1: launch_grouped_shape<Form::BF16X1>(g, lp, none, 0, nullptr, &occ); break;
     case Form::FP16: launch_grouped_shape<Form::FP16>(g, lp, none, 0, nullptr, &occ); break;
+    case Form::I8: launch_grouped_shape<Form::I8>(g, lp, none, 0, nullptr, &occ); break;
   }
   return occ > 0 ? occ : 1;
 }
@@ -549,12 +608,13 @@ uint64_t plan_partial_rows(uint64_t nrows_max, uint64_t kext_max, uint64_t kchun
 void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream_t s) {
   require(a.lp >= 16 && a.lp <= 128 && a.lp % 16 == 0, kErrArgument, "packed_gemm: bad lp");
   require(a.kchunk > 0 && a.kchunk % kKStep == 0 && a.k_begin % kKStep == 0 && a.k_end % kKStep == 0,
-          kErrArgument, "packed_gemm: K range must be multiples of 128");
+          kErrArgument, "packed_gemm: K range must be multiples of 256");
   require(gemm_shape_supported(f, g, a.lp), kErrArgument, "packed_gemm: unsupported shape");
   require(a.rows % (uint64_t)gemm_row_tile(g) == 0, kErrArgument, "packed_gemm: rows not a tile multiple");
   require(a.xpitch % 8 == 0 && a.xpitch * 16 >= a.k_end, kErrArgument, "packed_gemm: X pitch too small");
-  require(a.wpitch % 8 == 0 && a.wpitch >= a.k_end, kErrArgument, "packed_gemm: W pitch too small");
-  require(a.wpiece % 8 == 0, kErrArgument, "packed_gemm: W piece stride not 16-byte aligned");
+  require(a.wpitch % 16 == 0 && a.wpitch >= a.k_end, kErrArgument, "packed_gemm: W pitch too small");
+  require(a.wpiece % 16 == 0, kErrArgument, "packed_gemm: W piece stride not 16-byte aligned");
+  require(f != Form::I8 || a.kchunk <= kI8ChunkCap, kErrArgument, "packed_gemm: I8 chunks are limited to 65536 codes");
   if (a.rows == 0 || a.k_end <= a.k_begin) return;
   switch (f) {
     case Form::FP64: return launch_shape<Form::FP64>(g, a, s);
@@ -562,6 +622,7 @@ void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream
     case Form::BF16: return launch_shape<Form::BF16>(g, a, s);
     case Form::BF16X1: return launch_shape<Form::BF16X1>(g, a, s);
     case Form::FP16: return launch_shape<Form::FP16>(g, a, s);
+    case Form::I8: return launch_shape<Form::I8>(g, a, s);
   }
 }
 

@@ -27,7 +27,7 @@ template <Form F>
 __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint64_t rows_pad, int lp,
                               const double* __restrict__ rowscale, const double* __restrict__ colsign, void* Wt,
                               uint64_t wpitch, uint64_t wpiece, double* __restrict__ rounded,
-                              double* __restrict__ signed_out) {
+                              double* __restrict__ signed_out, const double* __restrict__ colscale) {
   const uint64_t n = rows_pad * (uint64_t)lp;
   GRID_STRIDE(t, n) {
     const uint64_t l = t / rows_pad, r = t - l * rows_pad;
@@ -51,6 +51,15 @@ __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint
       const uint16_t h = fp16_bits(y);
       static_cast<uint16_t*>(Wt)[l * wpitch + r] = h;
       back = fp16_value(h);
+    } else if constexpr (F == Form::I8) {
+      // q = rint(y / s_l) within +-kI8Max; q = 256 q_hi + q_lo, q_lo in [-128, 127]
+      const double sl = colscale[l];
+      const int q = (int)fmin(fmax(rint(y / sl), -(double)kI8Max), (double)kI8Max);
+      const int hi = (q + 128) >> 8, lo = q - 256 * hi;
+      uint8_t* w = static_cast<uint8_t*>(Wt);
+      w[l * wpitch + r] = (uint8_t)(int8_t)hi;
+      w[wpiece + l * wpitch + r] = (uint8_t)(int8_t)lo;
+      back = (double)q * sl;
     } else {
       const uint16_t hi = bf16_bits((float)y);
       uint16_t* w = static_cast<uint16_t*>(Wt);
@@ -66,10 +75,12 @@ __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint
   }
 }
 
+// colmul (nullable): per-column factor of the summed partials (the I8
+// form's int32 partials are sum (v/2) q, so the factor is 2 s_l).
 template <class P>
 __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t stride, uint64_t nreal,
                         uint64_t rows_pad, int lp, const double* __restrict__ sc, const double* __restrict__ cf,
-                        const double* __restrict__ colsum, double* __restrict__ out) {
+                        const double* __restrict__ colsum, double* __restrict__ out, const double* __restrict__ colmul) {
   const uint64_t n = rows_pad * (uint64_t)lp;
   GRID_STRIDE(t, n) {
     const uint64_t j = t / lp;
@@ -78,6 +89,7 @@ __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t s
     if (j < nreal) {
       double y = 0.0;
       for (uint64_t c = 0; c < nchunks; ++c) y += (double)part[c * stride + t];
+      if (colmul) y *= colmul[l];
       const double s = sc[j];
       v = 0.25 * s * y + s * (0.5 - cf[j]) * colsum[l];
     }
@@ -161,7 +173,8 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
 
 template <class P>
 __global__ void k_epi_h(const P* __restrict__ part, uint64_t nchunks, uint64_t stride, uint64_t nreal,
-                        uint64_t rows_pad, int lp, const double* __restrict__ b, double* __restrict__ out, int mode) {
+                        uint64_t rows_pad, int lp, const double* __restrict__ b, double* __restrict__ out, int mode,
+                        const double* __restrict__ colmul) {
   const uint64_t n = rows_pad * (uint64_t)lp;
   GRID_STRIDE(t, n) {
     const uint64_t i = t / lp;
@@ -169,12 +182,53 @@ __global__ void k_epi_h(const P* __restrict__ part, uint64_t nchunks, uint64_t s
     if (i < nreal) {
       double y = 0.0;
       for (uint64_t c = 0; c < nchunks; ++c) y += (double)part[c * stride + t];
+      if (colmul) y *= colmul[l];
       const double v = 0.25 * y + b[l];
       out[t] = mode ? out[t] + v : v;
     } else if (!mode) {
       out[t] = 0.0;
     }
   }
+}
+
+// Column maxima of |X[r][l] rowscale[r]| over rows [0, nreal): each block
+// reduces its rows in LDS and raises maxbits[l] (the bit pattern of a
+// non-negative double orders like the value) with one atomic per column.
+__global__ void __launch_bounds__(256) k_colmax(const double* __restrict__ X, uint64_t nreal, int lp,
+                                                const double* __restrict__ rowscale,
+                                                unsigned long long* __restrict__ maxbits) {
+  __shared__ double m[256];
+  const int per = 256 / lp;  // rows per pass
+  const int t = threadIdx.x, g = t / lp, l = t - g * lp;
+  double mx = 0.0;
+  if (g < per) {
+    for (uint64_t r = (uint64_t)blockIdx.x * per + g; r < nreal; r += (uint64_t)gridDim.x * per) {
+      const double rs = rowscale ? rowscale[r] : 1.0;
+      mx = fmax(mx, fabs(X[r * lp + l] * rs));
+    }
+  }
+  m[t] = mx;
+  __syncthreads();
+  if (t < lp) {
+    for (int q = 1; q < per; ++q) mx = fmax(mx, m[q * lp + t]);
+    if (mx > 0.0) atomicMax(&maxbits[t], (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// I8 column scales s_l = max_l / kI8Max (1 for an all-zero column) and the
+// epilogue factors 2 s_l.
+__global__ void k_i8_scales(const unsigned long long* __restrict__ maxbits, int lp, double* __restrict__ scale,
+                            double* __restrict__ mul) {
+  const int l = threadIdx.x;
+  if (l >= lp) return;
+  const double mx = __longlong_as_double((long long)maxbits[l]);
+  const double s = mx > 0.0 ? mx / (double)kI8Max : 1.0;
+  scale[l] = s;
+  mul[l] = 2.0 * s;
 }
 
 // Stage 1 of a column reduction: block b sums rows [r0 + b*rpb, ...) for each column.
@@ -495,43 +549,63 @@ void launch_rows_to_colmajor(const double* src, uint64_t n, int lp, int k, doubl
 
 void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
-                        double* signed_out, hipStream_t s) {
+                        double* signed_out, hipStream_t s, const double* colscale) {
   const unsigned g = grid_for(rows_pad * lp);
+  require(f != Form::I8 || colscale, kErrInternal, "make_pieces: the I8 form needs column scales");
   switch (f) {
     case Form::FP64:
       hipLaunchKernelGGL(k_make_pieces<Form::FP64>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out);
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
       break;
     case Form::FP32:
       hipLaunchKernelGGL(k_make_pieces<Form::FP32>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out);
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
       break;
     case Form::BF16:
       hipLaunchKernelGGL(k_make_pieces<Form::BF16>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out);
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
       break;
     case Form::BF16X1:
       hipLaunchKernelGGL(k_make_pieces<Form::BF16X1>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out);
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
       break;
     case Form::FP16:
       hipLaunchKernelGGL(k_make_pieces<Form::FP16>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out);
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
+      break;
+    case Form::I8:
+      hipLaunchKernelGGL(k_make_pieces<Form::I8>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
       break;
   }
   check_launch("make_pieces");
 }
 
 void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
-                  int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s) {
+                  int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s,
+                  const double* colmul) {
   const unsigned g = grid_for(rows_pad * lp);
+  require(f != Form::I8 || colmul, kErrInternal, "epi_g: the I8 form needs column factors");
   if (f == Form::FP64)
     hipLaunchKernelGGL(k_epi_g<double>, dim3(g), dim3(256), 0, s, static_cast<const double*>(P), nchunks,
-                       chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out);
+                       chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
+  else if (f == Form::I8)
+    hipLaunchKernelGGL(k_epi_g<int32_t>, dim3(g), dim3(256), 0, s, static_cast<const int32_t*>(P), nchunks,
+                       chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
   else
     hipLaunchKernelGGL(k_epi_g<float>, dim3(g), dim3(256), 0, s, static_cast<const float*>(P), nchunks, chunk_stride,
-                       nreal, rows_pad, lp, sc, cf, colsum, out);
+                       nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
   check_launch("epi_g");
+}
+
+void launch_i8_scales(const double* X, uint64_t nreal, int lp, const double* rowscale, unsigned long long* maxbits,
+                      double* scale, double* mul, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, maxbits, lp);
+  const uint64_t per = 256 / lp;
+  hipLaunchKernelGGL(k_colmax, dim3(grid_for(nreal, (unsigned)per, 2048)), dim3(256), 0, s, X, nreal, lp, rowscale,
+                     maxbits);
+  hipLaunchKernelGGL(k_i8_scales, dim3(1), dim3(128), 0, s, maxbits, lp, scale, mul);
+  check_launch("i8_scales");
 }
 
 uint64_t epi_g_fused_partial_elems(uint64_t nwin, int lp) { return (nwin + kEpiRows - 1) / kEpiRows * lp; }
@@ -566,6 +640,7 @@ void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride
     case Form::BF16: epi_fused_t<Form::BF16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
     case Form::BF16X1: epi_fused_t<Form::BF16X1, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
     case Form::FP16: epi_fused_t<Form::FP16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
+    case Form::I8: fail(kErrInternal, "epi_g_fused: the I8 form takes the unfused epilogue");
   }
   check_launch("epi_g_fused");
   const uint64_t nb = (nwin + kEpiRows - 1) / kEpiRows;
@@ -574,14 +649,18 @@ void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride
 }
 
 void launch_epi_h(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
-                  int lp, const double* b, double* out, int mode, hipStream_t s) {
+                  int lp, const double* b, double* out, int mode, hipStream_t s, const double* colmul) {
   const unsigned g = grid_for(rows_pad * lp);
+  require(f != Form::I8 || colmul, kErrInternal, "epi_h: the I8 form needs column factors");
   if (f == Form::FP64)
     hipLaunchKernelGGL(k_epi_h<double>, dim3(g), dim3(256), 0, s, static_cast<const double*>(P), nchunks,
-                       chunk_stride, nreal, rows_pad, lp, b, out, mode);
+                       chunk_stride, nreal, rows_pad, lp, b, out, mode, colmul);
+  else if (f == Form::I8)
+    hipLaunchKernelGGL(k_epi_h<int32_t>, dim3(g), dim3(256), 0, s, static_cast<const int32_t*>(P), nchunks,
+                       chunk_stride, nreal, rows_pad, lp, b, out, mode, colmul);
   else
     hipLaunchKernelGGL(k_epi_h<float>, dim3(g), dim3(256), 0, s, static_cast<const float*>(P), nchunks, chunk_stride,
-                       nreal, rows_pad, lp, b, out, mode);
+                       nreal, rows_pad, lp, b, out, mode, colmul);
   check_launch("epi_h");
 }
 

@@ -3,8 +3,10 @@
 Tolerances: the FP64 form must reproduce the reference outputs to their print
 precision (6 significant digits); the FP32 and BF16 hi/lo forms must meet the
 north-star bar against the FP64 oracle on the same Omega: eigenvalue relative
-error <= 1e-4 and every PC correlated >= 0.9999.  The single-piece forms
-(FP16, BF16MIX, FP16MIX) meet the eigenvalue bar and correlate >= 0.999.
+error <= 1e-4 and every PC correlated >= 0.9999; so must the int8 form (two
+int8 pieces of 16-bit per-column fixed point, exact int32 accumulation).  The
+single-piece forms (FP16, BF16MIX, FP16MIX) meet the eigenvalue bar and
+correlate >= 0.999.
 """
 import numpy as np
 import pytest
@@ -52,13 +54,29 @@ def test_packed_gemm_exact(ctx, form, nrows, ncodes, L):
     np.testing.assert_array_equal(Y, DMAP[codes] @ W)
 
 
+@pytest.mark.parametrize("nrows,ncodes,L", [(1, 1, 1), (37, 127, 20), (300, 129, 80), (513, 1000, 128),
+                                            (64, 4096, 16), (257, 333, 40), (100, 65536, 24)])
+def test_packed_gemm_int8_exact(ctx, nrows, ncodes, L):
+    """W = q 2^-7 with |q| <= 32639 and each column's maximum exactly 32639 2^-7:
+    the column scale is 2^-7, the fixed point is exact and so is the product."""
+    say(f"packed_gemm form=int8 nrows={nrows} ncodes={ncodes} L={L}")
+    rng = np.random.default_rng(nrows * 5 + ncodes + L)
+    codes = rng.integers(0, 4, (nrows, ncodes)).astype(np.uint8)
+    q = rng.integers(-32639, 32640, (ncodes, L))
+    q[rng.integers(0, ncodes, L), np.arange(L)] = rng.choice([-32639, 32639], L)
+    W = q / 128.0
+    Y = ctx.packed_gemm("int8", pack(codes), ncodes, W)
+    np.testing.assert_array_equal(Y, DMAP[codes] @ W)
+
+
 def test_packed_gemm_rounding_fp32(ctx):
     say("packed_gemm fp32 rounding")
     rng = np.random.default_rng(11)
     codes = rng.integers(0, 4, (200, 3000)).astype(np.uint8)
     W = rng.standard_normal((3000, 24))
     ref = DMAP[codes] @ W
-    for form, tol in (("fp64", 1e-12), ("fp32", 2e-5), ("bf16", 3e-5), ("bf16x1", 2e-2), ("fp16", 3e-3)):
+    for form, tol in (("fp64", 1e-12), ("fp32", 2e-5), ("bf16", 3e-5), ("bf16x1", 2e-2), ("fp16", 3e-3),
+                      ("int8", 2e-4)):
         Y = ctx.packed_gemm(form, pack(codes), 3000, W)
         err = np.abs(Y - ref).max() / np.abs(ref).max()
         say(f"  {form}: max rel err {err:.2e}")
@@ -166,14 +184,14 @@ def test_golden_fp64(ctx, golden_inputs, case):
         np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=2e-6)
 
 
-REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16")
+REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16", "int8")
            for c in ("f1_d1", "f1_d2", "f4_d1_miss", "f4_emu_d1", "cfg1_d1", "cfg1_d2")]
 # The mixed form (one BF16 piece of Omega) meets the bar on --svd 1 only; on
 # winSVD the rounded per-window Omega moves the unconverged noise PCs.
 REDUCED += [(c, f) for f in ("bf16mix", "fp16mix") for c in ("f1_d1", "f4_d1_miss", "cfg1_d1")]
 # Single-piece forms round Omega (or G) to 8-11 bits: the eigenvalues meet the
 # bar, the unconverged trailing PCs move by up to ~1e-3 in correlation.
-PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "fp16": 0.999, "bf16mix": 0.999, "fp16mix": 0.999}
+PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "int8": 0.9999, "fp16": 0.999, "bf16mix": 0.999, "fp16mix": 0.999}
 
 
 @pytest.mark.parametrize("case,form", REDUCED)
@@ -216,7 +234,8 @@ def test_device_generator_matches_host(ctx, spec_kw):
 
 @pytest.mark.parametrize("form,wm,waves", [("bf16", 4, 8), ("bf16", 2, 8), ("bf16", 8, 4), ("fp32", 4, 8),
                                            ("bf16mix", 4, 8), ("bf16x1", 8, 4), ("bf16x1", 4, 8), ("fp16", 4, 8),
-                                           ("fp16", 2, 8), ("fp16", 8, 4)])
+                                           ("fp16", 2, 8), ("fp16", 8, 4), ("int8", 2, 8), ("int8", 4, 8),
+                                           ("int8", 4, 4), ("int8", 1, 4)])
 def test_workgroup_shapes_agree(ctx, golden_inputs, form, wm, waves):
     """Every compiled workgroup shape gives the same decomposition (cfg1, 8 passes)."""
     say(f"shape form={form} wm={wm} waves={waves}")
