@@ -336,9 +336,8 @@ int pcaone_run_halko(pcaone_ctx* c, const pcaone_params* p, double* U, double* S
     require(x.M > 0 && !x.F.empty(), kErrState, "no genotype data (pcaone_bed_read / upload first)");
     c->last = RunInfo{};
     x.ktimes = Context::KernelTimes{};
-    const PcaResult r = run_pca_with_halko(x, q, &c->last);
-    const uint64_t N = x.N, M = x.M;
-    if (U) std::memcpy(U, r.U.data(), N * q.k * sizeof(double));
+    const PcaResult r = run_pca_with_halko(x, q, &c->last, U);
+    const uint64_t M = x.M;
     if (S) std::memcpy(S, r.S.data(), r.S.size() * sizeof(double));
     if (E) std::memcpy(E, r.E.data(), r.E.size() * sizeof(double));
     if (V && !r.V.empty()) std::memcpy(V, r.V.data(), M * q.k * sizeof(double));

@@ -170,27 +170,21 @@ void Context::finalize() {
   }
   perm.resize(M);
   for (uint64_t j = 0; j < M; ++j) perm[j] = (int)j;
+  perm_identity_ = true;
   dF_.resize(M);
   PCAONE_HIP(hipMemcpyAsync(dF_.get(), F.data(), M * 8, hipMemcpyHostToDevice, stream));
-  upload_col_state(F, std::vector<double>(M, 1.0));
+  pristine_columns();
   clear_impute();
   build_derived();
 }
 
-void Context::upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc) {
-  weights_dirty_ = true;
-  cols_pristine_ = false;
-  host_cf_ = cf;
-  host_sc_ = sc;
-  // padded to rows_s_ so window tiles may read past M
-  std::vector<double> pc(rows_s_, 0.0), ps(rows_s_, 0.0);
-  std::copy(cf.begin(), cf.end(), pc.begin());
-  std::copy(sc.begin(), sc.end(), ps.begin());
+void Context::pristine_columns() {
+  // padded to rows_s_ (zeros) so window tiles may read past M
   cF_.resize(rows_s_);
   sc_.resize(rows_s_);
-  PCAONE_HIP(hipMemcpyAsync(cF_.get(), pc.data(), rows_s_ * 8, hipMemcpyHostToDevice, stream));
-  PCAONE_HIP(hipMemcpyAsync(sc_.get(), ps.data(), rows_s_ * 8, hipMemcpyHostToDevice, stream));
-  PCAONE_HIP(hipStreamSynchronize(stream));
+  launch_pristine_cols(dF_.get(), M, rows_s_, cF_.get(), sc_.get(), stream);
+  weights_dirty_ = true;
+  cols_pristine_ = true;
 }
 
 void Context::build_derived() {
@@ -261,18 +255,24 @@ void Context::permute_columns(const std::vector<int>& p) {
   tmpx_.resize(rows_s_ * pitch_s_);
   launch_fill_rows(tmpx_.get(), pitch_s_, M, rows_s_, stream);
   launch_gather_rows(xs_.get(), pitch_s_, d.get(), M, tmpx_.get(), stream);
+  // the column state follows its columns (padding untouched)
+  DevBuf<double> t(2 * M);
+  launch_gather_f64(cF_.get(), d.get(), M, t.get(), stream);
+  launch_gather_f64(sc_.get(), d.get(), M, t.get() + M, stream);
+  PCAONE_HIP(hipMemcpyAsync(cF_.get(), t.get(), M * 8, hipMemcpyDeviceToDevice, stream));
+  PCAONE_HIP(hipMemcpyAsync(sc_.get(), t.get() + M, M * 8, hipMemcpyDeviceToDevice, stream));
   PCAONE_HIP(hipStreamSynchronize(stream));
   std::swap(xs_, tmpx_);
   tmpx_.release();
-  std::vector<double> cf(M), sc(M);
   std::vector<int> np(M);
+  perm_identity_ = true;
   for (uint64_t i = 0; i < M; ++i) {
-    cf[i] = host_cf_[idx[i]];
-    sc[i] = host_sc_[idx[i]];
     np[i] = perm[idx[i]];
+    perm_identity_ = perm_identity_ && np[i] == (int)i;
   }
   perm = np;
-  upload_col_state(cf, sc);
+  weights_dirty_ = true;
+  cols_pristine_ = false;
   if (emu_k_ > 0) {  // imputation loadings follow their columns
     std::vector<double> v(M * emu_k_), w(M * emu_k_);
     PCAONE_HIP(hipMemcpyAsync(v.data(), vimp_.get(), v.size() * 8, hipMemcpyDeviceToHost, stream));
@@ -285,14 +285,12 @@ void Context::permute_columns(const std::vector<int>& p) {
 }
 
 void Context::standardize(int ploidy) {
-  require(!host_sc_.empty(), kErrState, "no data");
-  std::vector<double> sc = host_sc_;
-  for (uint64_t i = 0; i < M; ++i) {
-    const double f = F[i];  // reference indexes F by the current column (src/Data.cpp:356)
-    const double sd = std::sqrt(f * (1.0 - f));
-    if (sd > kVarTol) sc[i] *= std::sqrt((double)ploidy) / sd;
-  }
-  upload_col_state(host_cf_, sc);
+  require(!F.empty() && sc_.size() >= M, kErrState, "no data");
+  // F as read, indexed by the current column: the reference indexes F by the
+  // column position after any permutation (src/Data.cpp:356)
+  launch_standardize(dF_.get(), M, std::sqrt((double)ploidy), kVarTol, sc_.get(), stream);
+  weights_dirty_ = true;
+  cols_pristine_ = false;
 }
 
 void Context::set_impute(const double* U, const double* S, const double* V, int kk) {
@@ -320,15 +318,12 @@ void Context::reset_columns() {
   require(!F.empty(), kErrState, "no data");
   if (cols_pristine_ && !emu_k_) return;  // nothing changed since the last reset
   clear_impute();
-  bool identity = true;
-  for (uint64_t i = 0; i < M && identity; ++i) identity = perm[i] == (int)i;
-  if (!identity) {
+  if (!perm_identity_) {
     std::vector<int> inv(M);
     for (uint64_t i = 0; i < M; ++i) inv[perm[i]] = (int)i;
     permute_columns(inv);
   }
-  upload_col_state(F, std::vector<double>(M, 1.0));
-  cols_pristine_ = true;
+  pristine_columns();
 }
 
 // ------------------------------------------------------------------ rsvd
@@ -450,6 +445,22 @@ void Context::set_omega(const double* om) {
   require(L > 0, kErrState, "configure() first");
   // column-major N x L as given, laid out row-major (ks_ x lp) on the device
   stage_in(om, (uint64_t)N * L, tmp_n_.get());
+  omega_pieces();
+}
+
+double* Context::omega_staging() {
+  require(L > 0, kErrState, "configure() first");
+  omega_pinned_.resize((uint64_t)N * L * 8);
+  return reinterpret_cast<double*>(omega_pinned_.get());
+}
+
+void Context::set_omega_staged() {
+  require(L > 0 && omega_pinned_.get(), kErrState, "set_omega_staged: nothing staged");
+  PCAONE_HIP(hipMemcpyAsync(tmp_n_.get(), omega_pinned_.get(), (uint64_t)N * L * 8, hipMemcpyHostToDevice, stream));
+  omega_pieces();
+}
+
+void Context::omega_pieces() {
   launch_colmajor_to_rows(tmp_n_.get(), N, L, lp, ks_, omega_c_.get(), stream);
   if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,

@@ -92,6 +92,10 @@ class Context {
   GemmTuning tuning;
 
   void set_omega(const double* omega_colmajor);  // N x L, also Omg2 := Omega
+  // Pinned host space for an N x L column-major test matrix; set_omega_staged()
+  // sends what was written there, as set_omega does, without a host copy.
+  double* omega_staging();
+  void set_omega_staged();
   // G rows [j0, j1) = A[j0:j1] Omega and the H contribution of those SNPs:
   // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ...
   void sketch(uint64_t j0, uint64_t j1, int hmode);
@@ -129,6 +133,7 @@ class Context {
   DevBuf<uint32_t> xs_, xt_, tmpx_;
   DevBuf<uint8_t> raw_;
   PinnedBuf pinned_;
+  PinnedBuf omega_pinned_;  // the run's test matrix, drawn straight into pinned memory
   DevBuf<double> dF_, cF_, sc_;
   DevBuf<uint64_t> rp_s_, rp_t_;
   DevBuf<uint32_t> ix_s_, ix_t_;
@@ -165,9 +170,10 @@ class Context {
   la::Mat gram_h_;    // Gram of H (L x L) for h_version_
 
   void build_derived();  // xt + missing lists from xs
-  void upload_col_state(const std::vector<double>& cf, const std::vector<double>& sc);
-  std::vector<double> host_cf_, host_sc_;
+  void pristine_columns();  // cF = F as read, sc = 1 (device)
+  bool perm_identity_ = true;  // perm[i] == i
   la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
+  void omega_pieces();  // tmp_n_ (column-major Omega) -> omega_c_, pieces, Omg2, column sums
   // I8 column scales / epilogue factors of the G (h = 0) or H (h = 1)
   // product's operand; null when that product is not in the I8 form.
   double* i8_scale(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h : nullptr; }

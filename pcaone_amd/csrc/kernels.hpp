@@ -86,6 +86,12 @@ void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, doub
 // flipOmg norms: a[l] = sum_i |O2 - Q|, b[l] = sum_i |O2 + Q|.
 void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
                        hipStream_t s);
+// Per-SNP column state: cf = F, sc = 1 for j < m, 0 up to rows.
+void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s);
+// sc_j *= sqrt_ploidy / sqrt(F_j (1 - F_j)) where that sd exceeds tol (j < m).
+void launch_standardize(const double* F, uint64_t m, double sqrt_ploidy, double tol, double* sc, hipStream_t s);
+// dst[i] = src[idx[i]], i < n.
+void launch_gather_f64(const double* src, const uint32_t* idx, uint64_t n, double* dst, hipStream_t s);
 // Column-major n x L (leading dimension n) -> row-major rows_pad x lp (zero padded).
 void launch_colmajor_to_rows(const double* src, uint64_t n, int L, int lp, uint64_t rows_pad, double* dst,
                              hipStream_t s);

@@ -247,35 +247,54 @@ __global__ void k_colsum_partial(const double* __restrict__ X, uint64_t r0, uint
   }
 }
 
-// out[t] = sum_b partial[b][t].  A workgroup owns 32 consecutive outputs; its
-// 8 thread rows take every 8th partial block (coalesced 32-wide reads) and
-// are combined in LDS in a fixed order, so the result is deterministic.
-__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partial, uint64_t nblocks,
-                                                      uint64_t width, double* __restrict__ out) {
+// Sums of per-block partials, deterministic.  A workgroup owns 32
+// consecutive outputs t of one slice of blocks: its 8 thread rows take every
+// 8th block of the slice (coalesced 32-wide reads), combined in LDS in a
+// fixed order.  Block b of the input sits at partial[b * bstride * width].
+// With dst == partial (the first stage of a long reduction) the slice's sum
+// replaces its first block; every workgroup reads its whole slice first.
+__global__ void __launch_bounds__(256) k_sum_partials(double* partial, uint64_t nblocks, uint64_t width,
+                                                      uint64_t bstride, uint64_t slice, double* dst) {
   __shared__ double red[8][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const uint64_t t = (uint64_t)blockIdx.x * 32 + tx;
+  const uint64_t b0 = (uint64_t)blockIdx.y * slice;
+  const uint64_t b1 = b0 + slice < nblocks ? b0 + slice : nblocks;
+  const uint64_t step = bstride * width;
   double s0 = 0.0, s1 = 0.0;
   if (t < width) {
-    uint64_t b = ty;
-    for (; b + 8 < nblocks; b += 16) {
-      s0 += partial[b * width + t];
-      s1 += partial[(b + 8) * width + t];
+    uint64_t b = b0 + ty;
+    for (; b + 8 < b1; b += 16) {
+      s0 += partial[b * step + t];
+      s1 += partial[(b + 8) * step + t];
     }
-    if (b < nblocks) s0 += partial[b * width + t];
+    if (b < b1) s0 += partial[b * step + t];
   }
   red[ty][tx] = s0 + s1;
   __syncthreads();
   if (ty == 0 && t < width) {
     double s = 0.0;
     for (int r = 0; r < 8; ++r) s += red[r][tx];
-    out[t] = s;
+    if (dst == partial)
+      partial[b0 * step + t] = s;
+    else
+      dst[t] = s;
   }
 }
 
-inline void sum_partials(const double* partial, uint64_t nblocks, uint64_t width, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((width + 31) / 32)), dim3(256), 0, s, partial, nblocks, width,
-                     out);
+// out[t] = sum_b partial[b][t] for b < nblocks (partial is scratch: a long
+// reduction first folds slices of 64 blocks in place).
+inline void sum_partials(double* partial, uint64_t nblocks, uint64_t width, double* out, hipStream_t s) {
+  const unsigned gx = (unsigned)((width + 31) / 32);
+  constexpr uint64_t kSlice = 64;
+  if (nblocks > 4 * kSlice) {
+    const uint64_t ns = (nblocks + kSlice - 1) / kSlice;
+    hipLaunchKernelGGL(k_sum_partials, dim3(gx, (unsigned)ns), dim3(256), 0, s, partial, nblocks, width, (uint64_t)1,
+                       kSlice, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(gx, 1), dim3(256), 0, s, partial, ns, width, kSlice, ns, out);
+    return;
+  }
+  hipLaunchKernelGGL(k_sum_partials, dim3(gx, 1), dim3(256), 0, s, partial, nblocks, width, (uint64_t)1, nblocks, out);
 }
 
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
@@ -397,6 +416,31 @@ __global__ void k_rows_to_colmajor(const double* __restrict__ src, uint64_t n, i
     const uint64_t j = t / n, r = t - j * n;
     dst[t] = src[r * lp + j];
   }
+}
+
+// Per-SNP column state as read: cF = F and sc = 1 for j < m, zero padding to rows.
+__global__ void k_pristine_cols(const double* __restrict__ F, uint64_t m, uint64_t rows, double* __restrict__ cf,
+                                double* __restrict__ sc) {
+  GRID_STRIDE(j, rows) {
+    cf[j] = j < m ? F[j] : 0.0;
+    sc[j] = j < m ? 1.0 : 0.0;
+  }
+}
+
+// standardize_E's scale (src/Data.cpp:351-362): sc_j *= sqrt(ploidy) / sd_j with
+// sd_j = sqrt(F_j (1 - F_j)) when sd_j > tol.
+__global__ void k_standardize(const double* __restrict__ F, uint64_t m, double sqrt_ploidy, double tol,
+                              double* __restrict__ sc) {
+  GRID_STRIDE(j, m) {
+    const double f = F[j];
+    const double sd = sqrt(f * (1.0 - f));
+    if (sd > tol) sc[j] *= sqrt_ploidy / sd;
+  }
+}
+
+__global__ void k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                             double* __restrict__ dst) {
+  GRID_STRIDE(i, n) dst[i] = src[idx[i]];
 }
 
 __global__ void k_add(const double* __restrict__ X, const double* __restrict__ Z, double* __restrict__ Y, uint64_t n) {
@@ -540,6 +584,21 @@ void launch_colmajor_to_rows(const double* src, uint64_t n, int L, int lp, uint6
                              hipStream_t s) {
   hipLaunchKernelGGL(k_colmajor_to_rows, dim3(grid_for(rows_pad * lp)), dim3(256), 0, s, src, n, L, lp, rows_pad, dst);
   check_launch("colmajor_to_rows");
+}
+
+void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_pristine_cols, dim3(grid_for(rows)), dim3(256), 0, s, F, m, rows, cf, sc);
+  check_launch("pristine_cols");
+}
+
+void launch_standardize(const double* F, uint64_t m, double sqrt_ploidy, double tol, double* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_standardize, dim3(grid_for(m)), dim3(256), 0, s, F, m, sqrt_ploidy, tol, sc);
+  check_launch("standardize");
+}
+
+void launch_gather_f64(const double* src, const uint32_t* idx, uint64_t n, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_f64, dim3(grid_for(n)), dim3(256), 0, s, src, idx, n, dst);
+  check_launch("gather_f64");
 }
 
 void launch_rows_to_colmajor(const double* src, uint64_t n, int lp, int k, double* dst, hipStream_t s) {

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <numeric>
 #include <random>
@@ -28,18 +29,18 @@ void Param::derive() {
 
 RsvdOpData::RsvdOpData(Context& ctx, const Param& p, int k, int os) : ctx_(ctx), params_(p), nk_(k), os_(os) {
   ctx_.configure(p.precision, k + os, p.tuning);
-  initOmg();
 }
 
 // src/Halko.cpp:15-23 with src/RSVD.hpp:25-59: libstdc++ default_random_engine
 // seeded with --seed; one draw per element in column-major order.  Every call
-// of one operator yields the same matrix, so it is drawn once and re-sent.
+// of one operator yields the same matrix, so it is drawn once (straight into
+// the context's pinned staging memory) and re-sent.
 void RsvdOpData::initOmg() {
-  if (omega_host_.empty()) {
-    omega_host_.resize(cols() * (uint64_t)size());
-    rng::omega(params_.seed, params_.rand != 0, cols(), (uint64_t)size(), omega_host_.data());
+  if (!omega_drawn_) {
+    rng::omega(params_.seed, params_.rand != 0, cols(), (uint64_t)size(), ctx_.omega_staging());
+    omega_drawn_ = true;
   }
-  ctx_.set_omega(omega_host_.data());
+  ctx_.set_omega_staged();
 }
 
 void RsvdOpData::fit_with_pi() {
@@ -83,8 +84,12 @@ void RsvdOpData::computeUSV(int p, double tol) {
       ctx_.save_prev();
     }
   }
-  U.resize(cols() * (uint64_t)nk);
-  ctx_.download_u(nk, U.data());
+  if (u_out && !params_.missme) {
+    ctx_.download_u(nk, u_out);  // straight into the caller's N x k buffer
+  } else {
+    U.resize(cols() * (uint64_t)nk);
+    ctx_.download_u(nk, U.data());
+  }
 }
 
 void NormalRsvdOpData::computeGandH(int pi) {
@@ -171,7 +176,7 @@ double mev_cols(const std::vector<double>& X, const std::vector<double>& Y, uint
   return res / k;
 }
 
-PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info) {
+PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out) {
   // src/Halko.cpp:271-345 (genetic input, in-core, no LD output)
   ctx.reset_columns();  // restores the as-read state between runs; not part of the timed work
   const auto t0 = std::chrono::steady_clock::now();
@@ -181,6 +186,7 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info) {
   else
     rsvd = std::make_unique<NormalRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
   rsvd->info = info;
+  rsvd->u_out = U_out;
   rsvd->need_v_host = params.printv || params.missme;
   const int k = (int)params.k;
   const uint64_t N = ctx.N, M = ctx.M;
@@ -213,7 +219,8 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info) {
   }
   PcaResult res;
   res.k = k;
-  res.U = rsvd->U;
+  res.U = std::move(rsvd->U);  // empty when U went straight to U_out
+  if (U_out && !res.U.empty()) std::memcpy(U_out, res.U.data(), res.U.size() * sizeof(double));
   res.S = rsvd->S;
   res.E.resize(res.S.size());
   for (size_t i = 0; i < res.S.size(); ++i) res.E[i] = res.S[i] * res.S[i] / (double)M;  // src/Halko.cpp:339

@@ -58,13 +58,14 @@ class RsvdOpData {
   bool update = false, standardize = false;
   std::vector<double> U, S, V;  // N x k, k, M x k (column-major; V in device column order)
   bool need_v_host = false;
+  double* u_out = nullptr;  // when set (and no EMU), U is written there instead of into U
   RunInfo* info = nullptr;
 
  protected:
   Context& ctx_;
   const Param& params_;
   int nk_, os_;
-  std::vector<double> omega_host_;  // the test matrix of initOmg (identical on every call)
+  bool omega_drawn_ = false;  // the test matrix of initOmg is in the context's staging memory
   void fit_with_pi();  // src/Data.cpp:293-349 via the device impute state
 };
 
@@ -89,7 +90,8 @@ struct PcaResult {
   int k = 0;
 };
 
-PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info);
+// U_out (nullable, N x k column-major): receives U directly; res.U is then empty.
+PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out = nullptr);
 
 // flip_UV(U, V, ubase=false) and mev (src/Utils.cpp:118-154, 194-200), column-major.
 void flip_uv_vbase(std::vector<double>& U, std::vector<double>& V, uint64_t n, uint64_t m, int k);
