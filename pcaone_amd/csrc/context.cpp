@@ -274,11 +274,11 @@ void Context::permute_columns(const std::vector<int>& p) {
   weights_dirty_ = true;
   cols_pristine_ = false;
   if (emu_k_ > 0) {  // imputation loadings follow their columns
-    std::vector<double> v(M * emu_k_), w(M * emu_k_);
+    const uint64_t kp = emu_kp_;
+    std::vector<double> v(M * kp), w(M * kp);
     PCAONE_HIP(hipMemcpyAsync(v.data(), vimp_.get(), v.size() * 8, hipMemcpyDeviceToHost, stream));
     PCAONE_HIP(hipStreamSynchronize(stream));
-    for (uint64_t i = 0; i < M; ++i)
-      std::memcpy(&w[i * emu_k_], &v[(uint64_t)idx[i] * emu_k_], emu_k_ * 8);
+    for (uint64_t i = 0; i < M; ++i) std::memcpy(&w[i * kp], &v[(uint64_t)idx[i] * kp], kp * 8);
     PCAONE_HIP(hipMemcpyAsync(vimp_.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, stream));
   }
   build_derived();
@@ -295,17 +295,20 @@ void Context::standardize(int ploidy) {
 
 void Context::set_impute(const double* U, const double* S, const double* V, int kk) {
   require(kk > 0, kErrArgument, "impute rank must be positive");
-  std::vector<double> us((uint64_t)N * kk), v(M * kk);
+  // rows padded to an even pitch (zeros) so the device reads them as 16-byte pairs
+  const int kp = (kk + 1) / 2 * 2;
+  std::vector<double> us((uint64_t)N * kp, 0.0), v(M * kp, 0.0);
   for (uint64_t i = 0; i < N; ++i)
-    for (int t = 0; t < kk; ++t) us[i * kk + t] = U[(uint64_t)t * N + i] * S[t];
+    for (int t = 0; t < kk; ++t) us[i * kp + t] = U[(uint64_t)t * N + i] * S[t];
   for (uint64_t j = 0; j < M; ++j)
-    for (int t = 0; t < kk; ++t) v[j * kk + t] = V[(uint64_t)t * M + j];
+    for (int t = 0; t < kk; ++t) v[j * kp + t] = V[(uint64_t)t * M + j];
   us_.resize(us.size());
   vimp_.resize(v.size());
   PCAONE_HIP(hipMemcpyAsync(us_.get(), us.data(), us.size() * 8, hipMemcpyHostToDevice, stream));
   PCAONE_HIP(hipMemcpyAsync(vimp_.get(), v.data(), v.size() * 8, hipMemcpyHostToDevice, stream));
   PCAONE_HIP(hipStreamSynchronize(stream));
   emu_k_ = kk;
+  emu_kp_ = kp;
   weights_dirty_ = true;
 }
 
@@ -630,10 +633,11 @@ void Context::update_missing_weights() {
   a.US = us_.get();
   a.V = vimp_.get();
   a.kk = emu_k_;
+  a.kp = emu_kp_;
   // index chunks keep the gathered rows of US (by SNP) or V (by sample) in L2
   const auto span = [&](uint64_t kext) {
     if (!emu_k_) return std::max<uint64_t>(kext, 1);
-    uint64_t sp = std::max<uint64_t>(1024, kL2Bytes / ((uint64_t)emu_k_ * 8));
+    uint64_t sp = std::max<uint64_t>(1024, kL2Bytes / ((uint64_t)emu_kp_ * 8));
     return std::max(sp, (kext + 65534) / 65535);
   };
   a.rp = rp_s_.get();

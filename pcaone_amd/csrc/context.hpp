@@ -148,6 +148,7 @@ class Context {
   uint64_t miss_span(uint64_t nrows, uint64_t kext) const;
   // EMU state
   int emu_k_ = 0;
+  int emu_kp_ = 0;  // row pitch of us_ / vimp_ (emu_k_ rounded up to even)
   DevBuf<double> us_, vimp_;
   // rsvd workspace
   DevBuf<double> omega_, omega_r_, omega_c_, colsum_, bvec_;

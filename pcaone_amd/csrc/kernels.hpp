@@ -113,9 +113,10 @@ struct MissWeightArgs {
   const double* sc;      // per SNP column
   const double* cf;      // per SNP column
   const double* Fclamp;  // per SNP column (EMU clamp bound), null = no EMU
-  const double* US;      // N x kk row-major (U * S), EMU only
-  const double* V;       // M x kk row-major, EMU only
+  const double* US;      // N x kk row-major (U * S), row pitch kp, EMU only
+  const double* V;       // M x kk row-major, row pitch kp, EMU only
   int kk;
+  int kp;                // even row pitch of US and V (zero padded)
   double* w;             // one weight per list entry
   uint64_t kext, kspan;
 };

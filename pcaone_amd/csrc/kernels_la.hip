@@ -465,9 +465,10 @@ __device__ __forceinline__ void entry_range(const uint64_t* rp, const uint32_t* 
   e1 = lower_bound_ix(ix, e0, e1, khi);
 }
 
-// Per-entry weights of the missing-entry corrections.  One wave per row of
-// the lists and index chunk (grid.y), lanes over the row's entries; chunking
-// the index range keeps the gathered rows of US (or V) within L2.
+// Per-entry weights of the missing-entry corrections without imputation:
+// w = -sc_c (1/2 - cF_c), which cancels the centering term the affine
+// epilogue gave the entry.  One wave per row of the lists and index chunk
+// (grid.y), lanes over the row's entries.
 __global__ void k_missing_weights(MissWeightArgs a) {
   const uint64_t r = blockIdx.x * (uint64_t)(blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
@@ -476,19 +477,68 @@ __global__ void k_missing_weights(MissWeightArgs a) {
   uint64_t e0, e1;
   entry_range(a.rp, a.ix, r, klo, khi, e0, e1);
   for (uint64_t e = e0 + lane; e < e1; e += 64) {
-    const uint64_t k = a.ix[e];
-    const uint64_t c = a.by_snp ? r : k;  // SNP column
-    const uint64_t i = a.by_snp ? k : r;  // sample
-    const double s = a.sc[c];
-    double w = -s * (0.5 - a.cf[c]);
-    if (a.Fclamp) {
-      double p = 0.0;
-      for (int t = 0; t < a.kk; ++t) p += a.US[i * a.kk + t] * a.V[c * a.kk + t];
-      const double f = a.Fclamp[c];
-      p = fmin(fmax(p, -f), 1.0 - f);
-      w += s * p;
+    const uint64_t c = a.by_snp ? r : a.ix[e];  // SNP column
+    a.w[e] = -a.sc[c] * (0.5 - a.cf[c]);
+  }
+}
+
+// EMU weights (src/Data.cpp:318-348): w = -sc_c (1/2 - cF_c) + sc_c clamp(p, -F_c,
+// 1 - F_c) with p = (U S)_i . V_c.  One wave per row of the lists (a SNP c for
+// the by-SNP list, a sample i for the by-sample list) and index chunk
+// (grid.y; chunks keep the gathered rows within L2).  The k-vectors are rows
+// of even pitch kp read as 16-byte pairs: the row's own vector sits in
+// registers, and the wave is split into 64 / LPE groups of LPE lanes (a power
+// of two >= kp / 2), each group taking one entry per step, reading the
+// entry's vector as one contiguous access and reducing the dot product by
+// butterfly shuffles.  The entries' indexes come 64 at a time (one coalesced
+// load, the next batch's fetched during this one).
+__global__ void __launch_bounds__(256) k_missing_weights_emu(MissWeightArgs a, int lpe) {
+  const uint64_t r = blockIdx.x * 4ull + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (r >= a.nrows) return;  // wave-uniform
+  const uint64_t klo = (uint64_t)blockIdx.y * a.kspan, khi = klo + a.kspan;
+  uint64_t e0, e1;
+  entry_range(a.rp, a.ix, r, klo, khi, e0, e1);
+  const int np = a.kp / 2;  // 16-byte pairs per vector
+  const int G = 64 / lpe, g = lane / lpe, cl = lane - g * lpe;
+  const bool has = cl < np;
+  const double2* own = reinterpret_cast<const double2*>(a.by_snp ? a.V : a.US) + r * np;
+  const double2* other = reinterpret_cast<const double2*>(a.by_snp ? a.US : a.V);
+  const double2 rv = has ? own[cl] : double2{0.0, 0.0};
+  constexpr int U = 4;
+  uint32_t nix = e0 + lane < e1 ? a.ix[e0 + lane] : 0u;
+  for (uint64_t eb = e0; eb < e1; eb += 64) {
+    const int nb = (int)min<uint64_t>(64, e1 - eb);
+    const uint32_t myix = nix;
+    nix = eb + 64 + lane < e1 ? a.ix[eb + 64 + lane] : 0u;
+    const int steps = (nb + G - 1) / G;
+    for (int st = 0; st < steps; st += U) {
+      uint32_t kk[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = (st + u) * G + g;
+        kk[u] = __shfl(myix, j < 63 ? j : 63);
+        ok[u] = j < nb;
+      }
+      double2 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        x[u] = has ? other[(ok[u] ? (uint64_t)kk[u] : 0) * np + cl] : double2{0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double p = rv.x * x[u].x + rv.y * x[u].y;
+        for (int off = lpe / 2; off > 0; off >>= 1) p += __shfl_xor(p, off);
+        const int j = (st + u) * G + g;
+        if (cl == 0 && ok[u]) {
+          const uint64_t c = a.by_snp ? r : kk[u];  // SNP column
+          const double sc = a.sc[c], f = a.Fclamp[c];
+          double w = -sc * (0.5 - a.cf[c]);
+          w += sc * fmin(fmax(p, -f), 1.0 - f);
+          a.w[eb + j] = w;
+        }
+      }
     }
-    a.w[e] = w;
   }
 }
 
@@ -503,66 +553,96 @@ __global__ void k_compact_rows(const double* __restrict__ src, uint64_t rows, in
   }
 }
 
+// Sparse correction out[r][c] (+)= sum_e w[e] src[ix[e] - src_k0][c], c < ncols,
+// one wave per (target row, index chunk = grid.z).  The row's entries are
+// taken 64 at a time (one coalesced load of their indexes and weights, the
+// next batch's fetched during this one).  The wave is split into G groups of
+// LPE = cp / VEC lanes, each lane holding VEC columns (16 bytes): per step
+// every group takes one entry, receives its index and weight by lane permutes
+// and reads the entry's compact source row as one contiguous access (16
+// bytes per lane), accumulating in FP64.  The groups' sums are combined in a
+// fixed order at the end.  With several index chunks each writes its own
+// partial row (summed later in chunk order), else it adds to out.
 template <class T>
-struct Vec16;
-template <>
-struct Vec16<float> {
-  using type = float4;
-  static constexpr int n = 4;
-};
-template <>
-struct Vec16<double> {
-  using type = double2;
-  static constexpr int n = 2;
-};
-
-// Sparse correction, one wave per (row, column chunk of CW = grid.y, index
-// chunk = grid.z): lane l takes the row's entries l, l+64, ... within the
-// index chunk and reads CW consecutive columns of each entry's compact source
-// row (16-byte loads), so 64 gathers are in flight per wave; the lanes' sums
-// are combined by butterfly shuffles.  With several index chunks each writes
-// its own partial row (summed later in chunk order), else it adds to out.
-template <class T, int CW>
 __global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
-  using V = typename Vec16<T>::type;
-  constexpr int NV = Vec16<T>::n;
+  constexpr int VEC = 16 / sizeof(T);
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
   const uint64_t r = blockIdx.x * 4ull + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
-  const int c0 = (int)blockIdx.y * CW;
-  const int ncol = min(CW, a.ncols - c0);
-  if (r >= a.nrows) return;
+  if (r >= a.nrows) return;  // wave-uniform
+  const int lpe = a.cp / VEC;  // lanes per entry (cp is a multiple of VEC)
+  const int G = 64 / lpe;      // entries per step
+  const int g = lane / lpe, cl = lane - g * lpe;
+  const bool act = g < G;
   const uint64_t klo = a.k_lo + (uint64_t)blockIdx.z * a.kspan;
   const uint64_t khi = min(a.k_hi, klo + a.kspan);
   uint64_t e0, e1;
   entry_range(a.rp, a.ix, r, klo, khi, e0, e1);
-  double acc[CW];
+  double acc[VEC];
 #pragma unroll
-  for (int c = 0; c < CW; ++c) acc[c] = 0.0;
-  const T* src = static_cast<const T*>(a.src);
-#pragma unroll 2
-  for (uint64_t e = e0 + lane; e < e1; e += 64) {
-    const double w = a.w[e];
-    const V* row = reinterpret_cast<const V*>(src + ((uint64_t)a.ix[e] - a.src_k0) * a.cp + c0);
+  for (int q = 0; q < VEC; ++q) acc[q] = 0.0;
+  const V* src = static_cast<const V*>(a.src);
+  constexpr int U = 4;  // steps per pass: their gathers are issued together
+  uint32_t nix = 0;
+  double nw = 0.0;
+  if (e0 + lane < e1) {
+    nix = a.ix[e0 + lane];
+    nw = a.w[e0 + lane];
+  }
+  for (uint64_t eb = e0; eb < e1; eb += 64) {
+    const int nb = (int)min<uint64_t>(64, e1 - eb);
+    const uint32_t myix = nix;
+    const double myw = nw;
+    if (eb + 64 + lane < e1) {
+      nix = a.ix[eb + 64 + lane];
+      nw = a.w[eb + 64 + lane];
+    }
+    const int steps = (nb + G - 1) / G;
+    // an entry past the batch (or an idle lane) reads source row 0 and is
+    // left out of the sum
+    for (int st = 0; st < steps; st += U) {
+      uint32_t kk[U];
+      double ww[U];
+      bool ok[U];
 #pragma unroll
-    for (int p = 0; p < CW / NV; ++p) {
-      if (p * NV < ncol) {
-        const V v = row[p];
-        const T* x = reinterpret_cast<const T*>(&v);
+      for (int u = 0; u < U; ++u) {
+        const int j = (st + u) * G + g;
+        const int jl = j < 63 ? j : 63;  // every lane takes part in the permutes
+        kk[u] = __shfl(myix, jl);
+        ww[u] = __shfl(myw, jl);
+        ok[u] = act && j < nb;
+      }
+      V xv[U];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[p * NV + q] += w * (double)x[q];
+      for (int u = 0; u < U; ++u) xv[u] = src[(ok[u] ? (uint64_t)kk[u] - a.src_k0 : 0) * lpe + cl];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const T* x = reinterpret_cast<const T*>(&xv[u]);
+        if (ok[u]) {
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) acc[q] += ww[u] * (double)x[q];
+        }
       }
     }
   }
-  const bool partial = gridDim.z > 1;
-  double* o = partial ? a.partial + ((uint64_t)blockIdx.z * a.nrows + r) * a.ncols + c0 : a.out + r * a.lp + c0;
+  // group 0 lane cl += the same lane of groups 1 .. G-1, in order
 #pragma unroll
-  for (int c = 0; c < CW; ++c) {
-    if (c < ncol) {
-      double v = acc[c];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0) o[c] = partial ? v : o[c] + v;
+  for (int q = 0; q < VEC; ++q) {
+    const double v = acc[q];
+    double tot = v;
+    for (int gg = 1; gg < G; ++gg) {
+      const int sl = cl + gg * lpe;
+      tot += __shfl(v, sl < 63 ? sl : 63);
     }
+    acc[q] = tot;
+  }
+  if (g != 0) return;
+  const bool partial = gridDim.z > 1;
+  double* o = partial ? a.partial + ((uint64_t)blockIdx.z * a.nrows + r) * a.ncols : a.out + r * a.lp;
+#pragma unroll
+  for (int q = 0; q < VEC; ++q) {
+    const int c = cl * VEC + q;
+    if (c < a.ncols) o[c] = partial ? acc[q] : o[c] + acc[q];
   }
 }
 
@@ -841,8 +921,16 @@ void launch_missing_weights(const MissWeightArgs& a, hipStream_t s) {
   require(a.kspan > 0, kErrArgument, "missing_weights: zero index span");
   const uint64_t nz = (a.kext + a.kspan - 1) / a.kspan;
   require(nz < 65536, kErrArgument, "missing_weights: too many index chunks");
-  hipLaunchKernelGGL(k_missing_weights, dim3((unsigned)((a.nrows + 3) / 4), (unsigned)std::max<uint64_t>(nz, 1)),
-                     dim3(256), 0, s, a);
+  const dim3 grid((unsigned)((a.nrows + 3) / 4), (unsigned)std::max<uint64_t>(nz, 1));
+  if (!a.Fclamp) {
+    hipLaunchKernelGGL(k_missing_weights, grid, dim3(256), 0, s, a);
+  } else {
+    require(a.kk >= 1 && a.kk <= 128 && a.kp % 2 == 0 && a.kp >= a.kk && a.kp <= 128, kErrArgument,
+            "missing_weights: imputation rank must be in [1, 128]");
+    int lpe = 1;
+    while (lpe < a.kp / 2) lpe *= 2;
+    hipLaunchKernelGGL(k_missing_weights_emu, grid, dim3(256), 0, s, a, lpe);
+  }
   check_launch("missing_weights");
 }
 
@@ -871,16 +959,11 @@ void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
           "missing_spmm: bad column counts");
   const uint64_t nz = missing_spmm_chunks(a);
   require(nz < 65536 && (nz == 1 || a.partial), kErrArgument, "missing_spmm: bad index chunking");
+  require(a.cp <= 128, kErrArgument, "missing_spmm: at most 128 columns");
   const dim3 block(256);
   const unsigned gx = (unsigned)((a.nrows + 3) / 4), gz = (unsigned)nz;
-  if (a.ncols <= 16) {
-    if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float, 16>), dim3(gx, 1, gz), block, 0, s, a);
-    else hipLaunchKernelGGL((k_missing_spmm<double, 16>), dim3(gx, 1, gz), block, 0, s, a);
-  } else {
-    const unsigned gy = (unsigned)((a.ncols + 31) / 32);
-    if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float, 32>), dim3(gx, gy, gz), block, 0, s, a);
-    else hipLaunchKernelGGL((k_missing_spmm<double, 32>), dim3(gx, gy, gz), block, 0, s, a);
-  }
+  if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float>), dim3(gx, 1, gz), block, 0, s, a);
+  else hipLaunchKernelGGL((k_missing_spmm<double>), dim3(gx, 1, gz), block, 0, s, a);
   check_launch("missing_spmm");
   if (nz > 1) {
     hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(a.nrows * (uint64_t)a.ncols)), dim3(256), 0, s, a.partial, nz,

@@ -151,6 +151,24 @@ def test_operator_epoch_matches_oracle(ctx, golden_inputs):
     assert abs(diff - (1 - O.mev(vt2.T[:, :k], vt.T[:, :k]))) < 1e-11
 
 
+@pytest.mark.parametrize("emu", [False, True])
+def test_missing_entries_with_chunked_sources(ctx, emu):
+    """Missing genotypes at a size where the FP64 form splits the H side's
+    compact source rows into two L2-sized index chunks (partial rows summed in
+    chunk order): the run matches the FP64 oracle."""
+    from pcaone_amd import synth
+    spec = synth.Spec(seed=77, nsamples=200, nsnps=50000, npops=3, miss_rate=0.1, high_miss_frac=0.05,
+                      high_miss_rate=0.3)
+    say(f"missing entries, chunked sources, emu={emu}")
+    ctx.generate(spec)
+    kw = dict(svd=1, k=4, oversamples=4, maxp=3, tol=0.0, emu=emu, maxiter=2, tolem=0.0)
+    ref = O.run_pca_with_halko(O.Data.from_packed(synth.packed(spec), spec.nsamples, O.Params(**kw)))
+    kw["emu"] = int(emu)
+    r = ctx.run(api.default_params(precision="fp64", **kw))
+    np.testing.assert_allclose(r.S, ref.S, rtol=1e-9)
+    np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-8)
+
+
 # ---------------------------------------------------------------- whole runs vs golden
 def run_case(ctx, golden_inputs, case, form):
     meta = load_meta(case)
