@@ -295,8 +295,9 @@ void Context::standardize(int ploidy) {
 
 void Context::set_impute(const double* U, const double* S, const double* V, int kk) {
   require(kk > 0, kErrArgument, "impute rank must be positive");
-  // rows padded to an even pitch (zeros) so the device reads them as 16-byte pairs
-  const int kp = (kk + 1) / 2 * 2;
+  // rows padded to whole 128-byte lines, with room for three per-SNP values
+  // after the k loadings of V (the device reads the rows as 16-byte pairs)
+  const int kp = (kk + 3 + 15) / 16 * 16;
   std::vector<double> us((uint64_t)N * kp, 0.0), v(M * kp, 0.0);
   for (uint64_t i = 0; i < N; ++i)
     for (int t = 0; t < kk; ++t) us[i * kp + t] = U[(uint64_t)t * N + i] * S[t];
@@ -379,10 +380,12 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
           "unsupported GEMM workgroup shape for this precision");
   miss_f32_ = precision != 0;  // FP64 form: FP64 sources
   cp_ = (int)round_up((uint64_t)L, miss_f32_ ? 4 : 2);
+  // compact rows start on 128-byte lines, so a row read touches the fewest lines
+  cpitch_ = (int)(round_up((uint64_t)cp_ * (miss_f32_ ? 4 : 8), 128) / (miss_f32_ ? 4 : 8));
   if (nmiss) {
     const uint64_t esz = miss_f32_ ? 4 : 8;
-    csg_.resize(nrows_n * cp_ * esz);
-    csh_.resize(rows_s_ * cp_ * esz);
+    csg_.resize(nrows_n * cpitch_ * esz);
+    csh_.resize(rows_s_ * cpitch_ * esz);
     mpart_.resize(kMissPartialElems);
   }
   fixed_g_ = tuning.wm_g || tuning.waves_g;
@@ -538,7 +541,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   }
   if (nmiss) {
     update_missing_weights();
-    launch_compact_rows(omega_r_.get(), N, lp, L, cp_, miss_f32_, csg_.get(), stream);
+    launch_compact_rows(omega_r_.get(), N, lp, L, cpitch_, miss_f32_, csg_.get(), stream);
     MissSpmmArgs m{};
     m.rp = rp_s_.get() + j0;
     m.ix = ix_s_.get();
@@ -550,13 +553,13 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     m.f32 = miss_f32_;
     m.src_k0 = 0;
     m.cp = cp_;
+    m.cpitch = cpitch_;
     m.ncols = L;
     m.kspan = miss_span(nwin, N);
     m.partial = mpart_.get();
+    m.partial_cap = mpart_.size();
     m.out = gwin;
     m.lp = lp;
-    require(missing_spmm_chunks(m) == 1 || missing_spmm_chunks(m) * nwin * L <= mpart_.size(), kErrInternal,
-            "missing-entry partials exceed their buffer");
     launch_missing_spmm(m, stream);
   }
   if (nmiss || form_g == Form::I8 || form_h == Form::I8) {
@@ -589,7 +592,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream,
                i8_mul(1));
   if (nmiss) {
-    launch_compact_rows(gwin, nwin, lp, L, cp_, miss_f32_, csh_.get(), stream);
+    launch_compact_rows(gwin, nwin, lp, L, cpitch_, miss_f32_, csh_.get(), stream);
     MissSpmmArgs m{};
     m.rp = rp_t_.get();
     m.ix = ix_t_.get();
@@ -601,13 +604,13 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     m.f32 = miss_f32_;
     m.src_k0 = j0;
     m.cp = cp_;
+    m.cpitch = cpitch_;
     m.ncols = L;
     m.kspan = miss_span(N, nwin);
     m.partial = mpart_.get();
+    m.partial_cap = mpart_.size();
     m.out = target;
     m.lp = lp;
-    require(missing_spmm_chunks(m) == 1 || missing_spmm_chunks(m) * N * L <= mpart_.size(), kErrInternal,
-            "missing-entry partials exceed their buffer");
     launch_missing_spmm(m, stream);
   }
   if (hmode == 0) ++h_version_;
@@ -616,7 +619,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
 uint64_t Context::miss_span(uint64_t nrows, uint64_t kext) const {
   // chunks of source rows that fit in L2, as many as the partial buffer holds
   const uint64_t esz = miss_f32_ ? 4 : 8;
-  uint64_t sp = std::max<uint64_t>(1024, kL2Bytes / ((uint64_t)cp_ * esz));
+  uint64_t sp = std::max<uint64_t>(1024, kL2Bytes / ((uint64_t)cpitch_ * esz));
   const uint64_t zmax = std::max<uint64_t>(1, mpart_.size() / std::max<uint64_t>(nrows * (uint64_t)L, 1));
   if ((kext + sp - 1) / sp > zmax) sp = (kext + zmax - 1) / zmax;
   return std::max(sp, (kext + 65534) / 65535);
@@ -634,6 +637,7 @@ void Context::update_missing_weights() {
   a.V = vimp_.get();
   a.kk = emu_k_;
   a.kp = emu_kp_;
+  if (emu_k_) launch_pack_col_state(vimp_.get(), M, emu_kp_, emu_k_, sc_.get(), cF_.get(), dF_.get(), stream);
   // index chunks keep the gathered rows of US (by SNP) or V (by sample) in L2
   const auto span = [&](uint64_t kext) {
     if (!emu_k_) return std::max<uint64_t>(kext, 1);

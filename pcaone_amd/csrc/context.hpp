@@ -144,11 +144,12 @@ class Context {
   DevBuf<unsigned char> csg_, csh_;
   DevBuf<double> mpart_;
   int cp_ = 0;
+  int cpitch_ = 0;  // elements between compact source rows (whole 128-byte lines)
   bool miss_f32_ = true;
   uint64_t miss_span(uint64_t nrows, uint64_t kext) const;
   // EMU state
   int emu_k_ = 0;
-  int emu_kp_ = 0;  // row pitch of us_ / vimp_ (emu_k_ rounded up to even)
+  int emu_kp_ = 0;  // row pitch of us_ / vimp_ (whole 128-byte lines, >= emu_k_ + 3)
   DevBuf<double> us_, vimp_;
   // rsvd workspace
   DevBuf<double> omega_, omega_r_, omega_c_, colsum_, bvec_;

@@ -116,14 +116,17 @@ struct MissWeightArgs {
   const double* US;      // N x kk row-major (U * S), row pitch kp, EMU only
   const double* V;       // M x kk row-major, row pitch kp, EMU only
   int kk;
-  int kp;                // even row pitch of US and V (zero padded)
+  int kp;                // row pitch of US and V: whole 128-byte lines; V row j holds sc_j, cF_j, F_j at kk..kk+2
   double* w;             // one weight per list entry
   uint64_t kext, kspan;
 };
 void launch_missing_weights(const MissWeightArgs& a, hipStream_t s);
+// V[j][kk + 0..2] = sc_j, cF_j, F_j (rows of pitch kp, j < m).
+void launch_pack_col_state(double* V, uint64_t m, int kp, int kk, const double* sc, const double* cf, const double* F,
+                           hipStream_t s);
 // Compact copy of the first L columns of rows x lp FP64 rows into rows x cp
 // (FP32 if f32, else FP64), padding columns zero.
-void launch_compact_rows(const double* src, uint64_t rows, int lp, int L, int cp, bool f32, void* out,
+void launch_compact_rows(const double* src, uint64_t rows, int lp, int L, int pitch, bool f32, void* out,
                          hipStream_t s);
 // (2) out[r][:ncols] += sum_e w[e] * src[ix[e] - src_k0][:ncols] over the
 //     entries of row r with index in [k_lo, k_hi).  src is a compact copy
@@ -140,14 +143,17 @@ struct MissSpmmArgs {
   const void* src;
   bool f32;
   uint64_t src_k0;     // index of the first source row
-  int cp;
+  int cp;              // columns read per source row (a multiple of 16 bytes)
+  int cpitch;          // elements between source rows (whole 128-byte lines)
   int ncols;
-  uint64_t kspan;
-  double* partial;
+  uint64_t kspan;      // index chunk (rows of src that fit in L2)
+  double* partial;     // [chunk][nrows][ncols] partial rows when the index range is split
+  uint64_t partial_cap;  // elements of partial
   double* out;         // nrows x lp
   int lp;
 };
-uint64_t missing_spmm_chunks(const MissSpmmArgs& a);
+// Adds the corrections to out; the source rows are gathered from L2 in index
+// chunks of kspan rows (partial rows summed in chunk order).
 void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s);
 
 }  // namespace pcaone

@@ -486,12 +486,14 @@ __global__ void k_missing_weights(MissWeightArgs a) {
 // 1 - F_c) with p = (U S)_i . V_c.  One wave per row of the lists (a SNP c for
 // the by-SNP list, a sample i for the by-sample list) and index chunk
 // (grid.y; chunks keep the gathered rows within L2).  The k-vectors are rows
-// of even pitch kp read as 16-byte pairs: the row's own vector sits in
-// registers, and the wave is split into 64 / LPE groups of LPE lanes (a power
-// of two >= kp / 2), each group taking one entry per step, reading the
-// entry's vector as one contiguous access and reducing the dot product by
-// butterfly shuffles.  The entries' indexes come 64 at a time (one coalesced
-// load, the next batch's fetched during this one).
+// of pitch kp (whole 128-byte lines) read as 16-byte pairs, and a V row also
+// carries its SNP's sc, cF and F, so an entry of the by-sample list touches
+// one line.  The row's own vector sits in registers; the wave is split into
+// 64 / LPE groups of LPE lanes (a power of two >= k / 2), each group taking
+// one entry per step, reading the entry's vector as one contiguous access
+// and reducing the dot product by butterfly shuffles.  The entries' indexes
+// come 64 at a time (one coalesced load, the next batch's fetched during this
+// one).
 __global__ void __launch_bounds__(256) k_missing_weights_emu(MissWeightArgs a, int lpe) {
   const uint64_t r = blockIdx.x * 4ull + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
@@ -499,12 +501,24 @@ __global__ void __launch_bounds__(256) k_missing_weights_emu(MissWeightArgs a, i
   const uint64_t klo = (uint64_t)blockIdx.y * a.kspan, khi = klo + a.kspan;
   uint64_t e0, e1;
   entry_range(a.rp, a.ix, r, klo, khi, e0, e1);
-  const int np = a.kp / 2;  // 16-byte pairs per vector
+  const int kk = a.kk, np = a.kp / 2;  // 16-byte pairs per row
+  const int nk = (kk + 1) / 2;          // pairs holding loadings
   const int G = 64 / lpe, g = lane / lpe, cl = lane - g * lpe;
-  const bool has = cl < np;
-  const double2* own = reinterpret_cast<const double2*>(a.by_snp ? a.V : a.US) + r * np;
+  const bool has = cl < nk;
+  const double* ownrow = (a.by_snp ? a.V : a.US) + r * (uint64_t)a.kp;
   const double2* other = reinterpret_cast<const double2*>(a.by_snp ? a.US : a.V);
-  const double2 rv = has ? own[cl] : double2{0.0, 0.0};
+  // the row's own loadings (a zero past k in the last pair)
+  double2 rv = double2{0.0, 0.0};
+  if (has) {
+    rv.x = ownrow[2 * cl];
+    rv.y = 2 * cl + 1 < kk ? ownrow[2 * cl + 1] : 0.0;
+  }
+  double sc_r = 0.0, cf_r = 0.0, f_r = 0.0;  // the SNP's state for the by-SNP list
+  if (a.by_snp) {
+    sc_r = a.sc[r];
+    cf_r = a.cf[r];
+    f_r = a.Fclamp[r];
+  }
   constexpr int U = 4;
   uint32_t nix = e0 + lane < e1 ? a.ix[e0 + lane] : 0u;
   for (uint64_t eb = e0; eb < e1; eb += 64) {
@@ -513,27 +527,32 @@ __global__ void __launch_bounds__(256) k_missing_weights_emu(MissWeightArgs a, i
     nix = eb + 64 + lane < e1 ? a.ix[eb + 64 + lane] : 0u;
     const int steps = (nb + G - 1) / G;
     for (int st = 0; st < steps; st += U) {
-      uint32_t kk[U];
+      uint32_t kx[U];
       bool ok[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int j = (st + u) * G + g;
-        kk[u] = __shfl(myix, j < 63 ? j : 63);
+        kx[u] = __shfl(myix, j < 63 ? j : 63);
         ok[u] = j < nb;
       }
       double2 x[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        x[u] = has ? other[(ok[u] ? (uint64_t)kk[u] : 0) * np + cl] : double2{0.0, 0.0};
+        x[u] = has ? other[(ok[u] ? (uint64_t)kx[u] : 0) * np + cl] : double2{0.0, 0.0};
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         double p = rv.x * x[u].x + rv.y * x[u].y;
         for (int off = lpe / 2; off > 0; off >>= 1) p += __shfl_xor(p, off);
         const int j = (st + u) * G + g;
         if (cl == 0 && ok[u]) {
-          const uint64_t c = a.by_snp ? r : kk[u];  // SNP column
-          const double sc = a.sc[c], f = a.Fclamp[c];
-          double w = -sc * (0.5 - a.cf[c]);
+          double sc = sc_r, cf = cf_r, f = f_r;
+          if (!a.by_snp) {  // the entry's SNP row carries its state after the loadings
+            const double* vr = a.V + (uint64_t)kx[u] * a.kp + kk;
+            sc = vr[0];
+            cf = vr[1];
+            f = vr[2];
+          }
+          double w = -sc * (0.5 - cf);
           w += sc * fmin(fmax(p, -f), 1.0 - f);
           a.w[eb + j] = w;
         }
@@ -542,13 +561,23 @@ __global__ void __launch_bounds__(256) k_missing_weights_emu(MissWeightArgs a, i
   }
 }
 
-// out (rows x cp, T) = the first L columns of src (rows x lp, FP64), zero padded.
+__global__ void k_pack_col_state(double* __restrict__ V, uint64_t m, int kp, int kk, const double* __restrict__ sc,
+                                 const double* __restrict__ cf, const double* __restrict__ F) {
+  GRID_STRIDE(j, m) {
+    double* vr = V + j * (uint64_t)kp + kk;
+    vr[0] = sc[j];
+    vr[1] = cf[j];
+    vr[2] = F[j];
+  }
+}
+
+// out (rows x pitch, T) = the first L columns of src (rows x lp, FP64), zero padded.
 template <class T>
-__global__ void k_compact_rows(const double* __restrict__ src, uint64_t rows, int lp, int L, int cp,
+__global__ void k_compact_rows(const double* __restrict__ src, uint64_t rows, int lp, int L, int pitch,
                                T* __restrict__ out) {
-  GRID_STRIDE(t, rows * (uint64_t)cp) {
-    const uint64_t r = t / cp;
-    const int c = (int)(t - r * cp);
+  GRID_STRIDE(t, rows * (uint64_t)pitch) {
+    const uint64_t r = t / pitch;
+    const int c = (int)(t - r * pitch);
     out[t] = c < L ? (T)src[r * lp + c] : (T)0;
   }
 }
@@ -572,6 +601,7 @@ __global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
   if (r >= a.nrows) return;  // wave-uniform
   const int lpe = a.cp / VEC;  // lanes per entry (cp is a multiple of VEC)
   const int G = 64 / lpe;      // entries per step
+  const uint64_t rpv = a.cpitch / VEC;  // 16-byte vectors per source row (whole cache lines)
   const int g = lane / lpe, cl = lane - g * lpe;
   const bool act = g < G;
   const uint64_t klo = a.k_lo + (uint64_t)blockIdx.z * a.kspan;
@@ -614,7 +644,7 @@ __global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
       }
       V xv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) xv[u] = src[(ok[u] ? (uint64_t)kk[u] - a.src_k0 : 0) * lpe + cl];
+      for (int u = 0; u < U; ++u) xv[u] = src[(ok[u] ? (uint64_t)kk[u] - a.src_k0 : 0) * rpv + cl];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const T* x = reinterpret_cast<const T*>(&xv[u]);
@@ -925,41 +955,45 @@ void launch_missing_weights(const MissWeightArgs& a, hipStream_t s) {
   if (!a.Fclamp) {
     hipLaunchKernelGGL(k_missing_weights, grid, dim3(256), 0, s, a);
   } else {
-    require(a.kk >= 1 && a.kk <= 128 && a.kp % 2 == 0 && a.kp >= a.kk && a.kp <= 128, kErrArgument,
+    require(a.kk >= 1 && a.kk <= 128 && a.kp % 16 == 0 && a.kp >= a.kk + 3, kErrArgument,
             "missing_weights: imputation rank must be in [1, 128]");
     int lpe = 1;
-    while (lpe < a.kp / 2) lpe *= 2;
+    while (lpe < (a.kk + 1) / 2) lpe *= 2;
     hipLaunchKernelGGL(k_missing_weights_emu, grid, dim3(256), 0, s, a, lpe);
   }
   check_launch("missing_weights");
 }
 
-void launch_compact_rows(const double* src, uint64_t rows, int lp, int L, int cp, bool f32, void* out,
-                         hipStream_t s) {
-  if (!rows) return;
-  const unsigned g = grid_for(rows * (uint64_t)cp);
-  if (f32)
-    hipLaunchKernelGGL(k_compact_rows<float>, dim3(g), dim3(256), 0, s, src, rows, lp, L, cp,
-                       static_cast<float*>(out));
-  else
-    hipLaunchKernelGGL(k_compact_rows<double>, dim3(g), dim3(256), 0, s, src, rows, lp, L, cp,
-                       static_cast<double*>(out));
-  check_launch("compact_rows");
+void launch_pack_col_state(double* V, uint64_t m, int kp, int kk, const double* sc, const double* cf, const double* F,
+                           hipStream_t s) {
+  if (!m) return;
+  hipLaunchKernelGGL(k_pack_col_state, dim3(grid_for(m)), dim3(256), 0, s, V, m, kp, kk, sc, cf, F);
+  check_launch("pack_col_state");
 }
 
-uint64_t missing_spmm_chunks(const MissSpmmArgs& a) {
-  const uint64_t span = a.k_hi > a.k_lo ? a.k_hi - a.k_lo : 0;
-  return a.kspan && span ? (span + a.kspan - 1) / a.kspan : 1;
+void launch_compact_rows(const double* src, uint64_t rows, int lp, int L, int pitch, bool f32, void* out,
+                         hipStream_t s) {
+  if (!rows) return;
+  const unsigned g = grid_for(rows * (uint64_t)pitch);
+  if (f32)
+    hipLaunchKernelGGL(k_compact_rows<float>, dim3(g), dim3(256), 0, s, src, rows, lp, L, pitch,
+                       static_cast<float*>(out));
+  else
+    hipLaunchKernelGGL(k_compact_rows<double>, dim3(g), dim3(256), 0, s, src, rows, lp, L, pitch,
+                       static_cast<double*>(out));
+  check_launch("compact_rows");
 }
 
 void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
   if (!a.nrows || a.k_hi <= a.k_lo) return;
   const int vec = a.f32 ? 4 : 2;
-  require(a.ncols >= 1 && a.ncols <= a.lp && a.cp >= a.ncols && a.cp % vec == 0, kErrArgument,
-          "missing_spmm: bad column counts");
-  const uint64_t nz = missing_spmm_chunks(a);
-  require(nz < 65536 && (nz == 1 || a.partial), kErrArgument, "missing_spmm: bad index chunking");
-  require(a.cp <= 128, kErrArgument, "missing_spmm: at most 128 columns");
+  require(a.ncols >= 1 && a.ncols <= a.lp && a.cp >= a.ncols && a.cp % vec == 0 && a.cp <= 128 &&
+              a.cpitch >= a.cp && a.cpitch % vec == 0,
+          kErrArgument, "missing_spmm: bad column counts");
+  const uint64_t span = a.k_hi - a.k_lo;
+  const uint64_t nz = a.kspan ? (span + a.kspan - 1) / a.kspan : 1;
+  require(nz < 65536 && (nz == 1 || (a.partial && nz * a.nrows * (uint64_t)a.ncols <= a.partial_cap)), kErrInternal,
+          "missing_spmm: bad index chunking");
   const dim3 block(256);
   const unsigned gx = (unsigned)((a.nrows + 3) / 4), gz = (unsigned)nz;
   if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float>), dim3(gx, 1, gz), block, 0, s, a);
