@@ -102,7 +102,9 @@ __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t s
 // correction (G_j = sc_j/4 y_j + sc_j (1/2 - cF_j) colsum), splits sc_j G_j
 // into operand pieces, writes G~ = sum(pieces)/sc_j (FP64, row-major), the
 // pieces transposed through LDS (coalesced along SNPs) and the per-block
-// column sums of beta_j G~_j with beta_j = sc_j (1/2 - cF_j).
+// column sums of beta_j G~_j with beta_j = sc_j (1/2 - cF_j).  Each thread
+// owns 4 adjacent columns (16-byte partial loads) of every RG-th row; its
+// beta sums stay in registers and are combined across row groups in LDS.
 constexpr int kEpiRows = 64;
 
 template <Form F, class PT>
@@ -115,48 +117,67 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
                                         typename std::conditional<F == Form::FP32, float, uint16_t>::type>::type;
   constexpr int P = F == Form::BF16 ? 2 : 1;
   extern __shared__ __align__(16) unsigned char sm[];
-  double* bt = reinterpret_cast<double*>(sm);                                  // kEpiRows x lp
-  w_t* pt = reinterpret_cast<w_t*>(sm + (size_t)kEpiRows * lp * sizeof(double));  // P x lp x kEpiRows
+  w_t* pt = reinterpret_cast<w_t*>(sm);  // P x lp x kEpiRows
+  double* bred = reinterpret_cast<double*>(sm + ((size_t)P * lp * kEpiRows * sizeof(w_t) + 15) / 16 * 16);
+  const int CG = lp / 4, RG = 256 / CG;  // column groups of 4, row groups
+  const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG;
+  const bool act = rg < RG;
   const uint64_t r0 = (uint64_t)blockIdx.x * kEpiRows;
   const int nr = (int)(nwin - r0 < (uint64_t)kEpiRows ? nwin - r0 : kEpiRows);
-  for (int v = threadIdx.x; v < kEpiRows * lp; v += 256) {
-    const int r = v / lp, l = v - r * lp;
-    double bval = 0.0;
-    w_t pc[P];
-    for (int p = 0; p < P; ++p) pc[p] = w_t(0);
-    if (r < nr) {
+  double b4[4] = {0.0, 0.0, 0.0, 0.0};
+  if (act) {
+    for (int r = rg; r < nr; r += RG) {
       const uint64_t j = r0 + r;
-      const uint64_t t = j * lp + l;
-      double y = 0.0;
-      for (uint64_t c = 0; c < nchunks; ++c) y += (double)part[c * stride + t];
-      const double s = sc[j];
-      const double beta = s * (0.5 - cf[j]);
-      const double g = 0.25 * s * y + beta * colsum[l];
-      const double yv = g * s;
-      double back;
-      if constexpr (F == Form::FP64) {
-        pc[0] = yv;
-        back = yv;
-      } else if constexpr (F == Form::FP32) {
-        pc[0] = (float)yv;
-        back = (double)pc[0];
-      } else if constexpr (F == Form::FP16) {
-        pc[0] = fp16_bits(yv);
-        back = fp16_value(pc[0]);
-      } else {
-        pc[0] = bf16_bits((float)yv);
-        back = (double)bf16_value(pc[0]);
-        if constexpr (P == 2) {
-          pc[1] = bf16_bits((float)(yv - back));
-          back += (double)bf16_value(pc[1]);
+      const uint64_t t = j * lp + 4 * cg;
+      double y[4] = {0.0, 0.0, 0.0, 0.0};
+      for (uint64_t c = 0; c < nchunks; ++c) {
+        const PT* pc4 = part + c * stride + t;
+        if constexpr (sizeof(PT) == 4) {
+          const float4 v = *reinterpret_cast<const float4*>(pc4);
+          y[0] += (double)v.x, y[1] += (double)v.y, y[2] += (double)v.z, y[3] += (double)v.w;
+        } else {
+          const double2 v0 = *reinterpret_cast<const double2*>(pc4);
+          const double2 v1 = *reinterpret_cast<const double2*>(pc4 + 2);
+          y[0] += v0.x, y[1] += v0.y, y[2] += v1.x, y[3] += v1.y;
         }
       }
-      const double gr = s != 0.0 ? back / s : 0.0;
-      gout[t] = gr;
-      bval = beta * gr;
+      const double s = sc[j];
+      const double beta = s * (0.5 - cf[j]);
+      double gr4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int l = 4 * cg + q;
+        const double g = 0.25 * s * y[q] + beta * colsum[l];
+        const double yv = g * s;
+        w_t pc[P];
+        double back;
+        if constexpr (F == Form::FP64) {
+          pc[0] = yv;
+          back = yv;
+        } else if constexpr (F == Form::FP32) {
+          pc[0] = (float)yv;
+          back = (double)pc[0];
+        } else if constexpr (F == Form::FP16) {
+          pc[0] = fp16_bits(yv);
+          back = fp16_value(pc[0]);
+        } else {
+          pc[0] = bf16_bits((float)yv);
+          back = (double)bf16_value(pc[0]);
+          if constexpr (P == 2) {
+            pc[1] = bf16_bits((float)(yv - back));
+            back += (double)bf16_value(pc[1]);
+          }
+        }
+        gr4[q] = s != 0.0 ? back / s : 0.0;
+        b4[q] += beta * gr4[q];
+#pragma unroll
+        for (int p = 0; p < P; ++p) pt[(p * lp + l) * kEpiRows + r] = pc[p];
+      }
+      *reinterpret_cast<double2*>(gout + t) = double2{gr4[0], gr4[1]};
+      *reinterpret_cast<double2*>(gout + t + 2) = double2{gr4[2], gr4[3]};
     }
-    bt[r * lp + l] = bval;
-    for (int p = 0; p < P; ++p) pt[(p * lp + l) * kEpiRows + r] = pc[p];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bred[rg * lp + 4 * cg + q] = b4[q];
   }
   __syncthreads();
   w_t* w = static_cast<w_t*>(Wt);
@@ -165,9 +186,9 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
     if (r < nr) w[p * wpiece + l * wpitch + r0 + r] = pt[v];
   }
   for (int l = threadIdx.x; l < lp; l += 256) {
-    double s = 0.0;
-    for (int r = 0; r < nr; ++r) s += bt[r * lp + l];
-    bpart[(uint64_t)blockIdx.x * lp + l] = s;
+    double sum = 0.0;
+    for (int q = 0; q < RG; ++q) sum += bred[q * lp + l];
+    bpart[(uint64_t)blockIdx.x * lp + l] = sum;
   }
 }
 
@@ -785,10 +806,12 @@ static void epi_fused_t(const void* P, uint64_t nchunks, uint64_t stride, uint64
                         uint64_t wpiece, double* bpart, hipStream_t s) {
   constexpr int NP = F == Form::BF16 ? 2 : 1;
   constexpr size_t wsz = F == Form::FP64 ? 8 : (F == Form::FP32 ? 4 : 2);
-  const size_t lds = (size_t)kEpiRows * lp * (sizeof(double) + NP * wsz);
+  // pieces (transposed) + the row groups' beta sums (256 / (lp / 4) groups x lp)
+  const auto lds_for = [](int l) { return ((size_t)kEpiRows * l * NP * wsz + 15) / 16 * 16 + (size_t)1024 * 8; };
+  const size_t lds = lds_for(lp);
   static bool attr = false;
   if (!attr) {
-    constexpr size_t lds_max = (size_t)kEpiRows * 128 * (sizeof(double) + NP * wsz);
+    const size_t lds_max = lds_for(128);
     PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_epi_g_fused<F, PT>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
     attr = true;
