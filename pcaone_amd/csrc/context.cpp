@@ -399,6 +399,12 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   part_.resize(std::max(pg, ph));
   h_version_ = 0;
   gram_version_ = ~0ull;
+  gram_dev_version_ = ~0ull;
+  if (thinq_on_device(lp)) {
+    ldev_.resize(3 * (uint64_t)lp * lp + lp);
+    lstatus_.resize(1);
+    PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
+  }
   ktimes = KernelTimes{};
 }
 
@@ -409,8 +415,8 @@ void Context::set_t(const la::Mat& T) {
   PCAONE_HIP(hipMemcpyAsync(small_.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
 }
 
-la::Mat Context::gram_l(const double* X, uint64_t rows, const double* Y) {
-  double* out = red_.get() + red_cap_;
+la::Mat Context::gram_l(const double* X, uint64_t rows, const double* Y, double* dev_out) {
+  double* out = dev_out ? dev_out : red_.get() + red_cap_;
   launch_gram(X, nullptr, Y, rows, lp, red_.get(), red_cap_, out, stream);
   std::vector<double> h((size_t)lp * lp);
   PCAONE_HIP(hipMemcpyAsync(h.data(), out, h.size() * 8, hipMemcpyDeviceToHost, stream));
@@ -668,6 +674,16 @@ void Context::zero_h(int which) {
   PCAONE_HIP(hipMemsetAsync(b.get(), 0, b.bytes(), stream));
 }
 
+void Context::check_device_status() {
+  if (!lstatus_.get()) return;
+  int st = 0;
+  PCAONE_HIP(hipMemcpy(&st, lstatus_.get(), sizeof(int), hipMemcpyDeviceToHost));
+  if (st) {
+    PCAONE_HIP(hipMemset(lstatus_.get(), 0, sizeof(int)));
+    fail(kErrNumeric, "Cholesky breakdown (Gram matrix not positive definite)");
+  }
+}
+
 void Context::omega_from(int src) {
   require(L > 0, kErrState, "configure() first");
   if (src == 1) {
@@ -675,6 +691,32 @@ void Context::omega_from(int src) {
     ++h_version_;
   }
   const double* X = H_.get();
+  if (thinq_on_device(lp)) {
+    // every step stream-ordered on the device: no host round trip
+    double* gh = ldev_.get();
+    double* g2 = gh + (uint64_t)lp * lp;
+    double* r1 = g2 + (uint64_t)lp * lp;
+    double* dsign = r1 + (uint64_t)lp * lp;
+    if (gram_dev_version_ != h_version_) {
+      launch_gram(X, nullptr, nullptr, N, lp, red_.get(), red_cap_, gh, stream);
+      gram_dev_version_ = h_version_;
+    }
+    launch_thinq_first(gh, L, lp, r1, small_.get(), lstatus_.get(), stream);
+    launch_apply(X, small_.get(), N, lp, tmp_n_.get(), stream);
+    launch_gram(tmp_n_.get(), nullptr, nullptr, N, lp, red_.get(), red_cap_, g2, stream);
+    launch_thinq_second(g2, r1, X, L, lp, small_.get(), lstatus_.get(), stream);
+    launch_apply(X, small_.get(), N, lp, omega_c_.get(), stream);
+    // flipOmg (src/RSVD.hpp:80-89) against Omg2
+    double* norms = red_.get() + red_cap_;
+    launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
+    launch_flip_signs(norms, L, lp, dsign, stream);
+    if (form_g == Form::I8)
+      launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
+    launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
+                       (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
+    launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+    return;
+  }
   la::Mat G1;
   if (gram_version_ == h_version_) {
     G1 = gram_h_;
@@ -724,6 +766,7 @@ EpochResult Context::finish_epoch(int k, bool have_prev) {
   // QR of G twice (src/Halko.cpp:55-65): R = R2 R1, CholeskyQR2 with the
   // second Gram formed on the L x L side: Q1^T Q1 = R1^-T (G^T G) R1^-1.
   const la::Mat G1 = gram_l(gt_.get(), M);
+  check_device_status();  // a breakdown in the device thin-Q of the last epoch
   const la::Mat R1 = la::chol_upper(G1);
   const la::Mat R1i = la::inv_upper(R1);
   const la::Mat R2 = la::chol_upper(la::matmul(la::transpose(R1i), la::matmul(G1, R1i)));
@@ -734,9 +777,12 @@ EpochResult Context::finish_epoch(int k, bool have_prev) {
   if (gram_version_ == h_version_) {
     HH = gram_h_;
   } else {
-    HH = gram_l(H_.get(), N);
+    // the device copy serves the next epoch's thin-Q of H
+    const bool dev = thinq_on_device(lp);
+    HH = gram_l(H_.get(), N, nullptr, dev ? ldev_.get() : nullptr);
     gram_h_ = HH;
     gram_version_ = h_version_;
+    if (dev) gram_dev_version_ = h_version_;
   }
   const la::Mat C = la::matmul(la::transpose(Rinv), la::matmul(HH, Rinv));
   std::vector<double> w;

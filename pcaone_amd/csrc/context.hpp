@@ -170,11 +170,18 @@ class Context {
   std::vector<double> sigma_;
   uint64_t h_version_ = 0, gram_version_ = ~0ull;
   la::Mat gram_h_;    // Gram of H (L x L) for h_version_
+  // device side of the thin-Q of H (lp <= 80): Gram of H for gram_dev_version_,
+  // second Gram, R1, column signs (lp x lp each, signs lp); breakdown status
+  DevBuf<double> ldev_;
+  DevBuf<int> lstatus_;
+  uint64_t gram_dev_version_ = ~0ull;
+  void check_device_status();
 
   void build_derived();  // xt + missing lists from xs
   void pristine_columns();  // cF = F as read, sc = 1 (device)
   bool perm_identity_ = true;  // perm[i] == i
-  la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr);  // L x L block of X^T Y
+  // L x L block of X^T Y; with dev_out the lp x lp result also stays there
+  la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr, double* dev_out = nullptr);
   void omega_pieces();  // tmp_n_ (column-major Omega) -> omega_c_, pieces, Omg2, column sums
   // I8 column scales / epilogue factors of the G (h = 0) or H (h = 1)
   // product's operand; null when that product is not in the I8 form.

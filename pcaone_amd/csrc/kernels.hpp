@@ -86,6 +86,19 @@ void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, doub
 // flipOmg norms: a[l] = sum_i |O2 - Q|, b[l] = sum_i |O2 + Q|.
 void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
                        hipStream_t s);
+// Thin-Q of an N x L matrix X (CholeskyQR2 with Householder column signs) with
+// its L x L steps on the device, for lp <= 80 (thinq_on_device):
+//   first:  R1 = chol(G1 = X^T X), T1 = R1^-1;
+//   second: R2 = chol(G2 = (X T1)^T (X T1)), T = (R2 R1)^-1 diag(s), s the
+//           Householder signs of Xtop (R2 R1)^-1, Xtop the first n rows of X.
+// A Cholesky breakdown sets *status = 1 (the outputs are then zero).
+bool thinq_on_device(int lp);
+void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s);
+void launch_thinq_second(const double* G2, const double* R1, const double* Xtop, int n, int lp, double* T,
+                         int* status, hipStream_t s);
+// flipOmg's column signs from the norms of launch_flip_norms: -1 where
+// ab[l] > 2 ab[lp + l] (l < n), else 1 (lp entries).
+void launch_flip_signs(const double* ab, int n, int lp, double* sign, hipStream_t s);
 // Per-SNP column state: cf = F, sc = 1 for j < m, 0 up to rows.
 void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s);
 // sc_j *= sqrt_ploidy / sqrt(F_j (1 - F_j)) where that sd exceeds tol (j < m).

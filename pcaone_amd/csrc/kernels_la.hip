@@ -697,6 +697,160 @@ __global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
   }
 }
 
+// ---- L x L steps of the thin-Q of H on the device (one workgroup each) ----
+// Matrices are row-major with pitch lp and live in LDS; the routines follow
+// the host ones (linalg_host.cpp) operation by operation except the
+// Cholesky, which is right-looking.
+constexpr int kSmallThreads = 256;
+
+// In place: A (n x n, symmetric positive definite) -> upper R with A = R^T R.
+// Returns false (in every thread) on breakdown.
+__device__ bool wg_chol_upper(double* A, int n, int lp, int* bad) {
+  const int tid = threadIdx.x;
+  if (tid == 0) *bad = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      const double d = A[j * lp + j];
+      if (!(d > 0.0) || !isfinite(d)) *bad = 1;
+      else A[j * lp + j] = sqrt(d);
+    }
+    __syncthreads();
+    if (*bad) return false;
+    const double rjj = A[j * lp + j];
+    for (int i = j + 1 + tid; i < n; i += kSmallThreads) A[j * lp + i] /= rjj;
+    __syncthreads();
+    const int m = n - j - 1;
+    for (int idx = tid; idx < m * m; idx += kSmallThreads) {
+      const int i = j + 1 + idx / m, k = j + 1 + idx % m;
+      if (k >= i) A[i * lp + k] -= A[j * lp + i] * A[j * lp + k];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < n * n; idx += kSmallThreads) {
+    const int i = idx / n, k = idx % n;
+    if (k < i) A[i * lp + k] = 0.0;
+  }
+  __syncthreads();
+  return true;
+}
+
+// X = R^-1 for upper R (column j by thread j).
+__device__ void wg_inv_upper(const double* R, double* X, int n, int lp) {
+  for (int j = threadIdx.x; j < n; j += kSmallThreads) {
+    X[j * lp + j] = 1.0 / R[j * lp + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += R[i * lp + k] * X[k * lp + j];
+      X[i * lp + j] = -s / R[i * lp + i];
+    }
+    for (int i = j + 1; i < n; ++i) X[i * lp + j] = 0.0;
+  }
+  __syncthreads();
+}
+
+// C = A B (n x n), summed in k order.
+__device__ void wg_matmul(const double* A, const double* B, double* C, int n, int lp) {
+  for (int idx = threadIdx.x; idx < n * n; idx += kSmallThreads) {
+    const int i = idx / n, j = idx % n;
+    double c = 0.0;
+    for (int k = 0; k < n; ++k) c += A[i * lp + k] * B[k * lp + j];
+    C[i * lp + j] = c;
+  }
+  __syncthreads();
+}
+
+// Householder diagonal signs of the thin QR whose Q has top block W (n x n,
+// destroyed): modified LU (Ballard et al. 2014), s_i = -sign(u_ii).
+__device__ void wg_householder_signs(double* W, int n, int lp, double* sgn) {
+  const int tid = threadIdx.x;
+  for (int i = 0; i < n; ++i) {
+    if (tid == 0) {
+      const double si = W[i * lp + i] >= 0.0 ? -1.0 : 1.0;
+      W[i * lp + i] -= si;
+      sgn[i] = si;
+    }
+    __syncthreads();
+    const double piv = W[i * lp + i];
+    for (int r = i + 1 + tid; r < n; r += kSmallThreads) W[r * lp + i] /= piv;
+    __syncthreads();
+    const int m = n - i - 1;
+    for (int idx = tid; idx < m * m; idx += kSmallThreads) {
+      const int r = i + 1 + idx / m, c = i + 1 + idx % m;
+      W[r * lp + c] -= W[r * lp + i] * W[i * lp + c];
+    }
+    __syncthreads();
+  }
+}
+
+// First pass: R1 = chol(G1), T1 = R1^-1 (lp x lp, zero padded).
+__global__ void __launch_bounds__(kSmallThreads) k_thinq_first(const double* __restrict__ G1, int n, int lp,
+                                                               double* __restrict__ R1, double* __restrict__ T1,
+                                                               int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char sm[];
+  __shared__ int bad;
+  double* A = reinterpret_cast<double*>(sm);
+  double* X = A + lp * lp;
+  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
+    A[v] = G1[v];
+    X[v] = 0.0;
+  }
+  __syncthreads();
+  const bool ok = wg_chol_upper(A, n, lp, &bad);
+  if (ok) wg_inv_upper(A, X, n, lp);
+  if (!ok && threadIdx.x == 0) *status = 1;
+  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
+    const int i = v / lp, j = v % lp;
+    const bool in = ok && i < n && j < n;
+    R1[v] = in ? A[v] : 0.0;
+    T1[v] = in ? X[v] : 0.0;
+  }
+}
+
+// Second pass: R2 = chol(G2), R^-1 = (R2 R1)^-1, the Householder signs s of
+// Xtop R^-1 (Xtop = the first n rows of the thinned matrix) and
+// T = R^-1 diag(s) (lp x lp, zero padded).
+__global__ void __launch_bounds__(kSmallThreads) k_thinq_second(const double* __restrict__ G2,
+                                                                const double* __restrict__ R1,
+                                                                const double* __restrict__ Xtop, int n, int lp,
+                                                                double* __restrict__ T, int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char sm[];
+  __shared__ int bad;
+  __shared__ double sgn[128];
+  double* M0 = reinterpret_cast<double*>(sm);
+  double* M1 = M0 + lp * lp;
+  double* M2 = M0 + 2 * lp * lp;
+  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
+    M0[v] = G2[v];
+    M1[v] = R1[v];
+    M2[v] = 0.0;
+  }
+  __syncthreads();
+  const bool ok = wg_chol_upper(M0, n, lp, &bad);  // R2
+  if (ok) {
+    wg_matmul(M0, M1, M2, n, lp);  // R = R2 R1
+    for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) M0[v] = 0.0;
+    __syncthreads();
+    wg_inv_upper(M2, M0, n, lp);  // R^-1
+    for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) M1[v] = (v / lp < n) ? Xtop[v] : 0.0;
+    __syncthreads();
+    wg_matmul(M1, M0, M2, n, lp);  // Xtop R^-1
+    wg_householder_signs(M2, n, lp, sgn);
+  }
+  if (!ok && threadIdx.x == 0) *status = 1;
+  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
+    const int i = v / lp, j = v % lp;
+    T[v] = (ok && i < n && j < n) ? M0[v] * sgn[j] : 0.0;
+  }
+}
+
+// flipOmg's decision (src/RSVD.hpp:80-89): column l flips when
+// sum |Omg2 - Q| > 2 sum |Omg2 + Q| (ab = the two norms, 2 x lp).
+__global__ void k_flip_signs(const double* __restrict__ ab, int n, int lp, double* __restrict__ sign) {
+  const int l = threadIdx.x;
+  if (l < lp) sign[l] = (l < n && ab[l] > 2.0 * ab[lp + l]) ? -1.0 : 1.0;
+}
+
 // out[r][l] += sum_z partial[z][r][l] (z in order), l < ncols.
 __global__ void k_sum_chunks(const double* __restrict__ partial, uint64_t nz, uint64_t nrows, int ncols,
                              double* __restrict__ out, int lp) {
@@ -1027,6 +1181,40 @@ void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
                        a.nrows, a.ncols, a.out, a.lp);
     check_launch("sum_chunks");
   }
+}
+
+bool thinq_on_device(int lp) { return lp <= 80; }
+
+void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s) {
+  require(thinq_on_device(lp) && n <= lp, kErrInternal, "thinq: lp too large for the device path");
+  const size_t lds = (size_t)2 * lp * lp * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_first),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * 80 * 80 * sizeof(double))));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_thinq_first, dim3(1), dim3(kSmallThreads), lds, s, G1, n, lp, R1, T1, status);
+  check_launch("thinq_first");
+}
+
+void launch_thinq_second(const double* G2, const double* R1, const double* Xtop, int n, int lp, double* T,
+                         int* status, hipStream_t s) {
+  require(thinq_on_device(lp) && n <= lp, kErrInternal, "thinq: lp too large for the device path");
+  const size_t lds = (size_t)3 * lp * lp * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_second),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(3 * 80 * 80 * sizeof(double))));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_thinq_second, dim3(1), dim3(kSmallThreads), lds, s, G2, R1, Xtop, n, lp, T, status);
+  check_launch("thinq_second");
+}
+
+void launch_flip_signs(const double* ab, int n, int lp, double* sign, hipStream_t s) {
+  hipLaunchKernelGGL(k_flip_signs, dim3(1), dim3(128), 0, s, ab, n, lp, sign);
+  check_launch("flip_signs");
 }
 
 }  // namespace pcaone
