@@ -533,10 +533,8 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   // the K margins around them must read as zero.
   const uint64_t off = j0 % kKStep;
   const uint64_t klen = round_up(off + nwin, kKStep);
-  if (off) PCAONE_HIP(hipMemset2DAsync(wh_.get(), wh_pitch_ * wb, 0, off * wb, (size_t)P * lp, stream));
-  if (klen > off + nwin)
-    PCAONE_HIP(hipMemset2DAsync(wh_.get() + (off + nwin) * wb, wh_pitch_ * wb, 0, (klen - off - nwin) * wb,
-                                (size_t)P * lp, stream));
+  launch_zero_ranges(wh_.get(), wh_pitch_ * wb, (uint64_t)P * lp, 0, off * wb, (off + nwin) * wb,
+                     (klen - off - nwin) * wb, stream);
   if (!nmiss && form_g != Form::I8 && form_h != Form::I8) {
     launch_epi_g_fused(form_h, part_.get(), nch_g, g.ychunk, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                        colsum_.get(), gwin, wh_.get() + off * wb, wh_pitch_, (uint64_t)lp * wh_pitch_, red_.get(),

@@ -170,7 +170,7 @@ class Context {
   std::vector<double> sigma_;
   uint64_t h_version_ = 0, gram_version_ = ~0ull;
   la::Mat gram_h_;    // Gram of H (L x L) for h_version_
-  // device side of the thin-Q of H (lp <= 80): Gram of H for gram_dev_version_,
+  // device side of the thin-Q of H (lp <= 48): Gram of H for gram_dev_version_,
   // second Gram, R1, column signs (lp x lp each, signs lp); breakdown status
   DevBuf<double> ldev_;
   DevBuf<int> lstatus_;

@@ -87,7 +87,7 @@ void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, doub
 void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
                        hipStream_t s);
 // Thin-Q of an N x L matrix X (CholeskyQR2 with Householder column signs) with
-// its L x L steps on the device, for lp <= 80 (thinq_on_device):
+// its L x L steps on the device, for lp <= 48 (thinq_on_device):
 //   first:  R1 = chol(G1 = X^T X), T1 = R1^-1;
 //   second: R2 = chol(G2 = (X T1)^T (X T1)), T = (R2 R1)^-1 diag(s), s the
 //           Householder signs of Xtop (R2 R1)^-1, Xtop the first n rows of X.
@@ -99,6 +99,9 @@ void launch_thinq_second(const double* G2, const double* R1, const double* Xtop,
 // flipOmg's column signs from the norms of launch_flip_norms: -1 where
 // ab[l] > 2 ab[lp + l] (l < n), else 1 (lp entries).
 void launch_flip_signs(const double* ab, int n, int lp, double* sign, hipStream_t s);
+// Zero bytes [b0, b0 + n0) and [b1, b1 + n1) of each of `rows` rows of `pitch` bytes.
+void launch_zero_ranges(void* base, uint64_t pitch, uint64_t rows, uint64_t b0, uint64_t n0, uint64_t b1, uint64_t n1,
+                        hipStream_t s);
 // Per-SNP column state: cf = F, sc = 1 for j < m, 0 up to rows.
 void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s);
 // sc_j *= sqrt_ploidy / sqrt(F_j (1 - F_j)) where that sd exceeds tol (j < m).

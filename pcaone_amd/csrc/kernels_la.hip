@@ -439,6 +439,16 @@ __global__ void k_rows_to_colmajor(const double* __restrict__ src, uint64_t n, i
   }
 }
 
+// Zero bytes [b0, b0 + n0) and [b1, b1 + n1) of each of `rows` rows (pitch bytes).
+__global__ void k_zero_ranges(unsigned char* __restrict__ base, uint64_t pitch, uint64_t rows, uint64_t b0,
+                              uint64_t n0, uint64_t b1, uint64_t n1) {
+  const uint64_t per = n0 + n1;
+  GRID_STRIDE(t, rows * per) {
+    const uint64_t r = t / per, c = t - r * per;
+    base[r * pitch + (c < n0 ? b0 + c : b1 + (c - n0))] = 0;
+  }
+}
+
 // Per-SNP column state as read: cF = F and sc = 1 for j < m, zero padding to rows.
 __global__ void k_pristine_cols(const double* __restrict__ F, uint64_t m, uint64_t rows, double* __restrict__ cf,
                                 double* __restrict__ sc) {
@@ -700,7 +710,10 @@ __global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
 // ---- L x L steps of the thin-Q of H on the device (one workgroup each) ----
 // Matrices are row-major with pitch lp and live in LDS; the routines follow
 // the host ones (linalg_host.cpp) operation by operation except the
-// Cholesky, which is right-looking.
+// Cholesky, which is right-looking.  Four waves run each kernel (one wave
+// leaves the LDS latency of the inner loops exposed); the 3 L barriers of the
+// Cholesky and the Householder steps bound it, so the device path is taken
+// for lp <= 48 only, where it is cheaper than the host's round trips.
 constexpr int kSmallThreads = 256;
 
 // In place: A (n x n, symmetric positive definite) -> upper R with A = R^T R.
@@ -869,6 +882,14 @@ void launch_colmajor_to_rows(const double* src, uint64_t n, int L, int lp, uint6
                              hipStream_t s) {
   hipLaunchKernelGGL(k_colmajor_to_rows, dim3(grid_for(rows_pad * lp)), dim3(256), 0, s, src, n, L, lp, rows_pad, dst);
   check_launch("colmajor_to_rows");
+}
+
+void launch_zero_ranges(void* base, uint64_t pitch, uint64_t rows, uint64_t b0, uint64_t n0, uint64_t b1, uint64_t n1,
+                        hipStream_t s) {
+  if (!rows || n0 + n1 == 0) return;
+  hipLaunchKernelGGL(k_zero_ranges, dim3(grid_for(rows * (n0 + n1))), dim3(256), 0, s,
+                     static_cast<unsigned char*>(base), pitch, rows, b0, n0, b1, n1);
+  check_launch("zero_ranges");
 }
 
 void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s) {
@@ -1183,7 +1204,7 @@ void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
   }
 }
 
-bool thinq_on_device(int lp) { return lp <= 80; }
+bool thinq_on_device(int lp) { return lp <= 48; }
 
 void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s) {
   require(thinq_on_device(lp) && n <= lp, kErrInternal, "thinq: lp too large for the device path");
@@ -1191,7 +1212,7 @@ void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1,
   static bool attr = false;
   if (!attr) {
     PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_first),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * 80 * 80 * sizeof(double))));
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * 48 * 48 * sizeof(double))));
     attr = true;
   }
   hipLaunchKernelGGL(k_thinq_first, dim3(1), dim3(kSmallThreads), lds, s, G1, n, lp, R1, T1, status);
@@ -1205,7 +1226,7 @@ void launch_thinq_second(const double* G2, const double* R1, const double* Xtop,
   static bool attr = false;
   if (!attr) {
     PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_second),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(3 * 80 * 80 * sizeof(double))));
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(3 * 48 * 48 * sizeof(double))));
     attr = true;
   }
   hipLaunchKernelGGL(k_thinq_second, dim3(1), dim3(kSmallThreads), lds, s, G2, R1, Xtop, n, lp, T, status);

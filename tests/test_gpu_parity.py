@@ -169,6 +169,22 @@ def test_missing_entries_with_chunked_sources(ctx, emu):
     np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-8)
 
 
+@pytest.mark.parametrize("k", [8, 30])
+def test_thin_q_paths_match_oracle(ctx, golden_inputs, k):
+    """The Omega update (thin-Q of H with Householder signs, then flipOmg) runs
+    its L x L steps on the device for small L (k = 8: L = 16) and on the host
+    for large L (k = 30: L = 60); both match the FP64 oracle over 5 epochs."""
+    say(f"thin-Q path k={k}")
+    prefix = golden_inputs["cfg1"]
+    packed, n, m = O.read_plink(prefix)
+    kw = dict(svd=1, k=k, oversamples=k, maxp=4, tol=0.0)
+    ref = O.run_pca_with_halko(O.Data.from_packed(packed, n, O.Params(**kw)))
+    ctx.read_bed(prefix)
+    r = ctx.run(api.default_params(precision="fp64", **kw))
+    np.testing.assert_allclose(r.S, ref.S, rtol=1e-9)
+    np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-7)
+
+
 # ---------------------------------------------------------------- whole runs vs golden
 def run_case(ctx, golden_inputs, case, form):
     meta = load_meta(case)
