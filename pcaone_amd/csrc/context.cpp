@@ -313,6 +313,43 @@ void Context::set_impute(const double* U, const double* S, const double* V, int 
   weights_dirty_ = true;
 }
 
+void Context::set_impute_device(const double* S, int kk) {
+  require(kk > 0 && kk <= L, kErrArgument, "impute rank must be in [1, L]");
+  const int kp = (kk + 3 + 15) / 16 * 16;  // as set_impute
+  double* dS = small_.get() + 128 * 128 + 256;
+  PCAONE_HIP(hipMemcpyAsync(dS, S, kk * sizeof(double), hipMemcpyHostToDevice, stream));
+  us_.resize((uint64_t)N * kp);
+  vimp_.resize(M * (uint64_t)kp);
+  launch_impute_rows(ucur_.get(), N, lp, kk, kp, dS, us_.get(), stream);
+  launch_impute_rows(vbuf_.get(), M, lp, kk, kp, nullptr, vimp_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));  // dS came from caller memory
+  emu_k_ = kk;
+  emu_kp_ = kp;
+  weights_dirty_ = true;
+}
+
+void Context::flip_uv_device(int k) {
+  require(k >= 1 && k <= L, kErrArgument, "k out of range");
+  launch_flip_cols_by_max(vbuf_.get(), M, ucur_.get(), N, lp, k, red_.get(), small_.get() + 128 * 128 + 128, stream);
+}
+
+void Context::save_vpre() {
+  vpre_.resize(vbuf_.size());
+  PCAONE_HIP(hipMemcpyAsync(vpre_.get(), vbuf_.get(), vbuf_.bytes(), hipMemcpyDeviceToDevice, stream));
+}
+
+double Context::mev_vpre(int k) {
+  // mev(V, Vpre) = (1/k) sum_i || V^T vpre_i ||   (src/Utils.cpp:194-200)
+  const la::Mat G = gram_l(vbuf_.get(), M, vpre_.get());
+  double m = 0.0;
+  for (int i = 0; i < k; ++i) {
+    double s2 = 0.0;
+    for (int a = 0; a < k; ++a) s2 += G(a, i) * G(a, i);
+    m += std::sqrt(s2);
+  }
+  return m / k;
+}
+
 void Context::clear_impute() {
   if (emu_k_) weights_dirty_ = true;
   emu_k_ = 0;

@@ -71,6 +71,13 @@ class Context {
   void permute_columns(const std::vector<int>& perm);   // column i <- column perm[i] (RSVD.hpp:61-78)
   void standardize(int ploidy);                         // Data.cpp:351-362 semantics
   void set_impute(const double* U, const double* S, const double* V, int kk);  // Data.cpp:318-348 (column order)
+  // The same from the device's own U (ucur) and V (vbuf), S on the host.
+  void set_impute_device(const double* S, int kk);
+  // EM bookkeeping on device-resident U and V (src/Halko.cpp:295-312):
+  // flip_UV with V as the base, the previous V kept, mev(V, Vpre).
+  void flip_uv_device(int k);
+  void save_vpre();
+  double mev_vpre(int k);
   void clear_impute();
   // Back to the state right after reading: original SNP order, centered and
   // unscaled, no imputation (each run starts from freshly read data).
@@ -160,7 +167,7 @@ class Context {
   DevBuf<unsigned long long> i8max_;
   DevBuf<unsigned char> wg_, wh_;
   uint64_t wg_pitch_ = 0, wh_pitch_ = 0;
-  DevBuf<double> gt_, H_, H1_, H2_, tmp_n_, tmp_m_, ucur_, upre_, vbuf_;
+  DevBuf<double> gt_, H_, H1_, H2_, tmp_n_, tmp_m_, ucur_, upre_, vbuf_, vpre_;
   DevBuf<unsigned char> part_;
   DevBuf<double> small_, red_;
   uint64_t red_cap_ = 0;

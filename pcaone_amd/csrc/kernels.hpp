@@ -102,6 +102,15 @@ void launch_flip_signs(const double* ab, int n, int lp, double* sign, hipStream_
 // Zero bytes [b0, b0 + n0) and [b1, b1 + n1) of each of `rows` rows of `pitch` bytes.
 void launch_zero_ranges(void* base, uint64_t pitch, uint64_t rows, uint64_t b0, uint64_t n0, uint64_t b1, uint64_t n1,
                         hipStream_t s);
+// flip_UV with V as the base (src/Utils.cpp:118-154) on device-resident
+// V (m x lp) and U (n x lp), row-major: column c < k of both changes sign when
+// V's entry of largest magnitude (the first such) is negative.  partial:
+// 2048 x lp scratch; sign: lp.
+void launch_flip_cols_by_max(double* V, uint64_t m, double* U, uint64_t n, int lp, int k, double* partial,
+                             double* sign, hipStream_t s);
+// dst (rows x kp) = first k columns of src (rows x lp) times scale[c] (null = 1), zero padded.
+void launch_impute_rows(const double* src, uint64_t rows, int lp, int k, int kp, const double* scale, double* dst,
+                        hipStream_t s);
 // Per-SNP column state: cf = F, sc = 1 for j < m, 0 up to rows.
 void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s);
 // sc_j *= sqrt_ploidy / sqrt(F_j (1 - F_j)) where that sd exceeds tol (j < m).

@@ -449,6 +449,55 @@ __global__ void k_zero_ranges(unsigned char* __restrict__ base, uint64_t pitch, 
   }
 }
 
+// flip_UV with V as the base (src/Utils.cpp:118-154): per column c < k of X
+// (rows [0, n), pitch lp) the value at its first maximum of |x|.  Stage 1:
+// block b scans rows [b rpb, ...) in order; stage 2 takes the blocks in
+// order, so the first maximum wins ties as Eigen's maxCoeff does.
+__global__ void k_col_absmax_partial(const double* __restrict__ X, uint64_t n, int lp, int k, uint64_t rpb,
+                                     double* __restrict__ pval) {
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+  const uint64_t r1 = r0 + rpb < n ? r0 + rpb : n;
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    double best = -1.0, v = 0.0;
+    for (uint64_t r = r0; r < r1; ++r) {
+      const double x = X[r * lp + c];
+      if (fabs(x) > best) best = fabs(x), v = x;
+    }
+    pval[(uint64_t)blockIdx.x * lp + c] = v;
+  }
+}
+
+__global__ void k_col_absmax_sign(const double* __restrict__ pval, uint64_t nb, int lp, int k,
+                                  double* __restrict__ sign) {
+  const int c = threadIdx.x;
+  if (c >= lp) return;
+  double s = 1.0;
+  if (c < k) {
+    double best = -1.0, v = 0.0;
+    for (uint64_t b = 0; b < nb; ++b) {
+      const double x = pval[b * lp + c];
+      if (fabs(x) > best) best = fabs(x), v = x;
+    }
+    s = v < 0.0 ? -1.0 : 1.0;
+  }
+  sign[c] = s;
+}
+
+__global__ void k_scale_cols(double* __restrict__ X, uint64_t n, int lp, const double* __restrict__ sign) {
+  GRID_STRIDE(t, n * (uint64_t)lp) X[t] *= sign[t % lp];
+}
+
+// dst (rows x kp) = the first k columns of src (rows x lp), times scale[t]
+// (null = 1), zero padded.
+__global__ void k_impute_rows(const double* __restrict__ src, uint64_t rows, int lp, int k, int kp,
+                              const double* __restrict__ scale, double* __restrict__ dst) {
+  GRID_STRIDE(t, rows * (uint64_t)kp) {
+    const uint64_t r = t / kp;
+    const int c = (int)(t - r * kp);
+    dst[t] = c < k ? src[r * lp + c] * (scale ? scale[c] : 1.0) : 0.0;
+  }
+}
+
 // Per-SNP column state as read: cF = F and sc = 1 for j < m, zero padding to rows.
 __global__ void k_pristine_cols(const double* __restrict__ F, uint64_t m, uint64_t rows, double* __restrict__ cf,
                                 double* __restrict__ sc) {
@@ -890,6 +939,26 @@ void launch_zero_ranges(void* base, uint64_t pitch, uint64_t rows, uint64_t b0, 
   hipLaunchKernelGGL(k_zero_ranges, dim3(grid_for(rows * (n0 + n1))), dim3(256), 0, s,
                      static_cast<unsigned char*>(base), pitch, rows, b0, n0, b1, n1);
   check_launch("zero_ranges");
+}
+
+void launch_flip_cols_by_max(double* V, uint64_t m, double* U, uint64_t n, int lp, int k, double* partial,
+                             double* sign, hipStream_t s) {
+  const uint64_t nb = std::min<uint64_t>(2048, std::max<uint64_t>(1, m));
+  const uint64_t rpb = (m + nb - 1) / nb;
+  const uint64_t nbu = (m + rpb - 1) / rpb;
+  hipLaunchKernelGGL(k_col_absmax_partial, dim3((unsigned)nbu), dim3(64), 0, s, V, m, lp, k, rpb, partial);
+  hipLaunchKernelGGL(k_col_absmax_sign, dim3(1), dim3(128), 0, s, partial, nbu, lp, k, sign);
+  hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(m * lp)), dim3(256), 0, s, V, m, lp, sign);
+  hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(n * lp)), dim3(256), 0, s, U, n, lp, sign);
+  check_launch("flip_cols_by_max");
+}
+
+void launch_impute_rows(const double* src, uint64_t rows, int lp, int k, int kp, const double* scale, double* dst,
+                        hipStream_t s) {
+  if (!rows) return;
+  hipLaunchKernelGGL(k_impute_rows, dim3(grid_for(rows * (uint64_t)kp)), dim3(256), 0, s, src, rows, lp, k, kp, scale,
+                     dst);
+  check_launch("impute_rows");
 }
 
 void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s) {

@@ -48,7 +48,12 @@ void RsvdOpData::fit_with_pi() {
   // EMU branch only runs with --emu; the device recomputes the imputed values
   // inside every product from (U S, V).
   if (!params_.emu) return;
-  require(!U.empty() && !S.empty() && !V.empty(), kErrState, "fit_with_pi before the first decomposition");
+  require(!S.empty(), kErrState, "fit_with_pi before the first decomposition");
+  if (device_uv) {
+    ctx_.set_impute_device(S.data(), (int)S.size());
+    return;
+  }
+  require(!U.empty() && !V.empty(), kErrState, "fit_with_pi before the first decomposition");
   ctx_.set_impute(U.data(), S.data(), V.data(), (int)S.size());
 }
 
@@ -71,7 +76,7 @@ void RsvdOpData::computeUSV(int p, double tol) {
         } else {
           S.assign(r.sigma.begin(), r.sigma.begin() + nk);
           ctx_.compute_v(nk);
-          if (need_v_host) {
+          if (need_v_host && !device_uv) {
             V.resize(rows() * (uint64_t)nk);
             ctx_.download_v(nk, V.data());
           }
@@ -84,6 +89,7 @@ void RsvdOpData::computeUSV(int p, double tol) {
       ctx_.save_prev();
     }
   }
+  if (device_uv) return;  // U stays in the device's ucur
   if (u_out && !params_.missme) {
     ctx_.download_u(nk, u_out);  // straight into the caller's N x k buffer
   } else {
@@ -145,37 +151,6 @@ void FancyRsvdOpData::computeGandH(int pi) {
   }
 }
 
-void flip_uv_vbase(std::vector<double>& U, std::vector<double>& V, uint64_t n, uint64_t m, int k) {
-  for (int i = 0; i < k; ++i) {
-    double* v = V.data() + (uint64_t)i * m;
-    uint64_t x = 0;
-    double best = -1.0;
-    for (uint64_t r = 0; r < m; ++r)
-      if (std::fabs(v[r]) > best) best = std::fabs(v[r]), x = r;  // first maximum, as Eigen maxCoeff
-    if (v[x] < 0) {
-      for (uint64_t r = 0; r < m; ++r) v[r] = -v[r];
-      double* u = U.data() + (uint64_t)i * n;
-      for (uint64_t r = 0; r < n; ++r) u[r] = -u[r];
-    }
-  }
-}
-
-double mev_cols(const std::vector<double>& X, const std::vector<double>& Y, uint64_t n, int k) {
-  double res = 0.0;
-  for (int i = 0; i < k; ++i) {
-    double s2 = 0.0;
-    for (int a = 0; a < k; ++a) {
-      double d = 0.0;
-      const double* x = X.data() + (uint64_t)a * n;
-      const double* y = Y.data() + (uint64_t)i * n;
-      for (uint64_t r = 0; r < n; ++r) d += x[r] * y[r];
-      s2 += d * d;
-    }
-    res += std::sqrt(s2);
-  }
-  return res / k;
-}
-
 PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out) {
   // src/Halko.cpp:271-345 (genetic input, in-core, no LD output)
   ctx.reset_columns();  // restores the as-read state between runs; not part of the timed work
@@ -195,15 +170,18 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, d
     rsvd->computeUSV((int)params.maxp, params.tol);
   } else {
     if (ctx.nmiss == 0 && params.verbose) std::printf("warning: there is no missing values\n");
+    // U and V stay on the device through the EM iterations (flip_UV, mev
+    // and the imputation inputs computed there); read back once at the end
+    rsvd->device_uv = true;
     rsvd->setFlags(false, false);
     rsvd->computeUSV((int)params.maxp, params.tol);
-    flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
+    ctx.flip_uv_device(k);
     for (uint32_t i = 0; i < params.maxiter; ++i) {
       rsvd->setFlags(true, false);
-      const std::vector<double> Vpre = rsvd->V;
+      ctx.save_vpre();
       rsvd->computeUSV((int)params.maxp, params.tol);
-      flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
-      const double diff = 1.0 - mev_cols(rsvd->V, Vpre, M, k);
+      ctx.flip_uv_device(k);
+      const double diff = 1.0 - ctx.mev_vpre(k);
       if (info) info->em_trace.push_back(diff), info->em_iters = (int)i + 1;
       if (params.verbose) {
         std::printf("individual allele frequencies estimated iter = %u, diff = %f\n", i + 1, diff);
@@ -214,8 +192,16 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, d
     if (params.emu) {
       rsvd->setFlags(true, true);
       rsvd->computeUSV((int)params.maxp, params.tol);
-      flip_uv_vbase(rsvd->U, rsvd->V, N, M, k);
+      ctx.flip_uv_device(k);
     }
+    if (U_out) {
+      ctx.download_u(k, U_out);
+    } else {
+      rsvd->U.resize(N * (uint64_t)k);
+      ctx.download_u(k, rsvd->U.data());
+    }
+    rsvd->V.resize(M * (uint64_t)k);
+    ctx.download_v(k, rsvd->V.data());
   }
   PcaResult res;
   res.k = k;

@@ -59,6 +59,7 @@ class RsvdOpData {
   std::vector<double> U, S, V;  // N x k, k, M x k (column-major; V in device column order)
   bool need_v_host = false;
   double* u_out = nullptr;  // when set (and no EMU), U is written there instead of into U
+  bool device_uv = false;   // U and V stay on the device (EM iterations); U, V left empty
   RunInfo* info = nullptr;
 
  protected:
@@ -93,8 +94,5 @@ struct PcaResult {
 // U_out (nullable, N x k column-major): receives U directly; res.U is then empty.
 PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out = nullptr);
 
-// flip_UV(U, V, ubase=false) and mev (src/Utils.cpp:118-154, 194-200), column-major.
-void flip_uv_vbase(std::vector<double>& U, std::vector<double>& V, uint64_t n, uint64_t m, int k);
-double mev_cols(const std::vector<double>& X, const std::vector<double>& Y, uint64_t n, int k);
 
 }  // namespace pcaone
