@@ -76,7 +76,7 @@ struct RowItem {
   uint64_t row0;
 };
 
-constexpr int kBatchEpiRows = 64;     // rows per G-epilogue item
+constexpr int kBatchEpiRows = 128;    // rows per G-epilogue item
 constexpr int kBatchRowsPerItem = 1024;  // rows per reduction / elementwise item
 
 struct BatchArgs {

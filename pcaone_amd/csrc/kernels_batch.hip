@@ -48,32 +48,58 @@ __device__ __forceinline__ double split(double x, piece_t<F> (&pc)[2]) {
 }
 
 // ---------------------------------------------------------------- Omega pieces
+// Rows of a piece tile in LDS: one element of padding keeps the transposed
+// writes (consecutive columns) on distinct banks.
+template <class T>
+constexpr int tile_pitch(int rows) {
+  return rows + (sizeof(T) == 2 ? 2 : 1);
+}
+
+// Omega (signed by the flipOmg signs or not) -> the FP64 copy, the value its
+// pieces sum to and the transposed pieces, in tiles of 64 rows: row-major
+// reads and writes, the pieces transposed through LDS.
 template <Form F>
 __global__ void __launch_bounds__(256) k_b_make_pieces(BatchArgs a, const RowItem* __restrict__ items, int srcsel,
                                                        int signed_) {
+  constexpr int P = pieces<F>();
+  constexpr int TR = 64;
+  using w_t = piece_t<F>;
+  constexpr int TRP = tile_pitch<w_t>(TR);
   const RowItem it = items[blockIdx.x];
   const BatchCohort& c = a.coh[it.cohort];
   if (c.st->state != 1) return;
+  __shared__ w_t pt[P * LP * TRP];
   const double* src = c.mat[srcsel];
   double* o2 = c.mat[kBufOmega];
   double* orr = c.mat[kBufOmegaR];
-  piece_t<F>* w = static_cast<piece_t<F>*>(c.wg);
+  w_t* w = static_cast<w_t*>(c.wg);
   const uint64_t wpiece = (uint64_t)LP * c.wg_pitch;
-  const int n = (int)it.nrows * LP;
-  for (int v = threadIdx.x; v < n; v += 256) {
-    const int l = v / (int)it.nrows;
-    const uint64_t r = it.row0 + (uint64_t)(v - l * (int)it.nrows);
-    double x = 0.0;
-    if (l < a.L) {
-      x = src[r * LP + l];
-      if (signed_) x *= c.st->colsign[l];
-    }
-    piece_t<F> pc[2];
-    const double back = split<F>(x, pc);
-    o2[r * LP + l] = x;
-    orr[r * LP + l] = back;
+  const int l = threadIdx.x % LP, r0 = threadIdx.x / LP;
+  const double sgn = (signed_ && l < a.L) ? c.st->colsign[l] : 1.0;
+  for (uint32_t t0 = 0; t0 < it.nrows; t0 += TR) {
+    const int nr = (int)min<uint32_t>(TR, it.nrows - t0);
+    for (int r = r0; r < TR; r += 256 / LP) {
+      w_t pc[2] = {w_t(0), w_t(0)};
+      if (r < nr) {
+        const uint64_t t = (it.row0 + t0 + r) * LP + l;
+        double x = 0.0;
+        if (l < a.L) {
+          x = src[t];
+          if (signed_) x *= sgn;
+        }
+        const double back = split<F>(x, pc);
+        o2[t] = x;
+        orr[t] = back;
+      }
 #pragma unroll
-    for (int p = 0; p < pieces<F>(); ++p) w[p * wpiece + l * c.wg_pitch + r] = pc[p];
+      for (int p = 0; p < P; ++p) pt[(p * LP + l) * TRP + r] = pc[p];
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < P * LP * TR; v += 256) {
+      const int r = v % TR, pl = v / TR, p = pl / LP, ll = pl - p * LP;
+      if (r < nr) w[p * wpiece + ll * c.wg_pitch + it.row0 + t0 + r] = pt[pl * TRP + r];
+    }
+    __syncthreads();
   }
 }
 
@@ -139,59 +165,86 @@ __global__ void __launch_bounds__(256) k_b_sum_items(BatchArgs a, const double* 
 }
 
 // ---------------------------------------------------------------- epilogues
+// Split-K partial element 4 l4 .. 4 l4 + 3 of one row, summed in FP64.
+__device__ __forceinline__ void add4(const float* p, double (&y)[4]) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  y[0] += (double)v.x, y[1] += (double)v.y, y[2] += (double)v.z, y[3] += (double)v.w;
+}
+__device__ __forceinline__ void add4(const double* p, double (&y)[4]) {
+  const double2 v0 = *reinterpret_cast<const double2*>(p), v1 = *reinterpret_cast<const double2*>(p + 2);
+  y[0] += v0.x, y[1] += v0.y, y[2] += v1.x, y[3] += v1.y;
+}
+
 // G epilogue for kBatchEpiRows rows of one cohort: sums the split-K partials
 // in FP64, applies G_j = sc_j y / 4 + sc_j (1/2 - cF_j) colsum, writes G~ (the
 // value the operand pieces sum to, over sc_j), the transposed pieces of
-// diag(sc) G~ and this item's share of the b-vector.
+// diag(sc) G~ and this item's share of the b-vector.  Each thread owns 4
+// adjacent columns of every (256 / (LP / 4))-th row; its b terms stay in
+// registers and the row slots are combined in order at the end.
 template <Form F, class PT>
 __global__ void __launch_bounds__(256) k_b_epi_g(BatchArgs a, const RowItem* __restrict__ items,
                                                  double* __restrict__ bpart) {
   constexpr int P = pieces<F>();
+  constexpr int CG = LP / 4, RG = 256 / CG;
   using w_t = piece_t<F>;
+  constexpr int TRP = tile_pitch<w_t>(kBatchEpiRows);
   const RowItem it = items[blockIdx.x];
   const BatchCohort& c = a.coh[it.cohort];
   if (c.st->state != 1) return;
-  __shared__ double bt[kBatchEpiRows * LP];
-  __shared__ w_t pt[P * LP * kBatchEpiRows];
+  __shared__ w_t pt[P * LP * TRP];
+  __shared__ double bred[RG * LP];
   const PT* part = static_cast<const PT*>(c.part);
   double* gout = c.mat[kBufG];
   const int nr = (int)it.nrows;
-  for (int v = threadIdx.x; v < kBatchEpiRows * LP; v += 256) {
-    const int r = v / LP, l = v - r * LP;
-    double bval = 0.0;
-    w_t pc[2] = {w_t(0), w_t(0)};
+  const int l0 = 4 * (threadIdx.x % CG), rs = threadIdx.x / CG;
+  double cs[4], bs[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) cs[q] = c.colsum[l0 + q];
+  for (int r = rs; r < kBatchEpiRows; r += RG) {
+    w_t pc[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pc[q][0] = pc[q][1] = w_t(0);
     if (r < nr) {
       const uint64_t j = it.row0 + r;
-      const uint64_t t = j * LP + l;
-      double y = 0.0;
-      for (uint64_t ch = 0; ch < c.nch_g; ++ch) y += (double)part[ch * c.ychunk_g + t];
+      const uint64_t t = j * LP + l0;
+      double y[4] = {0.0, 0.0, 0.0, 0.0};
+      for (uint64_t ch = 0; ch < c.nch_g; ++ch) add4(part + ch * c.ychunk_g + t, y);
       const double s = c.sc[j];
       const double beta = s * (0.5 - c.cf[j]);
-      const double g = 0.25 * s * y + beta * c.colsum[l];
-      const double back = split<F>(g * s, pc);
-      const double gr = s != 0.0 ? back / s : 0.0;
-      gout[t] = gr;
-      bval = beta * gr;
-    }
-    bt[r * LP + l] = bval;
+      double gr[4];
 #pragma unroll
-    for (int p = 0; p < P; ++p) pt[(p * LP + l) * kBatchEpiRows + r] = pc[p];
+      for (int q = 0; q < 4; ++q) {
+        const double g = 0.25 * s * y[q] + beta * cs[q];
+        const double back = split<F>(g * s, pc[q]);
+        gr[q] = s != 0.0 ? back / s : 0.0;
+        bs[q] += beta * gr[q];
+      }
+      *reinterpret_cast<double2*>(gout + t) = double2{gr[0], gr[1]};
+      *reinterpret_cast<double2*>(gout + t + 2) = double2{gr[2], gr[3]};
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int p = 0; p < P; ++p) pt[(p * LP + l0 + q) * TRP + r] = pc[q][p];
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) bred[rs * LP + l0 + q] = bs[q];
   __syncthreads();
   w_t* w = static_cast<w_t*>(c.wh);
   const uint64_t wpiece = (uint64_t)LP * c.wh_pitch;
   for (int v = threadIdx.x; v < P * LP * kBatchEpiRows; v += 256) {
     const int r = v % kBatchEpiRows, pl = v / kBatchEpiRows, p = pl / LP, l = pl - p * LP;
-    if (r < nr) w[p * wpiece + l * c.wh_pitch + it.row0 + r] = pt[v];
+    if (r < nr) w[p * wpiece + l * c.wh_pitch + it.row0 + r] = pt[pl * TRP + r];
   }
   if (threadIdx.x < LP) {
     const int l = threadIdx.x;
     double s = 0.0;
-    for (int r = 0; r < nr; ++r) s += bt[r * LP + l];
+    for (int q = 0; q < RG; ++q) s += bred[q * LP + l];
     bpart[(uint64_t)blockIdx.x * LP + l] = s;
   }
 }
 
+// H = y / 4 + b over the item's rows, 4 adjacent elements per thread.
 template <class PT>
 __global__ void __launch_bounds__(256) k_b_epi_h(BatchArgs a, const RowItem* __restrict__ items) {
   const RowItem it = items[blockIdx.x];
@@ -199,13 +252,17 @@ __global__ void __launch_bounds__(256) k_b_epi_h(BatchArgs a, const RowItem* __r
   if (c.st->state != 1) return;
   const PT* part = static_cast<const PT*>(c.part);
   double* h = c.mat[kBufH];
-  const int n = (int)it.nrows * LP;
-  for (int v = threadIdx.x; v < n; v += 256) {
-    const uint64_t t = it.row0 * LP + v;
-    const int l = v % LP;
-    double y = 0.0;
-    for (uint64_t ch = 0; ch < c.nch_h; ++ch) y += (double)part[ch * c.ychunk_h + t];
-    h[t] = 0.25 * y + c.bvec[l];
+  const int n4 = (int)it.nrows * (LP / 4);
+  const int l0 = 4 * (threadIdx.x % (LP / 4));  // the same columns on every pass (256 is a multiple of LP / 4)
+  double b[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) b[q] = c.bvec[l0 + q];
+  for (int v = threadIdx.x; v < n4; v += 256) {
+    const uint64_t t = it.row0 * LP + 4 * (uint64_t)v;
+    double y[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t ch = 0; ch < c.nch_h; ++ch) add4(part + ch * c.ychunk_h + t, y);
+    *reinterpret_cast<double2*>(h + t) = double2{0.25 * y[0] + b[0], 0.25 * y[1] + b[1]};
+    *reinterpret_cast<double2*>(h + t + 2) = double2{0.25 * y[2] + b[2], 0.25 * y[3] + b[3]};
   }
 }
 
