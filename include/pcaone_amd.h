@@ -75,6 +75,10 @@ typedef struct pcaone_params {
   int waves_g, waves_h; /* GEMM waves per workgroup, 4 or 8 (0 = tuned default)     */
   uint64_t kchunk_g;    /* split-K chunk of the G product in codes (0 = auto)       */
   uint64_t kchunk_h;    /* split-K chunk of the H product in codes (0 = auto)       */
+  double memory;        /* -m/--memory in GB: > 0 runs the out-of-core schedule     */
+                        /* (block plan, no flipOmg for --svd 1, blocks as winSVD    */
+                        /* windows, permute_plink order); the genotypes stay in HBM */
+  uint32_t buffer;      /* --buffer (2), GB of the permute_plink read buffer        */
 } pcaone_params;
 
 typedef struct pcaone_run_info {
@@ -236,6 +240,15 @@ int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix);
  * codes). */
 int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
                        const double* W, uint32_t L, double* Y);
+/* Data::prepare's out-of-core block plan (src/Data.cpp:42-84) for these
+ * params (memory > 0): block size, block count, winSVD band factor and the
+ * first SNP of each block (up to cap values into start, nullable). */
+int pcaone_block_plan(const pcaone_params* p, uint32_t nsamples, uint64_t nsnps, uint64_t* blocksize,
+                      uint64_t* nblocks, uint64_t* band_factor, uint64_t* start, uint64_t cap);
+/* permute_plink's SNP order (src/FilePlink.cpp:303-408): position i of the
+ * rewritten file holds original SNP order[i] (nsnps values). */
+int pcaone_permute_plink_order(uint64_t nsamples, uint64_t nsnps, uint32_t buffer_gb, uint32_t bands,
+                               int32_t* order);
 /* RsvdOpData::initOmg's matrix (src/Halko.cpp:15-23): nrows x ncols,
  * column-major, identical to the libstdc++ draw, computed in parallel. */
 int pcaone_host_omega(int seed, int gaussian, uint64_t nrows, uint64_t ncols, double* out);

@@ -21,7 +21,11 @@ Pinning (see tests/test_oracle.py):
                                            (tests/golden/*), dense SVD at convergence
   flip_omg / mev / flip_uv               - closed form + brute force
   fit_with_pi (EMU, incl. perm quirk)    - golden EMU outputs (-d 1 and -d 2)
-The out-of-core block plan (src/Data.cpp:42-84) is not restated here.
+  block_plan (out-of-core, -m)           - closed-form cases + golden -m runs (block count,
+                                           band factor, trajectory)
+  permute_plink_order                    - permutation invariants, the single-buffer closed form
+                                           and golden -d 2 -m runs (loadings in file order)
+  out-of-core NormalOp / FancyOp         - golden -m outputs of the reference binary
 """
 from __future__ import annotations
 
@@ -115,9 +119,14 @@ class Params:
     scale: int = -9
     ploidy: int = 2
     maf: float = 0.0
+    memory: float = 0.0  # -m/--memory (GB): out-of-core when > 0
+    buffer: int = 2  # --buffer (GB) of permute_plink
 
     def __post_init__(self):
         self.oversamples = max(self.oversamples, self.k)  # src/Cmd.cpp:216
+        self.out_of_core = self.memory > 0  # src/Cmd.cpp:218
+        if self.maf > 0 and self.out_of_core:  # src/Cmd.cpp:221-224
+            raise ValueError("does not support --maf filters for out-of-core mode yet! ")
         if not self.emu:
             self.maxiter = 0  # src/Cmd.cpp:230-232
         if self.bands < 4 or self.bands % 2 != 0:  # src/Cmd.cpp:235-236
@@ -136,6 +145,8 @@ class Data:
     perm: np.ndarray | None = None
     p_miss: float = 0.0
     keep: np.ndarray | None = None
+    plan: "BlockPlan | None" = None
+    G0: np.ndarray | None = None
 
     @property
     def nsamples(self):
@@ -148,6 +159,12 @@ class Data:
     @classmethod
     def from_packed(cls, packed: np.ndarray, n: int, params: Params) -> "Data":
         codes = decode_codes(packed, n)
+        order = None
+        if params.out_of_core and params.perm:
+            # src/Main.cpp:120-151: the file is rewritten in permute_plink order
+            # before it is read, so F is estimated in that order
+            order = permute_plink_order(n, codes.shape[0], params.buffer, params.bands)
+            codes = codes[order]
         F = allele_freq(codes)
         keep = None
         if params.maf > 0:  # src/Data.cpp:88-105 (MAF(a) = a > 0.5 ? 1-a : a, keep if MAF > maf)
@@ -157,9 +174,28 @@ class Data:
             F = F[keep]
         vals = BED2GENO[codes]
         G = np.where(codes == 1, 0.0, vals - F[:, None]).T.copy()  # src/FilePlink.cpp:102-109
-        C = (codes == 1) if params.missme else None
+        C = (codes == 1) if (params.missme or params.out_of_core) else None
         p_miss = float(C.mean()) if C is not None else 0.0
-        return cls(G=G, F=F, C=C, params=params, p_miss=p_miss, keep=keep)
+        d = cls(G=G, F=F, C=C, params=params, p_miss=p_miss, keep=keep, perm=order)
+        if params.out_of_core:
+            d.plan = block_plan(params, n, codes.shape[0])
+            d.G0 = G  # every block read starts from the centered codes
+        return d
+
+    def block_matrix(self, update, standardize, U, S, VT):
+        """What read_block_initial / read_block_update (src/FilePlink.cpp:122-298)
+        produce over all blocks: centered codes (missing -> 0); with update and
+        --emu the missing entries take clamp(U S VT, -F, 1-F) with F of the SNP's
+        own position; then the standardize scale sqrt(ploidy)/sd when sd > 1e-9."""
+        A = self.G0.copy()
+        if update and self.params.emu:
+            pred = (U * S[None, :]) @ VT
+            imp = np.minimum(np.maximum(pred, -self.F[None, :]), 1.0 - self.F[None, :])
+            A = np.where(self.C.T, imp, A)
+        if standardize and self.params.scale == -9:
+            sd = np.sqrt(self.F * (1.0 - self.F))
+            A = A * np.where(sd > VAR_TOL, math.sqrt(self.params.ploidy) / np.where(sd > VAR_TOL, sd, 1.0), 1.0)[None, :]
+        return A
 
     def standardize_E(self):
         """src/Data.cpp:351-362."""
@@ -183,6 +219,85 @@ class Data:
         hi = 1.0 - self.F[None, :]
         imp = np.minimum(np.maximum(pred, lo), hi)  # fmin(fmax(x, -F), 1-F)
         self.G = np.where(mask, imp, self.G)
+
+
+# ---------------------------------------------------------------- out-of-core
+def _u32(x: int) -> int:
+    return x & 0xFFFFFFFF
+
+
+@dataclass
+class BlockPlan:
+    blocksize: int
+    nblocks: int
+    band_factor: int
+    start: list
+    stop: list
+
+
+def block_plan(params: Params, nsamples: int, nsnps: int) -> BlockPlan:
+    """Data::prepare's out-of-core plan (src/Data.cpp:42-84, Halko branch with
+    dopca).  The RAM model 3nl + 2ml + 5m (doubles) is evaluated in 32-bit
+    unsigned arithmetic, as the reference's uint counts (src/Data.hpp:43-45) do."""
+    n, m = nsamples, nsnps
+    l = params.k + params.oversamples
+    t3, t2, t5 = _u32(3 * n * l), _u32(2 * m * l), _u32(5 * m)
+    ram = float(_u32(t3 + t2 + t5)) / 134217728
+    if params.memory > 1.1 * ram:
+        ram = 0.0
+    bs = math.ceil((((ram + params.memory) * 134217728 - t3) - t2 - t5) / n)
+    if bs < 1:
+        raise ValueError("out-of-core: -m/--memory leaves no room for a block of SNPs")
+    nblocks = math.ceil(m / bs)
+    if nblocks == 1:  # src/Data.cpp:63
+        raise ValueError("only one block exists. please remove -m option")
+    factor = 1
+    if params.svd == 2:  # src/Data.cpp:64-76
+        if nblocks < params.bands:
+            bs = math.ceil(m / params.bands)
+        else:
+            factor = math.ceil(nblocks / params.bands)
+            bs = math.ceil(m / (params.bands * factor))
+        nblocks = math.ceil(m / bs)
+    start = [i * bs for i in range(nblocks)]  # src/Data.cpp:79-83
+    stop = [min(s + bs - 1, m - 1) for s in start]
+    return BlockPlan(bs, nblocks, factor, start, stop)
+
+
+def permute_plink_order(nsamples: int, nsnps: int, gb: int, nbands: int) -> np.ndarray:
+    """permute_plink (src/FilePlink.cpp:303-408) as an index map: position ib
+    of the rewritten file holds original SNP order[ib].  Buffers of
+    bufsize * nbands SNPs are read in turn; SNP j * nbands + b of buffer i is
+    written at i * bufsize0 + bandidx[b] + j (bufsize0 = the first buffer's
+    bufsize); the last, shorter buffer gives its remainder to the first bands."""
+    bytes_per_snp = (nsamples + 3) >> 2
+    two = int(math.floor(1073741824.0 * gb / bytes_per_snp))
+    two = min(two, nsnps)
+    bufsize = two // nbands
+    if bufsize < 1:
+        raise ValueError("permute_plink: the --buffer holds fewer SNPs than -w bands")
+    two = bufsize * nbands
+    nblocks = (nsnps + two - 1) // two
+    modr2 = nsnps % two
+    modr = nsnps % nbands
+    bandsize = (nsnps + nbands - 1) // nbands
+    bandidx = [i * bandsize if (modr == 0 or i < modr) else modr * bandsize + (bandsize - 1) * (i - modr)
+               for i in range(nbands)]
+    bufidx = bufsize
+    order = np.full(nsnps, -1, dtype=np.int64)
+    for i in range(nblocks):
+        if i == nblocks - 1 and modr2 != 0:
+            last = nsnps - (nblocks - 1) * two
+            bufsize = (last + nbands - 1) // nbands
+            modr2 = last % nbands
+        for b in range(nbands):
+            j = 0
+            while j < bufsize - 1:
+                order[i * bufidx + bandidx[b] + j] = i * two + j * nbands + b
+                j += 1
+            if i != nblocks - 1 or b < modr2 or modr2 == 0:
+                order[i * bufidx + bandidx[b] + j] = i * two + j * nbands + b
+    return order
 
 
 # ---------------------------------------------------------------- helpers
@@ -289,6 +404,8 @@ class NormalOp(RsvdOp):
 
     def compute_gh(self, G, H, pi):
         d = self.data
+        if d.params.out_of_core:
+            return self._compute_gh_blocks(G, H, pi)
         if pi == 0:
             self.init_omg()
             if self.update:
@@ -300,6 +417,23 @@ class NormalOp(RsvdOp):
             self.Omg2 = self.Omg.copy()
         G = d.G.T @ self.Omg
         H = d.G @ G
+        return G, H
+
+    def _compute_gh_blocks(self, G, H, pi):
+        """src/Halko.cpp:129-152: no flipOmg; H summed over the blocks in order."""
+        d = self.data
+        if pi == 0:
+            self.init_omg()
+            self.A = d.block_matrix(self.update, self.standardize, self.U, self.S,
+                                    None if self.V is None else self.V.T)
+        if pi > 0:
+            self.Omg = thin_q(H)
+        G = np.zeros((d.nsnps, self.L))
+        H = np.zeros((d.nsamples, self.L))
+        for s, e in zip(d.plan.start, d.plan.stop):
+            blk = slice(s, e + 1)
+            G[blk] = self.A[:, blk].T @ self.Omg
+            H += self.A[:, blk] @ G[blk]
         return G, H
 
 
@@ -324,6 +458,8 @@ class FancyOp(RsvdOp):
         if 2.0 ** pi >= bands:
             self.H1[:] = 0
             self.H2[:] = 0
+        if d.params.out_of_core:
+            return self._compute_gh_blocks(G, H, pi)
         if pi == 0:
             if self.update:
                 d.fit_with_pi(self.U, self.S, self.V.T)
@@ -348,6 +484,46 @@ class FancyOp(RsvdOp):
                 else:
                     self.H2 += d.G[:, blk] @ G[blk]
             adjacent = pi > 0 and (b + 1) == 2 ** (pi - 1) and 2.0 ** pi < bands
+            if (b + 1) < bs and not adjacent:
+                i += 1
+                continue
+            if not (i == bs or i == bs // 2 or adjacent):
+                i += 1
+                continue
+            H = self.H1 + self.H2
+            self.Omg = flip_omg(self.Omg2, thin_q(H))
+            self.Omg2 = self.Omg.copy()
+            self.omega_updates += 1
+            if i == bs:
+                self.H1[:] = 0
+                i = 0
+            else:
+                self.H2[:] = 0
+            i += 1
+        return G, H
+
+
+    def _compute_gh_blocks(self, G, H, pi):
+        """src/Halko.cpp:225-268: the plan's blocks are the windows, bands grow
+        from band_factor blocks, flipOmg after every update."""
+        d = self.data
+        plan = d.plan
+        bands = d.params.bands
+        if pi == 0:
+            self.A = d.block_matrix(self.update, self.standardize, self.U, self.S,
+                                    None if self.V is None else self.V.T)
+            self.bandsize = plan.band_factor
+        self.bandsize = min(self.bandsize * 2, plan.nblocks)
+        bs = self.bandsize
+        i = 1
+        for b in range(plan.nblocks):
+            blk = slice(plan.start[b], plan.stop[b] + 1)
+            G[blk] = self.A[:, blk].T @ self.Omg
+            if i <= bs // 2:
+                self.H1 += self.A[:, blk] @ G[blk]
+            else:
+                self.H2 += self.A[:, blk] @ G[blk]
+            adjacent = pi > 0 and (b + 1) == 2 ** (pi - 1) * plan.band_factor and 2.0 ** pi < bands
             if (b + 1) < bs and not adjacent:
                 i += 1
                 continue

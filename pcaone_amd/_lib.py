@@ -35,7 +35,7 @@ class Params(C.Structure):
                 ("noshuffle", C.c_int), ("emu", C.c_int), ("printv", C.c_int), ("verbose", C.c_int),
                 ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("waves_g", C.c_int),
                 ("waves_h", C.c_int), ("kchunk_g", C.c_uint64),
-                ("kchunk_h", C.c_uint64)]
+                ("kchunk_h", C.c_uint64), ("memory", C.c_double), ("buffer", C.c_uint32)]
 
 
 class RunInfo(C.Structure):
@@ -100,6 +100,8 @@ _SIGS = {
     "pcaone_get_g": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
     "pcaone_get_h": (C.c_int, [_P, C.c_int, _P]),
     "pcaone_host_omega": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint64, _P]),
+    "pcaone_block_plan": (C.c_int, [_P, C.c_uint32, C.c_uint64, _P, _P, _P, _P, C.c_uint64]),
+    "pcaone_permute_plink_order": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _P]),
     "pcaone_host_sym_eig": (C.c_int, [C.c_int, _P, _P, _P]),
     "pcaone_host_householder_signs": (C.c_int, [C.c_int, _P, _P]),
     "pcaone_host_chol_upper": (C.c_int, [C.c_int, _P, _P]),

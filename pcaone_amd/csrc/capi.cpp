@@ -113,6 +113,8 @@ Param to_param(const pcaone_params* p) {
   q.tuning.waves_h = p->waves_h;
   q.tuning.kchunk_g = p->kchunk_g;
   q.tuning.kchunk_h = p->kchunk_h;
+  q.memory = p->memory;
+  q.buffer = p->buffer;
   require(q.k >= 1, kErrArgument, "k must be >= 1");
   require(q.ploidy == 1 || q.ploidy == 2, kErrArgument, "ploidy must be 1 or 2");
   q.derive();
@@ -180,6 +182,7 @@ void pcaone_params_default(pcaone_params* p) {
   p->rand = 1;
   p->ploidy = 2;
   p->precision = PCAONE_PREC_FP32;
+  p->buffer = 2;
 }
 
 int pcaone_bed_read(pcaone_ctx* c, const char* prefix, double maf, uint32_t* nsamples, uint64_t* nsnps) {
@@ -570,6 +573,28 @@ int pcaone_copy_probe(pcaone_ctx* c, uint64_t bytes) {
     PCAONE_HIP(hipMemsetAsync(a.get(), 1, n, x.stream));
     launch_copy_probe(a.get(), b.get(), n, x.stream);
     PCAONE_HIP(hipStreamSynchronize(x.stream));
+  });
+}
+
+int pcaone_block_plan(const pcaone_params* p, uint32_t nsamples, uint64_t nsnps, uint64_t* blocksize,
+                      uint64_t* nblocks, uint64_t* band_factor, uint64_t* start, uint64_t cap) {
+  return guard_global([&] {
+    const Param q = to_param(p);
+    const BlockPlan b = plan_blocks(q, nsamples, nsnps);
+    if (blocksize) *blocksize = b.blocksize;
+    if (nblocks) *nblocks = b.nblocks;
+    if (band_factor) *band_factor = b.band_factor;
+    if (start)
+      for (uint64_t i = 0; i < b.nblocks && i < cap; ++i) start[i] = b.start[i];
+  });
+}
+
+int pcaone_permute_plink_order(uint64_t nsamples, uint64_t nsnps, uint32_t buffer_gb, uint32_t bands,
+                               int32_t* order) {
+  return guard_global([&] {
+    require(order != nullptr, kErrArgument, "null output");
+    const std::vector<int> o = permute_plink_order(nsamples, nsnps, buffer_gb, bands);
+    std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
   });
 }
 

@@ -171,6 +171,7 @@ void Context::finalize() {
   perm.resize(M);
   for (uint64_t j = 0; j < M; ++j) perm[j] = (int)j;
   perm_identity_ = true;
+  freq_follows_ = false;
   dF_.resize(M);
   PCAONE_HIP(hipMemcpyAsync(dF_.get(), F.data(), M * 8, hipMemcpyHostToDevice, stream));
   pristine_columns();
@@ -243,8 +244,9 @@ void Context::filter_snps(const std::vector<uint32_t>& keep) {
   F = Fk;  // identical to the recount, kept explicit
 }
 
-void Context::permute_columns(const std::vector<int>& p) {
+void Context::permute_columns(const std::vector<int>& p, bool with_freq) {
   require(p.size() == M, kErrArgument, "permutation size mismatch");
+  require(with_freq || !freq_follows_, kErrState, "column order mixes two permutation kinds");
   std::vector<uint32_t> idx(M);
   for (uint64_t i = 0; i < M; ++i) {
     require(p[i] >= 0 && (uint64_t)p[i] < M, kErrArgument, "permutation index out of range");
@@ -261,6 +263,10 @@ void Context::permute_columns(const std::vector<int>& p) {
   launch_gather_f64(sc_.get(), d.get(), M, t.get() + M, stream);
   PCAONE_HIP(hipMemcpyAsync(cF_.get(), t.get(), M * 8, hipMemcpyDeviceToDevice, stream));
   PCAONE_HIP(hipMemcpyAsync(sc_.get(), t.get() + M, M * 8, hipMemcpyDeviceToDevice, stream));
+  if (with_freq) {
+    launch_gather_f64(dF_.get(), d.get(), M, t.get(), stream);
+    PCAONE_HIP(hipMemcpyAsync(dF_.get(), t.get(), M * 8, hipMemcpyDeviceToDevice, stream));
+  }
   PCAONE_HIP(hipStreamSynchronize(stream));
   std::swap(xs_, tmpx_);
   tmpx_.release();
@@ -271,6 +277,7 @@ void Context::permute_columns(const std::vector<int>& p) {
     perm_identity_ = perm_identity_ && np[i] == (int)i;
   }
   perm = np;
+  if (with_freq) freq_follows_ = !perm_identity_;
   weights_dirty_ = true;
   cols_pristine_ = false;
   if (emu_k_ > 0) {  // imputation loadings follow their columns
@@ -362,7 +369,7 @@ void Context::reset_columns() {
   if (!perm_identity_) {
     std::vector<int> inv(M);
     for (uint64_t i = 0; i < M; ++i) inv[perm[i]] = (int)i;
-    permute_columns(inv);
+    permute_columns(inv, freq_follows_);
   }
   pristine_columns();
 }
@@ -719,7 +726,7 @@ void Context::check_device_status() {
   }
 }
 
-void Context::omega_from(int src) {
+void Context::omega_from(int src, bool flip) {
   require(L > 0, kErrState, "configure() first");
   if (src == 1) {
     launch_add(H1_.get(), H2_.get(), H_.get(), H_.size(), stream);
@@ -741,10 +748,13 @@ void Context::omega_from(int src) {
     launch_gram(tmp_n_.get(), nullptr, nullptr, N, lp, red_.get(), red_cap_, g2, stream);
     launch_thinq_second(g2, r1, X, L, lp, small_.get(), lstatus_.get(), stream);
     launch_apply(X, small_.get(), N, lp, omega_c_.get(), stream);
-    // flipOmg (src/RSVD.hpp:80-89) against Omg2
-    double* norms = red_.get() + red_cap_;
-    launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
-    launch_flip_signs(norms, L, lp, dsign, stream);
+    if (flip) {  // flipOmg (src/RSVD.hpp:80-89) against Omg2
+      double* norms = red_.get() + red_cap_;
+      launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
+      launch_flip_signs(norms, L, lp, dsign, stream);
+    } else {
+      dsign = nullptr;
+    }
     if (form_g == Form::I8)
       launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
     launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
@@ -778,17 +788,19 @@ void Context::omega_from(int src) {
     for (int j = 0; j < L; ++j) T(i, j) *= s[j];
   set_t(T);
   launch_apply(X, small_.get(), N, lp, omega_c_.get(), stream);
-  // flipOmg (src/RSVD.hpp:80-89) against Omg2
-  double* norms = red_.get() + red_cap_;
-  launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
-  std::vector<double> ab(2 * (size_t)lp);
-  PCAONE_HIP(hipMemcpyAsync(ab.data(), norms, ab.size() * 8, hipMemcpyDeviceToHost, stream));
-  PCAONE_HIP(hipStreamSynchronize(stream));
-  std::vector<double> sign(lp, 1.0);
-  for (int l = 0; l < L; ++l)
-    if (ab[l] > 2.0 * ab[lp + l]) sign[l] = -1.0;
-  double* dsign = small_.get() + 128 * 128;
-  PCAONE_HIP(hipMemcpyAsync(dsign, sign.data(), lp * 8, hipMemcpyHostToDevice, stream));
+  double* dsign = nullptr;
+  if (flip) {  // flipOmg (src/RSVD.hpp:80-89) against Omg2
+    double* norms = red_.get() + red_cap_;
+    launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
+    std::vector<double> ab(2 * (size_t)lp);
+    PCAONE_HIP(hipMemcpyAsync(ab.data(), norms, ab.size() * 8, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    std::vector<double> sign(lp, 1.0);
+    for (int l = 0; l < L; ++l)
+      if (ab[l] > 2.0 * ab[lp + l]) sign[l] = -1.0;
+    dsign = small_.get() + 128 * 128;
+    PCAONE_HIP(hipMemcpyAsync(dsign, sign.data(), lp * 8, hipMemcpyHostToDevice, stream));
+  }
   if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));

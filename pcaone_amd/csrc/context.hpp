@@ -68,7 +68,11 @@ class Context {
   // (identity permutation, cF = F, sc = 1) and builds xt and the missing lists.
   void finalize();
   void filter_snps(const std::vector<uint32_t>& keep);  // keep SNPs (Data.cpp:88-105)
-  void permute_columns(const std::vector<int>& perm);   // column i <- column perm[i] (RSVD.hpp:61-78)
+  // column i <- column perm[i] (RSVD.hpp:61-78).  The allele frequencies used
+  // by standardize and the imputation clamp stay indexed by position (the
+  // in-core shuffle) unless with_freq, where they follow their SNPs (a file
+  // rewritten in permute_plink order before it is read).
+  void permute_columns(const std::vector<int>& perm, bool with_freq = false);
   void standardize(int ploidy);                         // Data.cpp:351-362 semantics
   void set_impute(const double* U, const double* S, const double* V, int kk);  // Data.cpp:318-348 (column order)
   // The same from the device's own U (ucur) and V (vbuf), S on the host.
@@ -106,9 +110,10 @@ class Context {
   // G rows [j0, j1) = A[j0:j1] Omega and the H contribution of those SNPs:
   // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ...
   void sketch(uint64_t j0, uint64_t j1, int hmode);
-  // Omega <- thin-Q(src) with Householder signs, then flipOmg against Omg2;
+  // Omega <- thin-Q(src) with Householder signs, then flipOmg against Omg2
+  // unless !flip (the out-of-core --svd 1 branch has none);
   // src 0: H, 1: H1 + H2 (also stored as the current H).
-  void omega_from(int src);
+  void omega_from(int src, bool flip = true);
   void zero_h(int which);  // 1: H1, 2: H2
   // QR of G twice, B = R^-T H^T, eigen-decomposition of B B^T, Ucur; mev
   // against the previous Ucur if have_prev.
@@ -187,6 +192,7 @@ class Context {
   void build_derived();  // xt + missing lists from xs
   void pristine_columns();  // cF = F as read, sc = 1 (device)
   bool perm_identity_ = true;  // perm[i] == i
+  bool freq_follows_ = false;  // the device allele frequencies are in the current column order
   // L x L block of X^T Y; with dev_out the lp x lp result also stays there
   la::Mat gram_l(const double* X, uint64_t rows, const double* Y = nullptr, double* dev_out = nullptr);
   void omega_pieces();  // tmp_n_ (column-major Omega) -> omega_c_, pieces, Omg2, column sums

@@ -25,6 +25,91 @@ void Param::derive() {
           "the -w/--batches must be a power of 2 and the minimun is 4.");  // src/Cmd.cpp:235-236
   perm = (svd == 2 && !noshuffle);                 // src/Cmd.cpp:238
   require(svd == 1 || svd == 2, kErrArgument, "only --svd 1 and --svd 2 run on this path");
+  out_of_core = memory > 0;                        // src/Cmd.cpp:218
+  require(!(filterSNP && out_of_core), kErrArgument,
+          "does not support --maf filters for out-of-core mode yet! ");  // src/Cmd.cpp:221-224
+}
+
+BlockPlan plan_blocks(const Param& p, uint64_t nsamples, uint64_t nsnps) {
+  // src/Data.cpp:42-84.  The reference holds the counts in 32-bit unsigned
+  // integers (src/Data.hpp:43-45) and evaluates its RAM model
+  // (3 n l + 2 m l + 5 m doubles) in that type.
+  require(nsamples > 0 && nsamples < (1ull << 32) && nsnps > 0 && nsnps < (1ull << 32), kErrArgument,
+          "out-of-core: sample and SNP counts must fit 32 bits");
+  const uint32_t n = (uint32_t)nsamples, m = (uint32_t)nsnps;
+  const uint32_t l = p.k + p.oversamples;
+  double ram = (double)(3u * n * l + 2u * m * l + 5u * m) / 134217728;
+  if (p.memory > 1.1 * ram) ram = 0;  // otherwise the reference warns and adds the minimum
+  const double bs = std::ceil((double)((ram + p.memory) * 134217728 - 3u * n * l - 2u * m * l - 5u * m) / n);
+  require(bs >= 1, kErrArgument, "out-of-core: -m/--memory leaves no room for a block of SNPs");
+  require(bs < (double)m, kErrArgument, "only one block exists. please remove -m option");  // src/Data.cpp:63
+  BlockPlan b;
+  uint32_t blocksize = (uint32_t)bs;
+  uint32_t nblocks = (uint32_t)std::ceil((double)m / blocksize);
+  uint32_t factor = 1;
+  if (p.svd == 2) {  // src/Data.cpp:64-76: blocks become the winSVD windows
+    if (nblocks < p.bands) {
+      blocksize = (uint32_t)std::ceil((double)m / p.bands);
+    } else {
+      factor = (uint32_t)std::ceil((double)nblocks / p.bands);
+      blocksize = (uint32_t)std::ceil((double)m / (p.bands * factor));
+    }
+    nblocks = (uint32_t)std::ceil((double)m / blocksize);
+  }
+  b.blocksize = blocksize;
+  b.nblocks = nblocks;
+  b.band_factor = factor;
+  b.start.resize(nblocks);
+  b.stop.resize(nblocks);
+  for (uint32_t i = 0; i < nblocks; ++i) {  // src/Data.cpp:79-83
+    b.start[i] = (uint64_t)i * blocksize;
+    b.stop[i] = std::min<uint64_t>(b.start[i] + blocksize - 1, (uint64_t)m - 1);
+  }
+  return b;
+}
+
+std::vector<int> permute_plink_order(uint64_t nsamples, uint64_t nsnps, uint32_t gb, uint32_t nbands) {
+  // src/FilePlink.cpp:303-408: the file is read in buffers of a multiple of
+  // nbands SNPs; SNP j nbands + b of buffer i goes to band b's run of the
+  // output, after the runs of the earlier buffers.  Only the index map is
+  // needed here (the device gathers its rows once).
+  require(nsnps > 0 && nsnps < (1ull << 31) && nbands > 0, kErrArgument, "permute_plink: bad sizes");
+  const uint32_t M = (uint32_t)nsnps;
+  const uint32_t bytes_per_snp = (uint32_t)((nsamples + 3) >> 2);
+  uint32_t buf_snps = (uint32_t)std::floor((double)1073741824 * gb / bytes_per_snp);
+  if (buf_snps > M) buf_snps = M;
+  uint32_t bufsize = (uint32_t)std::floor((double)buf_snps / nbands);
+  require(bufsize >= 1, kErrArgument, "permute_plink: the --buffer holds fewer SNPs than -w bands");
+  buf_snps = bufsize * nbands;
+  const uint32_t nbuf = (M + buf_snps - 1) / buf_snps;
+  uint32_t modr2 = M % buf_snps;
+  std::vector<uint64_t> bandidx(nbands);
+  const uint32_t modr = M % nbands, bandsize = (M + nbands - 1) / nbands;
+  for (uint32_t i = 0; i < nbands; ++i)
+    bandidx[i] = (modr == 0 || i < modr) ? (uint64_t)i * bandsize
+                                         : (uint64_t)modr * bandsize + (uint64_t)(bandsize - 1) * (i - modr);
+  std::vector<int> order(M, -1);
+  std::vector<char> seen(M, 0);
+  const uint64_t bufidx = bufsize;
+  auto place = [&](uint64_t ia, uint64_t ib) {
+    require(ia < M && ib < M && order[ib] < 0 && !seen[ia], kErrInternal, "permute_plink: not a permutation");
+    order[ib] = (int)ia;
+    seen[ia] = 1;
+  };
+  for (uint64_t i = 0; i < nbuf; ++i) {
+    if (i == nbuf - 1 && modr2 != 0) {  // the last, shorter buffer
+      const uint64_t last = M - (uint64_t)(nbuf - 1) * buf_snps;
+      bufsize = (uint32_t)((last + nbands - 1) / nbands);
+      modr2 = (uint32_t)(last % nbands);
+    }
+    for (uint64_t b = 0; b < nbands; ++b) {
+      uint64_t j = 0;
+      for (; j + 1 < bufsize; ++j) place(i * buf_snps + j * nbands + b, i * bufidx + bandidx[b] + j);
+      if (i != nbuf - 1 || b < modr2 || modr2 == 0) place(i * buf_snps + j * nbands + b, i * bufidx + bandidx[b] + j);
+    }
+  }
+  for (uint32_t i = 0; i < M; ++i) require(order[i] >= 0, kErrInternal, "permute_plink: not a permutation");
+  return order;
 }
 
 RsvdOpData::RsvdOpData(Context& ctx, const Param& p, int k, int os) : ctx_(ctx), params_(p), nk_(k), os_(os) {
@@ -99,13 +184,17 @@ void RsvdOpData::computeUSV(int p, double tol) {
 }
 
 void NormalRsvdOpData::computeGandH(int pi) {
-  // src/Halko.cpp:99-127 (in-core)
+  // src/Halko.cpp:99-127 (in-core) and :129-152 (out-of-core).  The block
+  // reads re-apply (update, standardize) on every pass from U, S, V, which
+  // stay fixed during computeUSV, so the device state set at pi == 0 is the
+  // matrix every pass reads.  H = sum over blocks of A_b^T G_b is one product
+  // over all blocks.  The out-of-core branch has no flipOmg.
   if (pi == 0) initOmg();
   if (pi == 0) {
     if (update) fit_with_pi();
     if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
   }
-  if (pi > 0) ctx_.omega_from(0);
+  if (pi > 0) ctx_.omega_from(0, /*flip=*/plan == nullptr);
   ctx_.sketch(0, rows(), 0);
 }
 
@@ -117,6 +206,10 @@ void FancyRsvdOpData::computeGandH(int pi) {
   if (std::pow(2, pi) >= (double)bands) {
     ctx_.zero_h(1);
     ctx_.zero_h(2);
+  }
+  if (plan) {
+    computeGandH_blocks(pi);
+    return;
   }
   if (pi == 0) {
     if (update) fit_with_pi();
@@ -151,15 +244,57 @@ void FancyRsvdOpData::computeGandH(int pi) {
   }
 }
 
+void FancyRsvdOpData::computeGandH_blocks(int pi) {
+  // src/Halko.cpp:225-268: the windows are the plan's blocks and the bands
+  // grow from band_factor blocks; (update, standardize) as for the Normal op.
+  const BlockPlan& bp = *plan;
+  const uint64_t bands = params_.bands;
+  if (pi == 0) {
+    if (update) fit_with_pi();
+    if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
+    bandsize_ = bp.band_factor;
+  }
+  bandsize_ = std::min<uint64_t>(bandsize_ * 2, bp.nblocks);
+  for (uint64_t b = 0, i = 1; b < bp.nblocks; ++b, ++i) {
+    ctx_.sketch(bp.start[b], bp.stop[b] + 1, i <= bandsize_ / 2 ? 1 : 2);
+    const bool adjacent = (pi > 0 && (double)(b + 1) == std::pow(2, pi - 1) * (double)bp.band_factor &&
+                           std::pow(2, pi) < (double)bands);
+    if ((b + 1) < bandsize_ && !adjacent) continue;
+    if (!((i == bandsize_) || (i == bandsize_ / 2) || adjacent)) continue;
+    ctx_.omega_from(1);
+    ++omega_updates;
+    if (i == bandsize_) {
+      ctx_.zero_h(1);
+      i = 0;
+    } else {
+      ctx_.zero_h(2);
+    }
+  }
+}
+
 PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out) {
   // src/Halko.cpp:271-345 (genetic input, in-core, no LD output)
   ctx.reset_columns();  // restores the as-read state between runs; not part of the timed work
+  BlockPlan plan;
+  if (params.out_of_core) {
+    // Data::prepare's plan (src/Data.cpp:42-84); with the shuffle, the file
+    // is first rewritten in permute_plink order (src/Main.cpp:120-151) and
+    // read from there, so allele frequencies follow their SNPs
+    plan = plan_blocks(params, ctx.N, ctx.M);
+    if (params.perm)
+      ctx.permute_columns(permute_plink_order(ctx.N, ctx.M, params.buffer, params.bands), /*with_freq=*/true);
+  }
   const auto t0 = std::chrono::steady_clock::now();
   std::unique_ptr<RsvdOpData> rsvd;
-  if (params.svd == 2)
-    rsvd = std::make_unique<FancyRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
-  else
-    rsvd = std::make_unique<NormalRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
+  if (params.svd == 2) {
+    auto f = std::make_unique<FancyRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
+    if (params.out_of_core) f->plan = &plan;
+    rsvd = std::move(f);
+  } else {
+    auto n = std::make_unique<NormalRsvdOpData>(ctx, params, (int)params.k, (int)params.oversamples);
+    if (params.out_of_core) n->plan = &plan;
+    rsvd = std::move(n);
+  }
   rsvd->info = info;
   rsvd->u_out = U_out;
   rsvd->need_v_host = params.printv || params.missme;

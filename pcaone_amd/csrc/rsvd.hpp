@@ -22,10 +22,24 @@ struct Param {
   bool noshuffle = false, emu = false, printv = false;
   int precision = 1;  // PCAONE_PREC_*
   GemmTuning tuning;
+  double memory = 0.0;  // -m/--memory (GB); > 0 selects the out-of-core schedule
+  uint32_t buffer = 2;  // --buffer (GB) of permute_plink
   // derived (src/Cmd.cpp:171-238)
-  bool perm = false, missme = false, filterSNP = false;
+  bool perm = false, missme = false, filterSNP = false, out_of_core = false;
   void derive();
 };
+
+// Data::prepare's out-of-core block plan (src/Data.cpp:42-84, the Halko
+// branch with dopca): SNP blocks [start[b], stop[b]] and the winSVD band factor.
+struct BlockPlan {
+  uint64_t blocksize = 0, nblocks = 1, band_factor = 1;
+  std::vector<uint64_t> start, stop;
+};
+BlockPlan plan_blocks(const Param& p, uint64_t nsamples, uint64_t nsnps);
+
+// permute_plink's SNP order (src/FilePlink.cpp:303-408): position i of the
+// permuted file holds original SNP order[i].
+std::vector<int> permute_plink_order(uint64_t nsamples, uint64_t nsnps, uint32_t gb, uint32_t nbands);
 
 struct RunInfo {
   int epochs = 0;                 // epochs of the last computeUSV
@@ -74,6 +88,7 @@ class NormalRsvdOpData : public RsvdOpData {
  public:
   using RsvdOpData::RsvdOpData;
   void computeGandH(int pi) override;
+  const BlockPlan* plan = nullptr;  // out-of-core block plan (null: in-core)
 };
 
 class FancyRsvdOpData : public RsvdOpData {
@@ -81,9 +96,11 @@ class FancyRsvdOpData : public RsvdOpData {
   using RsvdOpData::RsvdOpData;
   void computeGandH(int pi) override;
   uint64_t omega_updates = 0;
+  const BlockPlan* plan = nullptr;  // out-of-core block plan (null: in-core)
 
  private:
   uint64_t bandsize_ = 1, blocksize_ = 0;
+  void computeGandH_blocks(int pi);
 };
 
 struct PcaResult {

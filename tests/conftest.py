@@ -63,7 +63,8 @@ def params_from_args(args: str, **extra):
     i = 0
     flags = {"-d": ("svd", int), "-k": ("k", int), "--maxp": ("maxp", int), "--rand": ("rand", int),
              "--seed": ("seed", int), "--tol-rsvd": ("tol", float), "--maf": ("maf", float), "-w": ("bands", int),
-             "--maxiter": ("maxiter", int), "--oversamples": ("oversamples", int), "--tol-em": ("tolem", float)}
+             "--maxiter": ("maxiter", int), "--oversamples": ("oversamples", int), "--tol-em": ("tolem", float),
+             "-m": ("memory", float), "--buffer": ("buffer", int)}
     printv = False
     while i < len(toks):
         t = toks[i]

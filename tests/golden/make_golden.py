@@ -51,11 +51,23 @@ CASES = {
     "f4_emu_d2": ("f4", "-d 2 -w 16 -k 3 --emu --maxiter 5 -V"),
     "cfg1_d1": ("cfg1", "-d 1 -k 10 --oversamples 10 --maxp 7 --tol-rsvd 0"),
     "cfg1_d2": ("cfg1", "-d 2 -w 64 -k 10 --maxp 7 --tol-rsvd 0"),
+    # out-of-core (-m GB): 3 blocks; 10 blocks with the minimum-RAM branch of the plan
+    "f1_d1_ooc": ("f1", "-d 1 -k 5 --maxp 7 -V -m 0.002"),
+    "f1_d1_ooc_min": ("f1", "-d 1 -k 5 --maxp 7 --tol-rsvd 0 -m 0.0003"),
+    # winSVD over 48 blocks (band factor 3) in permute_plink order; 32 blocks unshuffled
+    "f1_d2_ooc": ("f1", "-d 2 -w 16 -k 5 --maxp 7 -V -m 0.00068"),
+    "f1_d2_ooc_noshuf": ("f1", "-d 2 -w 16 -k 5 --maxp 7 -S -m 0.0007"),
+    # EMU with per-pass block imputation; winSVD over 64 blocks (band factor 4),
+    # 3000 SNPs in 16 bands (the shuffle's remainder path)
+    "f4_emu_d1_ooc": ("f4", "-d 1 -k 3 --emu --maxiter 5 -V -m 0.002"),
+    "f4_emu_d2_ooc": ("f4", "-d 2 -w 16 -k 3 --emu --maxiter 5 -V -m 0.0009"),
 }
 
 EPOCH_RE = re.compile(r"running of epoch =\s*(\d+), diff = ([-0-9.e+]+)")
 STOP_RE = re.compile(r"stops at epoch =\s*(\d+)")
 EM_RE = re.compile(r"estimated iter =\s*(\d+), diff = ([-0-9.e+]+)")
+# Data::prepare's out-of-core plan as the reference logs it (src/Data.cpp:60-61, 73-74)
+PLAN_RE = re.compile(r"blocksize =\s*(\d+), nblocks =\s*(\d+), factor =\s*(\d+)")
 
 
 def sha256(path: Path) -> str:
@@ -87,11 +99,14 @@ def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
     trace = [float(m.group(2)) for m in EPOCH_RE.finditer(log)]
     stop = STOP_RE.findall(log)
     em = [float(m.group(2)) for m in EM_RE.finditer(log)]
+    plan = [[int(g) for g in m.groups()] for m in PLAN_RE.finditer(log)]
     for suf in (".eigvals", ".sigvals", ".eigvecs", ".loadings", ".mbim"):
         src = Path(str(outp) + suf)
         if src.exists():
             shutil.copy(src, dest / suf[1:])
     meta = dict(input=inp_name_of(inp), args=args, trace=trace, stop_epochs=[int(s) for s in stop], em_trace=em)
+    if plan:
+        meta["block_plan"] = plan[-1]  # [blocksize, nblocks, factor] after any winSVD adjustment
     (dest / "meta.json").write_text(json.dumps(meta, indent=1) + "\n")
     print(f"{name}: {len(trace)} checked epochs, stops {stop}, em {len(em)}")
 
@@ -113,8 +128,10 @@ def main() -> None:
             _names[inp["prefix"]] = n
         manifest = {n: dict(sha256=i["sha256"], spec=i["spec"], committed=i["committed"]) for n, i in inputs.items()}
         (HERE / "inputs.json").write_text(json.dumps(manifest, indent=1) + "\n")
+        only = set(sys.argv[1:])  # optional: regenerate only the named cases
         for name, (inp, args) in CASES.items():
-            run_case(name, inputs[inp], args, tmp)
+            if not only or name in only:
+                run_case(name, inputs[inp], args, tmp)
 
 
 if __name__ == "__main__":

@@ -111,3 +111,64 @@ def test_parallel_omega_is_the_libstdcxx_draw(seed, gaussian, n, l):
     _lib.check(_lib.lib().pcaone_host_omega(seed, gaussian, n, l, out.ctypes.data))
     ref = O.omega(seed, bool(gaussian), n, l)
     assert np.array_equal(out.T, ref)
+
+
+# ---------------------------------------------------------------- out-of-core plan and shuffle order
+PLAN_GRID = [
+    # (nsamples, nsnps, svd, k, oversamples, bands, memory)
+    (200, 2000, 1, 5, 10, 64, 0.002), (200, 2000, 1, 5, 10, 64, 0.0003), (200, 2000, 2, 5, 10, 16, 0.00068),
+    (200, 2000, 2, 5, 10, 16, 0.0007), (300, 3000, 1, 3, 10, 64, 0.002), (300, 3000, 2, 3, 10, 16, 0.0009),
+    (300, 3000, 2, 3, 10, 16, 0.002), (50000, 1000000, 1, 40, 20, 64, 16.0), (50000, 1000000, 2, 40, 20, 64, 1.0),
+    (10000, 500000, 2, 20, 10, 64, 2.0), (2000, 20000, 1, 10, 10, 64, 0.05), (20000, 300000, 1, 10, 10, 64, 0.4),
+]
+
+
+def _library_plan(n, m, svd, k, os, bands, memory):
+    p = api.default_params(svd=svd, k=k, oversamples=os, bands=bands, memory=memory)
+    bs, nb, bf = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    start = np.zeros(1 << 16, dtype=np.uint64)
+    rc = _lib.lib().pcaone_block_plan(C.byref(p), n, m, C.byref(bs), C.byref(nb), C.byref(bf), start.ctypes.data,
+                                      start.size)
+    return rc, bs.value, nb.value, bf.value, start[: nb.value]
+
+
+@pytest.mark.parametrize("n,m,svd,k,os,bands,memory", PLAN_GRID)
+def test_block_plan_library_matches_oracle(n, m, svd, k, os, bands, memory):
+    from oracle import pcaone_oracle as O
+    rc, bs, nb, bf, start = _library_plan(n, m, svd, k, os, bands, memory)
+    try:
+        ref = O.block_plan(O.Params(svd=svd, k=k, oversamples=os, bands=bands, memory=memory), n, m)
+    except ValueError:
+        assert rc == 1  # the reference refuses a plan with a single block
+        return
+    assert rc == 0
+    assert (bs, nb, bf) == (ref.blocksize, ref.nblocks, ref.band_factor)
+    assert list(start) == ref.start
+
+
+def test_block_plan_single_block_is_an_error():
+    rc, *_ = _library_plan(200, 2000, 1, 5, 10, 64, 1.0)  # 1 GB holds all 2,000 SNPs
+    assert rc == 1
+    assert b"only one block" in _lib.lib().pcaone_last_error(None)
+
+
+@pytest.mark.parametrize("nsamples,nsnps,gb,bands", [
+    (200, 2000, 2, 16), (300, 3000, 2, 16), (300, 3001, 2, 64), (777, 64, 2, 64), (5, 1000, 2, 4),
+    # nominal sample counts that make the read buffer hold fewer SNPs than the file: several buffers
+    (40_000_000, 1000, 1, 4), (40_000_000, 1003, 1, 4), (400_000_000, 997, 3, 8), (100_000_000, 4099, 2, 16),
+])
+def test_permute_plink_order_library_matches_oracle(nsamples, nsnps, gb, bands):
+    from oracle import pcaone_oracle as O
+    out = np.empty(nsnps, dtype=np.int32)
+    _lib.check(_lib.lib().pcaone_permute_plink_order(nsamples, nsnps, gb, bands, out.ctypes.data))
+    ref = O.permute_plink_order(nsamples, nsnps, gb, bands)
+    assert np.array_equal(out, ref)
+    assert np.array_equal(np.sort(out), np.arange(nsnps))  # a permutation
+
+
+def test_permute_plink_order_single_buffer_is_band_interleave():
+    """One buffer and nsnps a multiple of the bands: band b's run holds SNPs b, b + w, b + 2w, ..."""
+    n, m, w = 200, 2000, 16
+    out = np.empty(m, dtype=np.int32)
+    _lib.check(_lib.lib().pcaone_permute_plink_order(n, m, 2, w, out.ctypes.data))
+    assert np.array_equal(out, np.arange(m).reshape(m // w, w).T.ravel())

@@ -194,7 +194,7 @@ def run_case(ctx, golden_inputs, case, form):
     p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
                            tol=params.tol, seed=params.seed, rand=params.rand, bands=params.bands,
                            noshuffle=int(params.noshuffle), emu=int(params.emu), maxiter=params.maxiter,
-                           tolem=params.tolem, precision=form)
+                           tolem=params.tolem, memory=params.memory, buffer=params.buffer, precision=form)
     return meta, params, printv, ctx.run(p, want_v=printv)
 
 
@@ -223,6 +223,8 @@ REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16", "int8")
 # The mixed form (one BF16 piece of Omega) meets the bar on --svd 1 only; on
 # winSVD the rounded per-window Omega moves the unconverged noise PCs.
 REDUCED += [(c, f) for f in ("bf16mix", "fp16mix") for c in ("f1_d1", "f4_d1_miss", "cfg1_d1")]
+# the out-of-core schedule (block windows, permute_plink order, per-pass imputation)
+REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f1_d2_ooc", "f4_emu_d2_ooc")]
 # Single-piece forms round Omega (or G) to 8-11 bits: the eigenvalues meet the
 # bar, the unconverged trailing PCs move by up to ~1e-3 in correlation.
 PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "int8": 0.9999, "fp16": 0.999, "bf16mix": 0.999, "fp16mix": 0.999}
@@ -247,6 +249,59 @@ def test_bad_shapes_are_errors(ctx, golden_inputs):
     assert e.value.status == 1
     with pytest.raises(api._lib.PcaoneError):
         ctx.sketch(10, 5, 0)
+
+
+def _check_golden(r, case, printv):
+    d = GOLDEN / case
+    meta = load_meta(case)
+    if meta["trace"]:
+        np.testing.assert_allclose(r.trace, meta["trace"], atol=1.5e-6)
+    if meta["em_trace"]:
+        np.testing.assert_allclose(r.em_trace, meta["em_trace"], atol=1.5e-6)
+    Uref = O.read_matrix(d / "eigvecs")
+    np.testing.assert_allclose(O.sign_align(Uref, r.U), Uref, atol=2e-6)
+    if printv:
+        Vref = O.read_matrix(d / "loadings")
+        np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=2e-6)
+
+
+def test_out_of_core_and_in_core_runs_share_one_context(ctx, golden_inputs):
+    """The permute_plink order moves the allele frequencies with their SNPs and
+    the in-core shuffle does not; runs of both kinds on one loaded data set
+    must each restore the as-read state first."""
+    say("out-of-core / in-core alternation on f4 (EMU, winSVD)")
+    ctx.read_bed(golden_inputs["f4"])
+    for case in ("f4_emu_d2_ooc", "f4_emu_d2", "f4_emu_d2_ooc", "f4_emu_d1_ooc"):
+        params, printv = params_from_args(load_meta(case)["args"])
+        say(f"  {case}")
+        p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
+                               tol=params.tol, bands=params.bands, emu=1, maxiter=params.maxiter,
+                               tolem=params.tolem, memory=params.memory, precision="fp64")
+        _check_golden(ctx.run(p, want_v=printv), case, printv)
+
+
+def test_out_of_core_host_thin_q_without_flip(ctx, golden_inputs):
+    """L = 60 takes the host thin-Q path; the out-of-core --svd 1 branch skips flipOmg."""
+    say("out-of-core sSVD, L = 60, vs oracle")
+    prefix = golden_inputs["f1"]
+    packed, n, _ = O.read_plink(prefix)
+    kw = dict(svd=1, k=30, oversamples=30, maxp=4, tol=0.0, memory=0.004)
+    ref = O.run_pca_with_halko(O.Data.from_packed(packed, n, O.Params(**kw)))
+    assert ref.epochs == 5
+    ctx.read_bed(prefix)
+    r = ctx.run(api.default_params(precision="fp64", **kw))
+    np.testing.assert_allclose(r.trace, ref.trace, atol=1e-9)
+    np.testing.assert_allclose(r.S, ref.S, rtol=1e-9)
+    np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-7)
+
+
+def test_out_of_core_argument_errors(ctx, golden_inputs):
+    say("out-of-core argument errors")
+    ctx.read_bed(golden_inputs["f1"])
+    with pytest.raises(api._lib.PcaoneError, match="only one block"):
+        ctx.run(api.default_params(svd=1, k=5, maxp=1, memory=1.0, precision="fp64"))
+    with pytest.raises(api._lib.PcaoneError, match="--maf"):
+        ctx.run(api.default_params(svd=1, k=5, maxp=1, memory=0.002, maf=0.1, precision="fp64"))
 
 
 @pytest.mark.parametrize("spec_kw", [dict(seed=9, nsamples=301, nsnps=777, npops=3),

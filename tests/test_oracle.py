@@ -177,7 +177,54 @@ def _plant_swapped_clamp(monkeypatch):
     monkeypatch.setattr(O.Data, "fit_with_pi", bad)
 
 
+def _plant_ooc_band_factor_one(monkeypatch):
+    orig = O.block_plan
+
+    def bad(params, n, m):
+        p = orig(params, n, m)
+        p.band_factor = 1
+        return p
+    monkeypatch.setattr(O, "block_plan", bad)
+
+
+def _plant_ooc_incore_shuffle(monkeypatch):
+    monkeypatch.setattr(O, "permute_plink_order", lambda n, m, gb, w: O.shuffle_indices(m))
+
+
+def _plant_ooc_no_flip(monkeypatch):
+    monkeypatch.setattr(O, "flip_omg", lambda a, b: b)
+
+
+def _plant_ooc_impute_unclamped(monkeypatch):
+    orig = O.Data.block_matrix
+
+    def bad(self, update, standardize, U, S, VT):
+        A = orig(self, False, False, U, S, VT)
+        if update and self.params.emu:
+            A = np.where(self.C.T, (U * S[None, :]) @ VT, A)
+        if standardize:
+            sd = np.sqrt(self.F * (1.0 - self.F))
+            A = A * np.where(sd > O.VAR_TOL, np.sqrt(2.0) / np.where(sd > O.VAR_TOL, sd, 1.0), 1.0)[None, :]
+        return A
+    monkeypatch.setattr(O.Data, "block_matrix", bad)
+
+
+@pytest.mark.parametrize("case", [c for c in golden_cases() if "block_plan" in load_meta(c)])
+def test_block_plan_matches_reference_log(case):
+    """Data::prepare's plan (src/Data.cpp:42-84) as the reference binary logged it."""
+    meta = load_meta(case)
+    params, _ = params_from_args(meta["args"])
+    n, m = {"f1": (200, 2000), "f4": (300, 3000)}[meta["input"]]
+    plan = O.block_plan(params, n, m)
+    assert [plan.blocksize, plan.nblocks, plan.band_factor] == meta["block_plan"]
+    assert plan.start[-1] < m <= plan.stop[-1] + 1
+
+
 @pytest.mark.parametrize("case,plant", [
+    ("f1_d2_ooc", _plant_ooc_band_factor_one),
+    ("f1_d2_ooc", _plant_ooc_incore_shuffle),
+    ("f1_d2_ooc", _plant_ooc_no_flip),
+    ("f4_emu_d1_ooc", _plant_ooc_impute_unclamped),
     ("f1_d2", _plant_no_flip),
     ("f1_d1_p0", _plant_omega_row_major),
     ("f1_d1", _plant_no_scaling),
