@@ -172,34 +172,80 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
                 af[rb], __builtin_bit_cast(i32x4_t, bv[cb * T::P + p]), acc[p][rb][cb], 0, 0, 0);
     }
   } else if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
-    // All B fragments of a substep are read before its MFMAs, so the LDS
-    // latency is paid once per substep instead of once per fragment.
+    // The B fragments of substep s + 1 are read from LDS while the MFMAs of
+    // substep s run (two register sets), so the LDS latency stays hidden.
+    // Tiles with more than 10 fragments per substep read them just before
+    // their MFMAs instead: a second set would not fit in the registers.
+    constexpr int NS = T::KS / 32;
+    constexpr bool PF = NC * T::P <= 10;
+    if constexpr (PF) {
+      auto read_b = [&](int s, u32x4_t(&dst)[NC * T::P]) {
 #pragma unroll
-    for (int s = 0; s < T::KS / 32; ++s) {
-      u32x4_t bv[NC * T::P];
+        for (int cb = 0; cb < NC; ++cb)
 #pragma unroll
-      for (int cb = 0; cb < NC; ++cb)
+          for (int p = 0; p < T::P; ++p)
+            dst[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
+                                                                    swz(r16, (T::KS / 32) * q + s) * 8);
+      };
+      auto mfma_substep = [&](int s, const u32x4_t(&bv)[NC * T::P]) {
+        bf16x8_t af[WM];
 #pragma unroll
-        for (int p = 0; p < T::P; ++p)
-          bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
-                                                                 swz(r16, (T::KS / 32) * q + s) * 8);
-      __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
-      bf16x8_t af[WM];
+        for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(aw[rb][s >> 1], s & 1);
 #pragma unroll
-      for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(aw[rb][s >> 1], s & 1);
+        for (int cb = 0; cb < NC; ++cb) {
 #pragma unroll
-      for (int cb = 0; cb < NC; ++cb) {
+          for (int p = 0; p < T::P; ++p) {
 #pragma unroll
-        for (int p = 0; p < T::P; ++p) {
+            for (int rb = 0; rb < WM; ++rb) {
+              if constexpr (F == Form::FP16)  // +-2 and 0 have the same top byte in FP16 as in BF16
+                acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, af[rb]),
+                                                                        __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
+                                                                        acc[0][rb][cb], 0, 0, 0);
+              else
+                acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    af[rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]), acc[0][rb][cb], 0, 0, 0);
+            }
+          }
+        }
+      };
+      u32x4_t bvs[2][NC * T::P];
+      read_b(0, bvs[0]);
 #pragma unroll
-          for (int rb = 0; rb < WM; ++rb) {
-            if constexpr (F == Form::FP16)  // +-2 and 0 have the same top byte in FP16 as in BF16
-              acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, af[rb]),
-                                                                      __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
-                                                                      acc[0][rb][cb], 0, 0, 0);
-            else
-              acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  af[rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]), acc[0][rb][cb], 0, 0, 0);
+      for (int s = 0; s < NS; ++s) {
+        if (s + 1 < NS) read_b(s + 1, bvs[(s + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);  // keep the next reads ahead of these MFMAs
+        mfma_substep(s, bvs[s & 1]);
+      }
+    } else {
+      // All B fragments of a substep are read before its MFMAs, so the LDS
+      // latency is paid once per substep instead of once per fragment.
+#pragma unroll
+      for (int s = 0; s < T::KS / 32; ++s) {
+        u32x4_t bv[NC * T::P];
+#pragma unroll
+        for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+          for (int p = 0; p < T::P; ++p)
+            bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
+                                                                   swz(r16, (T::KS / 32) * q + s) * 8);
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
+        bf16x8_t af[WM];
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb) af[rb] = decode_bf16x8(aw[rb][s >> 1], s & 1);
+#pragma unroll
+        for (int cb = 0; cb < NC; ++cb) {
+#pragma unroll
+          for (int p = 0; p < T::P; ++p) {
+#pragma unroll
+            for (int rb = 0; rb < WM; ++rb) {
+              if constexpr (F == Form::FP16)  // +-2 and 0 have the same top byte in FP16 as in BF16
+                acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, af[rb]),
+                                                                        __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
+                                                                        acc[0][rb][cb], 0, 0, 0);
+              else
+                acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    af[rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]), acc[0][rb][cb], 0, 0, 0);
+            }
           }
         }
       }
@@ -296,23 +342,46 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
   const int vrow0 = tid / VPR, vpart = tid % VPR;  // NT is a multiple of VPR
   constexpr int ROWS_PER_I = NT / VPR;
 
-  u32x4_t wreg[NVT];
+  // Double-buffered tiles are filled by direct global -> LDS loads: a wave
+  // writes 64 consecutive 16-byte slots (64 / VPR whole rows), lane l slot
+  // l, and fetches the W vector that the swizzle places there (the XOR
+  // swizzle is its own inverse).  No registers hold the tile in flight.
+  constexpr bool DMA = DB;
+  static_assert(64 % VPR == 0, "a wave's 64 slots must cover whole W-tile rows");
+  auto tile_dma = [&](w_t* dst, uint64_t kk) {
+#pragma unroll
+    for (int i = 0; i < NVT; ++i) {
+      const int wrow = wave * (64 / VPR) + ROWS_PER_I * i;  // first row of this wave's slots (uniform)
+      if (EVEN || wrow < T::P * LP) {
+        const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
+        const w_t* src = wg + piece * wpiece + col * wpitch + swz(row, vpart) * VE + kk;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                         (__attribute__((address_space(3))) void*)(dst + wrow * LDSP), 16, 0, 0);
+      }
+    }
+  };
+
+  u32x4_t wreg[DMA ? 1 : NVT];
   aword_t<F> anext[WM], acur[WM];
 
   if (kb < ke) {
 #pragma unroll
     for (int rb = 0; rb < WM; ++rb) acur[rb] = *reinterpret_cast<const aword_t<F>*>(xr[rb] + (kb >> 4));
+    if constexpr (DMA) {
+      tile_dma(wl0, kb);
+    } else {
 #pragma unroll
-    for (int i = 0; i < NVT; ++i) {
-      const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-      if (EVEN || row < T::P * LP)
-        wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kb);
+      for (int i = 0; i < NVT; ++i) {
+        const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
+        if (EVEN || row < T::P * LP)
+          wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kb);
+      }
+#pragma unroll
+      for (int i = 0; i < NVT; ++i)
+        if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
+          *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP +
+                                      swz(vrow0 + ROWS_PER_I * i, vpart) * VE) = wreg[i];
     }
-#pragma unroll
-    for (int i = 0; i < NVT; ++i)
-      if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
-        *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP + swz(vrow0 + ROWS_PER_I * i, vpart) * VE) =
-            wreg[i];
     // Nothing may stay pending into the loop: the wait inserted for a load
     // still in flight at the loop head would also cover the loop's own
     // prefetch and expose its latency every step.
@@ -323,11 +392,16 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
       const bool more = k0 + T::KS < ke;
       const uint64_t kn = k0 + T::KS;
       if (more) {
+        if constexpr (DMA) {
+          // the other buffer was last read in the previous step, before its barrier
+          tile_dma(wl0 + (buf ^ 1) * TILE_ELEMS, kn);
+        } else {
 #pragma unroll
-        for (int i = 0; i < NVT; ++i) {
-          const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-          if (EVEN || row < T::P * LP)
-            wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kn);
+          for (int i = 0; i < NVT; ++i) {
+            const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
+            if (EVEN || row < T::P * LP)
+              wreg[i] = *reinterpret_cast<const u32x4_t*>(wg + piece * wpiece + col * wpitch + vpart * VE + kn);
+          }
         }
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) anext[rb] = *reinterpret_cast<const aword_t<F>*>(xr[rb] + (kn >> 4));
@@ -335,16 +409,19 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
       w_t* cur = wl0 + (DB ? buf * TILE_ELEMS : 0);
       compute_step<F, WM, NC>(acc, acur, cur, q, r16);
       if (more) {
-        w_t* nxt = wl0 + (DB ? (buf ^ 1) * TILE_ELEMS : 0);
-        if constexpr (!DB) __syncthreads();
+        if constexpr (!DMA) {
+          w_t* nxt = wl0 + (DB ? (buf ^ 1) * TILE_ELEMS : 0);
+          if constexpr (!DB) __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NVT; ++i)
-          if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
-            *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP + swz(vrow0 + ROWS_PER_I * i, vpart) * VE) =
-                wreg[i];
+          for (int i = 0; i < NVT; ++i)
+            if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
+              *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP +
+                                          swz(vrow0 + ROWS_PER_I * i, vpart) * VE) = wreg[i];
+        }
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) acur[rb] = anext[rb];
       }
+      if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);  // this wave's tile loads have landed
       __syncthreads();
       buf ^= 1;
     }
