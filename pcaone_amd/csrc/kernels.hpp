@@ -87,7 +87,7 @@ void launch_apply(const double* X, const double* T, uint64_t nrows, int lp, doub
 void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp, double* partial, double* out,
                        hipStream_t s);
 // Thin-Q of an N x L matrix X (CholeskyQR2 with Householder column signs) with
-// its L x L steps on the device, for lp <= 48 (thinq_on_device):
+// its L x L steps on the device, for lp <= 80 (thinq_on_device):
 //   first:  R1 = chol(G1 = X^T X), T1 = R1^-1;
 //   second: R2 = chol(G2 = (X T1)^T (X T1)), T = (R2 R1)^-1 diag(s), s the
 //           Householder signs of Xtop (R2 R1)^-1, Xtop the first n rows of X.

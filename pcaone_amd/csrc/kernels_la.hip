@@ -757,153 +757,291 @@ __global__ void __launch_bounds__(256) k_missing_spmm(MissSpmmArgs a) {
 }
 
 // ---- L x L steps of the thin-Q of H on the device (one workgroup each) ----
-// Matrices are row-major with pitch lp and live in LDS; the routines follow
-// the host ones (linalg_host.cpp) operation by operation except the
-// Cholesky, which is right-looking.  Four waves run each kernel (one wave
-// leaves the LDS latency of the inner loops exposed); the 3 L barriers of the
-// Cholesky and the Householder steps bound it, so the device path is taken
-// for lp <= 48 only, where it is cheaper than the host's round trips.
+// Register-blocked: 256 threads form a 16 x 16 grid and thread (ty, tx) holds
+// the elements (ty + 16 p, tx + 16 q), p, q < NB = lp / 16, of each L x L
+// matrix in registers.  The elimination steps exchange one row and one
+// column per step through double-buffered LDS vectors, one barrier per step.
+// The elimination order is the host routines' (linalg_host.cpp): right-
+// looking Cholesky, back substitution for R^-1, modified LU for the
+// Householder signs (Ballard et al. 2014); divisions by a step's pivot are
+// multiplications by its reciprocal.
 constexpr int kSmallThreads = 256;
+constexpr int kThinqMaxNB = 5;  // lp <= 80
 
-// In place: A (n x n, symmetric positive definite) -> upper R with A = R^T R.
-// Returns false (in every thread) on breakdown.
-__device__ bool wg_chol_upper(double* A, int n, int lp, int* bad) {
-  const int tid = threadIdx.x;
-  if (tid == 0) *bad = 0;
+template <int NB>
+struct RegMat {
+  double v[NB][NB];
+};
+
+template <int NB>
+__device__ __forceinline__ void rm_zero(RegMat<NB>& m) {
+#pragma unroll
+  for (int p = 0; p < NB; ++p)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) m.v[p][q] = 0.0;
+}
+
+// Rows and columns >= n read as zero.
+template <int NB>
+__device__ __forceinline__ void rm_load(RegMat<NB>& m, const double* src, int n, int lp, int ty, int tx) {
+#pragma unroll
+  for (int p = 0; p < NB; ++p)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = ty + 16 * p, k = tx + 16 * q;
+      m.v[p][q] = (i < n && k < n) ? src[i * lp + k] : 0.0;
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void rm_store(const RegMat<NB>& m, double* dst, int lp, int ty, int tx) {
+#pragma unroll
+  for (int p = 0; p < NB; ++p)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) dst[(ty + 16 * p) * lp + tx + 16 * q] = m.v[p][q];
+}
+
+// In place: A (n x n, symmetric positive definite, upper triangle used) ->
+// upper R with A = R^T R; rows and columns >= n and the lower triangle end
+// zero.  Returns false (in every thread) on breakdown.  rowbuf: 2 lp.
+template <int NB>
+__device__ bool rb_chol_upper(RegMat<NB>& a, int n, int lp, double* rowbuf, int ty, int tx) {
+  if (ty == 0) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) rowbuf[tx + 16 * q] = a.v[0][q];
+  }
   __syncthreads();
+  bool ok = true;
   for (int j = 0; j < n; ++j) {
-    if (tid == 0) {
-      const double d = A[j * lp + j];
-      if (!(d > 0.0) || !isfinite(d)) *bad = 1;
-      else A[j * lp + j] = sqrt(d);
+    const double* rb = rowbuf + (j & 1) * lp;
+    const double d = rb[j];
+    if (!(d > 0.0) || !isfinite(d)) {  // the same value in every thread
+      ok = false;
+      break;
     }
-    __syncthreads();
-    if (*bad) return false;
-    const double rjj = A[j * lp + j];
-    for (int i = j + 1 + tid; i < n; i += kSmallThreads) A[j * lp + i] /= rjj;
-    __syncthreads();
-    const int m = n - j - 1;
-    for (int idx = tid; idx < m * m; idx += kSmallThreads) {
-      const int i = j + 1 + idx / m, k = j + 1 + idx % m;
-      if (k >= i) A[i * lp + k] -= A[j * lp + i] * A[j * lp + k];
+    const double rjj = sqrt(d), inv = 1.0 / rjj;
+    double si[NB], sk[NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p) si[p] = rb[ty + 16 * p] * inv;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) sk[q] = rb[tx + 16 * q] * inv;
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int i = ty + 16 * p;
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int k = tx + 16 * q;
+        if (i > j && k >= i && k < n) a.v[p][q] -= si[p] * sk[q];
+        else if (i == j) a.v[p][q] = k > j ? sk[q] : (k == j ? rjj : 0.0);  // row j of R
+      }
+    }
+    if (j + 1 < n && ty == ((j + 1) & 15)) {  // publish row j + 1 for the next step
+      const int p1 = (j + 1) >> 4;
+#pragma unroll
+      for (int p = 0; p < NB; ++p)
+        if (p == p1) {
+#pragma unroll
+          for (int q = 0; q < NB; ++q) rowbuf[((j + 1) & 1) * lp + tx + 16 * q] = a.v[p][q];
+        }
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < n * n; idx += kSmallThreads) {
-    const int i = idx / n, k = idx % n;
-    if (k < i) A[i * lp + k] = 0.0;
-  }
-  __syncthreads();
-  return true;
+#pragma unroll
+  for (int p = 0; p < NB; ++p)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = ty + 16 * p, k = tx + 16 * q;
+      if (!ok || k < i || i >= n || k >= n) a.v[p][q] = 0.0;
+    }
+  return ok;
 }
 
-// X = R^-1 for upper R (column j by thread j).
-__device__ void wg_inv_upper(const double* R, double* X, int n, int lp) {
-  for (int j = threadIdx.x; j < n; j += kSmallThreads) {
-    X[j * lp + j] = 1.0 / R[j * lp + j];
-    for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int k = i + 1; k <= j; ++k) s += R[i * lp + k] * X[k * lp + j];
-      X[i * lp + j] = -s / R[i * lp + i];
-    }
-    for (int i = j + 1; i < n; ++i) X[i * lp + j] = 0.0;
-  }
+// X = R^-1 for upper R (n x n, zero outside), by rows from the bottom:
+// X[i][c] = (delta_ic - sum_{k>i} R[i][k] X[k][c]) / R[i][i], the sums
+// accumulated as each row of X is published.  buf: 5 lp (diag, 2 x row, 2 x column).
+template <int NB>
+__device__ void rb_inv_upper(const RegMat<NB>& r, RegMat<NB>& x, int n, int lp, double* buf, int ty, int tx) {
+  double* diag = buf;
+  double* xrow = buf + lp;
+  double* rcol = buf + 3 * lp;
+#pragma unroll
+  for (int p = 0; p < NB; ++p)
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (ty + 16 * p == tx + 16 * q) diag[ty + 16 * p] = r.v[p][q];
+  rm_zero(x);
   __syncthreads();
+  for (int i = n - 1; i >= 0; --i) {
+    const int b = i & 1;
+    if (ty == (i & 15)) {
+      const int pi = i >> 4;
+      const double inv = 1.0 / diag[i];
+#pragma unroll
+      for (int p = 0; p < NB; ++p)
+        if (p == pi) {
+#pragma unroll
+          for (int q = 0; q < NB; ++q) {
+            const int c = tx + 16 * q;
+            const double v = (c >= i && c < n) ? ((c == i ? 1.0 : 0.0) - x.v[p][q]) * inv : 0.0;
+            x.v[p][q] = v;
+            xrow[b * lp + c] = v;
+          }
+        }
+    }
+    if (tx == (i & 15)) {
+      const int qi = i >> 4;
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+        if (q == qi) {
+#pragma unroll
+          for (int p = 0; p < NB; ++p) rcol[b * lp + ty + 16 * p] = r.v[p][q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      if (ty + 16 * p < i) {
+        const double rv = rcol[b * lp + ty + 16 * p];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) x.v[p][q] += rv * xrow[b * lp + tx + 16 * q];
+      }
+    }
+  }
+  __syncthreads();  // the buffers are free again
 }
 
-// C = A B (n x n), summed in k order.
-__device__ void wg_matmul(const double* A, const double* B, double* C, int n, int lp) {
-  for (int idx = threadIdx.x; idx < n * n; idx += kSmallThreads) {
-    const int i = idx / n, j = idx % n;
-    double c = 0.0;
-    for (int k = 0; k < n; ++k) c += A[i * lp + k] * B[k * lp + j];
-    C[i * lp + j] = c;
+// C = A B over k < n (A, B in LDS, pitch lp, zero outside n x n).
+template <int NB>
+__device__ void rb_matmul(const double* A, const double* B, RegMat<NB>& c, int n, int lp, int ty, int tx) {
+  rm_zero(c);
+  for (int k = 0; k < n; ++k) {
+    double av[NB], bv[NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p) av[p] = A[(ty + 16 * p) * lp + k];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) bv[q] = B[k * lp + tx + 16 * q];
+#pragma unroll
+    for (int p = 0; p < NB; ++p)
+#pragma unroll
+      for (int q = 0; q < NB; ++q) c.v[p][q] += av[p] * bv[q];
   }
-  __syncthreads();
 }
 
 // Householder diagonal signs of the thin QR whose Q has top block W (n x n,
-// destroyed): modified LU (Ballard et al. 2014), s_i = -sign(u_ii).
-__device__ void wg_householder_signs(double* W, int n, int lp, double* sgn) {
-  const int tid = threadIdx.x;
+// destroyed): modified LU, s_i = -sign(u_ii).  buf: 4 lp.
+template <int NB>
+__device__ void rb_householder_signs(RegMat<NB>& w, int n, int lp, double* buf, double* sgn, int ty, int tx) {
+  double* rowb = buf;
+  double* colb = buf + 2 * lp;
   for (int i = 0; i < n; ++i) {
-    if (tid == 0) {
-      const double si = W[i * lp + i] >= 0.0 ? -1.0 : 1.0;
-      W[i * lp + i] -= si;
-      sgn[i] = si;
+    const int b = i & 1;
+    if (ty == (i & 15) && tx == (i & 15)) {  // the pivot: w[i][i] -= s_i
+      const int pi = i >> 4;
+#pragma unroll
+      for (int p = 0; p < NB; ++p)
+        if (p == pi) {
+          const double si = w.v[p][p] >= 0.0 ? -1.0 : 1.0;
+          w.v[p][p] -= si;
+          sgn[i] = si;
+        }
+    }
+    if (ty == (i & 15)) {
+      const int pi = i >> 4;
+#pragma unroll
+      for (int p = 0; p < NB; ++p)
+        if (p == pi) {
+#pragma unroll
+          for (int q = 0; q < NB; ++q) rowb[b * lp + tx + 16 * q] = w.v[p][q];
+        }
+    }
+    if (tx == (i & 15)) {
+      const int qi = i >> 4;
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+        if (q == qi) {
+#pragma unroll
+          for (int p = 0; p < NB; ++p) colb[b * lp + ty + 16 * p] = w.v[p][q];
+        }
     }
     __syncthreads();
-    const double piv = W[i * lp + i];
-    for (int r = i + 1 + tid; r < n; r += kSmallThreads) W[r * lp + i] /= piv;
-    __syncthreads();
-    const int m = n - i - 1;
-    for (int idx = tid; idx < m * m; idx += kSmallThreads) {
-      const int r = i + 1 + idx / m, c = i + 1 + idx % m;
-      W[r * lp + c] -= W[r * lp + i] * W[i * lp + c];
+    const double inv = 1.0 / rowb[b * lp + i];  // the pivot
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int r = ty + 16 * p;
+      if (r > i && r < n) {
+        const double l = colb[b * lp + r] * inv;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int c = tx + 16 * q;
+          if (c > i && c < n) w.v[p][q] -= l * rowb[b * lp + c];
+        }
+      }
     }
-    __syncthreads();
   }
+  __syncthreads();
 }
 
 // First pass: R1 = chol(G1), T1 = R1^-1 (lp x lp, zero padded).
+template <int NB>
 __global__ void __launch_bounds__(kSmallThreads) k_thinq_first(const double* __restrict__ G1, int n, int lp,
                                                                double* __restrict__ R1, double* __restrict__ T1,
                                                                int* __restrict__ status) {
-  extern __shared__ __align__(16) unsigned char sm[];
-  __shared__ int bad;
-  double* A = reinterpret_cast<double*>(sm);
-  double* X = A + lp * lp;
-  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
-    A[v] = G1[v];
-    X[v] = 0.0;
-  }
-  __syncthreads();
-  const bool ok = wg_chol_upper(A, n, lp, &bad);
-  if (ok) wg_inv_upper(A, X, n, lp);
+  __shared__ double buf[5 * 16 * kThinqMaxNB];
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  RegMat<NB> a, x;
+  rm_load(a, G1, n, lp, ty, tx);
+  const bool ok = rb_chol_upper(a, n, lp, buf, ty, tx);
+  if (ok) rb_inv_upper(a, x, n, lp, buf, ty, tx);
+  else rm_zero(x);
   if (!ok && threadIdx.x == 0) *status = 1;
-  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
-    const int i = v / lp, j = v % lp;
-    const bool in = ok && i < n && j < n;
-    R1[v] = in ? A[v] : 0.0;
-    T1[v] = in ? X[v] : 0.0;
-  }
+  rm_store(a, R1, lp, ty, tx);
+  rm_store(x, T1, lp, ty, tx);
 }
 
 // Second pass: R2 = chol(G2), R^-1 = (R2 R1)^-1, the Householder signs s of
 // Xtop R^-1 (Xtop = the first n rows of the thinned matrix) and
-// T = R^-1 diag(s) (lp x lp, zero padded).
+// T = R^-1 diag(s) (lp x lp, zero padded).  Dynamic LDS: 2 lp x lp matrices.
+template <int NB>
 __global__ void __launch_bounds__(kSmallThreads) k_thinq_second(const double* __restrict__ G2,
                                                                 const double* __restrict__ R1,
                                                                 const double* __restrict__ Xtop, int n, int lp,
                                                                 double* __restrict__ T, int* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char sm[];
-  __shared__ int bad;
-  __shared__ double sgn[128];
-  double* M0 = reinterpret_cast<double*>(sm);
-  double* M1 = M0 + lp * lp;
-  double* M2 = M0 + 2 * lp * lp;
-  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
-    M0[v] = G2[v];
-    M1[v] = R1[v];
-    M2[v] = 0.0;
-  }
-  __syncthreads();
-  const bool ok = wg_chol_upper(M0, n, lp, &bad);  // R2
+  __shared__ double buf[5 * 16 * kThinqMaxNB];
+  __shared__ double sgn[16 * kThinqMaxNB];
+  double* MA = reinterpret_cast<double*>(sm);
+  double* MB = MA + lp * lp;
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  RegMat<NB> a, x;
+  rm_load(a, G2, n, lp, ty, tx);
+  const bool ok = rb_chol_upper(a, n, lp, buf, ty, tx);  // R2
   if (ok) {
-    wg_matmul(M0, M1, M2, n, lp);  // R = R2 R1
-    for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) M0[v] = 0.0;
+    RegMat<NB> t;
+    rm_store(a, MA, lp, ty, tx);
+    rm_load(t, R1, n, lp, ty, tx);
+    rm_store(t, MB, lp, ty, tx);
     __syncthreads();
-    wg_inv_upper(M2, M0, n, lp);  // R^-1
-    for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) M1[v] = (v / lp < n) ? Xtop[v] : 0.0;
+    rb_matmul(MA, MB, a, n, lp, ty, tx);  // R = R2 R1
+    rb_inv_upper(a, x, n, lp, buf, ty, tx);  // R^-1 (ends with a barrier: MA, MB are free)
+    rm_store(x, MB, lp, ty, tx);
+    rm_load(t, Xtop, n, lp, ty, tx);
+    rm_store(t, MA, lp, ty, tx);
     __syncthreads();
-    wg_matmul(M1, M0, M2, n, lp);  // Xtop R^-1
-    wg_householder_signs(M2, n, lp, sgn);
+    rb_matmul(MA, MB, t, n, lp, ty, tx);  // Xtop R^-1
+    rb_householder_signs(t, n, lp, buf, sgn, ty, tx);
+  } else {
+    rm_zero(x);
+    if (threadIdx.x < 16 * NB) sgn[threadIdx.x] = 1.0;
+    if (threadIdx.x == 0) *status = 1;
+    __syncthreads();
   }
-  if (!ok && threadIdx.x == 0) *status = 1;
-  for (int v = threadIdx.x; v < lp * lp; v += kSmallThreads) {
-    const int i = v / lp, j = v % lp;
-    T[v] = (ok && i < n && j < n) ? M0[v] * sgn[j] : 0.0;
-  }
+#pragma unroll
+  for (int p = 0; p < NB; ++p)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int c = tx + 16 * q;
+      T[(ty + 16 * p) * lp + c] = c < n ? x.v[p][q] * sgn[c] : 0.0;
+    }
 }
 
 // flipOmg's decision (src/RSVD.hpp:80-89): column l flips when
@@ -1273,32 +1411,43 @@ void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
   }
 }
 
-bool thinq_on_device(int lp) { return lp <= 48; }
+bool thinq_on_device(int lp) { return lp % 16 == 0 && lp <= 16 * kThinqMaxNB; }
 
 void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s) {
   require(thinq_on_device(lp) && n <= lp, kErrInternal, "thinq: lp too large for the device path");
+  switch (lp / 16) {
+    case 1: hipLaunchKernelGGL(k_thinq_first<1>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
+    case 2: hipLaunchKernelGGL(k_thinq_first<2>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
+    case 3: hipLaunchKernelGGL(k_thinq_first<3>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
+    case 4: hipLaunchKernelGGL(k_thinq_first<4>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
+    default: hipLaunchKernelGGL(k_thinq_first<5>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
+  }
+  check_launch("thinq_first");
+}
+
+template <int NB>
+void thinq_second_nb(const double* G2, const double* R1, const double* Xtop, int n, int lp, double* T, int* status,
+                     hipStream_t s) {
   const size_t lds = (size_t)2 * lp * lp * sizeof(double);
   static bool attr = false;
   if (!attr) {
-    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_first),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * 48 * 48 * sizeof(double))));
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_second<NB>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * 256 * NB * NB * sizeof(double))));
     attr = true;
   }
-  hipLaunchKernelGGL(k_thinq_first, dim3(1), dim3(kSmallThreads), lds, s, G1, n, lp, R1, T1, status);
-  check_launch("thinq_first");
+  hipLaunchKernelGGL(k_thinq_second<NB>, dim3(1), dim3(kSmallThreads), lds, s, G2, R1, Xtop, n, lp, T, status);
 }
 
 void launch_thinq_second(const double* G2, const double* R1, const double* Xtop, int n, int lp, double* T,
                          int* status, hipStream_t s) {
   require(thinq_on_device(lp) && n <= lp, kErrInternal, "thinq: lp too large for the device path");
-  const size_t lds = (size_t)3 * lp * lp * sizeof(double);
-  static bool attr = false;
-  if (!attr) {
-    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_thinq_second),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(3 * 48 * 48 * sizeof(double))));
-    attr = true;
+  switch (lp / 16) {
+    case 1: thinq_second_nb<1>(G2, R1, Xtop, n, lp, T, status, s); break;
+    case 2: thinq_second_nb<2>(G2, R1, Xtop, n, lp, T, status, s); break;
+    case 3: thinq_second_nb<3>(G2, R1, Xtop, n, lp, T, status, s); break;
+    case 4: thinq_second_nb<4>(G2, R1, Xtop, n, lp, T, status, s); break;
+    default: thinq_second_nb<5>(G2, R1, Xtop, n, lp, T, status, s); break;
   }
-  hipLaunchKernelGGL(k_thinq_second, dim3(1), dim3(kSmallThreads), lds, s, G2, R1, Xtop, n, lp, T, status);
   check_launch("thinq_second");
 }
 

@@ -169,11 +169,12 @@ def test_missing_entries_with_chunked_sources(ctx, emu):
     np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-8)
 
 
-@pytest.mark.parametrize("k", [8, 30])
+@pytest.mark.parametrize("k", [8, 20, 30, 40, 50])
 def test_thin_q_paths_match_oracle(ctx, golden_inputs, k):
     """The Omega update (thin-Q of H with Householder signs, then flipOmg) runs
-    its L x L steps on the device for small L (k = 8: L = 16) and on the host
-    for large L (k = 30: L = 60); both match the FP64 oracle over 5 epochs."""
+    its L x L steps on the device up to L = 80 (k = 8, 20, 30, 40: one to five
+    16 x 16 register blocks per thread) and on the host above (k = 50: L = 100);
+    every path matches the FP64 oracle over 5 epochs."""
     say(f"thin-Q path k={k}")
     prefix = golden_inputs["cfg1"]
     packed, n, m = O.read_plink(prefix)
