@@ -4,6 +4,7 @@
 // eigen-decomposition, Householder signs, stop rule) in one workgroup each.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <type_traits>
 
 #include "batch_kernels.hpp"
@@ -316,26 +317,49 @@ __global__ void __launch_bounds__(128) k_b_gram(BatchArgs a, const RowItem* __re
 }
 
 // Y = X T for the item's rows, T = t (tsel 0) or tv (tsel 1) of the cohort.
+// Thread (row slot rs, column pair cp) keeps columns 2 cp, 2 cp + 1 of T in
+// registers; X arrives in tiles of 64 rows staged in LDS with coalesced
+// loads, and each output sums its 32 products in column order.
+static_assert(sizeof(BatchSmall) % 16 == 0 && offsetof(BatchSmall, t) % 16 == 0 && offsetof(BatchSmall, tv) % 16 == 0,
+              "the transforms are read as 16-byte pairs");
+
 __global__ void __launch_bounds__(256) k_b_apply(BatchArgs a, const RowItem* __restrict__ items, int xsel, int ysel,
                                                  int tsel, int state_req) {
+  constexpr int TR = 64;
   const RowItem it = items[blockIdx.x];
   const BatchCohort& c = a.coh[it.cohort];
   if (c.st->state != state_req) return;
-  __shared__ double T[LP * LP];
+  __shared__ double2 xs[TR * LP / 2];
   const double* src = tsel ? c.st->tv : c.st->t;
-  for (int v = threadIdx.x; v < LP * LP; v += 256) T[v] = src[v];
-  __syncthreads();
+  const int cp = threadIdx.x & 15, rs = threadIdx.x >> 4;
+  double t0[LP], t1[LP];
+#pragma unroll
+  for (int m = 0; m < LP; ++m) {
+    const double2 tv = *reinterpret_cast<const double2*>(src + m * LP + 2 * cp);
+    t0[m] = tv.x;
+    t1[m] = tv.y;
+  }
   const double* X = c.mat[xsel];
   double* Y = c.mat[ysel];
-  const int n = (int)it.nrows * LP;
-  for (int v = threadIdx.x; v < n; v += 256) {
-    const uint64_t r = it.row0 + (uint64_t)(v / LP);
-    const int j = v % LP;
-    const double* x = X + r * LP;
-    double s = 0.0;
-#pragma unroll 8
-    for (int m = 0; m < LP; ++m) s += x[m] * T[m * LP + j];
-    Y[r * LP + j] = s;
+  for (uint32_t r0 = 0; r0 < it.nrows; r0 += TR) {
+    const int nr = (int)min<uint32_t>(TR, it.nrows - r0);
+    const double2* xg = reinterpret_cast<const double2*>(X + (it.row0 + r0) * LP);
+    __syncthreads();  // the previous tile has been read
+    for (int v = threadIdx.x; v < nr * LP / 2; v += 256) xs[v] = xg[v];
+    __syncthreads();
+    for (int r = rs; r < nr; r += 16) {
+      const double2* xr = xs + r * (LP / 2);
+      double y0 = 0.0, y1 = 0.0;
+#pragma unroll
+      for (int m2 = 0; m2 < LP / 2; ++m2) {
+        const double2 xv = xr[m2];
+        y0 += xv.x * t0[2 * m2];
+        y1 += xv.x * t1[2 * m2];
+        y0 += xv.y * t0[2 * m2 + 1];
+        y1 += xv.y * t1[2 * m2 + 1];
+      }
+      reinterpret_cast<double2*>(Y + (it.row0 + r0 + r) * LP)[cp] = double2{y0, y1};
+    }
   }
 }
 
