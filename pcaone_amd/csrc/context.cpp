@@ -506,6 +506,7 @@ void Context::set_omega(const double* om) {
 
 double* Context::omega_staging() {
   require(L > 0, kErrState, "configure() first");
+  omega_key_valid_ = false;  // the caller is about to write it
   omega_pinned_.resize((uint64_t)N * L * 8);
   return reinterpret_cast<double*>(omega_pinned_.get());
 }

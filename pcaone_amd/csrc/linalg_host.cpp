@@ -66,8 +66,11 @@ Mat transpose(const Mat& A) {
 // Householder reduction of a symmetric matrix to tridiagonal form followed by
 // the implicit-shift QL iteration (the EISPACK tred2/tql2 method).  V holds
 // the accumulated orthogonal transformation; on return A = V diag(d) V^T.
+// V is addressed column-major (element (i, j) at j n + i; the symmetric input
+// reads the same either way), so the method's inner loops over a column, and
+// the QL rotations of two eigenvector columns, run over contiguous memory.
 static void tred2(int n, std::vector<double>& V, std::vector<double>& d, std::vector<double>& e) {
-  auto v = [&](int i, int j) -> double& { return V[(size_t)i * n + j]; };
+  auto v = [&](int i, int j) -> double& { return V[(size_t)j * n + i]; };
   for (int j = 0; j < n; ++j) d[j] = v(n - 1, j);
   for (int i = n - 1; i > 0; --i) {
     double scale = 0.0, h = 0.0;
@@ -140,8 +143,15 @@ static void tred2(int n, std::vector<double>& V, std::vector<double>& d, std::ve
   e[0] = 0.0;
 }
 
+// sqrt(a^2 + b^2): the plain formula where the squares neither overflow nor
+// lose precision to underflow, std::hypot's careful scaling otherwise.
+static inline double norm2(double a, double b) {
+  const double s = a * a + b * b;
+  return (s > 1e-280 && s < 1e280) ? std::sqrt(s) : std::hypot(a, b);
+}
+
 static void tql2(int n, std::vector<double>& V, std::vector<double>& d, std::vector<double>& e) {
-  auto v = [&](int i, int j) -> double& { return V[(size_t)i * n + j]; };
+  auto v = [&](int i, int j) -> double& { return V[(size_t)j * n + i]; };
   for (int i = 1; i < n; ++i) e[i - 1] = e[i];
   e[n - 1] = 0.0;
   double f = 0.0, tst1 = 0.0;
@@ -160,7 +170,7 @@ static void tql2(int n, std::vector<double>& V, std::vector<double>& d, std::vec
         if (++iter > 200) fail(kErrNumeric, "symmetric eigensolver did not converge");
         double g = d[l];
         double p = (d[l + 1] - g) / (2.0 * e[l]);
-        double r = std::hypot(p, 1.0);
+        double r = norm2(p, 1.0);
         if (p < 0) r = -r;
         d[l] = e[l] / (p + r);
         d[l + 1] = e[l] * (p + r);
@@ -178,7 +188,7 @@ static void tql2(int n, std::vector<double>& V, std::vector<double>& d, std::vec
           s2 = s;
           g = c * e[i];
           h = c * p;
-          r = std::hypot(p, e[i]);
+          r = norm2(p, e[i]);
           e[i + 1] = s * r;
           s = e[i] / r;
           c = p / r;
@@ -221,7 +231,7 @@ void sym_eig_desc(const Mat& A, std::vector<double>& w, Mat& Vout) {
   Vout = Mat(n);
   for (int j = 0; j < n; ++j) {
     w[j] = d[idx[j]];
-    for (int i = 0; i < n; ++i) Vout(i, j) = V[(size_t)i * n + idx[j]];
+    for (int i = 0; i < n; ++i) Vout(i, j) = V[(size_t)idx[j] * n + i];  // column-major V
   }
 }
 

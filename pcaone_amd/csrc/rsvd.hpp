@@ -80,7 +80,6 @@ class RsvdOpData {
   Context& ctx_;
   const Param& params_;
   int nk_, os_;
-  bool omega_drawn_ = false;  // the test matrix of initOmg is in the context's staging memory
   void fit_with_pi();  // src/Data.cpp:293-349 via the device impute state
 };
 
