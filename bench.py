@@ -266,8 +266,8 @@ def main() -> int:
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU only: the randomized-SVD hot path runs on a "
-                                   "single MI355X (multi-GPU sharding is described in DESIGN.md, not run)",
+        print(json.dumps({"error": "one GPU only: the randomized-SVD hot path is measured on a single MI355X; "
+                                   "multi-GPU sharding is not run",
                           "n_gpus": args.gpus}))
         return 2
 

@@ -10,7 +10,7 @@ sys.path.insert(0, str(ROOT))
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X")
 
 
 def _ensure_built():

@@ -4,7 +4,7 @@ the L x L steps on the device, one HIP graph per run).
 Each cohort must reproduce what the single-matrix path (itself pinned against
 the reference's golden outputs) returns for the same matrix and flags:
 FP64 form to 1e-8 relative on the singular values and 1e-6 on the
-sign-aligned PCs; BF16/FP32 forms to the north-star bar against the FP64
+sign-aligned PCs; BF16/FP32 forms to the accuracy bar against the FP64
 result (eigenvalue relative error <= 1e-4, every PC correlated >= 0.9999).
 """
 import numpy as np

@@ -2,7 +2,7 @@
 
 Tolerances: the FP64 form must reproduce the reference outputs to their print
 precision (6 significant digits); the FP32 and BF16 hi/lo forms must meet the
-north-star bar against the FP64 oracle on the same Omega: eigenvalue relative
+accuracy bar against the FP64 oracle on the same Omega: eigenvalue relative
 error <= 1e-4 and every PC correlated >= 0.9999; so must the int8 form (two
 int8 pieces of 16-bit per-column fixed point, exact int32 accumulation).  The
 single-piece forms (FP16, BF16MIX, FP16MIX) meet the eigenvalue bar and
