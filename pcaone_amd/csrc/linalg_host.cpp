@@ -30,9 +30,15 @@ Mat chol_upper(const Mat& A) {
   return R;
 }
 
+// Rows (matmul) and columns (inv_upper) are independent: at the larger L they
+// are spread over a few host threads (more only add start-up latency at these
+// sizes); each element's arithmetic order is unchanged.
+[[maybe_unused]] constexpr int kParallelN = 48, kThreads = 8;  // (unused in the device pass)
+
 Mat inv_upper(const Mat& R) {
   const int n = R.n;
   Mat X(n);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(kThreads) if (n >= kParallelN)
   for (int j = n - 1; j >= 0; --j) {
     X(j, j) = 1.0 / R(j, j);
     for (int i = j - 1; i >= 0; --i) {
@@ -47,6 +53,7 @@ Mat inv_upper(const Mat& R) {
 Mat matmul(const Mat& A, const Mat& B) {
   const int n = A.n;
   Mat C(n);
+#pragma omp parallel for schedule(static) num_threads(kThreads) if (n >= kParallelN)
   for (int i = 0; i < n; ++i)
     for (int k = 0; k < n; ++k) {
       const double a = A(i, k);
