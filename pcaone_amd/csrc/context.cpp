@@ -33,6 +33,7 @@ Context::~Context() {
   if (stream) {
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
+    if (gram_ev_) (void)hipEventDestroy(gram_ev_);
     (void)hipStreamDestroy(stream);
   }
 }
@@ -717,16 +718,6 @@ void Context::zero_h(int which) {
   PCAONE_HIP(hipMemsetAsync(b.get(), 0, b.bytes(), stream));
 }
 
-void Context::check_device_status() {
-  if (!lstatus_.get()) return;
-  int st = 0;
-  PCAONE_HIP(hipMemcpy(&st, lstatus_.get(), sizeof(int), hipMemcpyDeviceToHost));
-  if (st) {
-    PCAONE_HIP(hipMemset(lstatus_.get(), 0, sizeof(int)));
-    fail(kErrNumeric, "Cholesky breakdown (Gram matrix not positive definite)");
-  }
-}
-
 void Context::omega_from(int src, bool flip) {
   require(L > 0, kErrState, "configure() first");
   if (src == 1) {
@@ -810,11 +801,82 @@ void Context::omega_from(int src, bool flip) {
 }
 
 EpochResult Context::finish_epoch(int k, bool have_prev) {
+  finish_begin();
+  return finish_end(k, have_prev);
+}
+
+void Context::finish_begin() {
+  require(L > 0, kErrState, "configure() first");
+  const uint64_t l2 = (uint64_t)lp * lp;
+  gram_pinned_.resize(2 * l2 * 8 + 64);
+  double* hp = reinterpret_cast<double*>(gram_pinned_.get());
+  if (!gram_ev_) PCAONE_HIP(hipEventCreateWithFlags(&gram_ev_, hipEventDisableTiming));
+  held_ = false;
+  // Gram of G~ (first QR of G, src/Halko.cpp:55-60)
+  double* gout = red_.get() + red_cap_;
+  launch_gram(gt_.get(), nullptr, nullptr, M, lp, red_.get(), red_cap_, gout, stream);
+  PCAONE_HIP(hipMemcpyAsync(hp, gout, l2 * 8, hipMemcpyDeviceToHost, stream));
+  // Gram of H (B B^T); the device copy serves the next epoch's thin-Q of H
+  gram_h_sent_ = gram_version_ != h_version_;
+  begin_h_version_ = h_version_;
+  if (gram_h_sent_) {
+    const bool dev = thinq_on_device(lp);
+    double* hout = dev ? ldev_.get() : red_.get() + red_cap_ + l2;
+    launch_gram(H_.get(), nullptr, nullptr, N, lp, red_.get(), red_cap_, hout, stream);
+    PCAONE_HIP(hipMemcpyAsync(hp + l2, hout, l2 * 8, hipMemcpyDeviceToHost, stream));
+    if (dev) gram_dev_version_ = h_version_;
+  }
+  // a breakdown in the device thin-Q of the last epoch
+  int* st = reinterpret_cast<int*>(hp + 2 * l2);
+  *st = 0;
+  if (lstatus_.get()) PCAONE_HIP(hipMemcpyAsync(st, lstatus_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipEventRecord(gram_ev_, stream));
+}
+
+bool Context::overlap_ok() const { return L > 0 && thinq_on_device(lp) && !time_kernels; }
+
+void Context::hold() {
+  if (gt_alt_.size() < gt_.size()) {
+    gt_alt_.resize(gt_.size());
+    PCAONE_HIP(hipMemsetAsync(gt_alt_.get(), 0, gt_alt_.bytes(), stream));
+  }
+  if (H_alt_.size() < H_.size()) {
+    H_alt_.resize(H_.size());
+    PCAONE_HIP(hipMemsetAsync(H_alt_.get(), 0, H_alt_.bytes(), stream));
+  }
+  std::swap(gt_, gt_alt_);
+  std::swap(H_, H_alt_);
+  held_ = true;
+}
+
+void Context::unhold() {
+  if (!held_) return;
+  std::swap(gt_, gt_alt_);
+  std::swap(H_, H_alt_);
+  held_ = false;
+  h_version_ = begin_h_version_;  // H is the held epoch's again
+}
+
+EpochResult Context::finish_end(int k, bool have_prev) {
   require(k >= 1 && k <= L, kErrArgument, "k out of range");
+  PCAONE_HIP(hipEventSynchronize(gram_ev_));
+  const uint64_t l2 = (uint64_t)lp * lp;
+  const double* hp = reinterpret_cast<const double*>(gram_pinned_.get());
+  if (*reinterpret_cast<const int*>(hp + 2 * l2)) {
+    PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    unhold();
+    fail(kErrNumeric, "Cholesky breakdown (Gram matrix not positive definite)");
+  }
+  auto from_pinned = [&](const double* src) {
+    la::Mat G(L);
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) G(i, j) = src[(size_t)i * lp + j];
+    return G;
+  };
   // QR of G twice (src/Halko.cpp:55-65): R = R2 R1, CholeskyQR2 with the
   // second Gram formed on the L x L side: Q1^T Q1 = R1^-T (G^T G) R1^-1.
-  const la::Mat G1 = gram_l(gt_.get(), M);
-  check_device_status();  // a breakdown in the device thin-Q of the last epoch
+  const la::Mat G1 = from_pinned(hp);
   const la::Mat R1 = la::chol_upper(G1);
   const la::Mat R1i = la::inv_upper(R1);
   const la::Mat R2 = la::chol_upper(la::matmul(la::transpose(R1i), la::matmul(G1, R1i)));
@@ -822,15 +884,12 @@ EpochResult Context::finish_epoch(int k, bool have_prev) {
   const la::Mat Rinv = la::inv_upper(R_);
   // B B^T = R^-T (H^T H) R^-1; B = R^-T H^T (src/Halko.cpp:68)
   la::Mat HH;
-  if (gram_version_ == h_version_) {
-    HH = gram_h_;
-  } else {
-    // the device copy serves the next epoch's thin-Q of H
-    const bool dev = thinq_on_device(lp);
-    HH = gram_l(H_.get(), N, nullptr, dev ? ldev_.get() : nullptr);
+  if (gram_h_sent_) {
+    HH = from_pinned(hp + l2);
     gram_h_ = HH;
-    gram_version_ = h_version_;
-    if (dev) gram_dev_version_ = h_version_;
+    gram_version_ = begin_h_version_;
+  } else {
+    HH = gram_h_;
   }
   const la::Mat C = la::matmul(la::transpose(Rinv), la::matmul(HH, Rinv));
   std::vector<double> w;
@@ -843,7 +902,7 @@ EpochResult Context::finish_epoch(int k, bool have_prev) {
   for (int i = 0; i < L; ++i)
     for (int j = 0; j < k; ++j) T(i, j) = sigma_[j] > 0 ? RW(i, j) / sigma_[j] : 0.0;
   set_t(T);
-  launch_apply(H_.get(), small_.get(), N, lp, ucur_.get(), stream);
+  launch_apply(held_ ? H_alt_.get() : H_.get(), small_.get(), N, lp, ucur_.get(), stream);
   EpochResult r;
   r.sigma = sigma_;
   if (have_prev) {

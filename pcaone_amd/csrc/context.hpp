@@ -125,8 +125,19 @@ class Context {
   void omega_from(int src, bool flip = true);
   void zero_h(int which);  // 1: H1, 2: H2
   // QR of G twice, B = R^-T H^T, eigen-decomposition of B B^T, Ucur; mev
-  // against the previous Ucur if have_prev.
+  // against the previous Ucur if have_prev.  finish_epoch = finish_begin +
+  // finish_end; between them a caller may enqueue the next epoch's device work
+  // (omega_from, hold, sketch): finish_begin has sent the Gram matrices of G~
+  // and H to the host, hold keeps this epoch's G~ and H aside for finish_end
+  // and compute_v, and unhold puts them back when that next epoch is not run.
   EpochResult finish_epoch(int k, bool have_prev);
+  void finish_begin();
+  EpochResult finish_end(int k, bool have_prev);
+  void hold();
+  void unhold();
+  // The next epoch's device work can be enqueued before finish_end: the thin-Q
+  // of H runs on the device and no kernel timing waits on the host.
+  bool overlap_ok() const;
   void save_prev();  // Upre := Ucur
   // Outputs of the last finish_epoch: U (N x k), V (M x k), column-major FP64.
   void download_u(int k, double* U);
@@ -156,6 +167,13 @@ class Context {
   PinnedBuf pinned_;
   PinnedBuf omega_pinned_;  // the run's test matrix, drawn straight into pinned memory
   bool omega_key_valid_ = false, omega_key_gauss_ = false;
+  // epoch overlap: this epoch's G~ / H set aside while the next one is computed
+  DevBuf<double> gt_alt_, H_alt_;
+  bool held_ = false;
+  PinnedBuf gram_pinned_;  // Gram matrices of G~ and H, thin-Q status (finish_begin)
+  hipEvent_t gram_ev_ = nullptr;
+  bool gram_h_sent_ = false;
+  uint64_t begin_h_version_ = 0;
   int omega_key_seed_ = 0, omega_key_l_ = 0;
   uint64_t omega_key_n_ = 0;
   DevBuf<double> dF_, cF_, sc_;
@@ -199,7 +217,6 @@ class Context {
   DevBuf<double> ldev_;
   DevBuf<int> lstatus_;
   uint64_t gram_dev_version_ = ~0ull;
-  void check_device_status();
 
   void build_derived();  // xt + missing lists from xs
   void pristine_columns();  // cF = F as read, sc = 1 (device)

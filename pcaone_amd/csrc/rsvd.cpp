@@ -145,10 +145,17 @@ void RsvdOpData::fit_with_pi() {
 }
 
 void RsvdOpData::computeUSV(int p, double tol) {
+  // src/Halko.cpp:46-97.  Where the operator allows it, epoch pi + 1's device
+  // work is enqueued between the two halves of epoch pi's host steps, so the
+  // L x L work on the host overlaps the next products; the results are the
+  // same, and a stop at epoch pi puts epoch pi's G~ and H back.
   const int nk = ranks();
+  bool ahead = false;  // epoch pi's products were enqueued during epoch pi - 1
   for (int pi = 0; pi <= p; ++pi) {
-    computeGandH(pi);
-    const EpochResult r = ctx_.finish_epoch(nk, pi > 0);
+    if (!ahead) computeGandH(pi);
+    ctx_.finish_begin();
+    ahead = pi < p && enqueue_next(pi + 1);
+    const EpochResult r = ctx_.finish_end(nk, pi > 0);
     if (info) info->epochs = pi + 1, info->total_epochs += 1;
     if (pi > 0) {
       const double diff = r.diff;
@@ -161,6 +168,8 @@ void RsvdOpData::computeUSV(int p, double tol) {
         if (params_.svd == 2 && std::pow(2, pi) < params_.bands) {
           p = (int)std::log2(params_.bands);
         } else {
+          if (ahead) ctx_.unhold();  // the enqueued next epoch is not used
+          ahead = false;
           S.assign(r.sigma.begin(), r.sigma.begin() + nk);
           ctx_.compute_v(nk);
           if (need_v_host && !device_uv) {
@@ -198,6 +207,16 @@ void NormalRsvdOpData::computeGandH(int pi) {
   }
   if (pi > 0) ctx_.omega_from(0, /*flip=*/plan == nullptr);
   ctx_.sketch(0, rows(), 0);
+}
+
+bool NormalRsvdOpData::enqueue_next(int pi) {
+  // computeGandH for pi > 0 with this epoch's G~ and H held aside between the
+  // thin-Q (which reads H) and the products (which overwrite G~ and H)
+  if (pi == 0 || params_.missme || device_uv || !ctx_.overlap_ok()) return false;
+  ctx_.omega_from(0, /*flip=*/plan == nullptr);
+  ctx_.hold();
+  ctx_.sketch(0, rows(), 0);
+  return true;
 }
 
 void FancyRsvdOpData::computeGandH(int pi) {

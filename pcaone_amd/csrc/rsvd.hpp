@@ -62,6 +62,13 @@ class RsvdOpData {
   int size() const { return nk_ + os_; }
 
   virtual void computeGandH(int pi) = 0;
+  // Enqueues epoch pi's device work while the host steps of epoch pi - 1 are
+  // still to come (the previous epoch's G~ and H held aside); false where the
+  // operator cannot.
+  virtual bool enqueue_next(int pi) {
+    (void)pi;
+    return false;
+  }
   void setFlags(bool is_update, bool is_standardize) {
     update = is_update;
     standardize = is_standardize;
@@ -87,6 +94,7 @@ class NormalRsvdOpData : public RsvdOpData {
  public:
   using RsvdOpData::RsvdOpData;
   void computeGandH(int pi) override;
+  bool enqueue_next(int pi) override;
   const BlockPlan* plan = nullptr;  // out-of-core block plan (null: in-core)
 };
 
