@@ -296,6 +296,28 @@ def test_out_of_core_host_thin_q_without_flip(ctx, golden_inputs):
     np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-7)
 
 
+@pytest.mark.parametrize("memory", [0.0, 0.002])
+def test_early_stop_matches_oracle(ctx, golden_inputs, memory):
+    """--tol-rsvd 8e-4 stops f1 at epoch 13 of 21.  The next epoch's products
+    are already enqueued when the stop rule fires; U, S, the loadings V (from
+    the stopping epoch's G) and the trace must be the oracle's."""
+    say(f"early stop, memory={memory}")
+    prefix = golden_inputs["f1"]
+    packed, n, _ = O.read_plink(prefix)
+    kw = dict(svd=1, k=5, maxp=20, tol=8e-4, memory=memory)
+    ref = O.run_pca_with_halko(O.Data.from_packed(packed, n, O.Params(**kw)))
+    assert ref.epochs == 13
+    ctx.read_bed(prefix)
+    r = ctx.run(api.default_params(precision="fp64", **kw), want_v=True)
+    assert r.info["epochs"] == 13
+    np.testing.assert_allclose(r.trace, ref.trace, atol=1e-9)
+    np.testing.assert_allclose(r.S, ref.S, rtol=1e-9)
+    np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-7)
+    Vref = np.empty_like(ref.V)
+    Vref[ref.perm if ref.perm is not None else slice(None)] = ref.V
+    np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=1e-7)
+
+
 def test_out_of_core_argument_errors(ctx, golden_inputs):
     say("out-of-core argument errors")
     ctx.read_bed(golden_inputs["f1"])
