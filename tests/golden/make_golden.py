@@ -100,7 +100,10 @@ def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
     stop = STOP_RE.findall(log)
     em = [float(m.group(2)) for m in EM_RE.finditer(log)]
     plan = [[int(g) for g in m.groups()] for m in PLAN_RE.finditer(log)]
-    for suf in (".eigvals", ".sigvals", ".eigvecs", ".loadings", ".mbim"):
+    sufs = [".eigvals", ".sigvals", ".eigvecs", ".loadings", ".mbim"]
+    if inp["committed"]:
+        sufs.append(".eigvecs2")  # the small inputs also keep the plink2-style file (src/Utils.cpp:225-238)
+    for suf in sufs:
         src = Path(str(outp) + suf)
         if src.exists():
             shutil.copy(src, dest / suf[1:])

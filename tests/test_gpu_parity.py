@@ -8,6 +8,8 @@ int8 pieces of 16-bit per-column fixed point, exact int32 accumulation).  The
 single-piece forms (FP16, BF16MIX, FP16MIX) meet the eigenvalue bar and
 correlate >= 0.999.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -357,3 +359,47 @@ def test_workgroup_shapes_agree(ctx, golden_inputs, form, wm, waves):
                                    waves_g=waves, waves_h=waves))
     np.testing.assert_allclose(r.E, base.E, rtol=1e-9)
     assert O.pc_correlations(base.U, r.U).min() > 1 - 1e-9
+
+
+# ---------------------------------------------------------------- output files (pcaone_run_bfile)
+def _rows(path):
+    return [line.split("\t") for line in Path(path).read_text().splitlines()]
+
+
+@pytest.mark.parametrize("case", ["f1_d1", "f1_d1_maf", "f1_d2", "f4_emu_d1", "f1_d2_ooc", "f4_emu_d2_ooc"])
+def test_output_files_match_reference(ctx, golden_inputs, case, tmp_path):
+    """pcaone_run_bfile writes the reference's files (src/Data.cpp:211-240,
+    src/Utils.cpp:225-238): .sigvals with its '#N,M' header, .eigvals,
+    .eigvecs, .eigvecs2 with the FID/IID columns of the .fam, and with -V the
+    loadings and the .mbim (the .bim columns plus the allele frequency) in the
+    original SNP order; numbers printed to 6 significant digits, tab-separated."""
+    meta = load_meta(case)
+    params, printv = params_from_args(meta["args"])
+    say(f"output files {case} '{meta['args']}'")
+    out = str(tmp_path / "run")
+    p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
+                           tol=params.tol, bands=params.bands, noshuffle=int(params.noshuffle), emu=int(params.emu),
+                           maxiter=params.maxiter, tolem=params.tolem, maf=params.maf, memory=params.memory,
+                           printv=int(printv), precision="fp64")
+    ctx.run_bfile(golden_inputs[meta["input"]], out, p)
+    d = GOLDEN / case
+    sig, sig_ref = _rows(out + ".sigvals"), _rows(d / "sigvals")
+    assert sig[0] == sig_ref[0]  # '#N,M'
+    np.testing.assert_allclose(np.array(sig[1], float), np.array(sig_ref[1], float), rtol=1e-5)
+    np.testing.assert_allclose(O.read_matrix(out + ".eigvals"), O.read_matrix(d / "eigvals"), rtol=1e-5)
+    U, Uref = O.read_matrix(out + ".eigvecs"), O.read_matrix(d / "eigvecs")
+    assert U.shape == Uref.shape
+    np.testing.assert_allclose(O.sign_align(Uref, U), Uref, atol=2e-6)
+    e2, e2_ref = _rows(out + ".eigvecs2"), _rows(d / "eigvecs2")
+    assert e2[0] == e2_ref[0] and len(e2) == len(e2_ref)
+    assert [r[:2] for r in e2] == [r[:2] for r in e2_ref]
+    if printv:
+        V, Vref = O.read_matrix(out + ".loadings"), O.read_matrix(d / "loadings")
+        np.testing.assert_allclose(O.sign_align(Vref, V), Vref, atol=2e-6)
+        mb, mb_ref = _rows(out + ".mbim"), _rows(d / "mbim")
+        assert [r[:6] for r in mb] == [r[:6] for r in mb_ref]
+        np.testing.assert_allclose([float(r[6]) for r in mb], [float(r[6]) for r in mb_ref], rtol=5e-6)
+    for suffix in (".eigvals", ".eigvecs"):
+        for line in Path(out + suffix).read_text().splitlines():
+            for x in line.split("\t"):
+                assert x == f"{float(x):.6g}", (suffix, x)
