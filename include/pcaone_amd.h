@@ -156,6 +156,9 @@ int pcaone_sketch(pcaone_ctx* ctx, uint64_t snp_begin, uint64_t snp_end, int h_m
  * source 0: H, 1: H = H1 + H2.  CholeskyQR2 with Householder-convention
  * column signs. */
 int pcaone_update_omega(pcaone_ctx* ctx, int source);
+/* Omega <- thin-Q(src) without flipOmg: the out-of-core --svd 1 branch
+ * (src/Halko.cpp:133-136). */
+int pcaone_thin_q_omega(pcaone_ctx* ctx, int source);
 int pcaone_zero_h(pcaone_ctx* ctx, int which); /* 1: H1, 2: H2 */
 /* One epoch of computeUSV after computeGandH (src/Halko.cpp:55-77): QR of G
  * twice, B = R^-T H^T, SVD of B via B B^T, Ucur, diff = 1 - mev(Ucur, Upre)

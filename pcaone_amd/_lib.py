@@ -100,6 +100,7 @@ _SIGS = {
     "pcaone_get_g": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
     "pcaone_get_h": (C.c_int, [_P, C.c_int, _P]),
     "pcaone_host_omega": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint64, _P]),
+    "pcaone_thin_q_omega": (C.c_int, [_P, C.c_int]),
     "pcaone_block_plan": (C.c_int, [_P, C.c_uint32, C.c_uint64, _P, _P, _P, _P, C.c_uint64]),
     "pcaone_permute_plink_order": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _P]),
     "pcaone_host_sym_eig": (C.c_int, [C.c_int, _P, _P, _P]),

@@ -140,6 +140,9 @@ class Context:
     def update_omega(self, source: int = 0):
         self._check(lib().pcaone_update_omega(self._h, source))
 
+    def thin_q_omega(self, source: int = 0):
+        self._check(lib().pcaone_thin_q_omega(self._h, source))
+
     def zero_h(self, which: int):
         self._check(lib().pcaone_zero_h(self._h, which))
 

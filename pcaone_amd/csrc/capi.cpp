@@ -283,6 +283,13 @@ int pcaone_update_omega(pcaone_ctx* c, int source) {
   });
 }
 
+int pcaone_thin_q_omega(pcaone_ctx* c, int source) {
+  return guard(c, [&](Context& x) {
+    require(source == 0 || source == 1, kErrArgument, "source must be 0 or 1");
+    x.omega_from(source, /*flip=*/false);
+  });
+}
+
 int pcaone_zero_h(pcaone_ctx* c, int which) {
   return guard(c, [&](Context& x) {
     require(which == 1 || which == 2, kErrArgument, "which must be 1 or 2");

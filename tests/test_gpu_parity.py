@@ -153,6 +153,28 @@ def test_operator_epoch_matches_oracle(ctx, golden_inputs):
     assert abs(diff - (1 - O.mev(vt2.T[:, :k], vt.T[:, :k]))) < 1e-11
 
 
+def test_operator_thin_q_without_flip(ctx, golden_inputs):
+    """The out-of-core --svd 1 branch's Omega update through the C ABI:
+    thin-Q of H with Householder signs and no flipOmg, then the products."""
+    say("operator thin-Q without flipOmg")
+    prefix = golden_inputs["f1"]
+    packed, n, m = O.read_plink(prefix)
+    k, L = 5, 15
+    ctx.read_bed(prefix)
+    ctx.configure("fp64", L)
+    ctx.standardize(2)
+    om = O.omega(112, True, n, L)
+    ctx.set_omega(om)
+    ctx.sketch(0, m, 0)
+    d = O.Data.from_packed(packed, n, O.Params(svd=1, k=k))
+    d.standardize_E()
+    H = d.G @ (d.G.T @ om)
+    ctx.thin_q_omega(0)
+    ctx.sketch(0, m, 0)
+    G2 = d.G.T @ O.thin_q(H)
+    np.testing.assert_allclose(ctx.get_g(0, m, L), G2, rtol=1e-10, atol=1e-10 * np.abs(G2).max())
+
+
 @pytest.mark.parametrize("emu", [False, True])
 def test_missing_entries_with_chunked_sources(ctx, emu):
     """Missing genotypes at a size where the FP64 form splits the H side's
