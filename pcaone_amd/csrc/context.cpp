@@ -269,8 +269,7 @@ void Context::permute_columns(const std::vector<int>& p, bool with_freq) {
     PCAONE_HIP(hipMemcpyAsync(dF_.get(), t.get(), M * 8, hipMemcpyDeviceToDevice, stream));
   }
   PCAONE_HIP(hipStreamSynchronize(stream));
-  std::swap(xs_, tmpx_);
-  tmpx_.release();
+  std::swap(xs_, tmpx_);  // the old layout stays allocated for the next permutation (every winSVD run has two)
   std::vector<int> np(M);
   perm_identity_ = true;
   for (uint64_t i = 0; i < M; ++i) {
