@@ -321,11 +321,29 @@ inline void sum_partials(double* partial, uint64_t nblocks, uint64_t width, doub
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
 constexpr int kTileRows = 32;
 
+// Upper 16 x 16 blocks (i <= j) of an NB x NB block grid, numbered row by
+// row: block t is (sym_block_i(t), sym_block_j(t)).
+__host__ __device__ constexpr int sym_block_i(int t, int nb) {
+  int i = 0;
+  while (t >= nb - i) t -= nb - i, ++i;
+  return i;
+}
+__host__ __device__ constexpr int sym_block_j(int t, int nb) {
+  int i = 0;
+  while (t >= nb - i) t -= nb - i, ++i;
+  return i + t;
+}
+template <int NB>
+constexpr int kSymPerWave = (NB * (NB + 1) / 2 + NB - 1) / NB;
+
 // FP64-MFMA Gram partials: block b sums X[r]^T Y[r] over rows [b*rpb, ...).
-// One wave per 16-row block a of the output computes all NB blocks (a, :);
-// row tiles of X (and Y) are staged in LDS.  v_mfma_f64_16x16x4: A = X^T
-// (lane l: X[k0 + l/16][16a + l%16]), B = Y (lane l: Y[k0 + l/16][16b + l%16]),
-// accumulator lane group q, register r = output row q + 4r.
+// Cross products: one wave per 16-row block a of the output computes all NB
+// blocks (a, :).  X^T X: the NB (NB + 1) / 2 upper blocks are dealt round
+// robin to the NB waves and each is also stored mirrored (the products are
+// commutative, so the result is exactly symmetric).  Row tiles of X (and Y) are
+// staged in LDS.  v_mfma_f64_16x16x4: A = X^T (lane l: X[k0 + l/16][16i +
+// l%16]), B = Y (lane l: Y[k0 + l/16][16j + l%16]), accumulator lane group q,
+// register r = output row q + 4r.
 template <int NB, bool CROSS>
 __global__ void __launch_bounds__(64 * NB) k_gram_mfma(const double* __restrict__ X, const double* __restrict__ Y,
                                                        uint64_t nrows, uint64_t rpb, double* __restrict__ partial) {
@@ -336,6 +354,14 @@ __global__ void __launch_bounds__(64 * NB) k_gram_mfma(const double* __restrict_
   const int q = lane >> 4, r16 = lane & 15;
   f64x4_t acc[NB];
   for (int b = 0; b < NB; ++b) acc[b] = f64x4_t{0, 0, 0, 0};
+  constexpr int S = kSymPerWave<NB>, NT = NB * (NB + 1) / 2;
+  int ci[S], cj[S];  // this wave's upper blocks in X^T X (unused for cross products)
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = __builtin_amdgcn_readfirstlane(a + NB * s);
+    ci[s] = t < NT ? 16 * sym_block_i(t, NB) : 0;
+    cj[s] = t < NT ? 16 * sym_block_j(t, NB) : 0;
+  }
   const uint64_t rb = (uint64_t)blockIdx.x * rpb;
   const uint64_t re = rb + rpb < nrows ? rb + rpb : nrows;
   for (uint64_t r0 = rb; r0 < re; r0 += kTileRows) {
@@ -347,22 +373,50 @@ __global__ void __launch_bounds__(64 * NB) k_gram_mfma(const double* __restrict_
       if constexpr (CROSS) ys[v] = in ? Y[r0 * LP + v] : 0.0;
     }
     __syncthreads();
-    const double* yt = CROSS ? ys : xs;
+    if constexpr (CROSS) {
 #pragma unroll
-    for (int k0 = 0; k0 < kTileRows; k0 += 4) {
-      const double av = xs[(k0 + q) * LP + 16 * a + r16];
+      for (int k0 = 0; k0 < kTileRows; k0 += 4) {
+        const double av = xs[(k0 + q) * LP + 16 * a + r16];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const double bv = yt[(k0 + q) * LP + 16 * b + r16];
-        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[b], 0, 0, 0);
+        for (int b = 0; b < NB; ++b) {
+          const double bv = ys[(k0 + q) * LP + 16 * b + r16];
+          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[b], 0, 0, 0);
+        }
+      }
+    } else {
+      // X^T X is symmetric: wave a takes the upper blocks (i <= j) numbered
+      // a, a + NB, a + 2 NB, ... (NB (NB + 1) / 2 blocks over NB waves)
+#pragma unroll
+      for (int k0 = 0; k0 < kTileRows; k0 += 4) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          if (a + NB * s < NT) {
+            const double av = xs[(k0 + q) * LP + ci[s] + r16];
+            const double bv = xs[(k0 + q) * LP + cj[s] + r16];
+            acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[s], 0, 0, 0);
+          }
+        }
       }
     }
   }
   double* o = partial + (uint64_t)blockIdx.x * LP * LP;
+  if constexpr (CROSS) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o[(16 * a + q + 4 * r) * LP + 16 * b + r16] = acc[b][r];
+      for (int r = 0; r < 4; ++r) o[(16 * a + q + 4 * r) * LP + 16 * b + r16] = acc[b][r];
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (a + NB * s < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          o[(ci[s] + q + 4 * r) * LP + cj[s] + r16] = acc[s][r];
+          if (ci[s] != cj[s]) o[(cj[s] + r16) * LP + ci[s] + q + 4 * r] = acc[s][r];  // the mirror block
+        }
+      }
+    }
+  }
 }
 
 // FP64-MFMA right multiplication Y = X T for rows [0, nrows): one wave per 16
