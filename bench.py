@@ -120,6 +120,9 @@ def cpu_baseline(args, spec_full, k, os_, maxp):
 
 
 CONFIG_RUNS = {
+    # the headline workload through the out-of-core schedule (only with --memory)
+    3: dict(params=dict(svd=1, k=40, oversamples=20, maxp=7, tol=0.0), forms=("bf16", "fp32", "fp64"),
+            what="--svd 1, k=40, L=80, 7 power iterations"),
     # winSVD, 64 windows, 10 epochs (--maxp 9 --tol-rsvd 0), k = 20 (L = 40)
     2: dict(params=dict(svd=2, k=20, oversamples=10, bands=64, maxp=9, tol=0.0), forms=("fp32", "bf16", "fp64"),
             what="winSVD (--svd 2 -w 64 --maxp 9 --tol-rsvd 0), k=20, L=40"),
@@ -194,12 +197,27 @@ def batch_line(args) -> int:
     return 0
 
 
+def block_plan(p, nsamples, nsnps):
+    """The library's block plan for `-m` (blocks of SNPs per pass)."""
+    import ctypes as C
+    from pcaone_amd import _lib
+    bs, nb, bf = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    start = np.zeros(1 << 16, dtype=np.uint64)
+    rc = _lib.lib().pcaone_block_plan(C.byref(p), nsamples, nsnps, C.byref(bs), C.byref(nb), C.byref(bf),
+                                      start.ctypes.data, start.size)
+    return {"blocksize": bs.value, "nblocks": nb.value, "band_factor": bf.value} if rc == 0 else None
+
+
 def config_line(args) -> int:
-    """One line of timings and errors (against the FP64 form) for another configuration."""
+    """One line of timings and errors (against the FP64 form) for another
+    configuration, or for any of them through the out-of-core schedule
+    (`--memory`, the reference's `-m`: blocks of SNPs per pass, genotypes
+    resident in HBM)."""
     from pcaone_amd import api, synth
     if args.config == 4:
         return batch_line(args)
-    run = CONFIG_RUNS[args.config]
+    run = dict(CONFIG_RUNS[args.config])
+    run["params"] = dict(run["params"], **({"memory": args.memory} if args.memory > 0 else {}))
     spec = synth.config(args.config)
     ctx = api.Context(0)
     t0 = time.perf_counter()
@@ -222,6 +240,9 @@ def config_line(args) -> int:
            "epochs": {f: r.info["total_epochs"] for f, r in res.items()},
            "error_vs_fp64": {f: errors(ref, r) for f, r in res.items() if f != "fp64"},
            "g_ms_h_ms_per_run": None}
+    if args.memory > 0:
+        out["memory_gb"] = args.memory
+        out["block_plan"] = block_plan(api.default_params(**run["params"]), spec.nsamples, spec.nsnps)
     ctx.time_kernels(True)
     rk = ctx.run(api.default_params(precision=forms[0], **run["params"]))
     ctx.time_kernels(False)
@@ -258,6 +279,8 @@ def main() -> int:
     ap.add_argument("--sequential", type=int, default=256,
                     help="with --config 4: cohorts also run one by one through the single-matrix path (0 = none)")
     ap.add_argument("--verbose", action="store_true", help="with --config: the library prints epochs and EM steps")
+    ap.add_argument("--memory", type=float, default=0.0,
+                    help="with --config: run the out-of-core schedule (-m GB: blocks of SNPs per pass)")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--pcie", action="store_true",
                     help="also time the host-buffer boundary: .bed rows from host memory into HBM (upload + finalize)")
@@ -273,7 +296,7 @@ def main() -> int:
 
     from pcaone_amd import api, synth
 
-    if args.config != 3:
+    if args.config != 3 or args.memory > 0:
         return config_line(args)
     spec = synth.config(3, nsamples=args.nsamples, nsnps=args.nsnps)
     k, os_, maxp = args.k, args.oversamples, args.maxp
