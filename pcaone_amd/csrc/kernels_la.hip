@@ -364,15 +364,31 @@ __global__ void __launch_bounds__(64 * NB) k_gram_mfma(const double* __restrict_
   }
   const uint64_t rb = (uint64_t)blockIdx.x * rpb;
   const uint64_t re = rb + rpb < nrows ? rb + rpb : nrows;
-  for (uint64_t r0 = rb; r0 < re; r0 += kTileRows) {
+  // The next row tile is loaded into registers while the current one is
+  // multiplied; rows past the block's end read as zero.
+  constexpr int PER = kTileRows * LP / (64 * NB);
+  static_assert(PER * 64 * NB == kTileRows * LP, "whole tiles per thread");
+  double px[PER], py[CROSS ? PER : 1];
+  auto fetch = [&](uint64_t r0) {
     const uint64_t nr = re - r0 < (uint64_t)kTileRows ? re - r0 : kTileRows;
-    __syncthreads();
-    for (int v = tid; v < kTileRows * LP; v += 64 * NB) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int v = tid + e * 64 * NB;
       const bool in = (uint64_t)(v / LP) < nr;
-      xs[v] = in ? X[r0 * LP + v] : 0.0;
-      if constexpr (CROSS) ys[v] = in ? Y[r0 * LP + v] : 0.0;
+      px[e] = in ? X[r0 * LP + v] : 0.0;
+      if constexpr (CROSS) py[e] = in ? Y[r0 * LP + v] : 0.0;
+    }
+  };
+  fetch(rb);
+  for (uint64_t r0 = rb; r0 < re; r0 += kTileRows) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      xs[tid + e * 64 * NB] = px[e];
+      if constexpr (CROSS) ys[tid + e * 64 * NB] = py[e];
     }
     __syncthreads();
+    if (r0 + kTileRows < re) fetch(r0 + kTileRows);
     if constexpr (CROSS) {
 #pragma unroll
       for (int k0 = 0; k0 < kTileRows; k0 += 4) {
