@@ -289,15 +289,31 @@ __global__ void __launch_bounds__(128) k_b_gram(BatchArgs a, const RowItem* __re
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = f64x4_t{0, 0, 0, 0};
   const uint64_t re = it.row0 + it.nrows;
-  for (uint64_t r0 = it.row0; r0 < re; r0 += kTileRows) {
+  // The next row tile is loaded into registers while the current one is
+  // multiplied; rows past the item's end read as zero.
+  constexpr int PER = kTileRows * LP / (64 * NB);
+  static_assert(PER * 64 * NB == kTileRows * LP, "whole tiles per thread");
+  double px[PER], py[CROSS ? PER : 1];
+  auto fetch = [&](uint64_t r0) {
     const uint64_t nr = re - r0 < (uint64_t)kTileRows ? re - r0 : kTileRows;
-    __syncthreads();
-    for (int v = tid; v < kTileRows * LP; v += 64 * NB) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int v = tid + e * 64 * NB;
       const bool in = (uint64_t)(v / LP) < nr;
-      xs[v] = in ? X[r0 * LP + v] : 0.0;
-      if constexpr (CROSS) ys[v] = in ? Y[r0 * LP + v] : 0.0;
+      px[e] = in ? X[r0 * LP + v] : 0.0;
+      if constexpr (CROSS) py[e] = in ? Y[r0 * LP + v] : 0.0;
+    }
+  };
+  if (it.row0 < re) fetch(it.row0);
+  for (uint64_t r0 = it.row0; r0 < re; r0 += kTileRows) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      xs[tid + e * 64 * NB] = px[e];
+      if constexpr (CROSS) ys[tid + e * 64 * NB] = py[e];
     }
     __syncthreads();
+    if (r0 + kTileRows < re) fetch(r0 + kTileRows);
     const double* yt = CROSS ? ys : xs;
 #pragma unroll
     for (int k0 = 0; k0 < kTileRows; k0 += 4) {
