@@ -106,6 +106,9 @@ __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t s
 // owns 4 adjacent columns (16-byte partial loads) of every RG-th row; its
 // beta sums stay in registers and are combined across row groups in LDS.
 constexpr int kEpiRows = 64;
+// LDS row pitch of the transposed pieces (element size wsz bytes): padded so
+// that the column groups of one store land in different banks.
+constexpr int epi_pitch(size_t wsz) { return kEpiRows + (wsz == 2 ? 2 : 1); }
 
 template <Form F, class PT>
 __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part, uint64_t nchunks, uint64_t stride,
@@ -117,8 +120,9 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
                                         typename std::conditional<F == Form::FP32, float, uint16_t>::type>::type;
   constexpr int P = F == Form::BF16 ? 2 : 1;
   extern __shared__ __align__(16) unsigned char sm[];
-  w_t* pt = reinterpret_cast<w_t*>(sm);  // P x lp x kEpiRows
-  double* bred = reinterpret_cast<double*>(sm + ((size_t)P * lp * kEpiRows * sizeof(w_t) + 15) / 16 * 16);
+  constexpr int PITCH = epi_pitch(sizeof(w_t));
+  w_t* pt = reinterpret_cast<w_t*>(sm);  // P x lp rows of PITCH (kEpiRows used)
+  double* bred = reinterpret_cast<double*>(sm + ((size_t)P * lp * PITCH * sizeof(w_t) + 15) / 16 * 16);
   const int CG = lp / 4, RG = 256 / CG;  // column groups of 4, row groups
   const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG;
   const bool act = rg < RG;
@@ -171,7 +175,7 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
         gr4[q] = s != 0.0 ? back / s : 0.0;
         b4[q] += beta * gr4[q];
 #pragma unroll
-        for (int p = 0; p < P; ++p) pt[(p * lp + l) * kEpiRows + r] = pc[p];
+        for (int p = 0; p < P; ++p) pt[(p * lp + l) * PITCH + r] = pc[p];
       }
       *reinterpret_cast<double2*>(gout + t) = double2{gr4[0], gr4[1]};
       *reinterpret_cast<double2*>(gout + t + 2) = double2{gr4[2], gr4[3]};
@@ -183,7 +187,7 @@ __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part
   w_t* w = static_cast<w_t*>(Wt);
   for (int v = threadIdx.x; v < P * lp * kEpiRows; v += 256) {
     const int r = v % kEpiRows, pl = v / kEpiRows, p = pl / lp, l = pl - p * lp;
-    if (r < nr) w[p * wpiece + l * wpitch + r0 + r] = pt[v];
+    if (r < nr) w[p * wpiece + l * wpitch + r0 + r] = pt[pl * PITCH + r];
   }
   for (int l = threadIdx.x; l < lp; l += 256) {
     double sum = 0.0;
@@ -1306,7 +1310,7 @@ static void epi_fused_t(const void* P, uint64_t nchunks, uint64_t stride, uint64
   constexpr int NP = F == Form::BF16 ? 2 : 1;
   constexpr size_t wsz = F == Form::FP64 ? 8 : (F == Form::FP32 ? 4 : 2);
   // pieces (transposed) + the row groups' beta sums (256 / (lp / 4) groups x lp)
-  const auto lds_for = [](int l) { return ((size_t)kEpiRows * l * NP * wsz + 15) / 16 * 16 + (size_t)1024 * 8; };
+  const auto lds_for = [](int l) { return ((size_t)epi_pitch(wsz) * l * NP * wsz + 15) / 16 * 16 + (size_t)1024 * 8; };
   const size_t lds = lds_for(lp);
   static bool attr = false;
   if (!attr) {
