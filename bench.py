@@ -317,6 +317,7 @@ def main() -> int:
         log("copy probe: 1 GiB")
     only = set(args.only.split(",")) if args.only else None
     results = {}
+    omega_ms = {}  # host time of the Omega draw inside each timed run, per form
 
     def timed(form, steps, warmup):
         for i in range(warmup):
@@ -324,11 +325,13 @@ def main() -> int:
             log(f"{form} warmup {i}: {r.info['seconds']:.3f}s")
         ts = []
         r = None
+        omega_ms.setdefault(form, [])
         for i in range(steps):
             t0 = time.perf_counter()
             r = ctx.run(params(form))
             ts.append(time.perf_counter() - t0)
-            log(f"{form} step {i}: {ts[-1]:.3f}s ({r.info['epochs']} epochs)")
+            omega_ms[form].append(r.info["omega_ms"])
+            log(f"{form} step {i}: {ts[-1]:.3f}s ({r.info['epochs']} epochs, Omega draw {r.info['omega_ms']:.1f} ms)")
         return r, ts
 
     head = args.head
@@ -394,6 +397,9 @@ def main() -> int:
                      "algorithmic_flop_per_launch": flops_launch,
                      "other_product_avg_ms": round(g_avg if dom.startswith("H") else h_avg, 3)},
         "setup_s": round(setup_s, 2),
+        # every step draws Omega again on the host (libstdc++ engine, parallel), as the reference's
+        # computeUSV does at epoch 0; this is that draw's share of ms_per_step
+        "omega_draw_ms_per_step": round(float(np.mean(omega_ms[head])), 2),
     }
     if "fp32" in results:
         t32 = float(np.mean(ts_fp32))

@@ -90,6 +90,7 @@ typedef struct pcaone_run_info {
   double seconds;    /* wall time of the run (inputs resident)      */
   double g_ms, h_ms; /* summed device time of the two products      */
   uint64_t g_launches, h_launches;
+  double omega_ms;   /* host time of the Omega draws (initOmg), included in seconds */
 } pcaone_run_info;
 
 /* Synthetic Balding-Nichols PLINK data (test and benchmark input; no
@@ -116,6 +117,11 @@ int pcaone_close(pcaone_ctx* ctx);
 void pcaone_params_default(pcaone_params* p);
 
 /* ---- Data boundary (FileBed / Data) ------------------------------------ */
+/* FileBed::FileBed (src/FilePlink.hpp:10-30) without a device: N = lines of
+ * <prefix>.fam, M = lines of <prefix>.bim, the .bed opened and its 3-byte
+ * magic 6c 1b 01 checked.  PCAONE_ERR_IO with the reference's messages
+ * ("Cannot open bed file.", "Incorrect magic number in plink bed file."). */
+int pcaone_bed_dims(const char* prefix, uint32_t* nsamples, uint64_t* nsnps);
 /* Replaces FileBed::FileBed + read_all (src/FilePlink.hpp:10-30,
  * src/FilePlink.cpp:26-120) and the --maf filter (src/Data.cpp:88-105): reads
  * <prefix>.fam/.bim/.bed, checks the magic, moves the payload to HBM

@@ -41,7 +41,8 @@ class Params(C.Structure):
 class RunInfo(C.Structure):
     _fields_ = [("epochs", C.c_int), ("total_epochs", C.c_int), ("em_iters", C.c_int), ("n_trace", C.c_int),
                 ("n_sv", C.c_int), ("seconds", C.c_double), ("g_ms", C.c_double), ("h_ms", C.c_double),
-                ("g_launches", C.c_uint64), ("h_launches", C.c_uint64)]
+                ("g_launches", C.c_uint64), ("h_launches", C.c_uint64),
+                ("omega_ms", C.c_double)]
 
 
 _lib = None
@@ -61,6 +62,7 @@ _SIGS = {
     "pcaone_close": (C.c_int, [_P]),
     "pcaone_params_default": (None, [_P]),
     "pcaone_bed_read": (C.c_int, [_P, C.c_char_p, C.c_double, _P, _P]),
+    "pcaone_bed_dims": (C.c_int, [C.c_char_p, _P, _P]),
     "pcaone_bed_reserve": (C.c_int, [_P, C.c_uint32, C.c_uint64]),
     "pcaone_bed_upload": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
     "pcaone_bed_finalize": (C.c_int, [_P]),

@@ -134,6 +134,7 @@ void fill_info(pcaone_run_info* info, const RunInfo& r, const Context& c, int ns
   info->h_ms = c.ktimes.h_ms;
   info->g_launches = c.ktimes.g_launches;
   info->h_launches = c.ktimes.h_launches;
+  info->omega_ms = r.omega_ms;
 }
 
 }  // namespace
@@ -195,6 +196,15 @@ int pcaone_bed_read(pcaone_ctx* c, const char* prefix, double maf, uint32_t* nsa
     read_plink(x, prefix, p);
     if (nsamples) *nsamples = x.N;
     if (nsnps) *nsnps = x.M;
+  });
+}
+
+int pcaone_bed_dims(const char* prefix, uint32_t* nsamples, uint64_t* nsnps) {
+  return guard_global([&] {
+    require(prefix != nullptr, kErrArgument, "null prefix");
+    const PlinkInfo pi = plink_dims(prefix);
+    if (nsamples) *nsamples = (uint32_t)pi.nsamples;
+    if (nsnps) *nsnps = pi.nsnps_file;
   });
 }
 

@@ -506,7 +506,6 @@ void Context::set_omega(const double* om) {
 
 double* Context::omega_staging() {
   require(L > 0, kErrState, "configure() first");
-  omega_key_valid_ = false;  // the caller is about to write it
   omega_pinned_.resize((uint64_t)N * L * 8);
   return reinterpret_cast<double*>(omega_pinned_.get());
 }
@@ -854,6 +853,8 @@ void Context::unhold() {
   std::swap(H_, H_alt_);
   held_ = false;
   h_version_ = begin_h_version_;  // H is the held epoch's again
+  // the discarded epoch's thin-Q may have flagged a breakdown: not this run's
+  if (lstatus_.get()) PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
 }
 
 EpochResult Context::finish_end(int k, bool have_prev) {

@@ -107,15 +107,6 @@ class Context {
   // sends what was written there, as set_omega does, without a host copy.
   double* omega_staging();
   void set_omega_staged();
-  // The staged matrix is the draw of (seed, distribution) at the current N x L:
-  // runs with the same key reuse it instead of drawing again.
-  bool omega_staged_is(int seed, bool gaussian) const {
-    return omega_key_valid_ && omega_key_seed_ == seed && omega_key_gauss_ == gaussian && omega_key_n_ == N &&
-           omega_key_l_ == L;
-  }
-  void mark_omega_staged(int seed, bool gaussian) {
-    omega_key_valid_ = true, omega_key_seed_ = seed, omega_key_gauss_ = gaussian, omega_key_n_ = N, omega_key_l_ = L;
-  }
   // G rows [j0, j1) = A[j0:j1] Omega and the H contribution of those SNPs:
   // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ...
   void sketch(uint64_t j0, uint64_t j1, int hmode);
@@ -166,7 +157,6 @@ class Context {
   DevBuf<uint8_t> raw_;
   PinnedBuf pinned_;
   PinnedBuf omega_pinned_;  // the run's test matrix, drawn straight into pinned memory
-  bool omega_key_valid_ = false, omega_key_gauss_ = false;
   // epoch overlap: this epoch's G~ / H set aside while the next one is computed
   DevBuf<double> gt_alt_, H_alt_;
   bool held_ = false;
@@ -174,8 +164,6 @@ class Context {
   hipEvent_t gram_ev_ = nullptr;
   bool gram_h_sent_ = false;
   uint64_t begin_h_version_ = 0;
-  int omega_key_seed_ = 0, omega_key_l_ = 0;
-  uint64_t omega_key_n_ = 0;
   DevBuf<double> dF_, cF_, sc_;
   DevBuf<uint64_t> rp_s_, rp_t_;
   DevBuf<uint32_t> ix_s_, ix_t_;

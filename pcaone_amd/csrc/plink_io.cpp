@@ -67,7 +67,9 @@ std::vector<std::string> split_ws(const std::string& s) {
 
 }  // namespace
 
-PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& params) {
+PlinkInfo plink_dims(const std::string& prefix) {
+  // FileBed::FileBed (src/FilePlink.hpp:10-30): N and M from the line counts
+  // of .fam and .bim, then the .bed opened and its magic checked
   PlinkInfo info;
   info.nsamples = count_lines(prefix + ".fam");
   info.nsnps_file = count_lines(prefix + ".bim");
@@ -79,6 +81,14 @@ PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& param
   bed.read(reinterpret_cast<char*>(header), 3);
   require(bed.gcount() == 3 && header[0] == 0x6c && header[1] == 0x1b && header[2] == 0x01, kErrIO,
           "Incorrect magic number in plink bed file.");
+  return info;
+}
+
+PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& params) {
+  PlinkInfo info = plink_dims(prefix);
+  std::ifstream bed(prefix + ".bed", std::ios::binary);
+  require(bed.is_open(), kErrIO, "Cannot open bed file.");
+  bed.seekg(3, std::ios::beg);
   const uint64_t bps = bytes_per_snp(info.nsamples);
   ctx.reserve((uint32_t)info.nsamples, info.nsnps_file);
   const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / bps);

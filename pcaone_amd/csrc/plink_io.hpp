@@ -13,6 +13,8 @@ struct PlinkInfo {
   std::vector<uint32_t> keep;  // kept SNP indices when --maf filtered
 };
 
+// FileBed constructor: dimensions and the .bed magic (no device work).
+PlinkInfo plink_dims(const std::string& prefix);
 PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& params);
 void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, const std::string& bfile,
                    const std::string& out, const PlinkInfo& info);

@@ -83,38 +83,28 @@ void omega(int seed, bool gaussian, uint64_t nrows, uint64_t ncols, double* out)
     return;
   }
   // A round covers the attempts the remaining values need on average (the
-  // polar method accepts pi/4 of them) plus 2%, split into 256 blocks.
-  // Two passes per round: count the accepted attempts of every block (no
-  // log/sqrt), then redraw each block and write its values at its prefix
-  // offset directly into `out`.
+  // polar method accepts pi/4 of them) plus 2%, split into 256 blocks.  Each
+  // block evaluates its attempts once, keeping its values in a block-local
+  // buffer; the buffers are then concatenated in block order at their prefix
+  // offsets.
   const int nb = 256;
   std::vector<uint64_t> cnt(nb + 1);
+  // block buffers of the calling thread, kept across calls (no page faults on
+  // every draw); the reference makes the parallel region share this thread's copy
+  static thread_local std::vector<std::vector<double>> tl_vals;
+  std::vector<std::vector<double>>& vals = tl_vals;
+  vals.resize(nb);
   uint64_t filled = 0, next = 0;  // next attempt index
   while (filled < total) {
     const uint64_t want = (uint64_t)((double)(total - filled) / 2.0 / 0.785398 * 1.02) + 64;
     const uint64_t B = (want + nb - 1) / nb;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(dynamic, 4)
     for (int b = 0; b < nb; ++b) {
       uint64_t x = x0 * powmod(kA, 4 * (next + (uint64_t)b * B)) % kM;
-      uint64_t c = 0;
+      std::vector<double>& v = vals[b];
+      v.resize(2 * B);
+      uint64_t o = 0;
       for (uint64_t j = 0; j < B; ++j) {
-        const double px = 2.0 * canonical(x) - 1.0;
-        const double py = 2.0 * canonical(x) - 1.0;
-        const double r2 = px * px + py * py;
-        c += !(r2 > 1.0 || r2 == 0.0);
-      }
-      cnt[b + 1] = 2 * c;
-    }
-    cnt[0] = 0;
-    for (int b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
-    const uint64_t room = total - filled;
-#pragma omp parallel for schedule(static)
-    for (int b = 0; b < nb; ++b) {
-      uint64_t o = cnt[b];
-      if (o >= room) continue;
-      uint64_t x = x0 * powmod(kA, 4 * (next + (uint64_t)b * B)) % kM;
-      double* dst = out + filled;
-      for (uint64_t j = 0; j < B && o < room; ++j) {
         const double u1 = canonical(x);
         const double u2 = canonical(x);
         const double px = 2.0 * u1 - 1.0;
@@ -122,9 +112,20 @@ void omega(int seed, bool gaussian, uint64_t nrows, uint64_t ncols, double* out)
         const double r2 = px * px + py * py;
         if (r2 > 1.0 || r2 == 0.0) continue;
         const double mult = std::sqrt(-2 * std::log(r2) / r2);
-        dst[o++] = py * mult * 1.0 + 0.0;  // ret * stddev + mean
-        if (o < room) dst[o++] = px * mult * 1.0 + 0.0;
+        v[o++] = py * mult * 1.0 + 0.0;  // ret * stddev + mean
+        v[o++] = px * mult * 1.0 + 0.0;
       }
+      cnt[b + 1] = o;
+    }
+    cnt[0] = 0;
+    for (int b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    const uint64_t room = total - filled;
+    double* dst = out + filled;
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nb; ++b) {
+      if (cnt[b] >= room) continue;
+      const uint64_t n = std::min(cnt[b + 1], room) - cnt[b];
+      std::copy(vals[b].begin(), vals[b].begin() + (int64_t)n, dst + cnt[b]);
     }
     filled += std::min(cnt[nb], room);
     next += (uint64_t)nb * B;

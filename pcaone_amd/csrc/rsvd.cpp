@@ -117,16 +117,13 @@ RsvdOpData::RsvdOpData(Context& ctx, const Param& p, int k, int os) : ctx_(ctx),
 }
 
 // src/Halko.cpp:15-23 with src/RSVD.hpp:25-59: libstdc++ default_random_engine
-// seeded with --seed; one draw per element in column-major order.  The
-// matrix depends only on (seed, distribution, N, L), so it is drawn once into
-// the context's pinned staging memory and re-sent by every later call and run
-// with the same key.
+// seeded with --seed; one draw per element in column-major order, drawn
+// again at epoch 0 of every computeUSV as the reference does.  The draw runs
+// in parallel straight into the context's pinned staging memory.
 void RsvdOpData::initOmg() {
-  const bool gaussian = params_.rand != 0;
-  if (!ctx_.omega_staged_is(params_.seed, gaussian)) {
-    rng::omega(params_.seed, gaussian, cols(), (uint64_t)size(), ctx_.omega_staging());
-    ctx_.mark_omega_staged(params_.seed, gaussian);
-  }
+  const auto t0 = std::chrono::steady_clock::now();
+  rng::omega(params_.seed, params_.rand != 0, cols(), (uint64_t)size(), ctx_.omega_staging());
+  if (info) info->omega_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   ctx_.set_omega_staged();
 }
 

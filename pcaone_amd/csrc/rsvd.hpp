@@ -48,6 +48,7 @@ struct RunInfo {
   std::vector<double> trace;      // diff of every checked epoch (all computeUSV calls)
   std::vector<double> em_trace;   // EM convergence diffs
   double seconds = 0.0;           // wall time of run_pca_with_halko
+  double omega_ms = 0.0;          // host time of the Omega draws (part of seconds)
 };
 
 class RsvdOpData {

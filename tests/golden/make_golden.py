@@ -28,6 +28,8 @@ HERE = Path(__file__).resolve().parent
 ROOT = HERE.parents[1]
 sys.path.insert(0, str(ROOT))
 
+import numpy as np  # noqa: E402
+
 from pcaone_amd import synth  # noqa: E402
 
 REF = ROOT / "oracle" / "_ref" / "pcaone_ref"
@@ -37,7 +39,29 @@ INPUTS = {
     "f1": dict(spec=synth.Spec(seed=2001, nsamples=200, nsnps=2000, npops=3), commit=True),
     "f4": dict(spec=synth.Spec(seed=2004, nsamples=300, nsnps=3000, npops=3, miss_rate=0.1), commit=True),
     "cfg1": dict(spec=synth.config(1), commit=False),
+    # edge cases: ragged N (150), monomorphic, all-missing and single-genotype SNPs
+    "f5": dict(spec=synth.Spec(seed=2005, nsamples=150, nsnps=1200, npops=3, miss_rate=0.05), commit=True,
+               edit="edge"),
 }
+
+
+def edge_rows(rows, n):
+    """Overwrites a few SNPs of a packed matrix (rows: M x ceil(n/4) bytes) with
+    the reference's degenerate cases: all code 00 (F = 1), all code 11 (F = 0),
+    all missing (no count, F = 0, src/FilePlink.cpp:51-52), one genotype among
+    missing entries (F = 0.5, and F = 1), all heterozygous (centered to zero)."""
+    def fill(j, codes):
+        row = [0] * rows.shape[1]
+        for i, c in enumerate(codes):
+            row[i // 4] |= (c & 3) << (2 * (i % 4))
+        rows[j] = row
+    fill(3, [0] * n)
+    fill(10, [3] * n)
+    fill(17, [1] * n)
+    fill(40, [1] * 70 + [2] + [1] * (n - 71))
+    fill(41, [1] * 5 + [0] + [1] * (n - 6))
+    fill(100, [2] * n)
+    return rows
 
 CASES = {
     "f1_d1": ("f1", "-d 1 -k 5 --maxp 7 -V"),
@@ -61,6 +85,12 @@ CASES = {
     # 3000 SNPs in 16 bands (the shuffle's remainder path)
     "f4_emu_d1_ooc": ("f4", "-d 1 -k 3 --emu --maxiter 5 -V -m 0.002"),
     "f4_emu_d2_ooc": ("f4", "-d 2 -w 16 -k 3 --emu --maxiter 5 -V -m 0.0009"),
+    # degenerate SNPs (monomorphic, all missing, one genotype) through each driver
+    "f5_d1": ("f5", "-d 1 -k 4 --maxp 6 -V"),
+    "f5_d2": ("f5", "-d 2 -w 16 -k 4 --maxp 7 -V"),
+    "f5_emu": ("f5", "-d 1 -k 3 --emu --maxiter 4 -V"),
+    "f5_emu_d2": ("f5", "-d 2 -w 16 -k 3 --emu --maxiter 4 -V"),
+    "f5_maf": ("f5", "-d 1 -k 4 --maf 0.02 --maxp 5 -V"),
 }
 
 EPOCH_RE = re.compile(r"running of epoch =\s*(\d+), diff = ([-0-9.e+]+)")
@@ -81,6 +111,11 @@ def make_inputs(tmp: Path) -> dict:
         spec = d["spec"]
         prefix = (DATA if d["commit"] else tmp) / name
         synth.write_plink(spec, str(prefix))
+        if d.get("edit") == "edge":
+            bed = Path(str(prefix) + ".bed")
+            raw = bed.read_bytes()
+            rows = np.frombuffer(raw[3:], dtype=np.uint8).reshape(spec.nsnps, -1).copy()
+            bed.write_bytes(raw[:3] + edge_rows(rows, spec.nsamples).tobytes())
         out[name] = dict(prefix=str(prefix), sha256=sha256(Path(str(prefix) + ".bed")), spec=spec.__dict__,
                          committed=d["commit"])
     return out

@@ -172,3 +172,43 @@ def test_permute_plink_order_single_buffer_is_band_interleave():
     out = np.empty(m, dtype=np.int32)
     _lib.check(_lib.lib().pcaone_permute_plink_order(n, m, 2, w, out.ctypes.data))
     assert np.array_equal(out, np.arange(m).reshape(m // w, w).T.ravel())
+
+
+# ---------------------------------------------------------------- FileBed constructor (no device)
+def _bed_dims(prefix):
+    n, m = C.c_uint32(), C.c_uint64()
+    st = _lib.lib().pcaone_bed_dims(str(prefix).encode(), C.byref(n), C.byref(m))
+    return st, n.value, m.value, _lib.lib().pcaone_last_error(None).decode()
+
+
+def _write_set(prefix, n, m, header=b"\x6c\x1b\x01"):
+    Path(str(prefix) + ".fam").write_text("".join(f"f{i} i{i} 0 0 0 -9\n" for i in range(n)))
+    Path(str(prefix) + ".bim").write_text("".join(f"1 s{j} 0 {j + 1} A G\n" for j in range(m)))
+    Path(str(prefix) + ".bed").write_bytes(header + bytes(((n + 3) // 4) * m))
+
+
+def test_bed_dims_counts_fam_and_bim_lines(tmp_path):
+    """src/FilePlink.hpp:14-17: N = lines of .fam, M = lines of .bim."""
+    _write_set(tmp_path / "ok", 13, 7)
+    st, n, m, _ = _bed_dims(tmp_path / "ok")
+    assert (st, n, m) == (0, 13, 7)
+
+
+@pytest.mark.parametrize("header", [b"\x6c\x1b\x00", b"\x00\x1b\x01", b"\x6c\x00\x01", b"\x6c\x1b", b""])
+def test_bed_bad_magic_is_an_io_error(tmp_path, header):
+    """src/FilePlink.hpp:24-27: a .bed whose first three bytes are not 6c 1b 01
+    (SNP-major PLINK) is rejected with the reference's message."""
+    _write_set(tmp_path / "bad", 8, 3, header=header)
+    st, _, _, msg = _bed_dims(tmp_path / "bad")
+    assert st == 4
+    assert msg == "Incorrect magic number in plink bed file."
+
+
+def test_bed_missing_file_is_an_io_error(tmp_path):
+    """src/FilePlink.hpp:21-22: a missing .bed cannot be opened."""
+    _write_set(tmp_path / "nobed", 8, 3)
+    Path(str(tmp_path / "nobed") + ".bed").unlink()
+    st, _, _, msg = _bed_dims(tmp_path / "nobed")
+    assert (st, msg) == (4, "Cannot open bed file.")
+    st, _, _, msg = _bed_dims(tmp_path / "absent")
+    assert st == 4 and "can not open" in msg

@@ -248,6 +248,8 @@ REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16", "int8")
 # The mixed form (one BF16 piece of Omega) meets the bar on --svd 1 only; on
 # winSVD the rounded per-window Omega moves the unconverged noise PCs.
 REDUCED += [(c, f) for f in ("bf16mix", "fp16mix") for c in ("f1_d1", "f4_d1_miss", "cfg1_d1")]
+# degenerate SNPs (monomorphic, all missing, one genotype among missing entries)
+REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f5_d1", "f5_d2", "f5_emu")]
 # the out-of-core schedule (block windows, permute_plink order, per-pass imputation)
 REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f1_d2_ooc", "f4_emu_d2_ooc")]
 # Single-piece forms round Omega (or G) to 8-11 bits: the eigenvalues meet the
@@ -264,6 +266,65 @@ def test_golden_reduced_precision(ctx, golden_inputs, case, form):
     say(f"  eig rel err max {rel.max():.2e}  min PC corr {corr.min():.8f}")
     assert rel.max() <= 1e-4
     assert corr.min() >= PC_CORR_MIN[form]
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp32", "int8"])
+def test_precision_form_at_l80_matches_oracle(ctx, golden_inputs, form):
+    """The precision workload's settings (k = 40, oversampling 20 raised to 40
+    by src/Cmd.cpp:216, so L = 80; 7 power iterations, --tol-rsvd 0) on the
+    2,000 x 20,000 input: the reduced forms (BF16 hi/lo with the fused G
+    epilogue and the device thin-Q at L = 80, FP32, int8) against the FP64
+    oracle on the same Omega.  Bar: eigenvalue relative error <= 1e-4 and
+    every one of the 40 PCs correlated >= 0.9999."""
+    say(f"precision form {form} at k=40, L=80 on cfg1")
+    prefix = golden_inputs["cfg1"]
+    kw = dict(svd=1, k=40, oversamples=20, maxp=7, tol=0.0)
+    ref = O.run_plink(prefix, O.Params(**kw))
+    ctx.read_bed(prefix)
+    r = ctx.run(api.default_params(precision=form, **kw))
+    assert r.info["epochs"] == 8 and r.U.shape == (2000, 40)
+    rel = np.abs(r.E - ref.E) / ref.E
+    corr = O.pc_correlations(ref.U, r.U)
+    ang = O.principal_angles(ref.U, r.U)
+    say(f"  eig rel err max {rel.max():.2e}  min PC corr {corr.min():.8f}  max angle {ang.max():.2e}")
+    assert rel.max() <= 1e-4
+    assert corr.min() >= 0.9999
+
+
+@pytest.mark.parametrize("form", ["fp64", "bf16"])
+def test_winsvd_windows_past_the_last_snp(ctx, form):
+    """M = 1,000 < bands^2 = 4,096 with -w 64: windows of ceil(1000 / 64) = 16
+    SNPs, so window 63 starts at SNP 1,008, past the last one.  The reference
+    computes a negative width there and aborts (std::bad_alloc); here and in
+    the oracle such windows are empty (src/Halko.cpp:191-194) and the band
+    schedule is otherwise unchanged."""
+    from pcaone_amd import synth
+    spec = synth.Spec(seed=7, nsamples=150, nsnps=1000, npops=3)
+    say(f"winSVD with empty trailing windows, {form}")
+    kw = dict(svd=2, k=4, oversamples=10, bands=64, maxp=7, tol=0.0)
+    ref = O.run_pca_with_halko(O.Data.from_packed(synth.packed(spec), spec.nsamples, O.Params(**kw)))
+    ctx.generate(spec)
+    r = ctx.run(api.default_params(precision=form, **kw))
+    assert r.info["epochs"] == ref.epochs == 8
+    if form == "fp64":
+        np.testing.assert_allclose(r.trace, ref.trace, atol=1e-9)
+        np.testing.assert_allclose(r.S, ref.S, rtol=1e-9)
+        np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-7)
+    else:
+        assert (np.abs(r.E - ref.E) / ref.E).max() <= 1e-4
+        assert O.pc_correlations(ref.U, r.U).min() >= 0.9999
+
+
+def test_bed_bad_magic_through_the_device_reader(ctx, tmp_path):
+    """pcaone_bed_read checks the magic before any upload (src/FilePlink.hpp:24-27)."""
+    say("bad .bed magic")
+    prefix = tmp_path / "bad"
+    (tmp_path / "bad.fam").write_text("a a 0 0 0 -9\nb b 0 0 0 -9\n")
+    (tmp_path / "bad.bim").write_text("1 s1 0 1 A G\n")
+    (tmp_path / "bad.bed").write_bytes(b"\x6c\x1b\x00\x00")
+    with pytest.raises(api._lib.PcaoneError, match="Incorrect magic number in plink bed file.") as e:
+        ctx.read_bed(str(prefix))
+    assert e.value.status == 4
 
 
 def test_bad_shapes_are_errors(ctx, golden_inputs):

@@ -260,3 +260,44 @@ def test_dense_oracle_agrees_at_convergence(golden_inputs):
     Uref = O.read_matrix(GOLDEN / "cfg1_d1" / "eigvecs")
     np.testing.assert_allclose(Eref[:2], E[:2], rtol=1e-5)
     assert np.all(O.pc_correlations(U[:, :2], Uref[:, :2]) > 0.9999)
+
+
+# ---------------------------------------------------------------- metric helpers (closed forms)
+def test_pc_correlations_closed_forms():
+    """|Pearson r| per column: invariant to sign and to a positive scale and
+    shift (1), zero for a column orthogonal to the other after centering,
+    and cos(theta) for two centered unit vectors at angle theta."""
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((500, 3))
+    assert np.allclose(O.pc_correlations(x, x), 1.0, atol=1e-14)
+    assert np.allclose(O.pc_correlations(x, -3.0 * x + 7.0), 1.0, atol=1e-14)
+    a = x[:, :1] - x[:, :1].mean()
+    b = rng.standard_normal((500, 1))
+    b -= b.mean()
+    b -= a * (a.T @ b).item() / (a.T @ a).item()  # centered, orthogonal to a
+    assert O.pc_correlations(a, b)[0] < 1e-14
+    u = a / np.linalg.norm(a)
+    v = b / np.linalg.norm(b)
+    for theta in (0.1, 0.7, 1.3, 2.5):
+        w = np.cos(theta) * u + np.sin(theta) * v
+        assert abs(O.pc_correlations(u, w)[0] - abs(np.cos(theta))) < 1e-13
+    # column pairs are matched by index, not by best match
+    y = x[:, ::-1].copy()
+    r = O.pc_correlations(x, y)
+    assert abs(r[1] - 1.0) < 1e-14 and r[0] < 0.2 and r[2] < 0.2
+
+
+def test_principal_angles_closed_forms():
+    """Angles between the column spans: zero for the same span under any
+    invertible mixing, pi/2 for orthogonal spans, and the planted angles of
+    two planes built from orthonormal directions."""
+    rng = np.random.default_rng(9)
+    Q = np.linalg.qr(rng.standard_normal((300, 6)))[0]
+    A = Q[:, :3]
+    mix = rng.standard_normal((3, 3)) + 3 * np.eye(3)
+    assert np.max(O.principal_angles(A, A @ mix)) < 1e-7
+    assert np.allclose(O.principal_angles(A, Q[:, 3:]), np.pi / 2, atol=1e-12)
+    t = np.array([0.05, 0.4, 1.2])
+    B = A * np.cos(t) + Q[:, 3:] * np.sin(t)  # column i turned by t_i towards its own orthogonal direction
+    got = np.sort(O.principal_angles(A, B))
+    assert np.allclose(got, np.sort(t), atol=1e-12)
