@@ -79,6 +79,12 @@ typedef struct pcaone_params {
                         /* (block plan, no flipOmg for --svd 1, blocks as winSVD    */
                         /* windows, permute_plink order); the genotypes stay in HBM */
   uint32_t buffer;      /* --buffer (2), GB of the permute_plink read buffer        */
+  int ld;               /* -D/--ld: the PCA runs unscaled and pcaone_run_bfile also */
+                        /* writes <out>.residuals and <out>.mbim                    */
+  int ld_stats;         /* --ld-stats: 0 residuals of the PCs (default), 1 the      */
+                        /* centered genotypes                                       */
+  uint32_t imaxiter;    /* --imaxiter (1000): restarts of the IRAM solver, --svd 0  */
+  double itol;          /* --itol (1e-6): IRAM convergence tolerance                */
 } pcaone_params;
 
 typedef struct pcaone_run_info {
@@ -91,6 +97,8 @@ typedef struct pcaone_run_info {
   double g_ms, h_ms; /* summed device time of the two products      */
   uint64_t g_launches, h_launches;
   double omega_ms;   /* host time of the Omega draws (initOmg), included in seconds */
+  uint64_t householder_fallbacks; /* thin-Q of H by Householder QR (H near rank deficient) */
+  uint64_t gram_second_passes;    /* QR of G with a second pass over G (ill conditioned)   */
 } pcaone_run_info;
 
 /* Synthetic Balding-Nichols PLINK data (test and benchmark input; no
@@ -183,6 +191,16 @@ int pcaone_get_g(pcaone_ctx* ctx, uint64_t snp_begin, uint64_t snp_end, double* 
 /* H (which 0, src/Halko.cpp:126), H1 (1) or H2 (2) (src/Halko.cpp:198-203): N x L. */
 int pcaone_get_h(pcaone_ctx* ctx, int which, double* H);
 
+/* Data::write_residuals (src/Data.cpp:242-291): rows [snp_begin, snp_end)
+ * (current column order) of the residual matrix A - U S V^T, each SNP
+ * centered over samples, as float32 ((snp_end - snp_begin) x N, SNP-major: the
+ * .residuals payload).  U N x k, S k, V M x k (current column order),
+ * column-major; k = 0 subtracts nothing (--ld-stats 1).  as_read != 0: the
+ * out-of-core branch (blocks read again centered, unscaled, not imputed);
+ * otherwise A is the matrix the last products multiplied. */
+int pcaone_residuals(pcaone_ctx* ctx, uint32_t k, const double* U, const double* S, const double* V, int as_read,
+                     uint64_t snp_begin, uint64_t snp_end, float* out);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* run_pca_with_halko (src/Halko.cpp:271-345) on the resident data.  U: N x k,
  * S and E (= S^2/M): k, V (nullable; filled with printv or emu): M x k in the
@@ -268,6 +286,10 @@ int pcaone_copy_probe(pcaone_ctx* ctx, uint64_t bytes);
 int pcaone_host_sym_eig(int n, const double* A, double* w, double* V);
 int pcaone_host_householder_signs(int n, const double* Y, double* s);
 int pcaone_host_chol_upper(int n, const double* A, double* R);
+/* Thin Q of the Householder QR of X (nrows x ncols, row-major), in place: the
+ * fallback of the thin-Q of H when H is near rank deficient (the reference's
+ * HouseholderQR, src/Halko.cpp:121-122). */
+int pcaone_host_householder_q(uint64_t nrows, int ncols, double* X);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,8 @@ class Params:
     maf: float = 0.0
     memory: float = 0.0  # -m/--memory (GB): out-of-core when > 0
     buffer: int = 2  # --buffer (GB) of permute_plink
+    ld: bool = False  # -D/--ld: write the residual matrix (src/Cmd.cpp:92)
+    ld_stats: int = 0  # --ld-stats: 0 residuals of the PCs, 1 centered genotypes (src/Cmd.cpp:114)
 
     def __post_init__(self):
         self.oversamples = max(self.oversamples, self.k)  # src/Cmd.cpp:216
@@ -132,6 +134,8 @@ class Params:
         if self.bands < 4 or self.bands % 2 != 0:  # src/Cmd.cpp:235-236
             raise ValueError("the -w/--batches must be a power of 2 and the minimun is 4.")
         self.perm = self.svd == 2 and not self.noshuffle  # src/Cmd.cpp:238
+        if self.svd == 3:
+            self.out_of_core = False  # src/Cmd.cpp:218: -m does not apply to --svd 3
         self.missme = self.emu
 
 
@@ -306,6 +310,75 @@ def thin_q(X: np.ndarray) -> np.ndarray:
     return np.linalg.qr(X, mode="reduced")[0]
 
 
+def full_piv_solve(A: np.ndarray, Bm: np.ndarray) -> np.ndarray:
+    """X with A X = Bm by Eigen's FullPivHouseholderQR (the reference's
+    `fullPivHouseholderQr().solve`, src/Halko.cpp:68): Householder QR with
+    complete pivoting (external/Eigen/src/QR/FullPivHouseholderQR.h:470-530),
+    early exit when the remaining corner is below eps * size of the first pivot,
+    rank = pivots above eps * size * the largest |beta| (:246-255), and the
+    basic solution: unknowns of the pivots past the rank are zero (:544-577).
+    For an invertible A this is the ordinary solve."""
+    qr = np.array(A, dtype=np.float64, copy=True)
+    rows, cols = qr.shape
+    size = min(rows, cols)
+    prec = np.finfo(np.float64).eps * size
+    rowt = np.arange(size)
+    colt = np.arange(size)
+    hc = np.zeros(size)
+    nonzero, maxpivot, biggest = size, 0.0, 0.0
+    for k in range(size):
+        corner = np.abs(qr[k:, k:])
+        r, c = np.unravel_index(np.argmax(corner), corner.shape)
+        r += k
+        c += k
+        big = abs(qr[r, c])
+        if k == 0:
+            biggest = big
+        if big <= biggest * prec:
+            nonzero = k
+            break
+        rowt[k], colt[k] = r, c
+        if r != k:
+            qr[[k, r], k:] = qr[[r, k], k:]
+        if c != k:
+            qr[:, [k, c]] = qr[:, [c, k]]
+        x = qr[k:, k]
+        c0, tail = x[0], float(np.dot(x[1:], x[1:]))
+        if tail <= np.finfo(np.float64).tiny:
+            tau, beta = 0.0, c0
+            x[1:] = 0.0
+        else:
+            beta = math.sqrt(c0 * c0 + tail)
+            if c0 >= 0:
+                beta = -beta
+            x[1:] = x[1:] / (c0 - beta)
+            tau = (beta - c0) / beta
+        qr[k, k] = beta
+        hc[k] = tau
+        maxpivot = max(maxpivot, abs(beta))
+        if k + 1 < cols:
+            v = np.concatenate(([1.0], qr[k + 1:, k]))
+            w = v @ qr[k:, k + 1:]
+            qr[k:, k + 1:] -= tau * np.outer(v, w)
+    perm = np.arange(cols)
+    for k in range(size):
+        perm[[k, colt[k]]] = perm[[colt[k], k]]
+    diag = np.abs(np.diag(qr)[:nonzero])
+    rank = int(np.sum(diag > maxpivot * prec))
+    X = np.zeros((cols,) + Bm.shape[1:])
+    if rank == 0:
+        return X
+    c = np.array(Bm, dtype=np.float64, copy=True)
+    for k in range(rank):
+        if rowt[k] != k:
+            c[[k, rowt[k]]] = c[[rowt[k], k]]
+        v = np.concatenate(([1.0], qr[k + 1:, k]))
+        c[k:] -= hc[k] * np.outer(v, v @ c[k:])
+    z = np.linalg.solve(np.triu(qr[:rank, :rank]), c[:rank])
+    X[perm[:rank]] = z
+    return X
+
+
 def flip_omg(omg2: np.ndarray, omg: np.ndarray) -> np.ndarray:
     """src/RSVD.hpp:80-89; returns the new Omg (Omg2 := Omg is the caller's job)."""
     a = np.abs(omg2 - omg).sum(axis=0)
@@ -376,7 +449,7 @@ class RsvdOp:
             Q2, R2 = np.linalg.qr(Q1, mode="reduced")
             R = R2 @ R1
             G = Q2
-            B = np.linalg.solve(R.T, H.T)
+            B = full_piv_solve(R.T, H.T)
             UB, s, VBt = np.linalg.svd(B, full_matrices=False)
             Ucur = VBt.T[:, :k]
             self.epochs = pi + 1
@@ -553,6 +626,7 @@ class Result:
     trace: list = field(default_factory=list)
     em_trace: list = field(default_factory=list)
     perm: np.ndarray | None = None
+    residuals: np.ndarray | None = None  # M x N float32 in file (original SNP) order, with --ld
 
 
 def run_pca_with_halko(data: Data) -> Result:
@@ -561,7 +635,7 @@ def run_pca_with_halko(data: Data) -> Result:
     op = FancyOp(data, p.k, p.oversamples) if p.svd == 2 else NormalOp(data, p.k, p.oversamples)
     em_trace = []
     if not p.missme:
-        op.set_flags(False, True)
+        op.set_flags(False, not p.ld)  # src/Halko.cpp:283-285: no standardization with --ld
         op.compute_usv(p.maxp, p.tol)
     else:
         op.set_flags(False, False)
@@ -581,13 +655,119 @@ def run_pca_with_halko(data: Data) -> Result:
             op.compute_usv(p.maxp, p.tol)
             op.U, op.V = flip_uv(op.U, op.V, False)
     S = op.S if op.S is not None else np.zeros(0)
-    return Result(U=op.U, S=S, V=op.V if op.V is not None else np.zeros((0, 0)), E=S ** 2 / data.nsnps,
-                  epochs=op.epochs, trace=op.trace, em_trace=em_trace, perm=data.perm)
+    res = Result(U=op.U, S=S, V=op.V if op.V is not None else np.zeros((0, 0)), E=S ** 2 / data.nsnps,
+                 epochs=op.epochs, trace=op.trace, em_trace=em_trace, perm=data.perm)
+    if p.ld:  # src/Halko.cpp:338
+        res.residuals = residuals(data, op.S, op.U, op.V.T)
+    return res
+
+
+def residuals(data: Data, S, U, VT) -> np.ndarray:
+    """Data::write_residuals (src/Data.cpp:242-291) as the M x N float32 matrix
+    of the .residuals file (rows in file order).  In-core: the matrix as the
+    run left it (imputed and standardized as the run made it), minus U S V^T
+    with --ld-stats 0, each SNP centered over samples, cast to float; with the
+    in-core shuffle SNP column ib goes to row perm[ib].  Out-of-core: every
+    block is read again centered and unscaled, without imputation
+    (read_block_initial(.., false)), and the same follows."""
+    p = data.params
+    G = (data.G0 if p.out_of_core else data.G).copy()
+    if p.ld_stats == 0:
+        G -= (U * S[None, :]) @ VT
+    G -= G.mean(axis=0, keepdims=True)
+    out = np.empty((G.shape[1], G.shape[0]), dtype=np.float32)
+    rows = data.perm if (p.perm and data.perm is not None) else np.arange(G.shape[1])
+    out[rows] = G.T.astype(np.float32)
+    return out
+
+
+def write_residuals_file(path: str, R: np.ndarray) -> None:
+    """The .residuals layout (src/Data.cpp:252-256): uint32 nsnps, uint32
+    nsamples, then nsamples float32 values per SNP."""
+    with open(path, "wb") as f:
+        f.write(np.array([R.shape[0], R.shape[1]], dtype=np.uint32).tobytes())
+        f.write(np.ascontiguousarray(R, dtype=np.float32).tobytes())
+
+
+def read_residuals_file(path: str) -> np.ndarray:
+    raw = np.fromfile(path, dtype=np.uint8)
+    m, n = np.frombuffer(raw[:8].tobytes(), dtype=np.uint32)
+    return np.frombuffer(raw[8:].tobytes(), dtype=np.float32).reshape(int(m), int(n))
+
+
+def run_pca_full(data: Data) -> Result:
+    """--svd 3 (src/Main.cpp:180-217): standardize_E, then the eigen-decomposition
+    of G G^T / M (N <= M) or G^T G / M, the ncomp = min(k, N, M) largest
+    eigenvalues (clamped at 0) as evals, svals = sqrt(evals M), the other side
+    G^T U / s (or G V / s), flip_UV with U as the base."""
+    p = data.params
+    data.standardize_E()
+    G = data.G
+    N, M = G.shape
+    ncomp = min(p.k, N, M)
+    if N <= M:
+        w, Q = np.linalg.eigh(G @ G.T / M)
+        idx = np.arange(len(w) - 1, len(w) - 1 - ncomp, -1)
+        evals = np.maximum(w[idx], 0.0)
+        U = Q[:, idx]
+        S = np.sqrt(evals * M)
+        V = G.T @ U
+        V[:, S > 0] /= S[S > 0]
+    else:
+        w, Q = np.linalg.eigh(G.T @ G / M)
+        idx = np.arange(len(w) - 1, len(w) - 1 - ncomp, -1)
+        evals = np.maximum(w[idx], 0.0)
+        V = Q[:, idx]
+        S = np.sqrt(evals * M)
+        U = G @ V
+        U[:, S > 0] /= S[S > 0]
+    U, V = flip_uv(U, V, True)
+    return Result(U=U, S=S, V=V, E=evals, epochs=1)
+
+
+def run_pca_arnoldi(data: Data) -> Result:
+    """--svd 0, in-core without EMU (src/Arnoldi.cpp:48-118): Spectra's
+    PartialSVDSolver on the standardized G (unscaled with --ld) converged to
+    --itol: the k largest eigenpairs of G G^T (N <= M; G^T G otherwise), so
+    U, S = sqrt(eigenvalues), V = G^T U / S (or the tall mirror), E = S^2 / M;
+    no flip_UV.  Restated as the exact eigenpairs (what the converged IRAM
+    fixes); column signs are the solver's and are compared sign-aligned."""
+    p = data.params
+    if not p.ld:
+        data.standardize_E()
+    G = data.G
+    N, M = G.shape
+    k = p.k
+    if N <= M:
+        w, Q = np.linalg.eigh(G @ G.T)
+        idx = np.arange(len(w) - 1, len(w) - 1 - k, -1)
+        S = np.sqrt(np.maximum(w[idx], 0.0))
+        U = Q[:, idx]
+        V = G.T @ U / S
+    else:
+        w, Q = np.linalg.eigh(G.T @ G)
+        idx = np.arange(len(w) - 1, len(w) - 1 - k, -1)
+        S = np.sqrt(np.maximum(w[idx], 0.0))
+        V = Q[:, idx]
+        U = G @ V / S
+    res = Result(U=U, S=S, V=V, E=S ** 2 / M, epochs=1)
+    if p.ld:  # src/Arnoldi.cpp:117
+        res.residuals = residuals(data, S, U, V.T)
+    return res
+
+
+def run_pca(data: Data) -> Result:
+    """The dispatch of src/Main.cpp:176-217 by --svd."""
+    if data.params.svd == 3:
+        return run_pca_full(data)
+    if data.params.svd == 0:
+        return run_pca_arnoldi(data)
+    return run_pca_with_halko(data)
 
 
 def run_plink(prefix: str, params: Params) -> Result:
     packed, n, _ = read_plink(prefix)
-    return run_pca_with_halko(Data.from_packed(packed, n, params))
+    return run_pca(Data.from_packed(packed, n, params))
 
 
 # ---------------------------------------------------------------- outputs

@@ -29,6 +29,13 @@ ZSTDOBJ  := $(patsubst $(REF)/external/zstd/lib/%.c,$(OBJ)/zstd/%.o,$(ZSTD_C))
 
 all: $(OUT)/pcaone_ref
 
+# The same reference objects with libpcaone_amd bound in through
+# pcaone_amd/binding/ (the maintainer-side binding, compiled by the
+# reference's own compiler and flags): make -f oracle/ref.mk binding
+AMDINC   := -Iinclude -Ipcaone_amd/binding
+AMDLIB   := -Lpcaone_amd -lpcaone_amd -Wl,-rpath,'$$ORIGIN/../../pcaone_amd' -Wl,-rpath,/opt/rocm/lib
+binding: $(OUT)/pcaone_ref_amd
+
 $(OBJ)/%.o: $(REF)/src/%.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) $(INC) -c $< -o $@
@@ -36,6 +43,13 @@ $(OBJ)/%.o: $(REF)/src/%.cpp
 $(OBJ)/ref_driver.o: oracle/ref_driver.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) $(INC) -c $< -o $@
+
+$(OBJ)/ref_driver_amd.o: oracle/ref_driver_amd.cpp pcaone_amd/binding/pcaone_reference_binding.hpp include/pcaone_amd.h
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) $(INC) $(AMDINC) -c $< -o $@
+
+$(OUT)/pcaone_ref_amd: $(CPPOBJ) $(OBJ)/ref_driver_amd.o $(ZSTDOBJ) pcaone_amd/libpcaone_amd.so
+	$(CXX) $(CXXFLAGS) -o $@ $(CPPOBJ) $(OBJ)/ref_driver_amd.o $(ZSTDOBJ) $(AMDLIB) -lz -lgomp -lpthread -ldl -lm
 
 $(OBJ)/zstd/%.o: $(REF)/external/zstd/lib/%.c
 	@mkdir -p $(dir $@)
@@ -47,4 +61,4 @@ $(OUT)/pcaone_ref: $(CPPOBJ) $(OBJ)/ref_driver.o $(ZSTDOBJ)
 clean:
 	rm -rf $(OUT)
 
-.PHONY: all clean
+.PHONY: all clean binding

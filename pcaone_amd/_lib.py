@@ -35,14 +35,16 @@ class Params(C.Structure):
                 ("noshuffle", C.c_int), ("emu", C.c_int), ("printv", C.c_int), ("verbose", C.c_int),
                 ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("waves_g", C.c_int),
                 ("waves_h", C.c_int), ("kchunk_g", C.c_uint64),
-                ("kchunk_h", C.c_uint64), ("memory", C.c_double), ("buffer", C.c_uint32)]
+                ("kchunk_h", C.c_uint64), ("memory", C.c_double), ("buffer", C.c_uint32), ("ld", C.c_int),
+                ("ld_stats", C.c_int), ("imaxiter", C.c_uint32), ("itol", C.c_double)]
 
 
 class RunInfo(C.Structure):
     _fields_ = [("epochs", C.c_int), ("total_epochs", C.c_int), ("em_iters", C.c_int), ("n_trace", C.c_int),
                 ("n_sv", C.c_int), ("seconds", C.c_double), ("g_ms", C.c_double), ("h_ms", C.c_double),
                 ("g_launches", C.c_uint64), ("h_launches", C.c_uint64),
-                ("omega_ms", C.c_double)]
+                ("omega_ms", C.c_double), ("householder_fallbacks", C.c_uint64),
+                ("gram_second_passes", C.c_uint64)]
 
 
 _lib = None
@@ -108,6 +110,8 @@ _SIGS = {
     "pcaone_host_sym_eig": (C.c_int, [C.c_int, _P, _P, _P]),
     "pcaone_host_householder_signs": (C.c_int, [C.c_int, _P, _P]),
     "pcaone_host_chol_upper": (C.c_int, [C.c_int, _P, _P]),
+    "pcaone_host_householder_q": (C.c_int, [C.c_uint64, C.c_int, _P]),
+    "pcaone_residuals": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_int, C.c_uint64, C.c_uint64, _P]),
 }
 
 

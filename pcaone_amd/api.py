@@ -265,6 +265,20 @@ class Context:
                                            C.byref(info)))
         return {f: getattr(info, f) for f, _ in RunInfo._fields_}
 
+    def residuals(self, U, S, V, j0: int = 0, j1: int | None = None, as_read: bool = False) -> np.ndarray:
+        """Rows [j0, j1) of the centered residual matrix (float32, SNP-major);
+        U = None subtracts nothing."""
+        n, m, _ = self.dims()
+        j1 = m if j1 is None else j1
+        out = np.empty((j1 - j0, n), dtype=np.float32)
+        if U is None:
+            self._check(lib().pcaone_residuals(self._h, 0, None, None, None, int(as_read), j0, j1, _ptr(out)))
+        else:
+            U, S, V = _f64(U), np.ascontiguousarray(S, dtype=np.float64), _f64(V)
+            self._check(lib().pcaone_residuals(self._h, U.shape[1], _ptr(U), _ptr(S), _ptr(V), int(as_read), j0, j1,
+                                               _ptr(out)))
+        return out
+
     def packed_gemm(self, precision, packed: np.ndarray, ncodes: int, W: np.ndarray) -> np.ndarray:
         if isinstance(precision, str):
             precision = PRECISIONS[precision]

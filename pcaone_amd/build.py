@@ -61,8 +61,9 @@ def build(jobs: int | None = None) -> Path:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
+    # rocSOLVER (symmetric eigen-solver of the exact-PCA Gram matrix) and rocBLAS (its handle)
     cmd = [HIPCC, "-shared", "-fopenmp", f"--offload-arch={ARCH}", "-o", str(tmp)] + [str(o) for o in objs] + [
-        "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"]
+        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{res.stdout}{res.stderr}")

@@ -115,6 +115,10 @@ Param to_param(const pcaone_params* p) {
   q.tuning.kchunk_h = p->kchunk_h;
   q.memory = p->memory;
   q.buffer = p->buffer;
+  q.ld = p->ld != 0;
+  q.ld_stats = p->ld_stats;
+  q.imaxiter = p->imaxiter;
+  q.itol = p->itol;
   require(q.k >= 1, kErrArgument, "k must be >= 1");
   require(q.ploidy == 1 || q.ploidy == 2, kErrArgument, "ploidy must be 1 or 2");
   q.derive();
@@ -135,6 +139,8 @@ void fill_info(pcaone_run_info* info, const RunInfo& r, const Context& c, int ns
   info->g_launches = c.ktimes.g_launches;
   info->h_launches = c.ktimes.h_launches;
   info->omega_ms = r.omega_ms;
+  info->householder_fallbacks = c.householder_fallbacks;
+  info->gram_second_passes = c.gram_second_passes;
 }
 
 }  // namespace
@@ -184,6 +190,8 @@ void pcaone_params_default(pcaone_params* p) {
   p->ploidy = 2;
   p->precision = PCAONE_PREC_FP32;
   p->buffer = 2;
+  p->imaxiter = 1000;
+  p->itol = 1e-6;
 }
 
 int pcaone_bed_read(pcaone_ctx* c, const char* prefix, double maf, uint32_t* nsamples, uint64_t* nsnps) {
@@ -356,11 +364,11 @@ int pcaone_run_halko(pcaone_ctx* c, const pcaone_params* p, double* U, double* S
     require(x.M > 0 && !x.F.empty(), kErrState, "no genotype data (pcaone_bed_read / upload first)");
     c->last = RunInfo{};
     x.ktimes = Context::KernelTimes{};
-    const PcaResult r = run_pca_with_halko(x, q, &c->last, U);
+    const PcaResult r = run_pca(x, q, &c->last, U);
     const uint64_t M = x.M;
     if (S) std::memcpy(S, r.S.data(), r.S.size() * sizeof(double));
     if (E) std::memcpy(E, r.E.data(), r.E.size() * sizeof(double));
-    if (V && !r.V.empty()) std::memcpy(V, r.V.data(), M * q.k * sizeof(double));
+    if (V && !r.V.empty()) std::memcpy(V, r.V.data(), M * (uint64_t)r.k * sizeof(double));
     fill_info(info, c->last, x, (int)r.S.size());
   });
 }
@@ -492,9 +500,19 @@ int pcaone_run_bfile(pcaone_ctx* c, const char* bfile, const char* out, const pc
     const PlinkInfo pi = read_plink(x, bfile, q);
     c->last = RunInfo{};
     x.ktimes = Context::KernelTimes{};
-    const PcaResult r = run_pca_with_halko(x, q, &c->last);
+    const PcaResult r = run_pca(x, q, &c->last);
+    if (q.ld && q.svd != 3) write_residuals(x, r, q, out);  // src/Halko.cpp:338, src/Arnoldi.cpp:117
     write_outputs(x, r, q, bfile, out, pi);
     fill_info(info, c->last, x, (int)r.S.size());
+  });
+}
+
+int pcaone_residuals(pcaone_ctx* c, uint32_t k, const double* U, const double* S, const double* V, int as_read,
+                     uint64_t snp_begin, uint64_t snp_end, float* out) {
+  return guard(c, [&](Context& x) {
+    require(out != nullptr, kErrArgument, "null output");
+    x.residuals_prepare((int)k, U, S, V, as_read != 0);
+    x.residuals_rows(snp_begin, snp_end, out);
   });
 }
 
@@ -652,6 +670,13 @@ int pcaone_host_chol_upper(int n, const double* A, double* R) {
     std::memcpy(a.a.data(), A, (size_t)n * n * sizeof(double));
     const la::Mat r = la::chol_upper(a);
     std::memcpy(R, r.a.data(), (size_t)n * n * sizeof(double));
+  });
+}
+
+int pcaone_host_householder_q(uint64_t nrows, int ncols, double* X) {
+  return guard_global([&] {
+    require(ncols >= 1 && nrows >= (uint64_t)ncols && X, kErrArgument, "bad arguments");
+    la::householder_q(X, nrows, ncols, ncols);
   });
 }
 

@@ -443,6 +443,7 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   part_.resize(std::max(pg, ph));
   h_version_ = 0;
   gram_version_ = ~0ull;
+  h_safe_version_ = ~0ull;
   gram_dev_version_ = ~0ull;
   if (thinq_on_device(lp)) {
     ldev_.resize(3 * (uint64_t)lp * lp + lp);
@@ -450,6 +451,7 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
     PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
   }
   ktimes = KernelTimes{};
+  householder_fallbacks = gram_second_passes = 0;
 }
 
 void Context::set_t(const la::Mat& T) {
@@ -618,7 +620,18 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
   }
 
-  // (3) H contribution: sample-major rows, K = the window's SNPs
+  h_half(j0, j1, hmode);
+}
+
+void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
+  // H (or H1 / H2) += A[j0:j1]^T G~[j0:j1], with the operand pieces of
+  // diag(sc) G~ for these SNPs in wh_ at K offset j0 % kKStep and the b
+  // terms in bvec_.
+  const uint64_t nwin = j1 - j0;
+  const uint64_t off = j0 % kKStep;
+  const uint64_t klen = round_up(off + nwin, kKStep);
+  double* gwin = gt_.get() + j0 * lp;
+  // sample-major rows, K = the window's SNPs
   const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h, form_kcap(form_h, kChunkCapH));
   GemmArgs h{};
   h.X = xt_.get() + (j0 - off) / 16;
@@ -662,6 +675,21 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     launch_missing_spmm(m, stream);
   }
   if (hmode == 0) ++h_version_;
+}
+
+void Context::h_from_g(int hmode) {
+  // H = A^T G~ over all SNPs for a G~ the caller wrote into gt_ (rows 0..M)
+  require(L > 0, kErrState, "configure() first");
+  require(form_h != Form::I8, kErrArgument, "h_from_g: not in the int8 form");
+  const uint64_t wb = bytes_of(form_h);
+  const int P = form_pieces(form_h);
+  const uint64_t klen = round_up(M, kKStep);
+  launch_zero_ranges(wh_.get(), wh_pitch_ * wb, (uint64_t)P * lp, 0, 0, M * wb, (klen - M) * wb, stream);
+  launch_make_pieces(form_h, gt_.get(), M, M, lp, sc_.get(), nullptr, wh_.get(), wh_pitch_, (uint64_t)lp * wh_pitch_,
+                     gt_.get(), nullptr, stream, nullptr);
+  launch_colsum(gt_.get(), 0, M, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
+  if (nmiss) update_missing_weights();
+  h_half(0, M, hmode);
 }
 
 uint64_t Context::miss_span(uint64_t nrows, uint64_t kext) const {
@@ -723,6 +751,14 @@ void Context::omega_from(int src, bool flip) {
     ++h_version_;
   }
   const double* X = H_.get();
+  if (!h_thinq_safe()) {
+    // H is (nearly) rank deficient: CholeskyQR2 could break down or lose
+    // orthogonality.  The thin Q comes from a Householder QR instead, the
+    // reference's own factorization (src/Halko.cpp:121-122), which gives L
+    // orthonormal columns whatever the rank.
+    omega_householder(flip);
+    return;
+  }
   if (thinq_on_device(lp)) {
     // every step stream-ordered on the device: no host round trip
     double* gh = ldev_.get();
@@ -752,14 +788,7 @@ void Context::omega_from(int src, bool flip) {
     launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
     return;
   }
-  la::Mat G1;
-  if (gram_version_ == h_version_) {
-    G1 = gram_h_;
-  } else {
-    G1 = gram_l(X, N);
-    gram_h_ = G1;
-    gram_version_ = h_version_;
-  }
+  const la::Mat& G1 = gram_h_;  // h_thinq_safe() left the Gram of this H there
   const la::Mat R1 = la::chol_upper(G1);
   set_t(la::inv_upper(R1));
   launch_apply(X, small_.get(), N, lp, tmp_n_.get(), stream);
@@ -796,6 +825,112 @@ void Context::omega_from(int src, bool flip) {
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
   PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+la::Mat Context::from_pinned_gram(int which) const {
+  const double* src = reinterpret_cast<const double*>(gram_pinned_.get()) + (uint64_t)which * lp * lp;
+  la::Mat G(L);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < L; ++j) G(i, j) = src[(size_t)i * lp + j];
+  return G;
+}
+
+bool Context::h_thinq_safe() {
+  if (h_safe_version_ == h_version_) return h_safe_;
+  if (gram_version_ != h_version_) {
+    if (gram_ev_ && gram_h_sent_ && begin_h_version_ == h_version_) {
+      // finish_begin already sent this H's Gram to pinned memory
+      PCAONE_HIP(hipEventSynchronize(gram_ev_));
+      gram_h_ = from_pinned_gram(1);
+    } else if (thinq_on_device(lp)) {
+      // the device thin-Q reuses this Gram
+      gram_h_ = gram_l(H_.get(), N, nullptr, ldev_.get());
+      gram_dev_version_ = h_version_;
+    } else {
+      gram_h_ = gram_l(H_.get(), N);
+    }
+    gram_version_ = h_version_;
+  }
+  la::Mat R;
+  double ratio = 0.0;
+  h_safe_ = la::chol_try(gram_h_, R, ratio) && ratio > kThinqMinPivot;
+  h_safe_version_ = h_version_;
+  return h_safe_;
+}
+
+void Context::omega_householder(bool flip) {
+  ++householder_fallbacks;
+  std::vector<double> h((uint64_t)N * lp);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), H_.get(), h.size() * 8, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  la::householder_q(h.data(), N, L, lp);
+  for (uint64_t i = 0; i < N; ++i)
+    for (int j = L; j < lp; ++j) h[i * lp + j] = 0.0;
+  PCAONE_HIP(hipMemcpyAsync(omega_c_.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
+  double* dsign = nullptr;
+  if (flip) {  // flipOmg (src/RSVD.hpp:80-89) against Omg2
+    double* norms = red_.get() + red_cap_;
+    launch_flip_norms(omega_.get(), omega_c_.get(), N, lp, red_.get(), norms, stream);
+    dsign = small_.get() + 128 * 128;
+    launch_flip_signs(norms, L, lp, dsign, stream);
+  }
+  if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
+  launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
+  launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+  PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::orthonormal_factor(const la::Mat& G1, const double* g) {
+  // W such that Q = G~ W has orthonormal columns spanning range(G~).
+  // - Well conditioned (every Cholesky pivot above kOnePassMinPivot of the
+  //   largest diagonal entry): W = R1^-1, one CholeskyQR pass; Q is
+  //   orthonormal to about u / min pivot ratio <= 1e-10.
+  // - Ill conditioned: a real second pass over G~ on the device, R2 from the
+  //   Gram of G~ W1, W = W1 R2^-1 (CholeskyQR2).
+  // - (Nearly) rank deficient (breakdown, or a pivot below kThinqMinPivot):
+  //   W1 = P_r Lambda_r^-1/2 from the eigen-decomposition of the Gram, keeping
+  //   the r directions above kRankTol of the largest, then the second pass.
+  //   The other L - r columns of W are zero, like the zero rows that the
+  //   reference's rank-revealing solve (fullPivHouseholderQr, src/Halko.cpp:68)
+  //   gives B for pivots below its threshold.
+  la::Mat R1;
+  double ratio = 0.0;
+  const bool ok = la::chol_try(G1, R1, ratio);
+  if (ok && ratio > kOnePassMinPivot) {
+    W_ = la::inv_upper(R1);
+    return;
+  }
+  ++gram_second_passes;
+  la::Mat W1(L);
+  int r = L;
+  if (ok && ratio > kThinqMinPivot) {
+    W1 = la::inv_upper(R1);
+  } else {
+    std::vector<double> lam;
+    la::Mat P;
+    la::sym_eig_desc(G1, lam, P);
+    r = 0;
+    while (r < L && lam[r] > kRankTol * lam[0] && lam[r] > 0.0) ++r;
+    require(r > 0, kErrNumeric, "G is zero: no direction to orthonormalize");
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < r; ++j) W1(i, j) = P(i, j) / std::sqrt(lam[j]);
+  }
+  // second pass: G2 = (G~ W1)^T (G~ W1), R2 = chol of its leading r x r block
+  set_t(W1);
+  launch_apply(g, small_.get(), M, lp, tmp_m_.get(), stream);
+  const la::Mat G2 = gram_l(tmp_m_.get(), M);
+  la::Mat G2r(r);
+  for (int i = 0; i < r; ++i)
+    for (int j = 0; j < r; ++j) G2r(i, j) = G2(i, j);
+  const la::Mat R2i = la::inv_upper(la::chol_upper(G2r));
+  W_ = la::Mat(L);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < r; ++j) {
+      double v = 0.0;
+      for (int q = 0; q <= j; ++q) v += W1(i, q) * R2i(q, j);
+      W_(i, j) = v;
+    }
 }
 
 EpochResult Context::finish_epoch(int k, bool have_prev) {
@@ -853,6 +988,7 @@ void Context::unhold() {
   std::swap(H_, H_alt_);
   held_ = false;
   h_version_ = begin_h_version_;  // H is the held epoch's again
+  h_safe_version_ = ~0ull;
   // the discarded epoch's thin-Q may have flagged a breakdown: not this run's
   if (lstatus_.get()) PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
 }
@@ -862,30 +998,21 @@ EpochResult Context::finish_end(int k, bool have_prev) {
   PCAONE_HIP(hipEventSynchronize(gram_ev_));
   const uint64_t l2 = (uint64_t)lp * lp;
   const double* hp = reinterpret_cast<const double*>(gram_pinned_.get());
-  if (*reinterpret_cast<const int*>(hp + 2 * l2)) {
+  if (*reinterpret_cast<const int*>(hp + 2 * l2)) {  // a device thin-Q that passed h_thinq_safe() broke down
     PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
     PCAONE_HIP(hipStreamSynchronize(stream));
     unhold();
     fail(kErrNumeric, "Cholesky breakdown (Gram matrix not positive definite)");
   }
-  auto from_pinned = [&](const double* src) {
-    la::Mat G(L);
-    for (int i = 0; i < L; ++i)
-      for (int j = 0; j < L; ++j) G(i, j) = src[(size_t)i * lp + j];
-    return G;
-  };
-  // QR of G twice (src/Halko.cpp:55-65): R = R2 R1, CholeskyQR2 with the
-  // second Gram formed on the L x L side: Q1^T Q1 = R1^-T (G^T G) R1^-1.
-  const la::Mat G1 = from_pinned(hp);
-  const la::Mat R1 = la::chol_upper(G1);
-  const la::Mat R1i = la::inv_upper(R1);
-  const la::Mat R2 = la::chol_upper(la::matmul(la::transpose(R1i), la::matmul(G1, R1i)));
-  R_ = la::matmul(R2, R1);
-  const la::Mat Rinv = la::inv_upper(R_);
+  // QR of G twice (src/Halko.cpp:55-65): Q = G~ W with W = R^-1.  Only the
+  // L x L factor is formed: W from the Cholesky factor of the Gram of G~.
+  const la::Mat G1 = from_pinned_gram(0);
+  orthonormal_factor(G1, held_ ? gt_alt_.get() : gt_.get());
+  const la::Mat& Rinv = W_;
   // B B^T = R^-T (H^T H) R^-1; B = R^-T H^T (src/Halko.cpp:68)
   la::Mat HH;
   if (gram_h_sent_) {
-    HH = from_pinned(hp + l2);
+    HH = from_pinned_gram(1);
     gram_h_ = HH;
     gram_version_ = begin_h_version_;
   } else {
@@ -952,9 +1079,58 @@ void Context::download_h(int which, double* H) {
   download_rowmajor(which == 0 ? H_.get() : (which == 1 ? H1_.get() : H2_.get()), N, H);
 }
 
+void Context::residuals_prepare(int k, const double* U, const double* S, const double* V, bool as_read) {
+  require(N > 0 && M > 0, kErrState, "no data");
+  require(k >= 0 && k <= kResidMaxK, kErrArgument, "residuals: rank must be in [0, " + std::to_string(kResidMaxK) + "]");
+  require(k == 0 || (U && S && V), kErrArgument, "residuals: null U, S or V");
+  std::vector<double> ut((uint64_t)std::max(k, 1) * N, 0.0), sv(M * (uint64_t)std::max(k, 1), 0.0);
+  for (int t = 0; t < k; ++t)
+    for (uint64_t i = 0; i < N; ++i) ut[(uint64_t)t * N + i] = U[(uint64_t)t * N + i];
+  for (uint64_t j = 0; j < M; ++j)
+    for (int t = 0; t < k; ++t) sv[j * k + t] = S[t] * V[(uint64_t)t * M + j];
+  res_ut_.resize(ut.size());
+  res_sv_.resize(sv.size());
+  PCAONE_HIP(hipMemcpyAsync(res_ut_.get(), ut.data(), ut.size() * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipMemcpyAsync(res_sv_.get(), sv.data(), sv.size() * 8, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  res_k_ = k;
+  res_as_read_ = as_read;
+}
+
+void Context::residuals_rows(uint64_t j0, uint64_t j1, float* out) {
+  require(res_sv_.get(), kErrState, "residuals_prepare first");
+  require(j0 < j1 && j1 <= M, kErrArgument, "residuals: bad SNP range");
+  const uint64_t rows = j1 - j0;
+  res_out_.resize(rows * N);
+  res_part_.resize(residual_partial_elems(N, rows));
+  ResidualArgs a{};
+  a.xs = xs_.get();
+  a.pitch = pitch_s_;
+  a.n = N;
+  a.j0 = j0;
+  a.j1 = j1;
+  a.sc = res_as_read_ ? nullptr : sc_.get();
+  a.cf = res_as_read_ ? dF_.get() : cF_.get();
+  a.k = res_k_;
+  a.ut = res_ut_.get();
+  a.sv = res_sv_.get();
+  if (!res_as_read_ && emu_k_ > 0) {
+    a.kimp = emu_k_;
+    a.kpimp = emu_kp_;
+    a.us = us_.get();
+    a.vimp = vimp_.get();
+    a.fclamp = dF_.get();
+  }
+  a.part = res_part_.get();
+  a.out = res_out_.get();
+  launch_residuals(a, stream);
+  PCAONE_HIP(hipMemcpyAsync(out, res_out_.get(), rows * N * sizeof(float), hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
 void Context::compute_v(int k) {
   // V = Q U_B[:, :k] = G~ R^-1 W_B[:, :k]   (src/Halko.cpp:84)
-  const la::Mat RW = la::matmul(la::inv_upper(R_), WB_);
+  const la::Mat RW = la::matmul(W_, WB_);
   la::Mat T(L);
   for (int i = 0; i < L; ++i)
     for (int j = 0; j < k; ++j) T(i, j) = RW(i, j);

@@ -110,6 +110,20 @@ class Context {
   // G rows [j0, j1) = A[j0:j1] Omega and the H contribution of those SNPs:
   // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ...
   void sketch(uint64_t j0, uint64_t j1, int hmode);
+  // H (hmode 0; H1 1, H2 2) = A^T G~ over all SNPs for the G~ (M x lp,
+  // row-major) the caller wrote into gt() (the FP64, FP32 and BF16 forms).
+  void h_from_g(int hmode);
+  double* gt() { return gt_.get(); }
+  double* h() { return H_.get(); }
+  // the resident operands, for drivers that build their own dense blocks
+  const uint32_t* xs() const { return xs_.get(); }
+  uint64_t pitch_s() const { return pitch_s_; }
+  const double* sc() const { return sc_.get(); }
+  const double* cf() const { return cF_.get(); }
+  // Omega from an N x L column-major matrix the caller wrote on the device
+  // into omega_scratch(): set_omega without the host copy.
+  double* omega_scratch() { return tmp_n_.get(); }
+  void set_omega_scratch() { omega_pieces(); }
   // Omega <- thin-Q(src) with Householder signs, then flipOmg against Omg2
   // unless !flip (the out-of-core --svd 1 branch has none);
   // src 0: H, 1: H1 + H2 (also stored as the current H).
@@ -139,6 +153,15 @@ class Context {
   void download_g(uint64_t j0, uint64_t j1, double* G);
   void download_h(int which, double* H);
   const double* device_v() const { return vbuf_.get(); }
+  // Data::write_residuals (src/Data.cpp:242-291).  U (N x k), S (k) and V
+  // (M x k, current column order) column-major, subtracted as U S V^T (k = 0:
+  // --ld-stats 1, nothing subtracted).  as_read: the out-of-core branch, whose
+  // blocks are read again centered and unscaled without imputation; otherwise
+  // the matrix the last products multiplied (scale, imputation).
+  void residuals_prepare(int k, const double* U, const double* S, const double* V, bool as_read);
+  // Rows [j0, j1) (current column order) of the centered residual matrix:
+  // (j1 - j0) x N float32 into host memory.
+  void residuals_rows(uint64_t j0, uint64_t j1, float* out);
 
   // timing of the hot kernels (HIP events on the context stream)
   struct KernelTimes {
@@ -146,6 +169,9 @@ class Context {
     uint64_t g_launches = 0, h_launches = 0;
   } ktimes;
   bool time_kernels = false;
+  // how often the orthonormalization safeguards ran (Householder thin-Q of H,
+  // second pass over G~)
+  uint64_t householder_fallbacks = 0, gram_second_passes = 0;
 
  private:
   // data layout
@@ -195,13 +221,17 @@ class Context {
   DevBuf<double> small_, red_;
   uint64_t red_cap_ = 0;
   // host copies of the last epoch's small factors
-  la::Mat R_;         // R of QR(G), L x L
+  la::Mat W_;         // Q = G~ W_ orthonormal (W_ = R^-1 of QR(G) when G~ has full rank), L x L
   la::Mat WB_;        // eigenvectors of B B^T, L x L
   std::vector<double> sigma_;
   uint64_t h_version_ = 0, gram_version_ = ~0ull;
   la::Mat gram_h_;    // Gram of H (L x L) for h_version_
   // device side of the thin-Q of H (lp <= 48): Gram of H for gram_dev_version_,
   // second Gram, R1, column signs (lp x lp each, signs lp); breakdown status
+  DevBuf<double> res_ut_, res_sv_, res_part_;
+  DevBuf<float> res_out_;
+  int res_k_ = 0;
+  bool res_as_read_ = false;
   DevBuf<double> ldev_;
   DevBuf<int> lstatus_;
   uint64_t gram_dev_version_ = ~0ull;
@@ -218,6 +248,17 @@ class Context {
   double* i8_scale(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h : nullptr; }
   double* i8_mul(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h + 128 : nullptr; }
   void set_t(const la::Mat& T);
+  void h_half(uint64_t j0, uint64_t j1, int hmode);  // the H product of sketch for G~ rows [j0, j1)
+  // Orthonormalization safeguards (Cholesky pivots d_j over the largest diagonal entry).
+  static constexpr double kOnePassMinPivot = 1e-6;  // one CholeskyQR pass of G~ is enough above this
+  static constexpr double kThinqMinPivot = 1e-10;   // CholeskyQR2 is used above this, else the fallbacks
+  static constexpr double kRankTol = 1e-12;         // eigenvalues of a Gram kept above this x the largest
+  la::Mat from_pinned_gram(int which) const;        // 0: Gram of G~, 1: Gram of H (finish_begin)
+  bool h_thinq_safe();                              // CholeskyQR2 of the current H is safe
+  uint64_t h_safe_version_ = ~0ull;
+  bool h_safe_ = false;
+  void omega_householder(bool flip);                // Omega <- Householder thin Q of H (host)
+  void orthonormal_factor(const la::Mat& G1, const double* g);  // sets W_
   void download_rowmajor(const double* src, uint64_t rows, double* out);  // rows x lp -> column-major rows x L
   // host <-> device copies of count doubles through the pinned staging buffer
   void stage_in(const double* host, uint64_t count, double* dev);

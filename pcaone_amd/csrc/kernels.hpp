@@ -114,6 +114,41 @@ void launch_impute_rows(const double* src, uint64_t rows, int lp, int k, int kp,
 // Per-SNP column state: cf = F, sc = 1 for j < m, 0 up to rows.
 void launch_pristine_cols(const double* F, uint64_t m, uint64_t rows, double* cf, double* sc, hipStream_t s);
 // sc_j *= sqrt_ploidy / sqrt(F_j (1 - F_j)) where that sd exceeds tol (j < m).
+// ---- exact PCA / IRAM operands (kernels_exact.hip) -----------------------
+// gt (m x lp, row-major) <- the standardized columns of samples [s0, s0 + nc)
+void launch_sample_columns(const uint32_t* xs, uint64_t pitch, uint64_t m, uint64_t s0, int nc, int lp,
+                           const double* sc, const double* cf, double* gt, hipStream_t s);
+// om (n x L, column-major) <- the standardized rows of SNPs [t0, t0 + nc)
+void launch_snp_rows(const uint32_t* xs, uint64_t pitch, uint64_t n, uint64_t t0, int nc, int L, const double* sc,
+                     const double* cf, double* om, hipStream_t s);
+// K[:, col0:col0 + nc] (column-major, leading dimension ldk) <- scale X[:, :nc] (rows x lp row-major)
+void launch_rows_to_cols(const double* X, uint64_t rows, int lp, int nc, double scale, double* K, uint64_t ldk,
+                         uint64_t col0, hipStream_t s);
+
+// ---- residual matrix (kernels_resid.hip) --------------------------------
+constexpr int kResidMaxK = 64;  // rank of the subtracted U S V^T
+struct ResidualArgs {
+  const uint32_t* xs;  // SNP-major interleaved codes
+  uint64_t pitch;      // words per row
+  uint64_t n;          // samples
+  uint64_t j0, j1;     // SNP rows (current column order)
+  const double* sc;    // per-SNP scale, null = 1
+  const double* cf;    // per-SNP centering frequency
+  int k;               // rank of the subtracted part (0: none)
+  const double* ut;    // U, k x n
+  const double* sv;    // S_t V_jt, M x k
+  int kimp, kpimp;     // EMU imputation rank and row pitch (us null: no imputation)
+  const double* us;    // U S of the imputation, n x kpimp
+  const double* vimp;  // V of the imputation, M x kpimp
+  const double* fclamp;
+  double* part;        // nchunks x (j1 - j0) sums
+  uint64_t nchunks, part_stride;
+  float* out;          // (j1 - j0) x n
+};
+// Capacity of `part` needed for a launch over `snps` rows and n samples.
+inline uint64_t residual_partial_elems(uint64_t n, uint64_t snps) { return (n + 255) / 256 * snps; }
+void launch_residuals(const ResidualArgs& a, hipStream_t s);
+
 void launch_standardize(const double* F, uint64_t m, double sqrt_ploidy, double tol, double* sc, hipStream_t s);
 // dst[i] = src[idx[i]], i < n.
 void launch_gather_f64(const double* src, const uint32_t* idx, uint64_t n, double* dst, hipStream_t s);

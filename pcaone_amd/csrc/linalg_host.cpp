@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <numeric>
 
 #include "common.hpp"
@@ -28,6 +29,89 @@ Mat chol_upper(const Mat& A) {
     }
   }
   return R;
+}
+
+bool chol_try(const Mat& A, Mat& R, double& min_ratio) {
+  const int n = A.n;
+  R = Mat(n);
+  min_ratio = 1.0;
+  double dmax = 0.0;
+  for (int j = 0; j < n; ++j) dmax = std::max(dmax, A(j, j));
+  for (int j = 0; j < n; ++j) {
+    double d = A(j, j);
+    for (int k = 0; k < j; ++k) d -= R(k, j) * R(k, j);
+    if (!(d > 0.0) || !std::isfinite(d) || !(dmax > 0.0)) {
+      min_ratio = 0.0;
+      return false;
+    }
+    min_ratio = std::min(min_ratio, d / dmax);
+    const double rjj = std::sqrt(d);
+    R(j, j) = rjj;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A(j, i);
+      for (int k = 0; k < j; ++k) s -= R(k, j) * R(k, i);
+      R(j, i) = s / rjj;
+    }
+  }
+  return true;
+}
+
+void householder_q(double* X, uint64_t n, int L, int ld) {
+  require(n >= (uint64_t)L && L > 0, kErrArgument, "householder_q: needs rows >= columns");
+  std::vector<double> A(n * (uint64_t)L);  // column-major copy: reflectors work on contiguous columns
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    for (int j = 0; j < L; ++j) A[(uint64_t)j * n + i] = X[(uint64_t)i * ld + j];
+  std::vector<double> tau(L, 0.0);
+  [[maybe_unused]] const bool par = n * (uint64_t)L > 65536;
+  // dgeqr2: H_j = I - tau_j v_j v_j^T with v_j = [1; A(j+1:n, j)]
+  for (int j = 0; j < L; ++j) {
+    double* a = A.data() + (uint64_t)j * n;
+    const double c0 = a[j];
+    double tail = 0.0;
+    for (uint64_t i = j + 1; i < n; ++i) tail += a[i] * a[i];
+    double beta = c0, t = 0.0;
+    if (tail > std::numeric_limits<double>::min()) {
+      beta = std::sqrt(c0 * c0 + tail);
+      if (c0 >= 0.0) beta = -beta;
+      const double inv = 1.0 / (c0 - beta);
+      for (uint64_t i = j + 1; i < n; ++i) a[i] *= inv;
+      t = (beta - c0) / beta;
+    } else {
+      for (uint64_t i = j + 1; i < n; ++i) a[i] = 0.0;
+    }
+    a[j] = beta;
+    tau[j] = t;
+#pragma omp parallel for schedule(static) if (par)
+    for (int c = j + 1; c < L; ++c) {
+      double* b = A.data() + (uint64_t)c * n;
+      double w = b[j];
+      for (uint64_t i = j + 1; i < n; ++i) w += a[i] * b[i];
+      w *= t;
+      b[j] -= w;
+      for (uint64_t i = j + 1; i < n; ++i) b[i] -= w * a[i];
+    }
+  }
+  // dorg2r: Q = H_0 ... H_{L-1} [I; 0], formed in place from the last reflector
+  for (int j = L - 1; j >= 0; --j) {
+    double* a = A.data() + (uint64_t)j * n;
+    const double t = tau[j];
+    a[j] = 1.0;
+#pragma omp parallel for schedule(static) if (par)
+    for (int c = j + 1; c < L; ++c) {
+      double* b = A.data() + (uint64_t)c * n;
+      double w = 0.0;
+      for (uint64_t i = j; i < n; ++i) w += a[i] * b[i];
+      w *= t;
+      for (uint64_t i = j; i < n; ++i) b[i] -= w * a[i];
+    }
+    for (uint64_t i = j + 1; i < n; ++i) a[i] *= -t;
+    a[j] = 1.0 - t;
+    for (int i = 0; i < j; ++i) a[i] = 0.0;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    for (int j = 0; j < L; ++j) X[(uint64_t)i * ld + j] = A[(uint64_t)j * n + i];
 }
 
 // Rows (matmul) and columns (inv_upper) are independent: at the larger L they

@@ -114,6 +114,38 @@ PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& param
   return info;
 }
 
+void write_residuals(Context& ctx, const PcaResult& r, const Param& params, const std::string& out) {
+  // src/Data.cpp:242-291: uint32 nsnps, uint32 nsamples, then per SNP the
+  // nsamples float32 values of its centered residual column, at the SNP's
+  // position in the file order (the shuffle undone, src/Data.cpp:264-268).
+  const uint64_t N = ctx.N, M = ctx.M;
+  const int k = params.ld_stats == 0 ? r.k : 0;
+  std::vector<double> vcur;
+  if (k > 0) {
+    require(r.V.size() == M * (uint64_t)r.k && r.U.size() == N * (uint64_t)r.k, kErrState,
+            "residuals need U and V of the run");
+    vcur.resize(M * (uint64_t)k);
+    for (int c = 0; c < k; ++c)
+      for (uint64_t i = 0; i < M; ++i) vcur[(uint64_t)c * M + i] = r.V[(uint64_t)c * M + ctx.perm[i]];
+  }
+  ctx.residuals_prepare(k, r.U.data(), r.S.data(), vcur.data(), params.out_of_core);
+  std::ofstream f(out + ".residuals", std::ios::binary);
+  require(f.is_open(), kErrIO, "can not open " + out + ".residuals");
+  const uint32_t hdr[2] = {(uint32_t)M, (uint32_t)N};
+  f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+  const uint64_t rows = std::max<uint64_t>(1, std::min<uint64_t>(M, (256ull << 20) / (N * 4)));
+  std::vector<float> buf(rows * N);
+  for (uint64_t j0 = 0; j0 < M; j0 += rows) {
+    const uint64_t j1 = std::min(M, j0 + rows);
+    ctx.residuals_rows(j0, j1, buf.data());
+    for (uint64_t j = j0; j < j1; ++j) {
+      f.seekp((std::streamoff)(8 + (uint64_t)ctx.perm[j] * N * 4));
+      f.write(reinterpret_cast<const char*>(buf.data() + (j - j0) * N), (std::streamsize)(N * 4));
+    }
+  }
+  require(f.good(), kErrIO, "write failed: " + out + ".residuals");
+}
+
 void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, const std::string& bfile,
                    const std::string& out, const PlinkInfo& info) {
   const uint64_t N = ctx.N, M = ctx.M;
@@ -129,7 +161,7 @@ void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, 
     for (double e : r.E) o << fmt(e) << '\n';
   }
   write_matrix(out + ".eigvecs", r.U, N, k);
-  if (params.printv) {
+  if (params.printv || params.ld) {
     // .mbim: .bim line + allele frequency, kept SNPs only
     std::ifstream bim(bfile + ".bim");
     std::ofstream mb(out + ".mbim");
@@ -144,7 +176,7 @@ void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, 
         ++src;
       }
     }
-    write_matrix(out + ".loadings", r.V, M, k);
+    if (params.printv) write_matrix(out + ".loadings", r.V, M, k);
   }
   // .eigvecs2: "#FID IID PC1.." + the .eigvecs rows
   std::ifstream fam(bfile + ".fam");

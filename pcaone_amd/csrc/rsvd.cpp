@@ -11,11 +11,13 @@
 #include <random>
 
 #include "common.hpp"
+#include "exact.hpp"
 #include "rng_host.hpp"
 
 namespace pcaone {
 
 void Param::derive() {
+  ncv = 20 > (2 * k + 1) ? 20 : (2 * k + 1);        // src/Cmd.cpp:215
   oversamples = oversamples > k ? oversamples : k;  // src/Cmd.cpp:216
   filterSNP = maf > 0;                              // src/Cmd.cpp:220
   if (filterSNP) require(maf > 0 && maf < 0.5, kErrArgument, "--maf has to be between (0, 0.5)");
@@ -24,8 +26,8 @@ void Param::derive() {
   require(bands >= 4 && bands % 2 == 0, kErrArgument,
           "the -w/--batches must be a power of 2 and the minimun is 4.");  // src/Cmd.cpp:235-236
   perm = (svd == 2 && !noshuffle);                 // src/Cmd.cpp:238
-  require(svd == 1 || svd == 2, kErrArgument, "only --svd 1 and --svd 2 run on this path");
-  out_of_core = memory > 0;                        // src/Cmd.cpp:218
+  require(svd >= 0 && svd <= 3, kErrArgument, "--svd must be 0 (IRAM), 1, 2 or 3 (exact PCA)");
+  out_of_core = memory > 0 && svd != 3;            // src/Cmd.cpp:218
   require(!(filterSNP && out_of_core), kErrArgument,
           "does not support --maf filters for out-of-core mode yet! ");  // src/Cmd.cpp:221-224
 }
@@ -290,6 +292,16 @@ void FancyRsvdOpData::computeGandH_blocks(int pi) {
   }
 }
 
+PcaResult run_pca(Context& ctx, const Param& params, RunInfo* info, double* U_out) {
+  if (params.svd == 1 || params.svd == 2) return run_pca_with_halko(ctx, params, info, U_out);
+  PcaResult r = params.svd == 3 ? run_pca_full(ctx, params, info) : run_pca_arnoldi(ctx, params, info);
+  if (U_out && !r.U.empty()) {
+    std::memcpy(U_out, r.U.data(), r.U.size() * sizeof(double));
+    r.U.clear();
+  }
+  return r;
+}
+
 PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out) {
   // src/Halko.cpp:271-345 (genetic input, in-core, no LD output)
   ctx.reset_columns();  // restores the as-read state between runs; not part of the timed work
@@ -315,11 +327,11 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, d
   }
   rsvd->info = info;
   rsvd->u_out = U_out;
-  rsvd->need_v_host = params.printv || params.missme;
+  rsvd->need_v_host = params.printv || params.missme || params.ld;
   const int k = (int)params.k;
   const uint64_t N = ctx.N, M = ctx.M;
   if (!params.missme) {
-    rsvd->setFlags(false, true);
+    rsvd->setFlags(false, !params.ld);  // src/Halko.cpp:283-285: --ld keeps the matrix unscaled
     rsvd->computeUSV((int)params.maxp, params.tol);
   } else {
     if (ctx.nmiss == 0 && params.verbose) std::printf("warning: there is no missing values\n");

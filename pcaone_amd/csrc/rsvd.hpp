@@ -24,6 +24,11 @@ struct Param {
   GemmTuning tuning;
   double memory = 0.0;  // -m/--memory (GB); > 0 selects the out-of-core schedule
   uint32_t buffer = 2;  // --buffer (GB) of permute_plink
+  bool ld = false;      // -D/--ld: unscaled run, residual matrix written (src/Cmd.cpp:92)
+  int ld_stats = 0;     // --ld-stats (src/Cmd.cpp:114)
+  uint32_t imaxiter = 1000;  // --imaxiter (IRAM)
+  double itol = 1e-6;        // --itol (IRAM)
+  uint32_t ncv = 20;         // derived: max(20, 2k + 1) (src/Cmd.cpp:215)
   // derived (src/Cmd.cpp:171-238)
   bool perm = false, missme = false, filterSNP = false, out_of_core = false;
   void derive();
@@ -118,6 +123,9 @@ struct PcaResult {
 
 // U_out (nullable, N x k column-major): receives U directly; res.U is then empty.
 PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, double* U_out = nullptr);
+// The PCA the params select (src/Main.cpp:176-217): --svd 0 IRAM, 1/2 the
+// randomized SVD, 3 the exact eigen-decomposition.
+PcaResult run_pca(Context& ctx, const Param& params, RunInfo* info, double* U_out = nullptr);
 
 
 }  // namespace pcaone

@@ -64,6 +64,7 @@ def params_from_args(args: str, **extra):
     flags = {"-d": ("svd", int), "-k": ("k", int), "--maxp": ("maxp", int), "--rand": ("rand", int),
              "--seed": ("seed", int), "--tol-rsvd": ("tol", float), "--maf": ("maf", float), "-w": ("bands", int),
              "--maxiter": ("maxiter", int), "--oversamples": ("oversamples", int), "--tol-em": ("tolem", float),
+             "--ld-stats": ("ld_stats", int),
              "-m": ("memory", float), "--buffer": ("buffer", int)}
     printv = False
     while i < len(toks):
@@ -75,6 +76,8 @@ def params_from_args(args: str, **extra):
             continue
         if t == "-V":
             printv = True
+        elif t == "-D":
+            kw["ld"] = True
         elif t == "-S":
             kw["noshuffle"] = True
         elif t == "--emu":

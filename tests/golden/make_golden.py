@@ -33,6 +33,8 @@ import numpy as np  # noqa: E402
 from pcaone_amd import synth  # noqa: E402
 
 REF = ROOT / "oracle" / "_ref" / "pcaone_ref"
+# the complete reference program (oracle/ref_full.mk): the --svd 0 and --svd 3 branches
+REF_FULL = ROOT / "oracle" / "_ref" / "pcaone_full"
 DATA = HERE / "data"
 
 INPUTS = {
@@ -42,7 +44,31 @@ INPUTS = {
     # edge cases: ragged N (150), monomorphic, all-missing and single-genotype SNPs
     "f5": dict(spec=synth.Spec(seed=2005, nsamples=150, nsnps=1200, npops=3, miss_rate=0.05), commit=True,
                edit="edge"),
+    # rank deficient: 12 distinct samples, each present 16 times (N = 192), so the
+    # standardized matrix has rank 11 (centering removes one)
+    "f6": dict(spec=synth.Spec(seed=2006, nsamples=12, nsnps=1500, npops=3), commit=True, edit="dup16"),
+    # small input for the residual-matrix (-D/--ld) files: 64 x 500, 5 % missing
+    "f7": dict(spec=synth.Spec(seed=2007, nsamples=64, nsnps=500, npops=3, miss_rate=0.05), commit=True),
+    # more samples than SNPs (256 x 150): the tall branches of --svd 0 and --svd 3
+    "f8": dict(spec=synth.Spec(seed=2008, nsamples=256, nsnps=150, npops=4), commit=True),
+    # 1,008 SNPs in 64 windows of 16: window 63 starts at SNP 1,008 = M, an empty window
+    "f9": dict(spec=synth.Spec(seed=7, nsamples=150, nsnps=1008, npops=3), commit=True),
 }
+
+
+def dup_samples(rows, n, reps):
+    """Packed rows of n samples -> rows of n * reps samples, sample i a copy of
+    sample i % n."""
+    m = rows.shape[0]
+    codes = np.zeros((m, n), dtype=np.uint8)
+    for i in range(n):
+        codes[:, i] = (rows[:, i // 4] >> (2 * (i % 4))) & 3
+    full = np.tile(codes, (1, reps))
+    N = n * reps
+    out = np.zeros((m, (N + 3) // 4), dtype=np.uint8)
+    for i in range(N):
+        out[:, i // 4] |= full[:, i] << (2 * (i % 4))
+    return out
 
 
 def edge_rows(rows, n):
@@ -91,6 +117,26 @@ CASES = {
     "f5_emu": ("f5", "-d 1 -k 3 --emu --maxiter 4 -V"),
     "f5_emu_d2": ("f5", "-d 2 -w 16 -k 3 --emu --maxiter 4 -V"),
     "f5_maf": ("f5", "-d 1 -k 4 --maf 0.02 --maxp 5 -V"),
+    # rank 11 < L = 16: the Gram matrices of G and H are singular
+    "f6_d1": ("f6", "-d 1 -k 8 --maxp 7 -V"),
+    "f6_d2": ("f6", "-d 2 -w 16 -k 8 --maxp 7 -V"),
+    # L = 10 just under the rank
+    "f6_near": ("f6", "-d 1 -k 5 --oversamples 5 --maxp 7 --tol-rsvd 0 -V"),
+    # -D/--ld: the residual matrix (src/Data.cpp:242-291) after an unscaled run
+    "f7_ld": ("f7", "-d 1 -k 3 --maxp 7 -D"),
+    "f7_ld_std": ("f7", "-d 1 -k 3 --maxp 7 -D --ld-stats 1"),
+    "f7_ld_d2": ("f7", "-d 2 -w 8 -k 3 --maxp 7 -D"),
+    "f7_ld_emu": ("f7", "-d 1 -k 3 --emu --maxiter 4 -D"),
+    "f7_ld_ooc": ("f7", "-d 2 -w 8 -k 3 --maxp 7 -D -m 0.0002"),
+    # --svd 3: exact PCA (src/Main.cpp:180-217); --svd 0: IRAM (src/Arnoldi.cpp:48-118)
+    "f1_d3": ("f1", "-d 3 -k 5 -V"),
+    "f4_d3": ("f4", "-d 3 -k 3 -V"),
+    "f8_d3": ("f8", "-d 3 -k 4 -V"),
+    "f1_d0": ("f1", "-d 0 -k 4 -V"),
+    "f8_d0": ("f8", "-d 0 -k 3 -V"),
+    "f7_d0_ld": ("f7", "-d 0 -k 3 -D"),
+    # winSVD whose last window is empty (src/Halko.cpp:191-194 with a zero width)
+    "f9_d2_empty": ("f9", "-d 2 -w 64 -k 4 --maxp 7 --tol-rsvd 0 -V"),
 }
 
 EPOCH_RE = re.compile(r"running of epoch =\s*(\d+), diff = ([-0-9.e+]+)")
@@ -116,6 +162,14 @@ def make_inputs(tmp: Path) -> dict:
             raw = bed.read_bytes()
             rows = np.frombuffer(raw[3:], dtype=np.uint8).reshape(spec.nsnps, -1).copy()
             bed.write_bytes(raw[:3] + edge_rows(rows, spec.nsamples).tobytes())
+        if d.get("edit", "").startswith("dup"):
+            reps = int(d["edit"][3:])
+            bed = Path(str(prefix) + ".bed")
+            raw = bed.read_bytes()
+            rows = np.frombuffer(raw[3:], dtype=np.uint8).reshape(spec.nsnps, -1)
+            bed.write_bytes(raw[:3] + dup_samples(rows, spec.nsamples, reps).tobytes())
+            Path(str(prefix) + ".fam").write_text(
+                "".join(f"F{i} I{i} 0 0 0 -9\n" for i in range(spec.nsamples * reps)))
         out[name] = dict(prefix=str(prefix), sha256=sha256(Path(str(prefix) + ".bed")), spec=spec.__dict__,
                          committed=d["commit"])
     return out
@@ -127,7 +181,8 @@ def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
         shutil.rmtree(dest)
     dest.mkdir()
     outp = tmp / name
-    cmd = [str(REF), "-b", inp["prefix"], *args.split(), "-o", str(outp), "-v", "1"]
+    full = "-d 0" in args or "-d 3" in args
+    cmd = [str(REF_FULL if full else REF), "-b", inp["prefix"], *args.split(), "-o", str(outp), "-v", "1"]
     env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
     res = subprocess.run(cmd, capture_output=True, text=True, env=env, check=True)
     log = res.stdout
@@ -135,7 +190,7 @@ def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
     stop = STOP_RE.findall(log)
     em = [float(m.group(2)) for m in EM_RE.finditer(log)]
     plan = [[int(g) for g in m.groups()] for m in PLAN_RE.finditer(log)]
-    sufs = [".eigvals", ".sigvals", ".eigvecs", ".loadings", ".mbim"]
+    sufs = [".eigvals", ".sigvals", ".eigvecs", ".loadings", ".mbim", ".residuals"]
     if inp["committed"]:
         sufs.append(".eigvecs2")  # the small inputs also keep the plink2-style file (src/Utils.cpp:225-238)
     for suf in sufs:

@@ -1,6 +1,8 @@
 """CPU-side checks of libpcaone_amd.so: exported symbols, host FP64 helpers,
 parameter defaults and the no-device error path (no GPU compute here)."""
 import ctypes as C
+import os
+import subprocess
 import re
 from pathlib import Path
 
@@ -94,6 +96,23 @@ def test_host_cholesky_and_breakdown():
     st = _lib.lib().pcaone_host_chol_upper(3, B.ctypes.data, R.ctypes.data)
     assert st == 5  # PCAONE_ERR_NUMERIC
     assert b"Cholesky" in _lib.lib().pcaone_last_error(None)
+
+
+@pytest.mark.parametrize("n,L,rank", [(50, 7, 7), (300, 16, 11), (2000, 20, 20), (64, 12, 1)])
+def test_host_householder_q_is_lapacks(n, L, rank):
+    """The Householder thin Q (the fallback for a near rank-deficient H) equals
+    LAPACK's dgeqrf/dorgqr Q (numpy) column for column, signs included, for
+    full-rank X; for rank-deficient X its first `rank` columns equal LAPACK's
+    and all L columns are orthonormal."""
+    rng = np.random.default_rng(n + L)
+    X = rng.standard_normal((n, rank)) @ rng.standard_normal((rank, L))
+    Q = np.ascontiguousarray(X.copy())
+    _lib.check(_lib.lib().pcaone_host_householder_q(n, L, Q.ctypes.data))
+    np.testing.assert_allclose(Q.T @ Q, np.eye(L), atol=1e-13)
+    Qref = np.linalg.qr(X, mode="reduced")[0]
+    np.testing.assert_allclose(Q[:, :rank], Qref[:, :rank], atol=1e-12)
+    if rank == L:
+        np.testing.assert_allclose(Q, Qref, atol=1e-12)
 
 
 def test_null_arguments_are_rejected():
@@ -212,3 +231,20 @@ def test_bed_missing_file_is_an_io_error(tmp_path):
     assert (st, msg) == (4, "Cannot open bed file.")
     st, _, _, msg = _bed_dims(tmp_path / "absent")
     assert st == 4 and "can not open" in msg
+
+
+REF_SRC = Path(os.environ.get("PCAONE_REF", "/root/reference"))
+
+
+@pytest.mark.skipif(not (REF_SRC / "src" / "Halko.hpp").exists(), reason="reference sources absent")
+def test_binding_compiles_against_reference_headers():
+    """The maintainer-side binding (AmdFileBed : Data, AmdNormalRsvdOpData :
+    NormalRsvdOpData, AmdFancyRsvdOpData : FancyRsvdOpData, the level-1 and
+    level-2 drivers) compiles against the reference's own src/Halko.hpp and
+    src/Data.hpp with the reference's compiler flags."""
+    root = Path(__file__).resolve().parents[1]
+    cmd = ["g++", "-fsyntax-only", "-std=c++17", "-fopenmp", "-DVERSION=\"0.7.2\"", "-w",
+           f"-I{REF_SRC / 'src'}", f"-I{REF_SRC / 'external'}", f"-I{REF_SRC / 'external' / 'zstd' / 'lib'}",
+           f"-I{root / 'include'}", f"-I{root / 'pcaone_amd' / 'binding'}", str(root / "oracle" / "ref_driver_amd.cpp")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-3000:]

@@ -219,7 +219,8 @@ def run_case(ctx, golden_inputs, case, form):
     p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
                            tol=params.tol, seed=params.seed, rand=params.rand, bands=params.bands,
                            noshuffle=int(params.noshuffle), emu=int(params.emu), maxiter=params.maxiter,
-                           tolem=params.tolem, memory=params.memory, buffer=params.buffer, precision=form)
+                           tolem=params.tolem, memory=params.memory, buffer=params.buffer, ld=int(params.ld),
+                           ld_stats=params.ld_stats, precision=form)
     return meta, params, printv, ctx.run(p, want_v=printv)
 
 
@@ -238,6 +239,8 @@ def test_golden_fp64(ctx, golden_inputs, case):
         np.testing.assert_allclose(r.S, O.read_matrix(d / "sigvals").ravel(), rtol=1e-5)
     Uref = O.read_matrix(d / "eigvecs")
     np.testing.assert_allclose(O.sign_align(Uref, r.U), Uref, atol=2e-6)
+    if params.svd == 3:  # exact PCA: flip_UV fixes the signs (src/Main.cpp:215)
+        np.testing.assert_allclose(r.U, Uref, atol=2e-6)
     if printv and (d / "loadings").exists():
         Vref = O.read_matrix(d / "loadings")
         np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=2e-6)
@@ -250,6 +253,8 @@ REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16", "int8")
 REDUCED += [(c, f) for f in ("bf16mix", "fp16mix") for c in ("f1_d1", "f4_d1_miss", "cfg1_d1")]
 # degenerate SNPs (monomorphic, all missing, one genotype among missing entries)
 REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f5_d1", "f5_d2", "f5_emu")]
+# rank deficient (rank 11 < L = 16): the orthonormalization fallbacks in every form
+REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f6_d1", "f6_d2")]
 # the out-of-core schedule (block windows, permute_plink order, per-pass imputation)
 REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f1_d2_ooc", "f4_emu_d2_ooc")]
 # Single-piece forms round Omega (or G) to 8-11 bits: the eigenvalues meet the
@@ -291,15 +296,47 @@ def test_precision_form_at_l80_matches_oracle(ctx, golden_inputs, form):
     assert corr.min() >= 0.9999
 
 
+@pytest.mark.parametrize("case,form", [("f6_d1", "fp64"), ("f6_d1", "bf16"), ("f6_d2", "fp64"),
+                                       ("f6_near", "fp64")])
+def test_rank_deficient_input_takes_the_fallbacks(ctx, golden_inputs, case, form):
+    """12 distinct samples repeated 16 times: rank 11.  With L = 16 the Gram
+    matrices of G and H are singular, where CholeskyQR2 breaks down; the thin Q
+    of H falls back to a Householder QR and the QR of G to a rank-revealing
+    second pass, and the run matches the reference binary (golden FP64 test)
+    and the oracle.  With L = 10 (f6_near, one under the rank) no fallback is
+    needed for H."""
+    meta, params, printv, r = run_case(ctx, golden_inputs, case, form)
+    info = r.info
+    say(f"  {case} {form}: householder {info['householder_fallbacks']} second passes "
+        f"{info['gram_second_passes']} epochs {info['epochs']}")
+    ref = O.run_plink(golden_inputs[meta["input"]], params)
+    rel = np.abs(r.E - ref.E) / ref.E
+    corr = O.pc_correlations(ref.U, r.U)
+    say(f"  eig rel err max {rel.max():.2e}  min PC corr {corr.min():.10f}")
+    assert rel.max() <= (1e-10 if form == "fp64" else 1e-4)
+    assert corr.min() >= (1 - 1e-9 if form == "fp64" else 0.9999)
+    if case == "f6_near":
+        assert info["householder_fallbacks"] == 0
+    else:
+        assert info["householder_fallbacks"] >= 1 and info["gram_second_passes"] >= 1
+
+
+def test_well_conditioned_runs_take_no_fallback(ctx, golden_inputs):
+    meta, params, printv, r = run_case(ctx, golden_inputs, "cfg1_d1", "bf16")
+    assert r.info["householder_fallbacks"] == 0 and r.info["gram_second_passes"] == 0
+
+
 @pytest.mark.parametrize("form", ["fp64", "bf16"])
 def test_winsvd_windows_past_the_last_snp(ctx, form):
-    """M = 1,000 < bands^2 = 4,096 with -w 64: windows of ceil(1000 / 64) = 16
-    SNPs, so window 63 starts at SNP 1,008, past the last one.  The reference
-    computes a negative width there and aborts (std::bad_alloc); here and in
-    the oracle such windows are empty (src/Halko.cpp:191-194) and the band
-    schedule is otherwise unchanged."""
+    """M = 1,008 < bands^2 = 4,096 with -w 64: windows of ceil(1008 / 64) = 16
+    SNPs, so window 63 starts at SNP 1,008 = M and is empty (src/Halko.cpp:
+    191-194 give it width 0; the golden case f9_d2_empty holds the reference's
+    run).  Every other window has 16 SNPs, more than L = 14, so no H of a
+    band is rank deficient and the trajectory is fixed by the data (with a
+    shorter last window H can lose rank, and the Householder completion of
+    Omega then depends on rounding).  Here against the oracle."""
     from pcaone_amd import synth
-    spec = synth.Spec(seed=7, nsamples=150, nsnps=1000, npops=3)
+    spec = synth.Spec(seed=7, nsamples=150, nsnps=1008, npops=3)
     say(f"winSVD with empty trailing windows, {form}")
     kw = dict(svd=2, k=4, oversamples=10, bands=64, maxp=7, tol=0.0)
     ref = O.run_pca_with_halko(O.Data.from_packed(synth.packed(spec), spec.nsamples, O.Params(**kw)))
@@ -445,11 +482,65 @@ def test_workgroup_shapes_agree(ctx, golden_inputs, form, wm, waves):
 
 
 # ---------------------------------------------------------------- output files (pcaone_run_bfile)
+@pytest.mark.parametrize("case,form", [("f7_ld", "fp64"), ("f7_ld_std", "fp64"), ("f7_ld_d2", "fp64"),
+                                       ("f7_ld_emu", "fp64"), ("f7_ld_ooc", "fp64"), ("f7_ld", "bf16"),
+                                       ("f7_ld_emu", "bf16")])
+def test_residual_files_match_reference(ctx, golden_inputs, case, form, tmp_path):
+    """-D/--ld (src/Data.cpp:242-291): the run stays unscaled, the
+    .residuals file (uint32 M, uint32 N, then N float32 per SNP in file order)
+    holds A - U S V^T centered per SNP (the centered genotypes with
+    --ld-stats 1), with the winSVD shuffle undone, the EMU imputation in place,
+    and the out-of-core branch's unscaled, unimputed re-read; plus the .mbim.
+    FP64 runs to 2e-6 (the reference's values are float32); BF16 to 1e-4."""
+    meta = load_meta(case)
+    params, printv = params_from_args(meta["args"])
+    say(f"residual file {case} form={form} '{meta['args']}'")
+    out = str(tmp_path / "run")
+    p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
+                           tol=params.tol, bands=params.bands, noshuffle=int(params.noshuffle), emu=int(params.emu),
+                           maxiter=params.maxiter, tolem=params.tolem, memory=params.memory, ld=int(params.ld),
+                           ld_stats=params.ld_stats, precision=form)
+    ctx.run_bfile(golden_inputs[meta["input"]], out, p)
+    d = GOLDEN / case
+    R, Rref = O.read_residuals_file(out + ".residuals"), O.read_residuals_file(d / "residuals")
+    assert R.shape == Rref.shape
+    err = float(np.abs(R - Rref).max())
+    say(f"  max |R - R_ref| {err:.2e}")
+    assert err <= (2e-6 if form == "fp64" else 1e-4)
+    mb, mb_ref = _rows(out + ".mbim"), _rows(d / "mbim")
+    assert [r[:6] for r in mb] == [r[:6] for r in mb_ref]
+    np.testing.assert_allclose(O.read_matrix(out + ".eigvals"), O.read_matrix(d / "eigvals"),
+                               rtol=1e-5 if form == "fp64" else 1e-4)
+
+
+def test_residuals_entry_point_matches_oracle(ctx, golden_inputs):
+    """pcaone_residuals on the 2,000 x 20,000 input in row blocks (one not a
+    multiple of the 32-row tile) against the oracle's write_residuals with the
+    same U, S, V, after an unscaled FP64 run."""
+    prefix = golden_inputs["cfg1"]
+    params = O.Params(svd=1, k=10, maxp=4, tol=0.0, ld=True)
+    packed, n, m = O.read_plink(prefix)
+    data = O.Data.from_packed(packed, n, params)
+    ref = O.run_pca_with_halko(data)
+    ctx.read_bed(prefix)
+    r = ctx.run(api.default_params(svd=1, k=10, maxp=4, tol=0.0, ld=1, precision="fp64"), want_v=True)
+    blocks = [ctx.residuals(ref.U, ref.S, ref.V, j0, j1) for j0, j1 in ((0, 7001), (7001, 20000))]
+    R = np.concatenate(blocks)
+    err = float(np.abs(R - ref.residuals).max())
+    say(f"residuals cfg1 k=10: max err {err:.2e}")
+    assert err <= 2e-6
+    np.testing.assert_allclose(r.S, ref.S, rtol=1e-9)
+    R0 = ctx.residuals(None, None, None, 0, 100)  # --ld-stats 1: the centered matrix
+    G = data.G[:, :100]
+    np.testing.assert_allclose(R0, (G - G.mean(axis=0)).T.astype(np.float32), atol=1e-7)
+
+
 def _rows(path):
     return [line.split("\t") for line in Path(path).read_text().splitlines()]
 
 
-@pytest.mark.parametrize("case", ["f1_d1", "f1_d1_maf", "f1_d2", "f4_emu_d1", "f1_d2_ooc", "f4_emu_d2_ooc"])
+@pytest.mark.parametrize("case", ["f1_d1", "f1_d1_maf", "f1_d2", "f4_emu_d1", "f1_d2_ooc", "f4_emu_d2_ooc", "f1_d3",
+                                  "f8_d0"])
 def test_output_files_match_reference(ctx, golden_inputs, case, tmp_path):
     """pcaone_run_bfile writes the reference's files (src/Data.cpp:211-240,
     src/Utils.cpp:225-238): .sigvals with its '#N,M' header, .eigvals,

@@ -112,6 +112,8 @@ def _outputs_match(case, res, data, printv, tol_vec=2e-6):
     U = O.read_matrix(d / "eigvecs")
     assert U.shape == res.U.shape
     np.testing.assert_allclose(O.sign_align(U, res.U), U, atol=tol_vec)
+    if data.params.svd == 3:  # flip_UV fixes the signs of the exact PCA (src/Main.cpp:215)
+        np.testing.assert_allclose(res.U, U, atol=tol_vec)
     if printv and (d / "loadings").exists():
         V = O.read_matrix(d / "loadings")
         Vo = res.V
@@ -119,6 +121,12 @@ def _outputs_match(case, res, data, printv, tol_vec=2e-6):
             Vo = np.empty_like(res.V)
             Vo[data.perm] = res.V
         np.testing.assert_allclose(O.sign_align(V, Vo), V, atol=tol_vec)
+    if (d / "residuals").exists():
+        R = O.read_residuals_file(d / "residuals")
+        assert res.residuals is not None and res.residuals.shape == R.shape
+        # float32 values; with --ld-stats 1 no PCs are involved and only the
+        # summation order of the column means differs
+        np.testing.assert_allclose(res.residuals, R, atol=1e-7 if data.params.ld_stats == 1 else 2e-6)
 
 
 @pytest.mark.parametrize("case", golden_cases())
@@ -127,7 +135,7 @@ def test_oracle_matches_reference_binary(case, golden_inputs):
     params, printv = params_from_args(meta["args"])
     packed, n, _ = O.read_plink(golden_inputs[meta["input"]])
     data = O.Data.from_packed(packed, n, params)
-    res = O.run_pca_with_halko(data)
+    res = O.run_pca(data)
     if meta["trace"]:
         np.testing.assert_allclose(res.trace, meta["trace"], atol=1.5e-6)
     if meta["stop_epochs"]:
@@ -177,6 +185,57 @@ def _plant_swapped_clamp(monkeypatch):
     monkeypatch.setattr(O.Data, "fit_with_pi", bad)
 
 
+def _residuals_with(monkeypatch, center=True, unpermute=True, subtract=True, standardized_ooc=False):
+    def bad(data, S, U, VT):
+        p = data.params
+        G = (data.G0 if p.out_of_core and not standardized_ooc else data.G).copy()
+        if subtract:
+            G -= (U * S[None, :]) @ VT
+        if center:
+            G -= G.mean(axis=0, keepdims=True)
+        out = np.empty((G.shape[1], G.shape[0]), dtype=np.float32)
+        rows = data.perm if (unpermute and p.perm and data.perm is not None) else np.arange(G.shape[1])
+        out[rows] = G.T.astype(np.float32)
+        return out
+    monkeypatch.setattr(O, "residuals", bad)
+
+
+def _plant_residuals_uncentered(monkeypatch):
+    _residuals_with(monkeypatch, center=False)
+
+
+def _plant_residuals_not_unpermuted(monkeypatch):
+    _residuals_with(monkeypatch, unpermute=False)
+
+
+def _plant_residuals_not_subtracted(monkeypatch):
+    _residuals_with(monkeypatch, subtract=False)
+
+
+def _plant_residuals_ld_standardized(monkeypatch):
+    # --ld runs the PCA on the standardized matrix (the flag ignored)
+    orig = O.NormalOp.set_flags
+    monkeypatch.setattr(O.NormalOp, "set_flags", lambda self, u, s: orig(self, u, True))
+
+
+def _plant_full_no_flip(monkeypatch):
+    monkeypatch.setattr(O, "flip_uv", lambda U, V, ubase=False: (U, V))
+
+
+def _plant_full_gram_unscaled(monkeypatch):
+    orig = O.run_pca_full
+
+    def bad(data):
+        r = orig(data)
+        r.E = r.E * data.nsnps  # the Gram not divided by M
+        return r
+    monkeypatch.setattr(O, "run_pca_full", bad)
+
+
+def _plant_iram_unstandardized(monkeypatch):
+    monkeypatch.setattr(O.Data, "standardize_E", lambda self: None)
+
+
 def _plant_ooc_band_factor_one(monkeypatch):
     orig = O.block_plan
 
@@ -214,7 +273,7 @@ def test_block_plan_matches_reference_log(case):
     """Data::prepare's plan (src/Data.cpp:42-84) as the reference binary logged it."""
     meta = load_meta(case)
     params, _ = params_from_args(meta["args"])
-    n, m = {"f1": (200, 2000), "f4": (300, 3000)}[meta["input"]]
+    n, m = {"f1": (200, 2000), "f4": (300, 3000), "f7": (64, 500)}[meta["input"]]
     plan = O.block_plan(params, n, m)
     assert [plan.blocksize, plan.nblocks, plan.band_factor] == meta["block_plan"]
     assert plan.start[-1] < m <= plan.stop[-1] + 1
@@ -230,6 +289,13 @@ def test_block_plan_matches_reference_log(case):
     ("f1_d1", _plant_no_scaling),
     ("f1_d1", _plant_no_centering),
     ("f4_emu_d1", _plant_swapped_clamp),
+    ("f7_ld_emu", _plant_residuals_uncentered),  # imputed entries move the column means
+    ("f7_ld_d2", _plant_residuals_not_unpermuted),
+    ("f7_ld", _plant_residuals_not_subtracted),
+    ("f7_ld", _plant_residuals_ld_standardized),
+    ("f1_d3", _plant_full_no_flip),
+    ("f1_d3", _plant_full_gram_unscaled),
+    ("f1_d0", _plant_iram_unstandardized),
 ])
 def test_planted_mistake_is_caught(case, plant, golden_inputs, monkeypatch):
     """A plausible mistake planted in the oracle must fail the golden comparison."""
@@ -238,7 +304,7 @@ def test_planted_mistake_is_caught(case, plant, golden_inputs, monkeypatch):
     packed, n, _ = O.read_plink(golden_inputs[meta["input"]])
     plant(monkeypatch)
     data = O.Data.from_packed(packed, n, params)
-    res = O.run_pca_with_halko(data)
+    res = O.run_pca(data)
     with pytest.raises(AssertionError):
         if meta["trace"]:
             np.testing.assert_allclose(res.trace, meta["trace"], atol=1.5e-6)
@@ -263,6 +329,29 @@ def test_dense_oracle_agrees_at_convergence(golden_inputs):
 
 
 # ---------------------------------------------------------------- metric helpers (closed forms)
+def test_full_piv_solve_closed_forms():
+    """The restated fullPivHouseholderQr solve: the ordinary solve for an
+    invertible matrix; for a singular consistent system an exact solution with
+    exactly n - rank unknowns set to zero (the basic solution); a tiny planted
+    pivot below eps * n of the largest counts as zero."""
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((12, 12))
+    B = rng.standard_normal((12, 5))
+    np.testing.assert_allclose(O.full_piv_solve(A, B), np.linalg.solve(A, B), atol=1e-12)
+    A2 = rng.standard_normal((12, 7)) @ rng.standard_normal((7, 12))
+    B2 = A2 @ rng.standard_normal((12, 3))
+    X = O.full_piv_solve(A2, B2)
+    np.testing.assert_allclose(A2 @ X, B2, atol=1e-11)
+    assert int((np.abs(X).sum(axis=1) == 0).sum()) == 12 - 7
+    # upper triangular with one pivot at 1e-18 of the others: that unknown is 0
+    R = np.triu(rng.standard_normal((6, 6))) + 4 * np.eye(6)
+    R[5, 5] = 1e-18
+    X3 = O.full_piv_solve(R.T, rng.standard_normal((6, 2)))
+    assert np.all(X3[5] == 0.0) and np.all(np.abs(X3[:5]) > 0)
+    # the same with LAPACK's solve blows that unknown up instead
+    assert np.abs(np.linalg.solve(R.T, rng.standard_normal((6, 2)))[5]).max() > 1e10
+
+
 def test_pc_correlations_closed_forms():
     """|Pearson r| per column: invariant to sign and to a positive scale and
     shift (1), zero for a column orthogonal to the other after centering,
