@@ -267,6 +267,19 @@ int pcaone_synth_write_plink(const pcaone_synth_spec* spec, const char* prefix);
  * codes). */
 int pcaone_packed_gemm(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
                        const double* W, uint32_t L, double* Y);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pcaone_packed_gemm on DEVICE pointers (rows: nrows x
+ * ceil(ncodes/4) bytes, W: ncodes x L column-major FP64, Y: nrows x L
+ * column-major FP64), with every intermediate in the library's fixed 64 MiB
+ * device scratch array (one per GPU, held until the process ends); no
+ * hipMalloc/hipFree.  A workspace beyond the scratch (about rows_pad x
+ * ceil(ncodes/256) x 64 bytes + ncodes x L x 12 bytes + rows_pad x L x 4-8
+ * bytes) is PCAONE_ERR_ARGUMENT.  Calls on one GPU share the scratch: calls
+ * that overlap on one GPU race.  Runs on the context's stream and returns
+ * after Y is written. */
+int pcaone_packed_gemm_noalloc(pcaone_ctx* ctx, int precision, const uint8_t* rows, uint64_t nrows,
+                               uint32_t ncodes, const double* W, uint32_t L, double* Y);
 /* Data::prepare's out-of-core block plan (src/Data.cpp:42-84) for these
  * params (memory > 0): block size, block count, winSVD band factor and the
  * first SNP of each block (up to cap values into start, nullable). */

@@ -112,6 +112,7 @@ _SIGS = {
     "pcaone_host_chol_upper": (C.c_int, [C.c_int, _P, _P]),
     "pcaone_host_householder_q": (C.c_int, [C.c_uint64, C.c_int, _P]),
     "pcaone_residuals": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_int, C.c_uint64, C.c_uint64, _P]),
+    "pcaone_packed_gemm_noalloc": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.c_uint32, _P]),
 }
 
 

@@ -9,6 +9,7 @@
 #include "context.hpp"
 #include "kernels.hpp"
 #include "linalg_host.hpp"
+#include "noalloc.hpp"
 #include "plink_io.hpp"
 #include "rng_host.hpp"
 #include "rsvd.hpp"
@@ -598,6 +599,19 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
           v = reinterpret_cast<const float*>(h.data())[idx];
         Y[(uint64_t)l * nrows + r] = v;
       }
+  });
+}
+
+int pcaone_packed_gemm_noalloc(pcaone_ctx* c, int precision, const uint8_t* rows, uint64_t nrows, uint32_t ncodes,
+                               const double* W, uint32_t L, double* Y) {
+  return guard(c, [&](Context& x) {
+    require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
+    require(precision >= 0 && precision < kNumPrecisions && forms_of(precision).g == forms_of(precision).h,
+            kErrArgument, "packed_gemm_noalloc: precision must be a single-form precision (0..3, 5 or 7)");
+    const Form f = forms_of(precision).g;
+    require(f != Form::I8 || round_up(ncodes, kKStep) <= kI8ChunkCap, kErrArgument,
+            "packed_gemm_noalloc: the int8 form takes at most 65536 codes per call");
+    packed_gemm_noalloc(x, f, rows, nrows, ncodes, W, L, Y);
   });
 }
 
