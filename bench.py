@@ -244,8 +244,22 @@ def config_line(args) -> int:
     spec = synth.config(args.config)
     ctx = api.Context(0)
     t0 = time.perf_counter()
-    ctx.generate(spec)
-    log(f"config {args.config}: generated {spec.nsamples} x {spec.nsnps} in {time.perf_counter() - t0:.1f}s")
+    packed = None
+    if args.stream:
+        # the matrix stays in host memory; each product loads its -m block (data beyond HBM)
+        require_mem = args.memory > 0
+        if not require_mem:
+            raise SystemExit("--stream needs --memory GB (the -m plan sets the blocks)")
+        packed = synth.packed(spec)
+        plan = block_plan(api.default_params(**run["params"]), spec.nsamples, spec.nsnps)
+        log(f"config {args.config}: generated {spec.nsamples} x {spec.nsnps} on the host "
+            f"({packed.nbytes / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f}s")
+        t0 = time.perf_counter()
+        ctx.stream_host(packed, spec.nsamples, plan["blocksize"])
+        log(f"config {args.config}: streamed open (allele counts over all blocks) {time.perf_counter() - t0:.1f}s")
+    else:
+        ctx.generate(spec)
+        log(f"config {args.config}: generated {spec.nsamples} x {spec.nsnps} in {time.perf_counter() - t0:.1f}s")
     res, secs = {}, {}
     forms = [f for f in (args.forms.split(",") if args.forms else run["forms"]) if f != "fp64"] + ["fp64"]
     for form in forms:
@@ -266,6 +280,16 @@ def config_line(args) -> int:
     if args.memory > 0:
         out["memory_gb"] = args.memory
         out["block_plan"] = block_plan(api.default_params(**run["params"]), spec.nsamples, spec.nsnps)
+    if args.stream:
+        bps = (spec.nsamples + 3) // 4
+        blk = out["block_plan"]["blocksize"]
+        out["streamed"] = {
+            "host_matrix_bytes": int(packed.nbytes),
+            # the two packed layouts of one block on the device, against both layouts of the whole matrix
+            "device_genotype_bytes": int(2 * blk * bps),
+            "resident_genotype_bytes": int(2 * spec.nsnps * bps),
+            "what": "packed rows in host memory; every product loads its block (gather, DMA, interleave, "
+                    "transpose, missing lists) into block-sized layouts"}
     ctx.time_kernels(True)
     rk = ctx.run(api.default_params(precision=forms[0], **run["params"]))
     ctx.time_kernels(False)
@@ -306,6 +330,8 @@ def main() -> int:
                     help="power iterations of the converged FP64 reference (0 = none)")
     ap.add_argument("--cpu-memory", type=float, default=2.0,
                     help="-m GB of the reference binary's CPU baseline (out-of-core mode)")
+    ap.add_argument("--stream", action="store_true",
+                    help="with --config and --memory: keep the matrix in host memory, stream the -m blocks")
     ap.add_argument("--memory", type=float, default=0.0,
                     help="with --config: run the out-of-core schedule (-m GB: blocks of SNPs per pass)")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB plain copy (counter calibration)")

@@ -85,6 +85,9 @@ typedef struct pcaone_params {
                         /* centered genotypes                                       */
   uint32_t imaxiter;    /* --imaxiter (1000): restarts of the IRAM solver, --svd 0  */
   double itol;          /* --itol (1e-6): IRAM convergence tolerance                */
+  int stream;           /* pcaone_run_bfile with memory > 0: stream the .bed blocks */
+                        /* of the -m plan from the mapped file every pass (data    */
+                        /* beyond HBM) instead of loading the whole matrix         */
 } pcaone_params;
 
 typedef struct pcaone_run_info {
@@ -143,6 +146,17 @@ int pcaone_bed_finalize(pcaone_ctx* ctx);
 /* Synthetic data generated on the device, byte-identical to
  * pcaone_synth_packed (reserve + upload + finalize). */
 int pcaone_bed_generate(pcaone_ctx* ctx, const pcaone_synth_spec* spec);
+/* Streamed genotypes, for matrices beyond HBM (the reference's out-of-core
+ * reads, src/FilePlink.cpp:122-298, src/Halko.cpp:129-152,225-268): the
+ * packed rows stay on the host and each product loads the block of at most
+ * block_snps SNPs it covers.  pcaone_stream_bed maps <prefix>.bed (magic
+ * checked as src/FilePlink.hpp:24-27); pcaone_stream_host reads the caller's
+ * rows (nsnps x ceil(N/4) bytes, kept alive and unchanged while the context
+ * uses them).  Opening streams every block once to count the codes.  Runs
+ * on streamed genotypes take the out-of-core schedule (memory > 0) with
+ * plan blocks of at most block_snps SNPs. */
+int pcaone_stream_bed(pcaone_ctx* ctx, const char* prefix, uint64_t block_snps, uint32_t* nsamples, uint64_t* nsnps);
+int pcaone_stream_host(pcaone_ctx* ctx, const uint8_t* rows, uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
 /* The resident genotypes back as .bed rows (nsnps x ceil(N/4) bytes). */
 int pcaone_bed_download(pcaone_ctx* ctx, uint8_t* rows);
 int pcaone_dims(const pcaone_ctx* ctx, uint32_t* nsamples, uint64_t* nsnps, uint64_t* nmissing);

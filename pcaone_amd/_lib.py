@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("waves_g", C.c_int),
                 ("waves_h", C.c_int), ("kchunk_g", C.c_uint64),
                 ("kchunk_h", C.c_uint64), ("memory", C.c_double), ("buffer", C.c_uint32), ("ld", C.c_int),
-                ("ld_stats", C.c_int), ("imaxiter", C.c_uint32), ("itol", C.c_double)]
+                ("ld_stats", C.c_int), ("imaxiter", C.c_uint32), ("itol", C.c_double), ("stream", C.c_int)]
 
 
 class RunInfo(C.Structure):
@@ -112,6 +112,8 @@ _SIGS = {
     "pcaone_host_chol_upper": (C.c_int, [C.c_int, _P, _P]),
     "pcaone_host_householder_q": (C.c_int, [C.c_uint64, C.c_int, _P]),
     "pcaone_residuals": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_int, C.c_uint64, C.c_uint64, _P]),
+    "pcaone_stream_bed": (C.c_int, [_P, C.c_char_p, C.c_uint64, _P, _P]),
+    "pcaone_stream_host": (C.c_int, [_P, _P, C.c_uint32, C.c_uint64, C.c_uint64]),
     "pcaone_packed_gemm_noalloc": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.c_uint32, _P]),
 }
 

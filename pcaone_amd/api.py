@@ -265,6 +265,20 @@ class Context:
                                            C.byref(info)))
         return {f: getattr(info, f) for f, _ in RunInfo._fields_}
 
+    def stream_bed(self, prefix: str, block_snps: int):
+        """Streamed genotypes from <prefix>.bed (mapped; blocks loaded per product)."""
+        n, m = C.c_uint32(), C.c_uint64()
+        self._check(lib().pcaone_stream_bed(self._h, str(prefix).encode(), block_snps, C.byref(n), C.byref(m)))
+        self._stream_rows = None
+        return n.value, m.value
+
+    def stream_host(self, packed: np.ndarray, nsamples: int, block_snps: int):
+        """Streamed genotypes from host rows (M x ceil(N/4) bytes); the array is
+        kept referenced by the context while it streams from it."""
+        rows = np.ascontiguousarray(packed, dtype=np.uint8)
+        self._check(lib().pcaone_stream_host(self._h, _ptr(rows), nsamples, rows.shape[0], block_snps))
+        self._stream_rows = rows
+
     def residuals(self, U, S, V, j0: int = 0, j1: int | None = None, as_read: bool = False) -> np.ndarray:
         """Rows [j0, j1) of the centered residual matrix (float32, SNP-major);
         U = None subtracts nothing."""

@@ -120,6 +120,7 @@ Param to_param(const pcaone_params* p) {
   q.ld_stats = p->ld_stats;
   q.imaxiter = p->imaxiter;
   q.itol = p->itol;
+  q.stream = p->stream != 0;
   require(q.k >= 1, kErrArgument, "k must be >= 1");
   require(q.ploidy == 1 || q.ploidy == 2, kErrArgument, "ploidy must be 1 or 2");
   q.derive();
@@ -234,6 +235,20 @@ int pcaone_bed_finalize(pcaone_ctx* c) {
 
 int pcaone_bed_generate(pcaone_ctx* c, const pcaone_synth_spec* spec) {
   return guard(c, [&](Context& x) { x.generate(to_spec(spec)); });
+}
+
+int pcaone_stream_bed(pcaone_ctx* c, const char* prefix, uint64_t block_snps, uint32_t* nsamples, uint64_t* nsnps) {
+  return guard(c, [&](Context& x) {
+    require(prefix != nullptr, kErrArgument, "null prefix");
+    const PlinkInfo pi = plink_dims(prefix);
+    x.stream_bed(std::string(prefix) + ".bed", (uint32_t)pi.nsamples, pi.nsnps_file, block_snps);
+    if (nsamples) *nsamples = x.N;
+    if (nsnps) *nsnps = x.M;
+  });
+}
+
+int pcaone_stream_host(pcaone_ctx* c, const uint8_t* rows, uint32_t nsamples, uint64_t nsnps, uint64_t block_snps) {
+  return guard(c, [&](Context& x) { x.stream_host(rows, nsamples, nsnps, block_snps); });
 }
 
 int pcaone_bed_download(pcaone_ctx* c, uint8_t* rows) {
@@ -498,7 +513,16 @@ int pcaone_run_bfile(pcaone_ctx* c, const char* bfile, const char* out, const pc
   return guard(c, [&](Context& x) {
     require(bfile && out, kErrArgument, "null path");
     const Param q = to_param(p);
-    const PlinkInfo pi = read_plink(x, bfile, q);
+    PlinkInfo pi;
+    if (q.stream) {
+      // the -m plan's blocks are read from the mapped .bed on every pass
+      require(q.out_of_core, kErrArgument, "stream needs memory > 0 (the -m plan sets the blocks)");
+      pi = plink_dims(bfile);
+      const BlockPlan bp = plan_blocks(q, pi.nsamples, pi.nsnps_file);
+      x.stream_bed(std::string(bfile) + ".bed", (uint32_t)pi.nsamples, pi.nsnps_file, bp.blocksize);
+    } else {
+      pi = read_plink(x, bfile, q);
+    }
     c->last = RunInfo{};
     x.ktimes = Context::KernelTimes{};
     const PcaResult r = run_pca(x, q, &c->last);

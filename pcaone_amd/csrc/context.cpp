@@ -11,7 +11,6 @@
 namespace pcaone {
 
 namespace {
-constexpr uint64_t kRowPad = 512;  // largest GEMM row tile (4 waves x 8 x 16 rows)
 constexpr double kVarTol = 1e-9;   // src/Data.hpp:7
 constexpr uint64_t kChunkCapH = 16384;  // largest split-K chunk of the H product (codes)
 constexpr uint64_t kL2Bytes = 2ull << 20;  // working set per chunk of gathered rows (half an XCD's L2)
@@ -30,6 +29,7 @@ Context::Context(int dev) : device(dev) {
 }
 
 Context::~Context() {
+  stream_close();
   if (stream) {
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
@@ -42,6 +42,7 @@ Context::~Context() {
 void Context::reserve(uint32_t nsamples, uint64_t nsnps) {
   require(nsamples > 0 && nsnps > 0, kErrArgument, "empty genotype matrix");
   require(nsnps < (1ull << 32), kErrArgument, "more than 2^32-1 SNPs");
+  stream_close();  // resident rows replace streamed ones
   N = nsamples;
   M = nsnps;
   bps_ = bytes_per_snp(N);
@@ -50,6 +51,7 @@ void Context::reserve(uint32_t nsamples, uint64_t nsnps) {
   pitch_s_ = ks_ / 16;
   pitch_t_ = kt_ / 16;
   rows_s_ = round_up(M, kRowPad) + kRowPad;
+  rows_state_ = rows_s_;
   rows_t_ = round_up(N, kRowPad);
   xs_.resize(rows_s_ * pitch_s_);
   launch_fill_rows(xs_.get(), pitch_s_, 0, rows_s_, stream);
@@ -143,6 +145,7 @@ void Context::generate(const synth::Spec& spec) {
 
 void Context::download_rows(uint8_t* out) {
   require(!F.empty(), kErrState, "no data");
+  require(!streamed(), kErrArgument, "download: the genotypes are streamed from the host");
   const uint64_t chunk = std::max<uint64_t>(1, (256ull << 20) / bps_);
   raw_.resize(std::min(chunk, M) * bps_);
   for (uint64_t j0 = 0; j0 < M; j0 += chunk) {
@@ -181,22 +184,31 @@ void Context::finalize() {
 }
 
 void Context::pristine_columns() {
-  // padded to rows_s_ (zeros) so window tiles may read past M
-  cF_.resize(rows_s_);
-  sc_.resize(rows_s_);
-  launch_pristine_cols(dF_.get(), M, rows_s_, cF_.get(), sc_.get(), stream);
+  // padded (zeros) so window tiles may read past M
+  cF_.resize(rows_state_);
+  sc_.resize(rows_state_);
+  launch_pristine_cols(dF_.get(), M, rows_state_, cF_.get(), sc_.get(), stream);
   weights_dirty_ = true;
   cols_pristine_ = true;
 }
 
 void Context::build_derived() {
-  weights_dirty_ = true;
+  require(!streamed(), kErrInternal, "build_derived on streamed genotypes");
   xt_.resize(rows_t_ * pitch_t_);
   launch_transpose_codes(xs_.get(), pitch_s_, M, N, xt_.get(), pitch_t_, rows_t_, stream);
+  build_lists(M, M);
+}
+
+void Context::build_lists(uint64_t rows, uint64_t ncodes_t) {
+  // CSR lists of the code-01 entries of the loaded layouts: by SNP row of xs
+  // (sample indices) and by sample row of xt (SNP indices local to xt)
+  weights_dirty_ = true;
+  lrows_ = rows;
+  lnmiss_ = 0;
   if (nmiss == 0) {
-    rp_s_.resize(M + 1);
+    rp_s_.resize(rows + 1);
     rp_t_.resize((uint64_t)N + 1);
-    PCAONE_HIP(hipMemsetAsync(rp_s_.get(), 0, (M + 1) * 8, stream));
+    PCAONE_HIP(hipMemsetAsync(rp_s_.get(), 0, (rows + 1) * 8, stream));
     PCAONE_HIP(hipMemsetAsync(rp_t_.get(), 0, ((uint64_t)N + 1) * 8, stream));
     PCAONE_HIP(hipStreamSynchronize(stream));
     return;
@@ -217,13 +229,15 @@ void Context::build_derived() {
     PCAONE_HIP(hipStreamSynchronize(stream));
     return p[nrows];
   };
-  const uint64_t ns = lists(xs_.get(), pitch_s_, M, N, rp_s_, ix_s_);
-  const uint64_t nt = lists(xt_.get(), pitch_t_, N, M, rp_t_, ix_t_);
-  require(ns == nmiss && nt == nmiss, kErrInternal, "missing-entry lists disagree with the code counts");
+  const uint64_t ns = lists(xs_.get(), pitch_s_, rows, N, rp_s_, ix_s_);
+  const uint64_t nt = lists(xt_.get(), pitch_t_, N, ncodes_t, rp_t_, ix_t_);
+  require(ns == nt && (streamed() || ns == nmiss), kErrInternal, "missing-entry lists disagree with the code counts");
+  lnmiss_ = ns;
 }
 
 void Context::filter_snps(const std::vector<uint32_t>& keep) {
   require(!keep.empty(), kErrArgument, "no SNPs left after filtering!");
+  require(!streamed(), kErrArgument, "--maf filters do not apply to streamed genotypes (the reference's -m)");
   for (uint32_t j : keep) require(j < M, kErrArgument, "filter index out of range");
   DevBuf<uint32_t> idx(keep.size());
   PCAONE_HIP(hipMemcpyAsync(idx.get(), keep.data(), keep.size() * 4, hipMemcpyHostToDevice, stream));
@@ -241,6 +255,7 @@ void Context::filter_snps(const std::vector<uint32_t>& keep) {
   kt_ = round_up(M, kKStep);
   pitch_t_ = kt_ / 16;
   rows_s_ = new_rows;
+  rows_state_ = new_rows;
   finalize();
   F = Fk;  // identical to the recount, kept explicit
 }
@@ -255,9 +270,11 @@ void Context::permute_columns(const std::vector<int>& p, bool with_freq) {
   }
   DevBuf<uint32_t> d(M);
   PCAONE_HIP(hipMemcpyAsync(d.get(), idx.data(), M * 4, hipMemcpyHostToDevice, stream));
-  tmpx_.resize(rows_s_ * pitch_s_);
-  launch_fill_rows(tmpx_.get(), pitch_s_, M, rows_s_, stream);
-  launch_gather_rows(xs_.get(), pitch_s_, d.get(), M, tmpx_.get(), stream);
+  if (!streamed()) {
+    tmpx_.resize(rows_s_ * pitch_s_);
+    launch_fill_rows(tmpx_.get(), pitch_s_, M, rows_s_, stream);
+    launch_gather_rows(xs_.get(), pitch_s_, d.get(), M, tmpx_.get(), stream);
+  }
   // the column state follows its columns (padding untouched)
   DevBuf<double> t(2 * M);
   launch_gather_f64(cF_.get(), d.get(), M, t.get(), stream);
@@ -269,7 +286,7 @@ void Context::permute_columns(const std::vector<int>& p, bool with_freq) {
     PCAONE_HIP(hipMemcpyAsync(dF_.get(), t.get(), M * 8, hipMemcpyDeviceToDevice, stream));
   }
   PCAONE_HIP(hipStreamSynchronize(stream));
-  std::swap(xs_, tmpx_);  // the old layout stays allocated for the next permutation (every winSVD run has two)
+  if (!streamed()) std::swap(xs_, tmpx_);  // the old layout stays allocated for the next permutation (every winSVD run has two)
   std::vector<int> np(M);
   perm_identity_ = true;
   for (uint64_t i = 0; i < M; ++i) {
@@ -288,7 +305,12 @@ void Context::permute_columns(const std::vector<int>& p, bool with_freq) {
     for (uint64_t i = 0; i < M; ++i) std::memcpy(&w[i * kp], &v[(uint64_t)idx[i] * kp], kp * 8);
     PCAONE_HIP(hipMemcpyAsync(vimp_.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, stream));
   }
-  build_derived();
+  if (streamed()) {
+    blk1_ = 0;  // the next product loads its block in the new order
+    PCAONE_HIP(hipStreamSynchronize(stream));
+  } else {
+    build_derived();
+  }
 }
 
 void Context::standardize(int ploidy) {
@@ -401,9 +423,9 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   PCAONE_HIP(hipMemsetAsync(wg_.get(), 0, wg_.bytes(), stream));
   wh_pitch_ = kt_ + kKStep;
   wh_.resize((uint64_t)form_pieces(form_h) * lp * wh_pitch_ * bytes_of(form_h));
-  gt_.resize(rows_s_ * lp);
-  tmp_m_.resize(rows_s_ * lp);
-  vbuf_.resize(rows_s_ * lp);
+  gt_.resize(rows_state_ * lp);
+  tmp_m_.resize(rows_state_ * lp);
+  vbuf_.resize(rows_state_ * lp);
   H_.resize(nrows_n * lp);
   H1_.resize(nrows_n * lp);
   H2_.resize(nrows_n * lp);
@@ -552,14 +574,16 @@ void Context::run_gemm(bool g_product, const GemmShape& shape, uint64_t nchunks,
 void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   require(L > 0, kErrState, "configure() first");
   require(j0 < j1 && j1 <= M, kErrArgument, "sketch: bad SNP range");
+  if (streamed()) load_block(j0, j1);
   const uint64_t nwin = j1 - j0;
+  const uint64_t jl = j0 - xoff();  // the window's first row in the resident layouts
   const uint64_t wb = bytes_of(form_h);
   const int P = form_pieces(form_h);
 
   // (1) G[j0:j1] = A Omega : SNP-major rows, K = samples
   const GemmPlan pg = plan_gemm(form_g, shape_g, fixed_g_, lp, nwin, ks_, tuning.kchunk_g, form_kcap(form_g, ks_));
   GemmArgs g{};
-  g.X = xs_.get() + j0 * pitch_s_;
+  g.X = xs_.get() + jl * pitch_s_;
   g.xpitch = pitch_s_;
   g.rows = pg.rows;
   g.Wt = wg_.get();
@@ -571,17 +595,17 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   g.k_end = ks_;
   g.kchunk = pg.kchunk;
   g.lp = lp;
-  require(j0 + g.rows <= rows_s_, kErrInternal, "G window exceeds padded rows");
+  require(jl + g.rows <= rows_s_, kErrInternal, "G window exceeds padded rows");
   run_gemm(true, pg.shape, pg.nchunks, g);
   const uint64_t nch_g = pg.nchunks;
   double* gwin = gt_.get() + j0 * lp;
-  // The window's operand pieces of diag(sc) G~ sit at K offset j0 % 128 of wh_;
-  // the K margins around them must read as zero.
-  const uint64_t off = j0 % kKStep;
+  // The window's operand pieces of diag(sc) G~ sit at K offset jl % kKStep of
+  // wh_ (the window's K offset in xt); the K margins around them read as zero.
+  const uint64_t off = jl % kKStep;
   const uint64_t klen = round_up(off + nwin, kKStep);
   launch_zero_ranges(wh_.get(), wh_pitch_ * wb, (uint64_t)P * lp, 0, off * wb, (off + nwin) * wb,
                      (klen - off - nwin) * wb, stream);
-  if (!nmiss && form_g != Form::I8 && form_h != Form::I8) {
+  if (!lnmiss_ && form_g != Form::I8 && form_h != Form::I8) {
     launch_epi_g_fused(form_h, part_.get(), nch_g, g.ychunk, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                        colsum_.get(), gwin, wh_.get() + off * wb, wh_pitch_, (uint64_t)lp * wh_pitch_, red_.get(),
                        bvec_.get(), stream);
@@ -589,11 +613,11 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     launch_epi_g(form_g, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                  colsum_.get(), gwin, stream, i8_mul(0));
   }
-  if (nmiss) {
+  if (lnmiss_) {
     update_missing_weights();
     launch_compact_rows(omega_r_.get(), N, lp, L, cpitch_, miss_f32_, csg_.get(), stream);
     MissSpmmArgs m{};
-    m.rp = rp_s_.get() + j0;
+    m.rp = rp_s_.get() + jl;
     m.ix = ix_s_.get();
     m.w = w_s_.get();
     m.nrows = nwin;
@@ -612,7 +636,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     m.lp = lp;
     launch_missing_spmm(m, stream);
   }
-  if (nmiss || form_g == Form::I8 || form_h == Form::I8) {
+  if (lnmiss_ || form_g == Form::I8 || form_h == Form::I8) {
     if (form_h == Form::I8)
       launch_i8_scales(gwin, nwin, lp, sc_.get() + j0, i8max_.get(), i8_scale(1), i8_mul(1), stream);
     launch_make_pieces(form_h, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
@@ -628,13 +652,14 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
   // diag(sc) G~ for these SNPs in wh_ at K offset j0 % kKStep and the b
   // terms in bvec_.
   const uint64_t nwin = j1 - j0;
-  const uint64_t off = j0 % kKStep;
+  const uint64_t jl = j0 - xoff();
+  const uint64_t off = jl % kKStep;
   const uint64_t klen = round_up(off + nwin, kKStep);
   double* gwin = gt_.get() + j0 * lp;
   // sample-major rows, K = the window's SNPs
   const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h, form_kcap(form_h, kChunkCapH));
   GemmArgs h{};
-  h.X = xt_.get() + (j0 - off) / 16;
+  h.X = xt_.get() + (jl - off) / 16;
   h.xpitch = pitch_t_;
   h.rows = ph.rows;
   h.Wt = wh_.get();
@@ -649,21 +674,22 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
   require(h.rows <= rows_t_, kErrInternal, "H rows exceed padding");
   run_gemm(false, ph.shape, ph.nchunks, h);
   const uint64_t nch_h = ph.nchunks;
-  double* target = hmode == 0 ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
+  // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ..., 3: H += ... (blocks of one pass)
+  double* target = (hmode == 0 || hmode == 3) ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
   launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream,
                i8_mul(1));
-  if (nmiss) {
+  if (lnmiss_) {
     launch_compact_rows(gwin, nwin, lp, L, cpitch_, miss_f32_, csh_.get(), stream);
     MissSpmmArgs m{};
     m.rp = rp_t_.get();
     m.ix = ix_t_.get();
     m.w = w_t_.get();
     m.nrows = N;
-    m.k_lo = j0;
-    m.k_hi = j1;
+    m.k_lo = jl;
+    m.k_hi = jl + nwin;
     m.src = csh_.get();
     m.f32 = miss_f32_;
-    m.src_k0 = j0;
+    m.src_k0 = jl;
     m.cp = cp_;
     m.cpitch = cpitch_;
     m.ncols = L;
@@ -674,13 +700,14 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
     m.lp = lp;
     launch_missing_spmm(m, stream);
   }
-  if (hmode == 0) ++h_version_;
+  if (hmode == 0 || hmode == 3) ++h_version_;
 }
 
 void Context::h_from_g(int hmode) {
   // H = A^T G~ over all SNPs for a G~ the caller wrote into gt_ (rows 0..M)
   require(L > 0, kErrState, "configure() first");
   require(form_h != Form::I8, kErrArgument, "h_from_g: not in the int8 form");
+  require(!streamed(), kErrArgument, "h_from_g: not with streamed genotypes");
   const uint64_t wb = bytes_of(form_h);
   const int P = form_pieces(form_h);
   const uint64_t klen = round_up(M, kKStep);
@@ -688,7 +715,7 @@ void Context::h_from_g(int hmode) {
   launch_make_pieces(form_h, gt_.get(), M, M, lp, sc_.get(), nullptr, wh_.get(), wh_pitch_, (uint64_t)lp * wh_pitch_,
                      gt_.get(), nullptr, stream, nullptr);
   launch_colsum(gt_.get(), 0, M, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
-  if (nmiss) update_missing_weights();
+  if (lnmiss_) update_missing_weights();
   h_half(0, M, hmode);
 }
 
@@ -702,15 +729,17 @@ uint64_t Context::miss_span(uint64_t nrows, uint64_t kext) const {
 }
 
 void Context::update_missing_weights() {
-  if (!nmiss || !weights_dirty_) return;
-  w_s_.resize(nmiss);
-  w_t_.resize(nmiss);
+  if (!lnmiss_ || !weights_dirty_) return;
+  w_s_.resize(lnmiss_);
+  w_t_.resize(lnmiss_);
+  // the lists index SNPs from the first loaded row: the per-SNP state too
+  const uint64_t xo = xoff();
   MissWeightArgs a{};
-  a.sc = sc_.get();
-  a.cf = cF_.get();
-  a.Fclamp = emu_k_ ? dF_.get() : nullptr;
+  a.sc = sc_.get() + xo;
+  a.cf = cF_.get() + xo;
+  a.Fclamp = emu_k_ ? dF_.get() + xo : nullptr;
   a.US = us_.get();
-  a.V = vimp_.get();
+  a.V = emu_k_ ? vimp_.get() + xo * emu_kp_ : nullptr;
   a.kk = emu_k_;
   a.kp = emu_kp_;
   if (emu_k_) launch_pack_col_state(vimp_.get(), M, emu_kp_, emu_k_, sc_.get(), cF_.get(), dF_.get(), stream);
@@ -722,7 +751,7 @@ void Context::update_missing_weights() {
   };
   a.rp = rp_s_.get();
   a.ix = ix_s_.get();
-  a.nrows = M;
+  a.nrows = lrows_;
   a.by_snp = true;
   a.w = w_s_.get();
   a.kext = N;
@@ -733,15 +762,16 @@ void Context::update_missing_weights() {
   a.nrows = N;
   a.by_snp = false;
   a.w = w_t_.get();
-  a.kext = M;
-  a.kspan = span(M);
+  a.kext = lrows_;
+  a.kspan = span(lrows_);
   launch_missing_weights(a, stream);
   weights_dirty_ = false;
 }
 
 void Context::zero_h(int which) {
-  DevBuf<double>& b = which == 1 ? H1_ : H2_;
+  DevBuf<double>& b = which == 0 ? H_ : (which == 1 ? H1_ : H2_);
   PCAONE_HIP(hipMemsetAsync(b.get(), 0, b.bytes(), stream));
+  if (which == 0) ++h_version_;
 }
 
 void Context::omega_from(int src, bool flip) {
@@ -1081,6 +1111,7 @@ void Context::download_h(int which, double* H) {
 
 void Context::residuals_prepare(int k, const double* U, const double* S, const double* V, bool as_read) {
   require(N > 0 && M > 0, kErrState, "no data");
+  require(!streamed(), kErrArgument, "residuals need the genotypes resident (not streamed)");
   require(k >= 0 && k <= kResidMaxK, kErrArgument, "residuals: rank must be in [0, " + std::to_string(kResidMaxK) + "]");
   require(k == 0 || (U && S && V), kErrArgument, "residuals: null U, S or V");
   std::vector<double> ut((uint64_t)std::max(k, 1) * N, 0.0), sv(M * (uint64_t)std::max(k, 1), 0.0);

@@ -48,6 +48,8 @@ struct EpochResult {
   double diff = 0.0;          // 1 - mev(Ucur, Upre) (valid if have_prev)
 };
 
+constexpr uint64_t kRowPad = 512;  // largest GEMM row tile (4 waves x 8 x 16 rows)
+
 class Context {
  public:
   explicit Context(int device);
@@ -61,6 +63,18 @@ class Context {
 
   // ---------------------------------------------------------------- data
   void reserve(uint32_t nsamples, uint64_t nsnps);
+  // Streamed genotypes, for matrices beyond HBM (the reference's -m reads,
+  // src/FilePlink.cpp:122-298): the packed .bed rows stay in host memory, a
+  // caller's buffer (kept alive and unchanged while the context uses it) or
+  // the mapped payload of a .bed file, and each product loads the block of
+  // at most block_snps SNP rows it covers into block-sized device layouts.
+  // Opening streams every block once to count the codes (F, missing entries).
+  void stream_host(const uint8_t* rows, uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
+  void stream_bed(const std::string& bed, uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
+  bool streamed() const { return src_rows_ != nullptr; }
+  uint64_t stream_block_snps() const { return sblock_; }
+  uint64_t stream_loads = 0;   // blocks moved host -> device
+  double stream_load_ms = 0.0; // host time spent loading them
   void upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n);  // raw .bed rows (bps bytes each)
   void generate(const synth::Spec& spec);                          // synthetic data straight into HBM
   void download_rows(uint8_t* out);  // current column order, PLINK row format
@@ -108,7 +122,7 @@ class Context {
   double* omega_staging();
   void set_omega_staged();
   // G rows [j0, j1) = A[j0:j1] Omega and the H contribution of those SNPs:
-  // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ...
+  // hmode 0: H = ..., 1: H1 += ..., 2: H2 += ..., 3: H += ...
   void sketch(uint64_t j0, uint64_t j1, int hmode);
   // H (hmode 0; H1 1, H2 2) = A^T G~ over all SNPs for the G~ (M x lp,
   // row-major) the caller wrote into gt() (the FP64, FP32 and BF16 forms).
@@ -128,7 +142,7 @@ class Context {
   // unless !flip (the out-of-core --svd 1 branch has none);
   // src 0: H, 1: H1 + H2 (also stored as the current H).
   void omega_from(int src, bool flip = true);
-  void zero_h(int which);  // 1: H1, 2: H2
+  void zero_h(int which);  // 0: H, 1: H1, 2: H2
   // QR of G twice, B = R^-T H^T, eigen-decomposition of B B^T, Ucur; mev
   // against the previous Ucur if have_prev.  finish_epoch = finish_begin +
   // finish_end; between them a caller may enqueue the next epoch's device work
@@ -174,6 +188,20 @@ class Context {
   uint64_t householder_fallbacks = 0, gram_second_passes = 0;
 
  private:
+  // streamed genotypes
+  const uint8_t* src_rows_ = nullptr;  // packed rows in file order (host)
+  void* map_ = nullptr;                // mapped .bed file (stream_bed)
+  size_t map_len_ = 0;
+  uint64_t sblock_ = 0;                // device block capacity in SNP rows
+  uint64_t blk0_ = 0, blk1_ = 0;       // loaded rows [blk0_, blk1_) of the current column order; blk1_ == 0: none
+  uint64_t rows_state_ = 0;            // rows of the per-SNP device state (M padded)
+  uint64_t lnmiss_ = 0, lrows_ = 0;    // missing entries / SNP rows of the current missing lists
+  void stream_setup(uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
+  void stream_close();
+  void load_block(uint64_t j0, uint64_t j1);
+  void block_rows_to_device(uint64_t j0, uint64_t n);  // rows of the current order -> raw_ (device)
+  void build_lists(uint64_t rows, uint64_t ncodes_t);  // missing lists of the loaded layouts
+  uint64_t xoff() const { return src_rows_ ? blk0_ : 0; }
   // data layout
   uint64_t bps_ = 0;
   uint64_t ks_ = 0, kt_ = 0;           // K extents (codes) of xs and xt

@@ -87,6 +87,7 @@ PcaResult run_pca_full(Context& ctx, const Param& params, RunInfo* info) {
   ctx.reset_columns();
   const uint64_t N = ctx.N, M = ctx.M;
   require(N > 0 && M > 0, kErrState, "no data");
+  require(!ctx.streamed(), kErrArgument, "--svd 0 and --svd 3 need the genotypes resident (not streamed)");
   ctx.configure(PCAONE_PREC_FP64_ID, kGramBlock, {});
   ctx.standardize(params.ploidy);  // src/Main.cpp:181-182
   const bool wide = N <= M;
@@ -170,6 +171,7 @@ PcaResult run_pca_arnoldi(Context& ctx, const Param& params, RunInfo* info) {
   ctx.reset_columns();
   const uint64_t N = ctx.N, M = ctx.M;
   require(N > 0 && M > 0, kErrState, "no data");
+  require(!ctx.streamed(), kErrArgument, "--svd 0 and --svd 3 need the genotypes resident (not streamed)");
   require(!params.emu, kErrArgument, "--svd 0 with --emu does not run on this path");
   const int nev = (int)params.k;
   // PartialSVDSolver (external/Spectra/contrib/PartialSVDSolver.h): the

@@ -205,7 +205,19 @@ void NormalRsvdOpData::computeGandH(int pi) {
     if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
   }
   if (pi > 0) ctx_.omega_from(0, /*flip=*/plan == nullptr);
-  ctx_.sketch(0, rows(), 0);
+  products();
+}
+
+void NormalRsvdOpData::products() {
+  // G = A Omega and H = A^T G over all SNPs: one product on resident
+  // genotypes; on streamed ones block by block, H summed over the blocks
+  // (src/Halko.cpp:137-152)
+  if (!ctx_.streamed()) {
+    ctx_.sketch(0, rows(), 0);
+    return;
+  }
+  ctx_.zero_h(0);
+  for (uint64_t b = 0; b < plan->nblocks; ++b) ctx_.sketch(plan->start[b], plan->stop[b] + 1, 3);
 }
 
 bool NormalRsvdOpData::enqueue_next(int pi) {
@@ -214,7 +226,7 @@ bool NormalRsvdOpData::enqueue_next(int pi) {
   if (pi == 0 || params_.missme || device_uv || !ctx_.overlap_ok()) return false;
   ctx_.omega_from(0, /*flip=*/plan == nullptr);
   ctx_.hold();
-  ctx_.sketch(0, rows(), 0);
+  products();
   return true;
 }
 
@@ -306,11 +318,16 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, d
   // src/Halko.cpp:271-345 (genetic input, in-core, no LD output)
   ctx.reset_columns();  // restores the as-read state between runs; not part of the timed work
   BlockPlan plan;
+  require(!ctx.streamed() || params.out_of_core, kErrArgument,
+          "streamed genotypes run the out-of-core schedule: set memory (-m)");
   if (params.out_of_core) {
     // Data::prepare's plan (src/Data.cpp:42-84); with the shuffle, the file
     // is first rewritten in permute_plink order (src/Main.cpp:120-151) and
     // read from there, so allele frequencies follow their SNPs
     plan = plan_blocks(params, ctx.N, ctx.M);
+    require(!ctx.streamed() || plan.blocksize <= ctx.stream_block_snps(), kErrArgument,
+            "stream: the -m plan's blocks of " + std::to_string(plan.blocksize) +
+                " SNPs exceed the streamed block capacity of " + std::to_string(ctx.stream_block_snps()));
     if (params.perm)
       ctx.permute_columns(permute_plink_order(ctx.N, ctx.M, params.buffer, params.bands), /*with_freq=*/true);
   }

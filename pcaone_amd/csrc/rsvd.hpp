@@ -28,6 +28,7 @@ struct Param {
   int ld_stats = 0;     // --ld-stats (src/Cmd.cpp:114)
   uint32_t imaxiter = 1000;  // --imaxiter (IRAM)
   double itol = 1e-6;        // --itol (IRAM)
+  bool stream = false;       // run_bfile: stream the .bed blocks instead of loading the matrix
   uint32_t ncv = 20;         // derived: max(20, 2k + 1) (src/Cmd.cpp:215)
   // derived (src/Cmd.cpp:171-238)
   bool perm = false, missme = false, filterSNP = false, out_of_core = false;
@@ -102,6 +103,9 @@ class NormalRsvdOpData : public RsvdOpData {
   void computeGandH(int pi) override;
   bool enqueue_next(int pi) override;
   const BlockPlan* plan = nullptr;  // out-of-core block plan (null: in-core)
+
+ private:
+  void products();  // G and H over all SNPs (block by block on streamed genotypes)
 };
 
 class FancyRsvdOpData : public RsvdOpData {
