@@ -25,6 +25,9 @@ Context::Context(int dev) : device(dev) {
   if (e != hipSuccess || n <= 0) fail(kErrNoDevice, "no HIP device available");
   require(dev >= 0 && dev < n, kErrArgument, "device index out of range");
   PCAONE_HIP(hipSetDevice(dev));
+  int ncu = 0;
+  PCAONE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  cus_ = ncu > 0 ? (uint64_t)ncu : 0;
   PCAONE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 }
 
@@ -581,7 +584,8 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   const int P = form_pieces(form_h);
 
   // (1) G[j0:j1] = A Omega : SNP-major rows, K = samples
-  const GemmPlan pg = plan_gemm(form_g, shape_g, fixed_g_, lp, nwin, ks_, tuning.kchunk_g, form_kcap(form_g, ks_));
+  const GemmPlan pg = plan_gemm(form_g, shape_g, fixed_g_, lp, nwin, ks_, tuning.kchunk_g, form_kcap(form_g, ks_),
+                                 cus_);
   GemmArgs g{};
   g.X = xs_.get() + jl * pitch_s_;
   g.xpitch = pitch_s_;
@@ -657,7 +661,8 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
   const uint64_t klen = round_up(off + nwin, kKStep);
   double* gwin = gt_.get() + j0 * lp;
   // sample-major rows, K = the window's SNPs
-  const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h, form_kcap(form_h, kChunkCapH));
+  const GemmPlan ph = plan_gemm(form_h, shape_h, fixed_h_, lp, N, klen, tuning.kchunk_h,
+                                 form_kcap(form_h, kChunkCapH), cus_);
   GemmArgs h{};
   h.X = xt_.get() + (jl - off) / 16;
   h.xpitch = pitch_t_;

@@ -192,6 +192,7 @@ class Context {
   const uint8_t* src_rows_ = nullptr;  // packed rows in file order (host)
   void* map_ = nullptr;                // mapped .bed file (stream_bed)
   size_t map_len_ = 0;
+  uint64_t cus_ = 0;                   // compute units of the device (split-K planning)
   uint64_t sblock_ = 0;                // device block capacity in SNP rows
   uint64_t blk0_ = 0, blk1_ = 0;       // loaded rows [blk0_, blk1_) of the current column order; blk1_ == 0: none
   uint64_t rows_state_ = 0;            // rows of the per-SNP device state (M padded)

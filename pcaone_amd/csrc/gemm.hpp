@@ -114,6 +114,8 @@ void launch_grouped_gemm(Form f, const GemmShape& g, int lp, const GroupedGemmAr
                          hipStream_t s);
 // Workgroups of the grouped kernel of this shape resident on one CU.
 int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp);
+// The same for the single-product kernel (cached per form, shape and lp).
+int gemm_blocks_per_cu(Form f, const GemmShape& g, int lp);
 
 // Launch plan of one product: shape, padded rows and split-K chunk.
 struct GemmPlan {
@@ -127,9 +129,11 @@ struct GemmPlan {
 // under-filled and `fixed_shape` is false, a smaller row tile) for a product
 // of `nrows` rows over `kext` codes: at least 2,048 workgroups where the
 // matrix allows it, at least 4 K-steps per workgroup, chunks of at most
-// `kcap` codes.  kchunk_user > 0 fixes the chunk.
+// `kcap` codes.  kchunk_user > 0 fixes the chunk.  With the device's CU count
+// `cus` (0 = not known) a product of at most 4 Mi rows takes up to two more
+// chunks when they fill its last round of resident workgroups.
 GemmPlan plan_gemm(Form f, const GemmShape& preferred, bool fixed_shape, int lp, uint64_t nrows, uint64_t kext,
-                   uint64_t kchunk_user, uint64_t kcap);
+                   uint64_t kchunk_user, uint64_t kcap, uint64_t cus = 0);
 // Upper bound of nchunks * rows over every plan_gemm result for these
 // arguments' extremes (partial-product buffer sizing).
 uint64_t plan_partial_rows(uint64_t nrows_max, uint64_t kext_max, uint64_t kchunk_user, uint64_t kcap);

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "device.hpp"
@@ -133,9 +135,16 @@ using accv_t = typename std::conditional<F == Form::FP64, f64x4_t,
 template <Form F>
 using aword_t = typename std::conditional<FT<F>::KS == 256, u32x4_t, u32x2_t>::type;
 
-// Position of the 16-byte vector v of LDS W-tile row `row`.  Sixteen
-// consecutive rows read at one v land in sixteen different bank groups.
-__device__ __forceinline__ int swz(int row, int v) { return v ^ (row & 15); }
+// Position of the 16-byte vector v of LDS W-tile row `row` (an XOR within
+// each aligned group of 8 or 16 vectors).  For the 2-byte forms the B reads
+// (16 rows at vectors 8q + s, rows 512 bytes apart) are free of bank
+// conflicts with the 8-row period and take two passes per read with the
+// 16-row one (measured with SQ_LDS_BANK_CONFLICT on MI355X); the other forms
+// keep the 16-row period.
+template <Form F>
+__device__ __forceinline__ int swz(int row, int v) {
+  return v ^ (row & (sizeof(typename FT<F>::w_t) == 2 ? 7 : 15));
+}
 
 // All MFMAs of one K-step for one wave.
 template <Form F, int WM, int NC>
@@ -157,7 +166,7 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
 #pragma unroll
         for (int p = 0; p < T::P; ++p)
           bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
-                                                                 swz(r16, (T::KS / 64) * q + s) * 16);
+                                                                 swz<F>(r16, (T::KS / 64) * q + s) * 16);
       __builtin_amdgcn_sched_barrier(0);
       i32x4_t af[WM];
 #pragma unroll
@@ -172,20 +181,61 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
                 af[rb], __builtin_bit_cast(i32x4_t, bv[cb * T::P + p]), acc[p][rb][cb], 0, 0, 0);
     }
   } else if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
-    // The B fragments of substep s + 1 are read from LDS while the MFMAs of
-    // substep s run (two register sets), so the LDS latency stays hidden.
-    // Tiles with more than 10 fragments per substep read them just before
-    // their MFMAs instead: a second set would not fit in the registers.
+    // Fragment j of substep s + 1 is read from LDS into the registers of
+    // fragment j of substep s right after that fragment's last MFMA, and the A
+    // fragments of substep s + 1 are decoded behind substep s's MFMAs (two A
+    // sets, one B set), so LDS and decode latency stay hidden.  Wave tiles of
+    // 8 row blocks keep one A set and two B sets instead (their second A set
+    // would spill), and tiles with more than 10 fragments per substep read
+    // them just before their MFMAs.
     constexpr int NS = T::KS / 32;
     constexpr bool PF = NC * T::P <= 10;
-    if constexpr (PF) {
+    if constexpr (PF && WM <= 4) {
+      auto read1 = [&](int s, int cb, int p) {
+        return *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
+                                                 swz<F>(r16, (T::KS / 32) * q + s) * 8);
+      };
+      auto decode = [&](int s, bf16x8_t(&dst)[WM]) {
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb) dst[rb] = decode_bf16x8(aw[rb][s >> 1], s & 1);
+      };
+      u32x4_t bv[NC * T::P];
+      bf16x8_t af[2][WM];
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+        for (int p = 0; p < T::P; ++p) bv[cb * T::P + p] = read1(0, cb, p);
+      decode(0, af[0]);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+          for (int p = 0; p < T::P; ++p) {
+#pragma unroll
+            for (int rb = 0; rb < WM; ++rb) {
+              if constexpr (F == Form::FP16)  // +-2 and 0 have the same top byte in FP16 as in BF16
+                acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                    __builtin_bit_cast(f16x8_t, af[s & 1][rb]), __builtin_bit_cast(f16x8_t, bv[cb * T::P + p]),
+                    acc[0][rb][cb], 0, 0, 0);
+              else
+                acc[0][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    af[s & 1][rb], __builtin_bit_cast(bf16x8_t, bv[cb * T::P + p]), acc[0][rb][cb], 0, 0, 0);
+            }
+            if (s + 1 < NS) bv[cb * T::P + p] = read1(s + 1, cb, p);
+          }
+        if (s + 1 < NS) decode(s + 1, af[(s + 1) & 1]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    } else if constexpr (PF) {
       auto read_b = [&](int s, u32x4_t(&dst)[NC * T::P]) {
 #pragma unroll
         for (int cb = 0; cb < NC; ++cb)
 #pragma unroll
           for (int p = 0; p < T::P; ++p)
             dst[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
-                                                                    swz(r16, (T::KS / 32) * q + s) * 8);
+                                                                    swz<F>(r16, (T::KS / 32) * q + s) * 8);
       };
       auto mfma_substep = [&](int s, const u32x4_t(&bv)[NC * T::P]) {
         bf16x8_t af[WM];
@@ -227,7 +277,7 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
 #pragma unroll
           for (int p = 0; p < T::P; ++p)
             bv[cb * T::P + p] = *reinterpret_cast<const u32x4_t*>(wl + (p * LP + cb * 16 + r16) * LDSP +
-                                                                   swz(r16, (T::KS / 32) * q + s) * 8);
+                                                                   swz<F>(r16, (T::KS / 32) * q + s) * 8);
         __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
         bf16x8_t af[WM];
 #pragma unroll
@@ -271,11 +321,11 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
       for (int cb = 0; cb < NC; ++cb) {
         const w_t* row = wl + (cb * 16 + r16) * LDSP;
         if constexpr (F == Form::FP32) {
-          const float4 b4 = *reinterpret_cast<const float4*>(row + swz(r16, (T::KS / 16) * q + u) * 4);
+          const float4 b4 = *reinterpret_cast<const float4*>(row + swz<F>(r16, (T::KS / 16) * q + u) * 4);
           bw[cb][0] = b4.x, bw[cb][1] = b4.y, bw[cb][2] = b4.z, bw[cb][3] = b4.w;
         } else {
-          const double2 b0 = *reinterpret_cast<const double2*>(row + swz(r16, (T::KS / 8) * q + 2 * u) * 2);
-          const double2 b1 = *reinterpret_cast<const double2*>(row + swz(r16, (T::KS / 8) * q + 2 * u + 1) * 2);
+          const double2 b0 = *reinterpret_cast<const double2*>(row + swz<F>(r16, (T::KS / 8) * q + 2 * u) * 2);
+          const double2 b1 = *reinterpret_cast<const double2*>(row + swz<F>(r16, (T::KS / 8) * q + 2 * u + 1) * 2);
           bw[cb][0] = b0.x, bw[cb][1] = b0.y, bw[cb][2] = b1.x, bw[cb][3] = b1.y;
         }
       }
@@ -354,7 +404,7 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
       const int wrow = wave * (64 / VPR) + ROWS_PER_I * i;  // first row of this wave's slots (uniform)
       if (EVEN || wrow < T::P * LP) {
         const int row = vrow0 + ROWS_PER_I * i, piece = row / LP, col = row - piece * LP;
-        const w_t* src = wg + piece * wpiece + col * wpitch + swz(row, vpart) * VE + kk;
+        const w_t* src = wg + piece * wpiece + col * wpitch + swz<F>(row, vpart) * VE + kk;
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                          (__attribute__((address_space(3))) void*)(dst + wrow * LDSP), 16, 0, 0);
       }
@@ -380,7 +430,7 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
       for (int i = 0; i < NVT; ++i)
         if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
           *reinterpret_cast<u32x4_t*>(wl0 + (vrow0 + ROWS_PER_I * i) * LDSP +
-                                      swz(vrow0 + ROWS_PER_I * i, vpart) * VE) = wreg[i];
+                                      swz<F>(vrow0 + ROWS_PER_I * i, vpart) * VE) = wreg[i];
     }
     // Nothing may stay pending into the loop: the wait inserted for a load
     // still in flight at the loop head would also cover the loop's own
@@ -416,7 +466,7 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
           for (int i = 0; i < NVT; ++i)
             if (EVEN || vrow0 + ROWS_PER_I * i < T::P * LP)
               *reinterpret_cast<u32x4_t*>(nxt + (vrow0 + ROWS_PER_I * i) * LDSP +
-                                          swz(vrow0 + ROWS_PER_I * i, vpart) * VE) = wreg[i];
+                                          swz<F>(vrow0 + ROWS_PER_I * i, vpart) * VE) = wreg[i];
         }
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) acur[rb] = anext[rb];
@@ -495,14 +545,20 @@ void launch_grouped_t(const GroupedGemmArgs& a, unsigned grid, hipStream_t s, in
   check_launch("grouped_gemm");
 }
 
+// Launches the kernel, or (occ != null) reports how many of its workgroups
+// fit on one CU.
 template <Form F, int WM, int NC, int WAVES>
-void launch_t(const GemmArgs& a, hipStream_t s) {
+void launch_t(const GemmArgs& a, hipStream_t s, int* occ) {
   constexpr size_t lds = lds_bytes<F, NC>();
+  const void* fn = reinterpret_cast<const void*>(&k_packed_gemm<F, WM, NC, WAVES>);
   static bool attr = false;
   if (!attr) {
-    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_packed_gemm<F, WM, NC, WAVES>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    PCAONE_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr = true;
+  }
+  if (occ) {
+    PCAONE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, fn, 64 * WAVES, lds));
+    return;
   }
   const uint64_t tile = (uint64_t)WAVES * WM * 16;
   const uint64_t nchunks = (a.k_end - a.k_begin + a.kchunk - 1) / a.kchunk;
@@ -536,9 +592,9 @@ bool with_nc(int lp, Fn&& fn) {
 }
 
 template <Form F, int WM, int WAVES>
-void launch_nc(const GemmArgs& a, hipStream_t s) {
+void launch_nc(const GemmArgs& a, hipStream_t s, int* occ) {
   if constexpr (shape_ok<F>(WM, WAVES)) {
-    if (with_nc<F, WAVES>(a.lp, [&]<int NC>() { launch_t<F, WM, NC, WAVES>(a, s); })) return;
+    if (with_nc<F, WAVES>(a.lp, [&]<int NC>() { launch_t<F, WM, NC, WAVES>(a, s, occ); })) return;
   }
   fail(kErrArgument, "packed_gemm: unsupported shape (wm " + std::to_string(WM) + ", waves " +
                          std::to_string(WAVES) + ", lp " + std::to_string(a.lp) + ")");
@@ -569,15 +625,15 @@ void launch_grouped_shape(const GemmShape& g, int lp, const GroupedGemmArgs& a, 
 }
 
 template <Form F>
-void launch_shape(const GemmShape& g, const GemmArgs& a, hipStream_t s) {
+void launch_shape(const GemmShape& g, const GemmArgs& a, hipStream_t s, int* occ = nullptr) {
   if (g.waves == 8) {
-    if (g.wm == 2) return launch_nc<F, 2, 8>(a, s);
-    if (g.wm == 4) return launch_nc<F, 4, 8>(a, s);
+    if (g.wm == 2) return launch_nc<F, 2, 8>(a, s, occ);
+    if (g.wm == 4) return launch_nc<F, 4, 8>(a, s, occ);
   } else if (g.waves == 4) {
-    if (g.wm == 1) return launch_nc<F, 1, 4>(a, s);
-    if (g.wm == 2) return launch_nc<F, 2, 4>(a, s);
-    if (g.wm == 4) return launch_nc<F, 4, 4>(a, s);
-    if (g.wm == 8) return launch_nc<F, 8, 4>(a, s);
+    if (g.wm == 1) return launch_nc<F, 1, 4>(a, s, occ);
+    if (g.wm == 2) return launch_nc<F, 2, 4>(a, s, occ);
+    if (g.wm == 4) return launch_nc<F, 4, 4>(a, s, occ);
+    if (g.wm == 8) return launch_nc<F, 8, 4>(a, s, occ);
   }
   fail(kErrArgument, "packed_gemm: unsupported wave tile " + std::to_string(g.wm) + " x " + std::to_string(g.waves));
 }
@@ -637,22 +693,75 @@ int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
   return occ > 0 ? occ : 1;
 }
 
+int gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
+  require(gemm_shape_supported(f, g, lp), kErrArgument, "packed_gemm: unsupported shape");
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  const int key = ((int)f * 16 + g.wm) * 16 * 16 + g.waves * 16 + lp / 16;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int occ = 0;
+  GemmArgs none{};
+  none.lp = lp;
+  switch (f) {
+    case Form::FP64: launch_shape<Form::FP64>(g, none, nullptr, &occ); break;
+    case Form::FP32: launch_shape<Form::FP32>(g, none, nullptr, &occ); break;
+    case Form::BF16: launch_shape<Form::BF16>(g, none, nullptr, &occ); break;
+    case Form::BF16X1: launch_shape<Form::BF16X1>(g, none, nullptr, &occ); break;
+    case Form::FP16: launch_shape<Form::FP16>(g, none, nullptr, &occ); break;
+    case Form::I8: launch_shape<Form::I8>(g, none, nullptr, &occ); break;
+  }
+  occ = occ > 0 ? occ : 1;
+  cache[key] = occ;
+  return occ;
+}
+
 namespace {
 constexpr uint64_t kTargetWorkgroups = 2048;  // 8 per CU
 constexpr uint64_t kMinStepsPerChunk = 4;
+// Products of at most this many rows may take up to two more split-K chunks
+// than the rules below ask for, when that fills the last workgroup round.
+constexpr uint64_t kQuantMaxRows = 1ull << 22;
+constexpr uint64_t kQuantExtraChunks = 2;
 
-GemmPlan plan_for(const GemmShape& sh, uint64_t nrows, uint64_t kext, uint64_t kchunk_user, uint64_t kcap) {
+// Share of the launch's workgroup rounds that is filled: w workgroups run in
+// ceil(w / slots) rounds of `slots` resident workgroups.
+double round_fill(uint64_t w, uint64_t slots) {
+  const uint64_t r = (w + slots - 1) / slots;
+  return (double)w / (double)(r * slots);
+}
+
+GemmPlan plan_for(const GemmShape& sh, uint64_t nrows, uint64_t kext, uint64_t kchunk_user, uint64_t kcap,
+                  uint64_t slots) {
   GemmPlan p;
   p.shape = sh;
   const uint64_t tile = (uint64_t)gemm_row_tile(sh);
   p.rows = (nrows + tile - 1) / tile * tile;
   const uint64_t nrt = p.rows / tile;
+  auto chunk_of = [&](uint64_t nch) {
+    const uint64_t kc = (kext + nch - 1) / nch;
+    return std::max<uint64_t>(kKStep, (kc + kKStep - 1) / kKStep * kKStep);
+  };
   uint64_t kc = kchunk_user;
   if (!kc) {
     uint64_t nch = nrt >= kTargetWorkgroups / 2 ? 1 : (kTargetWorkgroups + nrt - 1) / nrt;
     nch = std::max<uint64_t>(nch, (kext + kcap - 1) / kcap);
-    nch = std::min<uint64_t>(nch, std::max<uint64_t>(1, kext / (kMinStepsPerChunk * kKStep)));
-    kc = (kext + nch - 1) / nch;
+    const uint64_t nmax = std::max<uint64_t>(1, kext / (kMinStepsPerChunk * kKStep));
+    nch = std::min<uint64_t>(nch, nmax);
+    kc = chunk_of(nch);
+    if (slots && p.rows <= kQuantMaxRows) {
+      // A last round with few workgroups leaves most CUs idle (1,954 row
+      // tiles on 256 CUs fill 7.63 of 8 rounds; three chunks fill 22.9 of 23).
+      // Each extra chunk costs one more partial block: a point of fill.
+      double best = round_fill(nrt * ((kext + kc - 1) / kc), slots);
+      for (uint64_t n = nch + 1; n <= std::min(nch + kQuantExtraChunks, nmax); ++n) {
+        const uint64_t c = chunk_of(n);
+        if (c > kcap) continue;
+        const double fill = round_fill(nrt * ((kext + c - 1) / c), slots) - 0.01 * (double)(n - nch);
+        if (fill > best + 1e-9) best = fill, kc = c;
+      }
+    }
   }
   p.kchunk = std::max<uint64_t>(kKStep, (kc + kKStep - 1) / kKStep * kKStep);
   p.nchunks = std::max<uint64_t>(1, (kext + p.kchunk - 1) / p.kchunk);
@@ -661,13 +770,14 @@ GemmPlan plan_for(const GemmShape& sh, uint64_t nrows, uint64_t kext, uint64_t k
 }  // namespace
 
 GemmPlan plan_gemm(Form f, const GemmShape& preferred, bool fixed_shape, int lp, uint64_t nrows, uint64_t kext,
-                   uint64_t kchunk_user, uint64_t kcap) {
-  GemmPlan best = plan_for(preferred, nrows, kext, kchunk_user, kcap);
+                   uint64_t kchunk_user, uint64_t kcap, uint64_t cus) {
+  auto slots = [&](const GemmShape& sh) { return cus ? cus * (uint64_t)gemm_blocks_per_cu(f, sh, lp) : 0; };
+  GemmPlan best = plan_for(preferred, nrows, kext, kchunk_user, kcap, slots(preferred));
   if (fixed_shape || best.workgroups() >= kTargetWorkgroups / 2) return best;
   static const GemmShape smaller[] = {{4, 8}, {4, 4}, {2, 8}, {2, 4}, {1, 4}};
   for (const GemmShape& sh : smaller) {
     if (gemm_row_tile(sh) >= gemm_row_tile(preferred) || !gemm_shape_supported(f, sh, lp)) continue;
-    const GemmPlan p = plan_for(sh, nrows, kext, kchunk_user, kcap);
+    const GemmPlan p = plan_for(sh, nrows, kext, kchunk_user, kcap, slots(sh));
     if (p.workgroups() > best.workgroups()) best = p;
     if (best.workgroups() >= kTargetWorkgroups / 2) break;
   }
@@ -676,10 +786,13 @@ GemmPlan plan_gemm(Form f, const GemmShape& preferred, bool fixed_shape, int lp,
 
 uint64_t plan_partial_rows(uint64_t nrows_max, uint64_t kext_max, uint64_t kchunk_user, uint64_t kcap) {
   // nchunks <= max(target / row tiles + 1, kext / kcap + 1) when automatic, so
-  // nchunks * rows <= target * 512 + rows or (kext / kcap + 1) * rows.
+  // nchunks * rows <= target * 512 + rows or (kext / kcap + 1) * rows, plus
+  // the extra chunks that fill the last round on products of up to
+  // kQuantMaxRows rows.
   const uint64_t rows = nrows_max + 512;
   if (kchunk_user) return ((kext_max + kchunk_user - 1) / kchunk_user + 1) * rows;
-  return std::max(kTargetWorkgroups * 512 + rows, ((kext_max + kcap - 1) / kcap + 1) * rows);
+  return std::max(kTargetWorkgroups * 512 + rows, ((kext_max + kcap - 1) / kcap + 1) * rows) +
+         kQuantExtraChunks * std::min(rows, kQuantMaxRows + 512);
 }
 
 void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream_t s) {
