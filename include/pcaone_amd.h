@@ -48,6 +48,11 @@ extern "C" {
  * 32639) split into two int8 pieces; exact int32 accumulation.  Single-matrix
  * path only (not the batched form). */
 #define PCAONE_PREC_INT8 7
+/* FP8 MFMA (f8f6f4, E4M3 on both sides): genotypes exact in E4M3; each column
+ * of Omega and of diag(sc) G scaled by a power of two and split into three
+ * E4M3 pieces (about 12 significant bits per value); FP32 accumulation.
+ * Single-matrix path only (not the batched form). */
+#define PCAONE_PREC_FP8 8
 
 typedef struct pcaone_ctx pcaone_ctx;
 

@@ -10,9 +10,10 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpcaone_amd.so"
 
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X1, PREC_BF16MIX, PREC_FP16, PREC_FP16MIX, PREC_INT8 = range(8)
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X1, PREC_BF16MIX, PREC_FP16, PREC_FP16MIX, PREC_INT8, PREC_FP8 = range(9)
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x1": PREC_BF16X1,
-              "bf16mix": PREC_BF16MIX, "fp16": PREC_FP16, "fp16mix": PREC_FP16MIX, "int8": PREC_INT8}
+              "bf16mix": PREC_BF16MIX, "fp16": PREC_FP16, "fp16mix": PREC_FP16MIX, "int8": PREC_INT8,
+              "fp8": PREC_FP8}
 ERR_NO_DEVICE = 6
 
 

@@ -416,7 +416,7 @@ void Batch::run(const Param& p, bool want_v, bool use_graph, BatchResult& out) {
             "batch: k + oversamples exceeds the dimensions of cohort " + std::to_string(c));
   const ProductForms pf = forms_of(p.precision);
   require(pf.g == pf.h, kErrArgument, "batch: mixed operand forms are not supported");
-  require(pf.g != Form::I8, kErrArgument, "batch: the int8 form runs on the single-matrix path only");
+  require(!form_colscaled(pf.g), kErrArgument, "batch: the int8 and fp8 forms run on the single-matrix path only");
   const int form = (int)pf.g;
   // configuration: explicit (tuning fields) or autotuned once per (form, L)
   const GemmTuning& t = p.tuning;

@@ -564,7 +564,7 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
   return guard(c, [&](Context& x) {
     require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
     require(precision >= 0 && precision < kNumPrecisions && forms_of(precision).g == forms_of(precision).h,
-            kErrArgument, "packed_gemm: precision must be a single-form precision (0..3, 5 or 7)");
+            kErrArgument, "packed_gemm: precision must be a single-form precision (0..3, 5, 7 or 8)");
     const Form f = forms_of(precision).g;
     require(f != Form::I8 || round_up(ncodes, kKStep) <= kI8ChunkCap, kErrArgument,
             "packed_gemm: the int8 form takes at most 65536 codes per call");
@@ -587,7 +587,8 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
     DevBuf<unsigned char> wt((uint64_t)P * lp * K * form_wbytes(f));
     DevBuf<double> i8s(2 * 128);
     DevBuf<unsigned long long> i8max(128);
-    if (f == Form::I8) launch_i8_scales(dw.get(), ncodes, lp, nullptr, i8max.get(), i8s.get(), i8s.get() + 128, x.stream);
+    if (form_colscaled(f))
+      launch_col_scales(f, dw.get(), ncodes, lp, nullptr, i8max.get(), i8s.get(), i8s.get() + 128, x.stream);
     launch_make_pieces(f, dw.get(), ncodes, K, lp, nullptr, nullptr, wt.get(), K, (uint64_t)lp * K, nullptr,
                        nullptr, x.stream, i8s.get());
     DevBuf<unsigned char> out(rows_pad * lp * form_accbytes(f));
@@ -608,7 +609,7 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
     std::vector<unsigned char> h(out.bytes());
     std::vector<double> mul(128, 1.0);
     PCAONE_HIP(hipMemcpyAsync(h.data(), out.get(), h.size(), hipMemcpyDeviceToHost, x.stream));
-    if (f == Form::I8)
+    if (form_colscaled(f))
       PCAONE_HIP(hipMemcpyAsync(mul.data(), i8s.get() + 128, 128 * 8, hipMemcpyDeviceToHost, x.stream));
     PCAONE_HIP(hipStreamSynchronize(x.stream));
     for (uint64_t r = 0; r < nrows; ++r)
@@ -619,6 +620,8 @@ int pcaone_packed_gemm(pcaone_ctx* c, int precision, const uint8_t* rows, uint64
           v = reinterpret_cast<const double*>(h.data())[idx];
         else if (f == Form::I8)  // int32 sum of (v/2) q, times 2 s_l
           v = (double)reinterpret_cast<const int32_t*>(h.data())[idx] * mul[l];
+        else if (f == Form::F8)  // FP32 sum of v (w / s_l), times s_l
+          v = (double)reinterpret_cast<const float*>(h.data())[idx] * mul[l];
         else
           v = reinterpret_cast<const float*>(h.data())[idx];
         Y[(uint64_t)l * nrows + r] = v;
@@ -631,7 +634,7 @@ int pcaone_packed_gemm_noalloc(pcaone_ctx* c, int precision, const uint8_t* rows
   return guard(c, [&](Context& x) {
     require(rows && W && Y && nrows > 0 && ncodes > 0 && L >= 1 && L <= 128, kErrArgument, "bad gemm arguments");
     require(precision >= 0 && precision < kNumPrecisions && forms_of(precision).g == forms_of(precision).h,
-            kErrArgument, "packed_gemm_noalloc: precision must be a single-form precision (0..3, 5 or 7)");
+            kErrArgument, "packed_gemm_noalloc: precision must be a single-form precision (0..3, 5, 7 or 8)");
     const Form f = forms_of(precision).g;
     require(f != Form::I8 || round_up(ncodes, kKStep) <= kI8ChunkCap, kErrArgument,
             "packed_gemm_noalloc: the int8 form takes at most 65536 codes per call");

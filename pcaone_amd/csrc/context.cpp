@@ -417,7 +417,7 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
   omega_c_.resize(nrows_n * lp);
   colsum_.resize(lp);
   bvec_.resize(lp);
-  if (form_g == Form::I8 || form_h == Form::I8) {
+  if (form_colscaled(form_g) || form_colscaled(form_h)) {
     i8s_.resize(4 * 128);
     i8max_.resize(128);
   }
@@ -545,7 +545,8 @@ void Context::set_omega_staged() {
 
 void Context::omega_pieces() {
   launch_colmajor_to_rows(tmp_n_.get(), N, L, lp, ks_, omega_c_.get(), stream);
-  if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
+  if (form_colscaled(form_g))
+    launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
@@ -609,7 +610,7 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
   const uint64_t klen = round_up(off + nwin, kKStep);
   launch_zero_ranges(wh_.get(), wh_pitch_ * wb, (uint64_t)P * lp, 0, off * wb, (off + nwin) * wb,
                      (klen - off - nwin) * wb, stream);
-  if (!lnmiss_ && form_g != Form::I8 && form_h != Form::I8) {
+  if (!lnmiss_ && !form_colscaled(form_g) && !form_colscaled(form_h)) {
     launch_epi_g_fused(form_h, part_.get(), nch_g, g.ychunk, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                        colsum_.get(), gwin, wh_.get() + off * wb, wh_pitch_, (uint64_t)lp * wh_pitch_, red_.get(),
                        bvec_.get(), stream);
@@ -640,9 +641,9 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     m.lp = lp;
     launch_missing_spmm(m, stream);
   }
-  if (lnmiss_ || form_g == Form::I8 || form_h == Form::I8) {
-    if (form_h == Form::I8)
-      launch_i8_scales(gwin, nwin, lp, sc_.get() + j0, i8max_.get(), i8_scale(1), i8_mul(1), stream);
+  if (lnmiss_ || form_colscaled(form_g) || form_colscaled(form_h)) {
+    if (form_colscaled(form_h))
+      launch_col_scales(form_h, gwin, nwin, lp, sc_.get() + j0, i8max_.get(), i8_scale(1), i8_mul(1), stream);
     launch_make_pieces(form_h, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
                        (uint64_t)lp * wh_pitch_, gwin, nullptr, stream, i8_scale(1));
     launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
@@ -711,7 +712,7 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
 void Context::h_from_g(int hmode) {
   // H = A^T G~ over all SNPs for a G~ the caller wrote into gt_ (rows 0..M)
   require(L > 0, kErrState, "configure() first");
-  require(form_h != Form::I8, kErrArgument, "h_from_g: not in the int8 form");
+  require(!form_colscaled(form_h), kErrArgument, "h_from_g: not in the int8 or fp8 form");
   require(!streamed(), kErrArgument, "h_from_g: not with streamed genotypes");
   const uint64_t wb = bytes_of(form_h);
   const int P = form_pieces(form_h);
@@ -816,8 +817,8 @@ void Context::omega_from(int src, bool flip) {
     } else {
       dsign = nullptr;
     }
-    if (form_g == Form::I8)
-      launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
+    if (form_colscaled(form_g))
+      launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
     launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
                        (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
     launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
@@ -855,7 +856,8 @@ void Context::omega_from(int src, bool flip) {
     dsign = small_.get() + 128 * 128;
     PCAONE_HIP(hipMemcpyAsync(dsign, sign.data(), lp * 8, hipMemcpyHostToDevice, stream));
   }
-  if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
+  if (form_colscaled(form_g))
+    launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
@@ -909,7 +911,8 @@ void Context::omega_householder(bool flip) {
     dsign = small_.get() + 128 * 128;
     launch_flip_signs(norms, L, lp, dsign, stream);
   }
-  if (form_g == Form::I8) launch_i8_scales(omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
+  if (form_colscaled(form_g))
+    launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
   launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);

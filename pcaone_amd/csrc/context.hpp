@@ -274,8 +274,9 @@ class Context {
   void omega_pieces();  // tmp_n_ (column-major Omega) -> omega_c_, pieces, Omg2, column sums
   // I8 column scales / epilogue factors of the G (h = 0) or H (h = 1)
   // product's operand; null when that product is not in the I8 form.
-  double* i8_scale(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h : nullptr; }
-  double* i8_mul(int h) { return (h ? form_h : form_g) == Form::I8 ? i8s_.get() + 256 * h + 128 : nullptr; }
+  // column scales and epilogue factors of the column-scaled forms (null otherwise)
+  double* i8_scale(int h) { return form_colscaled(h ? form_h : form_g) ? i8s_.get() + 256 * h : nullptr; }
+  double* i8_mul(int h) { return form_colscaled(h ? form_h : form_g) ? i8s_.get() + 256 * h + 128 : nullptr; }
   void set_t(const la::Mat& T);
   void h_half(uint64_t j0, uint64_t j1, int hmode);  // the H product of sketch for G~ rows [j0, j1)
   // Orthonormalization safeguards (Cholesky pivots d_j over the largest diagonal entry).

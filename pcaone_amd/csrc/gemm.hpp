@@ -24,8 +24,13 @@ constexpr int kKStep = 256;       // codes consumed per K-step (and row padding 
 // FP16 = one FP16 piece (11 significand bits).  I8 = 16-bit fixed point per
 // column, q = rint(w / s_col) with |q| <= kI8Max, split into two int8 pieces
 // q = 256 q_hi + q_lo on the int8 matrix cores; the genotype operand is
-// v/2 in {+1, 0, -1} and the products accumulate exactly in int32.
-enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3, FP16 = 4, I8 = 5 };
+// v/2 in {+1, 0, -1} and the products accumulate exactly in int32.  F8 =
+// three FP8 E4M3 pieces of w / s_col (s_col a power of two that brings the
+// column maximum into (224, 448]): w = s_col (p1 + p2 + p3) to about 2^-12
+// of |w|, on the FP8 matrix cores (v_mfma_f32_16x16x128_f8f6f4); the
+// genotype operand v in {+2, 0, -2} is exact in E4M3 and the pieces
+// accumulate together in FP32.
+enum class Form : int { FP64 = 0, FP32 = 1, BF16 = 2, BF16X1 = 3, FP16 = 4, I8 = 5, F8 = 6 };
 
 // Largest |q| of the I8 form: q_hi = (q + 128) >> 8 stays within int8.
 constexpr int kI8Max = 32639;
@@ -36,7 +41,7 @@ constexpr uint64_t kI8ChunkCap = 65536;
 // User-facing precision (PCAONE_PREC_*) -> forms of the two products
 // (G = A Omega, H = A^T G).  4 = BF16 mixed: one BF16 piece of Omega, hi/lo of
 // G; 5 = FP16, one piece of each; 6 = FP16 mixed: one FP16 piece of Omega,
-// BF16 hi/lo of G; 7 = I8 for both.
+// BF16 hi/lo of G; 7 = I8 for both; 8 = F8 for both.
 struct ProductForms {
   Form g, h;
 };
@@ -49,15 +54,18 @@ inline ProductForms forms_of(int precision) {
     case 4: return {Form::BF16X1, Form::BF16};
     case 5: return {Form::FP16, Form::FP16};
     case 6: return {Form::FP16, Form::BF16};
-    default: return {Form::I8, Form::I8};
+    case 7: return {Form::I8, Form::I8};
+    default: return {Form::F8, Form::F8};
   }
 }
-constexpr int kNumPrecisions = 8;
+constexpr int kNumPrecisions = 9;
 
-inline int form_pieces(Form f) { return f == Form::BF16 || f == Form::I8 ? 2 : 1; }
+inline int form_pieces(Form f) { return f == Form::F8 ? 3 : (f == Form::BF16 || f == Form::I8 ? 2 : 1); }
 inline size_t form_wbytes(Form f) {
-  return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : (f == Form::I8 ? 1 : 2));
+  return f == Form::FP64 ? 8 : (f == Form::FP32 ? 4 : (f == Form::I8 || f == Form::F8 ? 1 : 2));
 }
+// Forms whose operand pieces carry a per-column scale (undone by the epilogues).
+inline bool form_colscaled(Form f) { return f == Form::I8 || f == Form::F8; }
 // Partial products: FP64, int32 (I8) or FP32.
 inline size_t form_accbytes(Form f) { return f == Form::FP64 ? 8 : 4; }
 // Largest split-K chunk a form accepts.

@@ -46,18 +46,18 @@ void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uin
 // Wt[p][l][r] for r < rows_pad (rows >= nreal written as 0).  Row r is first
 // multiplied by rowscale[r] (null = 1) and column l by colsign[l] (null = 1).
 // rounded (nullable, row-major rows x lp) receives sum(pieces) / rowscale.
-// The I8 form takes its column scales from colscale (launch_i8_scales).
+// The I8 and F8 forms take their column scales from colscale (launch_col_scales).
 void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
                         double* signed_out, hipStream_t s, const double* colscale = nullptr);
-// I8 column scales of X (rows [0, nreal), each row times rowscale[r], null =
-// 1): scale[l] = max_r |X[r][l] rowscale[r]| / kI8Max (1 if the column is
-// zero), mul[l] = 2 scale[l].  maxbits: lp words of scratch.
-void launch_i8_scales(const double* X, uint64_t nreal, int lp, const double* rowscale, unsigned long long* maxbits,
-                      double* scale, double* mul, hipStream_t s);
-// G epilogue: out[j][l] = alpha_j * sum_c P[c][j][l] + beta_j * colsum[l]  (j < nreal; 0 above)
-// with alpha_j = sc_j / 4, beta_j = sc_j * (1/2 - cf_j).  colmul (required for
-// I8, whose int32 partials are sum (v/2) q): per-column factor of the sum.
+// Column scales of X for a column-scaled form (rows [0, nreal), each row
+// times rowscale[r], null = 1), from m_l = max_r |X[r][l] rowscale[r]|:
+// I8: scale[l] = m_l / kI8Max, mul[l] = 2 scale[l] (the genotype operand is
+// v/2); F8: scale[l] = the power of two that brings m_l into (224, 448],
+// mul[l] = scale[l].  An all-zero column has scale 1.  maxbits: lp words of
+// scratch.
+void launch_col_scales(Form f, const double* X, uint64_t nreal, int lp, const double* rowscale,
+                       unsigned long long* maxbits, double* scale, double* mul, hipStream_t s);
 void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
                   int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s,
                   const double* colmul = nullptr);

@@ -738,6 +738,7 @@ void launch_batch_make_pieces(const BatchArgs& a, const RowItem* items_n, uint32
     case Form::BF16X1: hipLaunchKernelGGL(k_b_make_pieces<Form::BF16X1>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::FP16: hipLaunchKernelGGL(k_b_make_pieces<Form::FP16>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::I8: fail(kErrArgument, "batch: no int8 form");
+    case Form::F8: fail(kErrArgument, "batch: no fp8 form");
   }
   check_launch("batch_make_pieces");
 }
@@ -770,6 +771,7 @@ void launch_batch_epi_g(const BatchArgs& a, const RowItem* items_e, uint32_t nit
     case Form::BF16X1: hipLaunchKernelGGL((k_b_epi_g<Form::BF16X1, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
     case Form::FP16: hipLaunchKernelGGL((k_b_epi_g<Form::FP16, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
     case Form::I8: fail(kErrArgument, "batch: no int8 form");
+    case Form::F8: fail(kErrArgument, "batch: no fp8 form");
   }
   hipLaunchKernelGGL(k_b_sum_items, dim3(nblocks(a.ncoh)), dim3(256), 0, s, a, partial, LP, (int)kListE,
                      (int)kDestBvec, 1);

@@ -33,6 +33,7 @@ typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 
@@ -99,6 +100,12 @@ struct FT<Form::I8> {
   using w_t = uint8_t;
   using acc_t = int32_t;
   static constexpr int P = 2, KS = 256, NACC = 2;
+};
+template <>
+struct FT<Form::F8> {
+  using w_t = uint8_t;
+  using acc_t = float;
+  static constexpr int P = 3, KS = 256, NACC = 1;
 };
 template <>
 struct FT<Form::FP32> {
@@ -180,6 +187,42 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
             acc[p][rb][cb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                 af[rb], __builtin_bit_cast(i32x4_t, bv[cb * T::P + p]), acc[p][rb][cb], 0, 0, 0);
     }
+  } else if constexpr (F == Form::F8) {
+    // 16x16x128 FP8 MFMAs, E4M3 on both sides: substep m takes words 2m and
+    // 2m + 1 of the lane group's four (codes 64 q + 32 m + [0, 32), in order),
+    // decoded to E4M3 (+2 = 0x40, 0, -2 = 0xC0: the same bytes as the BF16
+    // top bytes); the B fragment of a piece is the same 32 K slots of its W
+    // row, two 16-byte vectors.  The pieces accumulate into one FP32 set.
+    i32x8_t af[2][WM];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int rb = 0; rb < WM; ++rb)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) af[m][rb][4 * i + t] = (int)top_bytes(spread(aw[rb][2 * m + i], t));
+    auto read_b = [&](int m, int cb, int p) {
+      const w_t* row = wl + (p * LP + cb * 16 + r16) * LDSP;
+      const u32x4_t v0 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, 4 * q + 2 * m) * 16);
+      const u32x4_t v1 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, 4 * q + 2 * m + 1) * 16);
+      i32x8_t b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) b[e] = (int)v0[e], b[4 + e] = (int)v1[e];
+      return b;
+    };
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+        for (int p = 0; p < T::P; ++p) {
+          const i32x8_t bv = read_b(m, cb, p);
+#pragma unroll
+          for (int rb = 0; rb < WM; ++rb)  // scale operands 0: the plain (unscaled) product
+            acc[0][rb][cb] =
+                __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[m][rb], bv, acc[0][rb][cb], 0, 0, 0, 0, 0, 0);
+        }
   } else if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
     // Fragment j of substep s + 1 is read from LDS into the registers of
     // fragment j of substep s right after that fragment's last MFMA, and the A
@@ -570,6 +613,7 @@ void launch_t(const GemmArgs& a, hipStream_t s, int* occ) {
 // Which (form, wm, waves) combinations are compiled.
 template <Form F>
 constexpr bool shape_ok(int wm, int waves) {
+  if (F == Form::F8) return (waves == 4 && (wm == 1 || wm == 2 || wm == 4)) || (waves == 8 && (wm == 2 || wm == 4));
   if (waves == 4) return wm == 1 || wm == 2 || (wm == 4 && F != Form::FP64) || (wm == 8 && F != Form::FP64 && F != Form::FP32);
   if (waves == 8) return (wm == 2 || wm == 4) && F != Form::FP64;
   return false;
@@ -647,6 +691,7 @@ GemmShape gemm_default_shape(Form f, int lp) {
   if (f == Form::FP32) return lp > 96 ? GemmShape{2, 4} : GemmShape{4, 8};
   if (f == Form::BF16) return lp > 80 ? GemmShape{4, 4} : GemmShape{4, 8};
   if (f == Form::I8) return lp > 64 ? GemmShape{2, 8} : GemmShape{4, 8};  // two accumulator sets
+  if (f == Form::F8) return lp > 80 ? GemmShape{2, 8} : GemmShape{4, 8};  // 16 A registers per row block
   return lp > 96 ? GemmShape{4, 4} : GemmShape{4, 8};
 }
 
@@ -659,6 +704,7 @@ bool gemm_shape_supported(Form f, const GemmShape& g, int lp) {
     case Form::BF16X1: return shape_ok<Form::BF16X1>(g.wm, g.waves);
     case Form::FP16: return shape_ok<Form::FP16>(g.wm, g.waves);
     case Form::I8: return shape_ok<Form::I8>(g.wm, g.waves);
+    case Form::F8: return shape_ok<Form::F8>(g.wm, g.waves);
   }
   return false;
 }
@@ -675,6 +721,7 @@ void launch_grouped_gemm(Form f, const GemmShape& g, int lp, const GroupedGemmAr
     case Form::BF16X1: return launch_grouped_shape<Form::BF16X1>(g, lp, a, grid, s, nullptr);
     case Form::FP16: return launch_grouped_shape<Form::FP16>(g, lp, a, grid, s, nullptr);
     case Form::I8: return launch_grouped_shape<Form::I8>(g, lp, a, grid, s, nullptr);
+    case Form::F8: return launch_grouped_shape<Form::F8>(g, lp, a, grid, s, nullptr);
   }
 }
 
@@ -689,6 +736,7 @@ int grouped_gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
     case Form::BF16X1: launch_grouped_shape<Form::BF16X1>(g, lp, none, 0, nullptr, &occ); break;
     case Form::FP16: launch_grouped_shape<Form::FP16>(g, lp, none, 0, nullptr, &occ); break;
     case Form::I8: launch_grouped_shape<Form::I8>(g, lp, none, 0, nullptr, &occ); break;
+    case Form::F8: launch_grouped_shape<Form::F8>(g, lp, none, 0, nullptr, &occ); break;
   }
   return occ > 0 ? occ : 1;
 }
@@ -711,6 +759,7 @@ int gemm_blocks_per_cu(Form f, const GemmShape& g, int lp) {
     case Form::BF16X1: launch_shape<Form::BF16X1>(g, none, nullptr, &occ); break;
     case Form::FP16: launch_shape<Form::FP16>(g, none, nullptr, &occ); break;
     case Form::I8: launch_shape<Form::I8>(g, none, nullptr, &occ); break;
+    case Form::F8: launch_shape<Form::F8>(g, none, nullptr, &occ); break;
   }
   occ = occ > 0 ? occ : 1;
   cache[key] = occ;
@@ -813,6 +862,7 @@ void launch_packed_gemm(Form f, const GemmShape& g, const GemmArgs& a, hipStream
     case Form::BF16X1: return launch_shape<Form::BF16X1>(g, a, s);
     case Form::FP16: return launch_shape<Form::FP16>(g, a, s);
     case Form::I8: return launch_shape<Form::I8>(g, a, s);
+    case Form::F8: return launch_shape<Form::F8>(g, a, s);
   }
 }
 

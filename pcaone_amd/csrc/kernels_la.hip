@@ -51,6 +51,20 @@ __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint
       const uint16_t h = fp16_bits(y);
       static_cast<uint16_t*>(Wt)[l * wpitch + r] = h;
       back = fp16_value(h);
+    } else if constexpr (F == Form::F8) {
+      // y / s_l (exact: s_l is a power of two, |y / s_l| <= 448) as three
+      // E4M3 pieces, each the nearest E4M3 to what the previous ones left
+      const double sl = colscale[l];
+      const double x1 = y / sl;
+      double v1, v2, v3;
+      const uint8_t b1 = e4m3_bits(x1, &v1);
+      const uint8_t b2 = e4m3_bits(x1 - v1, &v2);
+      const uint8_t b3 = e4m3_bits(x1 - v1 - v2, &v3);
+      uint8_t* w = static_cast<uint8_t*>(Wt);
+      w[l * wpitch + r] = b1;
+      w[wpiece + l * wpitch + r] = b2;
+      w[2 * wpiece + l * wpitch + r] = b3;
+      back = (v1 + v2 + v3) * sl;
     } else if constexpr (F == Form::I8) {
       // q = rint(y / s_l) within +-kI8Max; q = 256 q_hi + q_lo, q_lo in [-128, 127]
       const double sl = colscale[l];
@@ -76,7 +90,8 @@ __global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint
 }
 
 // colmul (nullable): per-column factor of the summed partials (the I8
-// form's int32 partials are sum (v/2) q, so the factor is 2 s_l).
+// form's int32 partials are sum (v/2) q, so the factor is 2 s_l; the F8
+// form's are sum v (w / s_l), so it is s_l).
 template <class P>
 __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t stride, uint64_t nreal,
                         uint64_t rows_pad, int lp, const double* __restrict__ sc, const double* __restrict__ cf,
@@ -291,16 +306,26 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
-// I8 column scales s_l = max_l / kI8Max (1 for an all-zero column) and the
-// epilogue factors 2 s_l.
-__global__ void k_i8_scales(const unsigned long long* __restrict__ maxbits, int lp, double* __restrict__ scale,
-                            double* __restrict__ mul) {
+// Column scales from the column maxima (1 for an all-zero column): I8
+// s_l = max_l / kI8Max with epilogue factor 2 s_l; F8 s_l = 2^ceil(log2(max_l
+// / 448)) (the column maximum lands in (224, 448]) with factor s_l.
+__global__ void k_col_scales(const unsigned long long* __restrict__ maxbits, int lp, int f8, double* __restrict__ scale,
+                             double* __restrict__ mul) {
   const int l = threadIdx.x;
   if (l >= lp) return;
   const double mx = __longlong_as_double((long long)maxbits[l]);
-  const double s = mx > 0.0 ? mx / (double)kI8Max : 1.0;
+  double s = 1.0;
+  if (mx > 0.0) {
+    if (f8) {
+      int e;
+      const double f = frexp(mx / 448.0, &e);  // mx / 448 = f 2^e, f in [0.5, 1)
+      s = ldexp(1.0, f == 0.5 ? e - 1 : e);    // smallest power of two >= mx / 448
+    } else {
+      s = mx / (double)kI8Max;
+    }
+  }
   scale[l] = s;
-  mul[l] = 2.0 * s;
+  mul[l] = f8 ? s : 2.0 * s;
 }
 
 // Stage 1 of a column reduction: block b sums rows [r0 + b*rpb, ...) for each column.
@@ -1244,7 +1269,7 @@ void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_p
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
                         double* signed_out, hipStream_t s, const double* colscale) {
   const unsigned g = grid_for(rows_pad * lp);
-  require(f != Form::I8 || colscale, kErrInternal, "make_pieces: the I8 form needs column scales");
+  require(!form_colscaled(f) || colscale, kErrInternal, "make_pieces: this form needs column scales");
   switch (f) {
     case Form::FP64:
       hipLaunchKernelGGL(k_make_pieces<Form::FP64>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
@@ -1270,6 +1295,10 @@ void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_p
       hipLaunchKernelGGL(k_make_pieces<Form::I8>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
                          colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
       break;
+    case Form::F8:
+      hipLaunchKernelGGL(k_make_pieces<Form::F8>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
+                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
+      break;
   }
   check_launch("make_pieces");
 }
@@ -1278,7 +1307,7 @@ void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride
                   int lp, const double* sc, const double* cf, const double* colsum, double* out, hipStream_t s,
                   const double* colmul) {
   const unsigned g = grid_for(rows_pad * lp);
-  require(f != Form::I8 || colmul, kErrInternal, "epi_g: the I8 form needs column factors");
+  require(!form_colscaled(f) || colmul, kErrInternal, "epi_g: this form needs column factors");
   if (f == Form::FP64)
     hipLaunchKernelGGL(k_epi_g<double>, dim3(g), dim3(256), 0, s, static_cast<const double*>(P), nchunks,
                        chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
@@ -1291,14 +1320,15 @@ void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride
   check_launch("epi_g");
 }
 
-void launch_i8_scales(const double* X, uint64_t nreal, int lp, const double* rowscale, unsigned long long* maxbits,
-                      double* scale, double* mul, hipStream_t s) {
+void launch_col_scales(Form f, const double* X, uint64_t nreal, int lp, const double* rowscale,
+                       unsigned long long* maxbits, double* scale, double* mul, hipStream_t s) {
+  require(form_colscaled(f), kErrInternal, "col_scales: not a column-scaled form");
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, maxbits, lp);
   const uint64_t per = 256 / lp;
   hipLaunchKernelGGL(k_colmax, dim3(grid_for(nreal, (unsigned)per, 2048)), dim3(256), 0, s, X, nreal, lp, rowscale,
                      maxbits);
-  hipLaunchKernelGGL(k_i8_scales, dim3(1), dim3(128), 0, s, maxbits, lp, scale, mul);
-  check_launch("i8_scales");
+  hipLaunchKernelGGL(k_col_scales, dim3(1), dim3(128), 0, s, maxbits, lp, f == Form::F8 ? 1 : 0, scale, mul);
+  check_launch("col_scales");
 }
 
 uint64_t epi_g_fused_partial_elems(uint64_t nwin, int lp) { return (nwin + kEpiRows - 1) / kEpiRows * lp; }
@@ -1335,7 +1365,8 @@ void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride
     case Form::BF16: epi_fused_t<Form::BF16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
     case Form::BF16X1: epi_fused_t<Form::BF16X1, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
     case Form::FP16: epi_fused_t<Form::FP16, float>(P, nchunks, stride, nwin, lp, sc, cf, colsum, gout, Wt, wpitch, wpiece, bpart, s); break;
-    case Form::I8: fail(kErrInternal, "epi_g_fused: the I8 form takes the unfused epilogue");
+    case Form::I8:
+    case Form::F8: fail(kErrInternal, "epi_g_fused: the column-scaled forms take the unfused epilogue");
   }
   check_launch("epi_g_fused");
   const uint64_t nb = (nwin + kEpiRows - 1) / kEpiRows;
@@ -1346,7 +1377,7 @@ void launch_epi_g_fused(Form f, const void* P, uint64_t nchunks, uint64_t stride
 void launch_epi_h(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
                   int lp, const double* b, double* out, int mode, hipStream_t s, const double* colmul) {
   const unsigned g = grid_for(rows_pad * lp);
-  require(f != Form::I8 || colmul, kErrInternal, "epi_h: the I8 form needs column factors");
+  require(!form_colscaled(f) || colmul, kErrInternal, "epi_h: this form needs column factors");
   if (f == Form::FP64)
     hipLaunchKernelGGL(k_epi_h<double>, dim3(g), dim3(256), 0, s, static_cast<const double*>(P), nchunks,
                        chunk_stride, nreal, rows_pad, lp, b, out, mode, colmul);

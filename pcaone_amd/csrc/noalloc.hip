@@ -54,7 +54,8 @@ unsigned char* scratch_of(int device) {
 }
 
 // Y (nrows x L, column-major FP64) from the product rows (rows x lp) in the
-// form's accumulator: FP64, FP32, or int32 times the int8 column factor.
+// form's accumulator: FP64, FP32, int32 times the int8 column factor, or FP32
+// times the fp8 column factor.
 __global__ void k_gemm_out(const void* __restrict__ acc, int kind, uint64_t nrows, int L, int lp,
                            const double* __restrict__ mul, double* __restrict__ Y) {
   GRID_STRIDE(t, nrows * (uint64_t)L) {
@@ -65,6 +66,8 @@ __global__ void k_gemm_out(const void* __restrict__ acc, int kind, uint64_t nrow
       v = static_cast<const double*>(acc)[idx];
     else if (kind == 2)
       v = (double)static_cast<const int32_t*>(acc)[idx] * mul[l];
+    else if (kind == 3)
+      v = (double)static_cast<const float*>(acc)[idx] * mul[l];
     else
       v = static_cast<const float*>(acc)[idx];
     Y[t] = v;
@@ -107,7 +110,7 @@ void packed_gemm_noalloc(Context& x, Form f, const uint8_t* rows, uint64_t nrows
   launch_fill_rows(X, pitch, 0, rows_pad, s);
   launch_interleave(rows, nrows, bps, ncodes, X, pitch, s);
   launch_colmajor_to_rows(W, ncodes, (int)L, lp, K, dw, s);
-  if (f == Form::I8) launch_i8_scales(dw, ncodes, lp, nullptr, i8max, i8s, i8s + 128, s);
+  if (form_colscaled(f)) launch_col_scales(f, dw, ncodes, lp, nullptr, i8max, i8s, i8s + 128, s);
   launch_make_pieces(f, dw, ncodes, K, lp, nullptr, nullptr, wt, K, (uint64_t)lp * K, nullptr, nullptr, s, i8s);
   GemmArgs a{};
   a.X = X;
@@ -123,7 +126,7 @@ void packed_gemm_noalloc(Context& x, Form f, const uint8_t* rows, uint64_t nrows
   a.kchunk = K;
   a.lp = lp;
   launch_packed_gemm(f, shape, a, s);
-  const int kind = f == Form::FP64 ? 0 : (f == Form::I8 ? 2 : 1);
+  const int kind = f == Form::FP64 ? 0 : (f == Form::I8 ? 2 : (f == Form::F8 ? 3 : 1));
   hipLaunchKernelGGL(k_gemm_out, dim3(grid_for(nrows * (uint64_t)L)), dim3(256), 0, s, acc, kind, nrows, (int)L, lp,
                      i8s + 128, Y);
   check_launch("gemm_out");

@@ -4,7 +4,8 @@ Tolerances: the FP64 form must reproduce the reference outputs to their print
 precision (6 significant digits); the FP32 and BF16 hi/lo forms must meet the
 accuracy bar against the FP64 oracle on the same Omega: eigenvalue relative
 error <= 1e-4 and every PC correlated >= 0.9999; so must the int8 form (two
-int8 pieces of 16-bit per-column fixed point, exact int32 accumulation).  The
+int8 pieces of 16-bit per-column fixed point, exact int32 accumulation) and
+the fp8 form (three E4M3 pieces per value, FP32 accumulation).  The
 single-piece forms (FP16, BF16MIX, FP16MIX) meet the eigenvalue bar and
 correlate >= 0.999.
 """
@@ -71,6 +72,38 @@ def test_packed_gemm_int8_exact(ctx, nrows, ncodes, L):
     np.testing.assert_array_equal(Y, DMAP[codes] @ W)
 
 
+@pytest.mark.parametrize("nrows,ncodes,L", [(1, 1, 1), (37, 127, 20), (300, 129, 80), (513, 1000, 128),
+                                            (257, 333, 40), (64, 1024, 16)])
+def test_packed_gemm_fp8_exact(ctx, nrows, ncodes, L):
+    """W = q 2^-7 with |q| <= 4095 and each column's maximum exactly 4095 2^-7:
+    the column scale is 2^-3, so W / s = q / 16 has at most 12 significant bits
+    above 2^-4 and three E4M3 pieces (4 bits each, round to nearest) hold it
+    exactly; every partial sum is a multiple of 2^-3 below 2^21, exact in FP32."""
+    say(f"packed_gemm form=fp8 nrows={nrows} ncodes={ncodes} L={L}")
+    rng = np.random.default_rng(nrows * 3 + ncodes + L)
+    codes = rng.integers(0, 4, (nrows, ncodes)).astype(np.uint8)
+    q = rng.integers(-4095, 4096, (ncodes, L))
+    q[rng.integers(0, ncodes, L), np.arange(L)] = rng.choice([-4095, 4095], L)
+    W = q / 128.0
+    Y = ctx.packed_gemm("fp8", pack(codes), ncodes, W)
+    np.testing.assert_array_equal(Y, DMAP[codes] @ W)
+
+
+def test_packed_gemm_fp8_small_and_signed_values(ctx):
+    """Columns of one value each: a power of two, a value needing all three
+    pieces, values near the top of E4M3 after scaling, and a zero column."""
+    say("packed_gemm fp8 special columns")
+    codes = np.array([[0, 3, 2, 0, 1, 3, 0, 0]], dtype=np.uint8)
+    n = codes.shape[1]
+    W = np.zeros((n, 5))
+    W[:, 0] = 2.0 ** -20
+    W[:, 1] = 1.0 + 2.0 ** -11          # 12 significant bits
+    W[:, 2] = -(447.0 + 0.5)            # scaled into (224, 448]
+    W[:, 3] = 3.0 * 2.0 ** 20
+    Y = ctx.packed_gemm("fp8", pack(codes), n, W)
+    np.testing.assert_array_equal(Y, DMAP[codes] @ W)
+
+
 def test_packed_gemm_rounding_fp32(ctx):
     say("packed_gemm fp32 rounding")
     rng = np.random.default_rng(11)
@@ -78,7 +111,7 @@ def test_packed_gemm_rounding_fp32(ctx):
     W = rng.standard_normal((3000, 24))
     ref = DMAP[codes] @ W
     for form, tol in (("fp64", 1e-12), ("fp32", 2e-5), ("bf16", 3e-5), ("bf16x1", 2e-2), ("fp16", 3e-3),
-                      ("int8", 2e-4)):
+                      ("int8", 2e-4), ("fp8", 3e-4)):
         Y = ctx.packed_gemm(form, pack(codes), 3000, W)
         err = np.abs(Y - ref).max() / np.abs(ref).max()
         say(f"  {form}: max rel err {err:.2e}")
@@ -246,7 +279,7 @@ def test_golden_fp64(ctx, golden_inputs, case):
         np.testing.assert_allclose(O.sign_align(Vref, r.V), Vref, atol=2e-6)
 
 
-REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16", "int8")
+REDUCED = [(c, f) for f in ("fp32", "bf16", "fp16", "int8", "fp8")
            for c in ("f1_d1", "f1_d2", "f4_d1_miss", "f4_emu_d1", "cfg1_d1", "cfg1_d2")]
 # The mixed form (one BF16 piece of Omega) meets the bar on --svd 1 only; on
 # winSVD the rounded per-window Omega moves the unconverged noise PCs.
@@ -259,7 +292,8 @@ REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f6_d1", "f6_d2")]
 REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f1_d2_ooc", "f4_emu_d2_ooc")]
 # Single-piece forms round Omega (or G) to 8-11 bits: the eigenvalues meet the
 # bar, the unconverged trailing PCs move by up to ~1e-3 in correlation.
-PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "int8": 0.9999, "fp16": 0.999, "bf16mix": 0.999, "fp16mix": 0.999}
+PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "int8": 0.9999, "fp8": 0.9999, "fp16": 0.999, "bf16mix": 0.999,
+               "fp16mix": 0.999}
 
 
 @pytest.mark.parametrize("case,form", REDUCED)
@@ -273,12 +307,12 @@ def test_golden_reduced_precision(ctx, golden_inputs, case, form):
     assert corr.min() >= PC_CORR_MIN[form]
 
 
-@pytest.mark.parametrize("form", ["bf16", "fp32", "int8"])
+@pytest.mark.parametrize("form", ["bf16", "fp32", "int8", "fp8"])
 def test_precision_form_at_l80_matches_oracle(ctx, golden_inputs, form):
     """The precision workload's settings (k = 40, oversampling 20 raised to 40
     by src/Cmd.cpp:216, so L = 80; 7 power iterations, --tol-rsvd 0) on the
     2,000 x 20,000 input: the reduced forms (BF16 hi/lo with the fused G
-    epilogue and the device thin-Q at L = 80, FP32, int8) against the FP64
+    epilogue and the device thin-Q at L = 80, FP32, int8, fp8) against the FP64
     oracle on the same Omega.  Bar: eigenvalue relative error <= 1e-4 and
     every one of the 40 PCs correlated >= 0.9999."""
     say(f"precision form {form} at k=40, L=80 on cfg1")
@@ -469,7 +503,8 @@ def test_device_generator_matches_host(ctx, spec_kw):
 @pytest.mark.parametrize("form,wm,waves", [("bf16", 4, 8), ("bf16", 2, 8), ("bf16", 8, 4), ("fp32", 4, 8),
                                            ("bf16mix", 4, 8), ("bf16x1", 8, 4), ("bf16x1", 4, 8), ("fp16", 4, 8),
                                            ("fp16", 2, 8), ("fp16", 8, 4), ("int8", 2, 8), ("int8", 4, 8),
-                                           ("int8", 4, 4), ("int8", 1, 4)])
+                                           ("int8", 4, 4), ("int8", 1, 4), ("fp8", 4, 8), ("fp8", 2, 8),
+                                           ("fp8", 4, 4), ("fp8", 1, 4)])
 def test_workgroup_shapes_agree(ctx, golden_inputs, form, wm, waves):
     """Every compiled workgroup shape gives the same decomposition (cfg1, 8 passes)."""
     say(f"shape form={form} wm={wm} waves={waves}")

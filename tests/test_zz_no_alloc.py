@@ -28,7 +28,7 @@ def _noalloc(ctx, precision, packed, ncodes, W):
     return Y.cpu().numpy().reshape(W.shape[1], packed.shape[0]).T
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16", "int8"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16", "int8", "fp8"])
 def test_noalloc_matches_allocating_entry_point(precision):
     rng = np.random.default_rng(11)
     with api.Context(0) as ctx:
