@@ -93,6 +93,10 @@ typedef struct pcaone_params {
   int stream;           /* pcaone_run_bfile with memory > 0: stream the .bed blocks */
                         /* of the -m plan from the mapped file every pass (data    */
                         /* beyond HBM) instead of loading the whole matrix         */
+  int hardcall;         /* --hardcall: read the hardcall track of a .pgen that also */
+                        /* stores dosages (without it such a file is rejected)      */
+  uint32_t threads;     /* -n/--threads (12): with -m, a .pgen's shuffle depends on */
+                        /* min(threads, host cores) (src/Main.cpp:53-54,169-172)    */
 } pcaone_params;
 
 typedef struct pcaone_run_info {
@@ -143,6 +147,28 @@ int pcaone_bed_dims(const char* prefix, uint32_t* nsamples, uint64_t* nsnps);
  * <prefix>.fam/.bim/.bed, checks the magic, moves the payload to HBM
  * undecoded and counts allele frequencies on the device. */
 int pcaone_bed_read(pcaone_ctx* ctx, const char* prefix, double maf, uint32_t* nsamples, uint64_t* nsnps);
+/* PLINK 2 input (the hardcall path of src/FilePgen.cpp:27-287).
+ * pcaone_pgen_dims replaces FilePgen::FilePgen (src/FilePgen.hpp:14-35)
+ * without a device: N = data lines of <prefix>.psam (not starting with '#'),
+ * M = the variant count of <prefix>.pgen, whose header must give the same N.
+ * pcaone_pgen_read replaces FilePgen::read_all (src/FilePgen.cpp:27-89) and
+ * the --maf filter: the records (every storage mode and record type of the
+ * format; the index may sit in <prefix>.pgen.pgi) are decoded on the host
+ * into .bed rows (ALT count 2 / 1 / 0 / missing -> code 00 / 10 / 11 / 01,
+ * the codes whose values are the reference's pgen2dosage) and uploaded as
+ * pcaone_bed_read uploads them.  A file that stores dosages is read only with
+ * hardcall != 0 (the reference's --hardcall); multi-allelic files are
+ * rejected, as the reference's reader rejects them (PCAONE_ERR_IO).
+ * pcaone_pgen_decode writes the rows [snp_begin, snp_end) as .bed rows
+ * (ceil(N/4) bytes each) into host memory, without a device.
+ * pcaone_stream_pgen opens the file for streamed runs (records decoded at
+ * every block load, src/FilePgen.cpp:153-287). */
+int pcaone_pgen_dims(const char* prefix, uint32_t* nsamples, uint64_t* nsnps);
+int pcaone_pgen_read(pcaone_ctx* ctx, const char* prefix, double maf, int hardcall, uint32_t* nsamples,
+                     uint64_t* nsnps);
+int pcaone_pgen_decode(const char* prefix, int hardcall, uint64_t snp_begin, uint64_t snp_end, uint8_t* rows);
+int pcaone_stream_pgen(pcaone_ctx* ctx, const char* prefix, int hardcall, uint64_t block_snps, uint32_t* nsamples,
+                       uint64_t* nsnps);
 /* The same from memory: reserve, then upload raw .bed rows (ceil(N/4) bytes
  * per SNP, the payload after the 3-byte magic), then finalize. */
 int pcaone_bed_reserve(pcaone_ctx* ctx, uint32_t nsamples, uint64_t nsnps);
@@ -233,6 +259,13 @@ int pcaone_run_trace(const pcaone_ctx* ctx, int em, double* out, int cap, int* n
  * printv). */
 int pcaone_run_bfile(pcaone_ctx* ctx, const char* bfile, const char* out, const pcaone_params* p,
                      pcaone_run_info* info);
+/* The same for --pgen <pfile> (src/Main.cpp:150-231 with FilePgen): the
+ * .mbim takes the .pvar lines as .bim lines (src/Utils.cpp:405-430), the
+ * .eigvecs2 the FID/IID of the .psam (src/Utils.cpp:240-265), and an -m run
+ * with the winSVD shuffle takes compute_pgen_perm's order
+ * (src/FilePgen.cpp:91-151, src/Main.cpp:169-172). */
+int pcaone_run_pgen(pcaone_ctx* ctx, const char* pfile, const char* out, const pcaone_params* p,
+                    pcaone_run_info* info);
 /* Kernel timing with HIP events around the two products (on/off). */
 int pcaone_time_kernels(pcaone_ctx* ctx, int on);
 
@@ -308,6 +341,10 @@ int pcaone_block_plan(const pcaone_params* p, uint32_t nsamples, uint64_t nsnps,
  * rewritten file holds original SNP order[i] (nsnps values). */
 int pcaone_permute_plink_order(uint64_t nsamples, uint64_t nsnps, uint32_t buffer_gb, uint32_t bands,
                                int32_t* order);
+/* compute_pgen_perm (src/FilePgen.cpp:91-151): the logical SNP order of an
+ * out-of-core winSVD run on a .pgen: position i reads variant order[i]. */
+int pcaone_pgen_perm_order(uint64_t nsnps, uint32_t nbatches, uint64_t blocksize, uint32_t nthreads, int seed,
+                           int32_t* order);
 /* RsvdOpData::initOmg's matrix (src/Halko.cpp:15-23): nrows x ncols,
  * column-major, identical to the libstdc++ draw, computed in parallel. */
 int pcaone_host_omega(int seed, int gaussian, uint64_t nrows, uint64_t ncols, double* out);

@@ -26,11 +26,14 @@ Pinning (see tests/test_oracle.py):
   permute_plink_order                    - permutation invariants, the single-buffer closed form
                                            and golden -d 2 -m runs (loadings in file order)
   out-of-core NormalOp / FancyOp         - golden -m outputs of the reference binary
+  pgen_perm_order (.pgen -m shuffle)     - permutation invariants, a hand-worked small case and
+                                           the golden .pgen -d 2 -m runs of the reference binary
 """
 from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -52,6 +55,7 @@ def _rnglib():
         _rng = C.CDLL(str(path))
         _rng.oracle_omega.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, C.c_void_p]
         _rng.oracle_shuffle.argtypes = [C.c_long, C.c_void_p]
+        _rng.oracle_shuffle_parts.argtypes = [C.c_int, C.c_long, C.c_void_p, C.c_void_p]
     return _rng
 
 
@@ -60,6 +64,20 @@ def omega(seed: int, gaussian: bool, n: int, l: int) -> np.ndarray:
     out = np.empty(n * l, dtype=np.float64)
     _rnglib().oracle_omega(int(seed), 1 if gaussian else 0, n, l, out.ctypes.data)
     return out.reshape(l, n).T.copy()
+
+
+def shuffle_parts(seed: int, parts: list) -> list:
+    """src/FilePgen.cpp:124-126: one std::default_random_engine seeded with
+    `seed` shuffles each partition in turn (std::shuffle)."""
+    sizes = np.array([len(p) for p in parts], dtype=np.int64)
+    data = np.concatenate([np.asarray(p, dtype=np.int32) for p in parts]) if parts else np.zeros(0, np.int32)
+    data = np.ascontiguousarray(data, dtype=np.int32)
+    _rnglib().oracle_shuffle_parts(int(seed), len(parts), sizes.ctypes.data, data.ctypes.data)
+    out, at = [], 0
+    for s in sizes:
+        out.append(data[at:at + s])
+        at += s
+    return out
 
 
 def shuffle_indices(m: int) -> np.ndarray:
@@ -123,6 +141,9 @@ class Params:
     buffer: int = 2  # --buffer (GB) of permute_plink
     ld: bool = False  # -D/--ld: write the residual matrix (src/Cmd.cpp:92)
     ld_stats: int = 0  # --ld-stats: 0 residuals of the PCs, 1 centered genotypes (src/Cmd.cpp:114)
+    pgen: bool = False  # the input is a .pgen (src/Main.cpp:169-172: its -m shuffle is compute_pgen_perm)
+    threads: int = 12  # -n/--threads (src/Cmd.hpp:31); the .pgen -m shuffle uses min(threads, cores)
+    hardcall: bool = False  # --hardcall (src/Cmd.cpp:78)
 
     def __post_init__(self):
         self.oversamples = max(self.oversamples, self.k)  # src/Cmd.cpp:216
@@ -166,8 +187,15 @@ class Data:
         order = None
         if params.out_of_core and params.perm:
             # src/Main.cpp:120-151: the file is rewritten in permute_plink order
-            # before it is read, so F is estimated in that order
-            order = permute_plink_order(n, codes.shape[0], params.buffer, params.bands)
+            # before it is read, so F is estimated in that order; a .pgen is read
+            # in compute_pgen_perm's logical order instead (src/Main.cpp:168-172,
+            # F(snp_idx) of the logical index, src/FilePgen.cpp:229-234)
+            if params.pgen:
+                plan = block_plan(params, n, codes.shape[0])
+                order = pgen_perm_order(codes.shape[0], params.bands, plan.blocksize,
+                                        min(params.threads, os.cpu_count() or 1), params.seed)
+            else:
+                order = permute_plink_order(n, codes.shape[0], params.buffer, params.bands)
             codes = codes[order]
         F = allele_freq(codes)
         keep = None
@@ -266,6 +294,43 @@ def block_plan(params: Params, nsamples: int, nsnps: int) -> BlockPlan:
     start = [i * bs for i in range(nblocks)]  # src/Data.cpp:79-83
     stop = [min(s + bs - 1, m - 1) for s in start]
     return BlockPlan(bs, nblocks, factor, start, stop)
+
+
+def pgen_perm_order(nsnps: int, nbatches: int, blocksize: int, nthreads: int, seed: int) -> np.ndarray:
+    """compute_pgen_perm (src/FilePgen.cpp:91-151): each batch of the SNP range
+    is cut into blocks; a block's SNPs are split over the reader threads (the
+    first len % active threads take one more); thread t reads a contiguous
+    source partition of the file, shuffled (one engine seeded with --seed,
+    partitions in thread order), and hands out its indices in turn."""
+    if nsnps == 0:
+        return np.zeros(0, dtype=np.int64)
+    nbatches = max(1, min(nbatches, nsnps))
+    blocksize = max(1, blocksize)
+    nthreads = max(1, min(nthreads, min(blocksize, nsnps)))
+
+    def blocks():
+        for b in range(nbatches):
+            bs, be = b * nsnps // nbatches, (b + 1) * nsnps // nbatches
+            for s0 in range(bs, be, blocksize):
+                ln = min(blocksize, be - s0)
+                act = min(nthreads, ln)
+                yield s0, [ln // act + (1 if t < ln % act else 0) for t in range(act)]
+
+    count = [0] * nthreads
+    for _, lens in blocks():
+        for t, ln in enumerate(lens):
+            count[t] += ln
+    starts = np.cumsum([0] + count)
+    parts = shuffle_parts(seed, [np.arange(starts[t], starts[t + 1]) for t in range(nthreads)])
+    nxt = [0] * nthreads
+    order = np.full(nsnps, -1, dtype=np.int64)
+    for s0, lens in blocks():
+        off = 0
+        for t, ln in enumerate(lens):
+            order[s0 + off:s0 + off + ln] = parts[t][nxt[t]:nxt[t] + ln]
+            nxt[t] += ln
+            off += ln
+    return order
 
 
 def permute_plink_order(nsamples: int, nsnps: int, gb: int, nbands: int) -> np.ndarray:

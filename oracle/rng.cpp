@@ -7,6 +7,8 @@
 //            order, one draw per element).
 //   shuffle: src/RSVD.hpp:61-70 (std::shuffle of 0..M-1 with a default-seeded
 //            std::default_random_engine).
+//   parts:   src/FilePgen.cpp:124-126 (one engine seeded with --seed shuffles
+//            consecutive partitions in turn).
 #include <algorithm>
 #include <numeric>
 #include <random>
@@ -30,6 +32,15 @@ void oracle_shuffle(long m, int* out) {
   std::iota(out, out + m, 0);
   auto rng = std::default_random_engine{};
   std::shuffle(out, out + m, rng);
+}
+
+void oracle_shuffle_parts(int seed, long nparts, const long* sizes, int* data) {
+  auto rng = std::default_random_engine{};
+  rng.seed(seed);
+  for (long p = 0; p < nparts; ++p) {
+    std::shuffle(data, data + sizes[p], rng);
+    data += sizes[p];
+  }
 }
 
 }  // extern "C"

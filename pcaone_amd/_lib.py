@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("wm_g", C.c_int), ("wm_h", C.c_int), ("waves_g", C.c_int),
                 ("waves_h", C.c_int), ("kchunk_g", C.c_uint64),
                 ("kchunk_h", C.c_uint64), ("memory", C.c_double), ("buffer", C.c_uint32), ("ld", C.c_int),
-                ("ld_stats", C.c_int), ("imaxiter", C.c_uint32), ("itol", C.c_double), ("stream", C.c_int)]
+                ("ld_stats", C.c_int), ("imaxiter", C.c_uint32), ("itol", C.c_double), ("stream", C.c_int),
+                ("hardcall", C.c_int), ("threads", C.c_uint32)]
 
 
 class RunInfo(C.Structure):
@@ -116,6 +117,12 @@ _SIGS = {
     "pcaone_stream_bed": (C.c_int, [_P, C.c_char_p, C.c_uint64, _P, _P]),
     "pcaone_stream_host": (C.c_int, [_P, _P, C.c_uint32, C.c_uint64, C.c_uint64]),
     "pcaone_packed_gemm_noalloc": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.c_uint32, _P]),
+    "pcaone_pgen_dims": (C.c_int, [C.c_char_p, _P, _P]),
+    "pcaone_pgen_read": (C.c_int, [_P, C.c_char_p, C.c_double, C.c_int, _P, _P]),
+    "pcaone_pgen_decode": (C.c_int, [C.c_char_p, C.c_int, C.c_uint64, C.c_uint64, _P]),
+    "pcaone_stream_pgen": (C.c_int, [_P, C.c_char_p, C.c_int, C.c_uint64, _P, _P]),
+    "pcaone_run_pgen": (C.c_int, [_P, C.c_char_p, C.c_char_p, _P, _P]),
+    "pcaone_pgen_perm_order": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int, _P]),
 }
 
 

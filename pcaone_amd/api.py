@@ -34,6 +34,28 @@ def default_params(**kw) -> Params:
     return p
 
 
+def pgen_dims(prefix: str):
+    n, m = C.c_uint32(), C.c_uint64()
+    check(lib().pcaone_pgen_dims(str(prefix).encode(), C.byref(n), C.byref(m)))
+    return n.value, m.value
+
+
+def pgen_decode(prefix: str, j0: int = 0, j1: int | None = None, hardcall: bool = False) -> np.ndarray:
+    """Variants [j0, j1) of <prefix>.pgen as .bed rows ((j1 - j0) x ceil(N/4)
+    bytes), decoded on the host."""
+    n, m = pgen_dims(prefix)
+    j1 = m if j1 is None else j1
+    out = np.empty((max(j1 - j0, 0), (n + 3) // 4), dtype=np.uint8)
+    check(lib().pcaone_pgen_decode(str(prefix).encode(), int(hardcall), j0, j1, _ptr(out)))
+    return out
+
+
+def pgen_perm_order(nsnps: int, nbatches: int, blocksize: int, nthreads: int, seed: int = 112) -> np.ndarray:
+    out = np.empty(max(nsnps, 1), dtype=np.int32)
+    check(lib().pcaone_pgen_perm_order(nsnps, nbatches, blocksize, nthreads, seed, _ptr(out)))
+    return out[:nsnps]
+
+
 def device_count() -> int:
     n = C.c_int(0)
     check(lib().pcaone_device_count(C.byref(n)))
@@ -264,6 +286,28 @@ class Context:
         self._check(lib().pcaone_run_bfile(self._h, str(bfile).encode(), str(out).encode(), C.byref(p),
                                            C.byref(info)))
         return {f: getattr(info, f) for f, _ in RunInfo._fields_}
+
+    def run_pgen(self, pfile: str, out: str, params: Params | None = None, **kw) -> dict:
+        """The --pgen path: read <pfile>.pgen/.pvar/.psam, run, write the files."""
+        p = params if params is not None else default_params(**kw)
+        info = RunInfo()
+        self._check(lib().pcaone_run_pgen(self._h, str(pfile).encode(), str(out).encode(), C.byref(p),
+                                          C.byref(info)))
+        return {f: getattr(info, f) for f, _ in RunInfo._fields_}
+
+    def read_pgen(self, prefix: str, maf: float = 0.0, hardcall: bool = False):
+        n, m = C.c_uint32(), C.c_uint64()
+        self._check(lib().pcaone_pgen_read(self._h, str(prefix).encode(), maf, int(hardcall), C.byref(n),
+                                           C.byref(m)))
+        return n.value, m.value
+
+    def stream_pgen(self, prefix: str, block_snps: int, hardcall: bool = False):
+        """Streamed genotypes from <prefix>.pgen (records decoded at every block load)."""
+        n, m = C.c_uint32(), C.c_uint64()
+        self._check(lib().pcaone_stream_pgen(self._h, str(prefix).encode(), int(hardcall), block_snps, C.byref(n),
+                                             C.byref(m)))
+        self._stream_rows = None
+        return n.value, m.value
 
     def stream_bed(self, prefix: str, block_snps: int):
         """Streamed genotypes from <prefix>.bed (mapped; blocks loaded per product)."""

@@ -10,6 +10,7 @@
 #include "kernels.hpp"
 #include "linalg_host.hpp"
 #include "noalloc.hpp"
+#include "pgen_io.hpp"
 #include "plink_io.hpp"
 #include "rng_host.hpp"
 #include "rsvd.hpp"
@@ -121,6 +122,9 @@ Param to_param(const pcaone_params* p) {
   q.imaxiter = p->imaxiter;
   q.itol = p->itol;
   q.stream = p->stream != 0;
+  q.hardcall = p->hardcall != 0;
+  q.threads = p->threads;
+  require(q.threads >= 1, kErrArgument, "threads must be >= 1");
   require(q.k >= 1, kErrArgument, "k must be >= 1");
   require(q.ploidy == 1 || q.ploidy == 2, kErrArgument, "ploidy must be 1 or 2");
   q.derive();
@@ -192,6 +196,7 @@ void pcaone_params_default(pcaone_params* p) {
   p->ploidy = 2;
   p->precision = PCAONE_PREC_FP32;
   p->buffer = 2;
+  p->threads = 12;
   p->imaxiter = 1000;
   p->itol = 1e-6;
 }
@@ -215,6 +220,52 @@ int pcaone_bed_dims(const char* prefix, uint32_t* nsamples, uint64_t* nsnps) {
     const PlinkInfo pi = plink_dims(prefix);
     if (nsamples) *nsamples = (uint32_t)pi.nsamples;
     if (nsnps) *nsnps = pi.nsnps_file;
+  });
+}
+
+int pcaone_pgen_dims(const char* prefix, uint32_t* nsamples, uint64_t* nsnps) {
+  return guard_global([&] {
+    require(prefix != nullptr, kErrArgument, "null prefix");
+    const PlinkInfo pi = pgen_dims(prefix);
+    if (nsamples) *nsamples = (uint32_t)pi.nsamples;
+    if (nsnps) *nsnps = pi.nsnps_file;
+  });
+}
+
+int pcaone_pgen_read(pcaone_ctx* c, const char* prefix, double maf, int hardcall, uint32_t* nsamples,
+                     uint64_t* nsnps) {
+  return guard(c, [&](Context& x) {
+    require(prefix != nullptr, kErrArgument, "null prefix");
+    Param p;
+    p.maf = maf;
+    p.svd = 1;
+    p.hardcall = hardcall != 0;
+    p.derive();
+    read_pgen(x, prefix, p);
+    if (nsamples) *nsamples = x.N;
+    if (nsnps) *nsnps = x.M;
+  });
+}
+
+int pcaone_pgen_decode(const char* prefix, int hardcall, uint64_t snp_begin, uint64_t snp_end, uint8_t* rows) {
+  return guard_global([&] {
+    require(prefix != nullptr && rows != nullptr, kErrArgument, "null argument");
+    const uint64_t n = psam_samples(std::string(prefix) + ".psam");
+    const PgenSource src(std::string(prefix) + ".pgen", n, hardcall != 0);
+    require(snp_begin <= snp_end && snp_end <= src.nsnps(), kErrArgument, "pgen_decode: SNP range out of bounds");
+    src.decode_rows(snp_begin, snp_end - snp_begin, nullptr, rows);
+  });
+}
+
+int pcaone_stream_pgen(pcaone_ctx* c, const char* prefix, int hardcall, uint64_t block_snps, uint32_t* nsamples,
+                       uint64_t* nsnps) {
+  return guard(c, [&](Context& x) {
+    require(prefix != nullptr, kErrArgument, "null prefix");
+    Param p;
+    p.hardcall = hardcall != 0;
+    stream_pgen(x, prefix, p, block_snps);
+    if (nsamples) *nsamples = x.N;
+    if (nsnps) *nsnps = x.M;
   });
 }
 
@@ -532,6 +583,30 @@ int pcaone_run_bfile(pcaone_ctx* c, const char* bfile, const char* out, const pc
   });
 }
 
+int pcaone_run_pgen(pcaone_ctx* c, const char* pfile, const char* out, const pcaone_params* p,
+                    pcaone_run_info* info) {
+  return guard(c, [&](Context& x) {
+    require(pfile && out, kErrArgument, "null path");
+    const Param q = to_param(p);
+    PlinkInfo pi;
+    if (q.stream) {
+      // the -m plan's blocks are decoded from the mapped .pgen on every pass
+      require(q.out_of_core, kErrArgument, "stream needs memory > 0 (the -m plan sets the blocks)");
+      pi = pgen_dims(pfile);
+      const BlockPlan bp = plan_blocks(q, pi.nsamples, pi.nsnps_file);
+      pi = stream_pgen(x, pfile, q, bp.blocksize);
+    } else {
+      pi = read_pgen(x, pfile, q);
+    }
+    c->last = RunInfo{};
+    x.ktimes = Context::KernelTimes{};
+    const PcaResult r = run_pca(x, q, &c->last);
+    if (q.ld && q.svd != 3) write_residuals(x, r, q, out);  // src/Halko.cpp:338, src/Arnoldi.cpp:117
+    write_outputs(x, r, q, pfile, out, pi);
+    fill_info(info, c->last, x, (int)r.S.size());
+  });
+}
+
 int pcaone_residuals(pcaone_ctx* c, uint32_t k, const double* U, const double* S, const double* V, int as_read,
                      uint64_t snp_begin, uint64_t snp_end, float* out) {
   return guard(c, [&](Context& x) {
@@ -671,6 +746,15 @@ int pcaone_permute_plink_order(uint64_t nsamples, uint64_t nsnps, uint32_t buffe
     require(order != nullptr, kErrArgument, "null output");
     const std::vector<int> o = permute_plink_order(nsamples, nsnps, buffer_gb, bands);
     std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
+  });
+}
+
+int pcaone_pgen_perm_order(uint64_t nsnps, uint32_t nbatches, uint64_t blocksize, uint32_t nthreads, int seed,
+                           int32_t* order) {
+  return guard_global([&] {
+    require(order != nullptr || nsnps == 0, kErrArgument, "null output");
+    const std::vector<int> o = pgen_perm_order(nsnps, nbatches, blocksize, nthreads, seed);
+    if (!o.empty()) std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
   });
 }
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,7 @@ namespace pcaone {
 namespace synth {
 struct Spec;
 }
+class PgenSource;
 
 // Synthetic genotypes of `spec` straight into interleaved rows (row j = SNP j,
 // `pitch` words per row) on `stream`, byte-identical to the host generator.
@@ -71,7 +73,9 @@ class Context {
   // Opening streams every block once to count the codes (F, missing entries).
   void stream_host(const uint8_t* rows, uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
   void stream_bed(const std::string& bed, uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
-  bool streamed() const { return src_rows_ != nullptr; }
+  // The same from a .pgen: every block load decodes its records again.
+  void stream_pgen(std::shared_ptr<const PgenSource> src, uint64_t block_snps);
+  bool streamed() const { return src_rows_ != nullptr || pgen_ != nullptr; }
   uint64_t stream_block_snps() const { return sblock_; }
   uint64_t stream_loads = 0;   // blocks moved host -> device
   double stream_load_ms = 0.0; // host time spent loading them
@@ -104,6 +108,7 @@ class Context {
   uint32_t N = 0;
   uint64_t M = 0;
   uint64_t nmiss = 0;
+  bool pgen_input = false;     // read from a .pgen: the -m shuffle is compute_pgen_perm's order
   std::vector<double> F;       // unpermuted allele frequencies (after filtering)
   std::vector<int> perm;       // device column i holds SNP perm[i]
   double p_miss() const { return N && M ? (double)nmiss / ((double)N * (double)M) : 0.0; }
@@ -191,6 +196,7 @@ class Context {
   // streamed genotypes
   const uint8_t* src_rows_ = nullptr;  // packed rows in file order (host)
   void* map_ = nullptr;                // mapped .bed file (stream_bed)
+  std::shared_ptr<const PgenSource> pgen_;  // .pgen records decoded per block (stream_pgen)
   size_t map_len_ = 0;
   uint64_t cus_ = 0;                   // compute units of the device (split-K planning)
   uint64_t sblock_ = 0;                // device block capacity in SNP rows
@@ -202,7 +208,7 @@ class Context {
   void load_block(uint64_t j0, uint64_t j1);
   void block_rows_to_device(uint64_t j0, uint64_t n);  // rows of the current order -> raw_ (device)
   void build_lists(uint64_t rows, uint64_t ncodes_t);  // missing lists of the loaded layouts
-  uint64_t xoff() const { return src_rows_ ? blk0_ : 0; }
+  uint64_t xoff() const { return streamed() ? blk0_ : 0; }
   // data layout
   uint64_t bps_ = 0;
   uint64_t ks_ = 0, kt_ = 0;           // K extents (codes) of xs and xt

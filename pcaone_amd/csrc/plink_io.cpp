@@ -3,9 +3,14 @@
 //                     src/FilePlink.cpp:26-120): the .bed payload goes to HBM
 //                     undecoded; allele frequencies are counted on the device.
 //   MAF filter        Data::filter_snps_resize_F (src/Data.cpp:88-105)
+//   read_pgen         FilePgen::FilePgen + read_all (src/FilePgen.hpp:14-35,
+//                     src/FilePgen.cpp:27-89): hardcalls decoded on the host
+//                     (pgen_io.cpp) into .bed rows, then the same upload
 //   write_outputs     Data::write_eigs_files (src/Data.cpp:211-240),
-//                     save_snps_in_mbim (src/Data.cpp:110-170, in-core PLINK),
-//                     make_plink2_eigenvec_file (src/Utils.cpp:225-238)
+//                     save_snps_in_mbim (src/Data.cpp:110-170; .pvar lines
+//                     as .bim lines, src/Utils.cpp:405-430),
+//                     make_plink2_eigenvec_file (src/Utils.cpp:225-238) and
+//                     make_plink2_eigenvec_from_psam (src/Utils.cpp:240-265)
 #include "plink_io.hpp"
 
 #include <fstream>
@@ -13,6 +18,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pgen_io.hpp"
 
 namespace pcaone {
 
@@ -65,6 +71,44 @@ std::vector<std::string> split_ws(const std::string& s) {
   return out;
 }
 
+std::vector<std::string> tokens(const std::string& s) {
+  // split on runs of blanks and tabs
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == ' ' || ch == '\t') {
+      if (!cur.empty()) out.push_back(cur);
+      cur.clear();
+    } else {
+      cur += ch;
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+// pvar_line_to_bim_line (src/Utils.cpp:405-430): CHROM ID 0 POS ALT(first) REF
+std::string pvar_to_bim(const std::string& line, const std::string& path) {
+  const auto t = tokens(line);
+  require(t.size() >= 5, kErrIO, "the input pvar file is not valid!\n => " + path + "\n" + line);
+  const std::string alt = t[4].substr(0, t[4].find(','));
+  return t[0] + "\t" + t[2] + "\t0\t" + t[1] + "\t" + alt + "\t" + t[3];
+}
+
+void filter_by_maf(Context& ctx, const Param& params, PlinkInfo& info) {
+  // Data::filter_snps_resize_F (src/Data.cpp:88-105)
+  if (!params.filterSNP) return;
+  std::vector<uint32_t> keep;
+  for (uint64_t j = 0; j < ctx.M; ++j) {
+    const double f = ctx.F[j];
+    const double maf = f > 0.5 ? 1 - f : f;  // MAF(a) (src/Common.hpp:26)
+    if (maf > params.maf) keep.push_back((uint32_t)j);
+  }
+  require(!keep.empty(), kErrArgument, "no SNPs left after filtering!");
+  ctx.filter_snps(keep);
+  info.keep = keep;
+}
+
 }  // namespace
 
 PlinkInfo plink_dims(const std::string& prefix) {
@@ -100,17 +144,49 @@ PlinkInfo read_plink(Context& ctx, const std::string& prefix, const Param& param
     ctx.upload_rows(buf.data(), j0, m);
   }
   ctx.finalize();
-  if (params.filterSNP) {
-    std::vector<uint32_t> keep;
-    for (uint64_t j = 0; j < ctx.M; ++j) {
-      const double f = ctx.F[j];
-      const double maf = f > 0.5 ? 1 - f : f;  // MAF(a) (src/Common.hpp:26)
-      if (maf > params.maf) keep.push_back((uint32_t)j);
-    }
-    require(!keep.empty(), kErrArgument, "no SNPs left after filtering!");
-    ctx.filter_snps(keep);
-    info.keep = keep;
+  filter_by_maf(ctx, params, info);
+  return info;
+}
+
+PlinkInfo pgen_dims(const std::string& prefix) {
+  PlinkInfo info;
+  info.pgen = true;
+  info.nsamples = psam_samples(prefix + ".psam");
+  require(info.nsamples > 0, kErrIO, "no samples in " + prefix + ".psam");
+  info.nsnps_file = PgenSource(prefix + ".pgen", info.nsamples, true).nsnps();
+  return info;
+}
+
+PlinkInfo read_pgen(Context& ctx, const std::string& prefix, const Param& params) {
+  PlinkInfo info;
+  info.pgen = true;
+  info.nsamples = psam_samples(prefix + ".psam");
+  require(info.nsamples > 0, kErrIO, "no samples in " + prefix + ".psam");
+  const PgenSource src(prefix + ".pgen", info.nsamples, params.hardcall);
+  info.nsnps_file = src.nsnps();
+  const uint64_t bps = bytes_per_snp(info.nsamples);
+  ctx.reserve((uint32_t)info.nsamples, info.nsnps_file);
+  const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / bps);
+  std::vector<uint8_t> buf(std::min(chunk, info.nsnps_file) * bps);
+  for (uint64_t j0 = 0; j0 < info.nsnps_file; j0 += chunk) {
+    const uint64_t m = std::min(chunk, info.nsnps_file - j0);
+    src.decode_rows(j0, m, nullptr, buf.data());
+    ctx.upload_rows(buf.data(), j0, m);
   }
+  ctx.finalize();
+  ctx.pgen_input = true;
+  filter_by_maf(ctx, params, info);
+  return info;
+}
+
+PlinkInfo stream_pgen(Context& ctx, const std::string& prefix, const Param& params, uint64_t block_snps) {
+  PlinkInfo info;
+  info.pgen = true;
+  info.nsamples = psam_samples(prefix + ".psam");
+  require(info.nsamples > 0, kErrIO, "no samples in " + prefix + ".psam");
+  auto src = std::make_shared<const PgenSource>(prefix + ".pgen", info.nsamples, params.hardcall);
+  info.nsnps_file = src->nsnps();
+  ctx.stream_pgen(std::move(src), block_snps);
   return info;
 }
 
@@ -162,13 +238,18 @@ void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, 
   }
   write_matrix(out + ".eigvecs", r.U, N, k);
   if (params.printv || params.ld) {
-    // .mbim: .bim line + allele frequency, kept SNPs only
-    std::ifstream bim(bfile + ".bim");
+    // .mbim: .bim line (a .pvar line as a .bim line) + allele frequency, kept SNPs only
+    const std::string meta = bfile + (info.pgen ? ".pvar" : ".bim");
+    std::ifstream bim(meta);
     std::ofstream mb(out + ".mbim");
     if (bim.is_open() && mb.is_open()) {
       std::string line;
       uint64_t src = 0, kept = 0;
       while (std::getline(bim, line) && kept < M) {
+        if (info.pgen) {
+          if (!line.empty() && line[0] == '#') continue;
+          line = pvar_to_bim(line, meta);
+        }
         if (info.keep.empty() || info.keep[kept] == src) {
           mb << line << '\t' << fmt(ctx.F[kept]) << '\n';
           ++kept;
@@ -179,13 +260,36 @@ void write_outputs(const Context& ctx, const PcaResult& r, const Param& params, 
     if (params.printv) write_matrix(out + ".loadings", r.V, M, k);
   }
   // .eigvecs2: "#FID IID PC1.." + the .eigvecs rows
-  std::ifstream fam(bfile + ".fam");
   std::ifstream ev(out + ".eigvecs");
   std::ofstream o2(out + ".eigvecs2");
   o2 << "#FID\tIID";
   for (int i = 0; i < k; ++i) o2 << "\tPC" << i + 1;
   o2 << '\n';
   std::string l1, l2;
+  if (info.pgen) {
+    // src/Utils.cpp:240-265: '##' lines skipped up to the '#' column header;
+    // FID and IID when it starts '#FID', else FID 0 and the first column as IID
+    std::ifstream psam(bfile + ".psam");
+    std::string header;
+    while (std::getline(psam, header))
+      if (!header.empty() && header[0] == '#' && (header.size() < 2 || header[1] != '#')) break;
+    const bool has_fid = header.compare(0, 4, "#FID") == 0;
+    while (std::getline(psam, l1)) {
+      if (l1.empty() || l1[0] == '#') continue;
+      const auto tok = tokens(l1);
+      std::getline(ev, l2);
+      std::string fid = "0", iid;
+      if (has_fid && tok.size() >= 2) {
+        fid = tok[0];
+        iid = tok[1];
+      } else if (!tok.empty()) {
+        iid = tok[0];
+      }
+      o2 << fid << '\t' << iid << '\t' << l2 << '\n';
+    }
+    return;
+  }
+  std::ifstream fam(bfile + ".fam");
   while (std::getline(fam, l1)) {
     auto tok = split_ws(l1);
     std::getline(ev, l2);

@@ -9,9 +9,11 @@
 #include <memory>
 #include <numeric>
 #include <random>
+#include <thread>
 
 #include "common.hpp"
 #include "exact.hpp"
+#include "pgen_io.hpp"
 #include "rng_host.hpp"
 
 namespace pcaone {
@@ -328,8 +330,15 @@ PcaResult run_pca_with_halko(Context& ctx, const Param& params, RunInfo* info, d
     require(!ctx.streamed() || plan.blocksize <= ctx.stream_block_snps(), kErrArgument,
             "stream: the -m plan's blocks of " + std::to_string(plan.blocksize) +
                 " SNPs exceed the streamed block capacity of " + std::to_string(ctx.stream_block_snps()));
-    if (params.perm)
-      ctx.permute_columns(permute_plink_order(ctx.N, ctx.M, params.buffer, params.bands), /*with_freq=*/true);
+    if (params.perm) {
+      // a .pgen is not rewritten: its reader threads take a logical order
+      // (src/Main.cpp:169-172, max_threads of src/Main.cpp:53-54)
+      const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+      ctx.permute_columns(ctx.pgen_input ? pgen_perm_order(ctx.M, params.bands, plan.blocksize,
+                                                           std::min(params.threads, hw), params.seed)
+                                         : permute_plink_order(ctx.N, ctx.M, params.buffer, params.bands),
+                          /*with_freq=*/true);
+    }
   }
   const auto t0 = std::chrono::steady_clock::now();
   std::unique_ptr<RsvdOpData> rsvd;

@@ -29,6 +29,8 @@ struct Param {
   uint32_t imaxiter = 1000;  // --imaxiter (IRAM)
   double itol = 1e-6;        // --itol (IRAM)
   bool stream = false;       // run_bfile: stream the .bed blocks instead of loading the matrix
+  bool hardcall = false;     // --hardcall: the hardcall track of a .pgen that also stores dosages
+  uint32_t threads = 12;     // -n/--threads: the reader threads that shape a .pgen's -m shuffle
   uint32_t ncv = 20;         // derived: max(20, 2k + 1) (src/Cmd.cpp:215)
   // derived (src/Cmd.cpp:171-238)
   bool perm = false, missme = false, filterSNP = false, out_of_core = false;

@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
+#include "pgen_io.hpp"
 
 namespace pcaone {
 
@@ -28,6 +29,8 @@ void Context::stream_close() {
   map_ = nullptr;
   map_len_ = 0;
   src_rows_ = nullptr;
+  pgen_.reset();
+  pgen_input = false;
   sblock_ = 0;
   blk0_ = blk1_ = 0;
 }
@@ -130,6 +133,19 @@ void Context::stream_bed(const std::string& bed, uint32_t nsamples, uint64_t nsn
   }
 }
 
+void Context::stream_pgen(std::shared_ptr<const PgenSource> src, uint64_t block_snps) {
+  require(src != nullptr, kErrArgument, "stream: null .pgen source");
+  stream_close();
+  pgen_ = std::move(src);
+  try {
+    stream_setup((uint32_t)pgen_->nsamples(), pgen_->nsnps(), block_snps);
+    pgen_input = true;
+  } catch (...) {
+    stream_close();
+    throw;
+  }
+}
+
 void Context::block_rows_to_device(uint64_t j0, uint64_t n) {
   // rows j0..j0+n of the current column order (file rows perm[j]) -> raw_
   const auto t0 = std::chrono::steady_clock::now();
@@ -138,10 +154,14 @@ void Context::block_rows_to_device(uint64_t j0, uint64_t n) {
   raw_.resize(std::max<uint64_t>(bytes, 1));
   unsigned char* host = pinned_.get();
   const bool ident = perm_identity_;
+  if (pgen_) {
+    pgen_->decode_rows(j0, n, ident ? nullptr : perm.data(), host);
+  } else {
 #pragma omp parallel for schedule(static)
-  for (int64_t r = 0; r < (int64_t)n; ++r) {
-    const uint64_t src = ident ? j0 + (uint64_t)r : (uint64_t)perm[j0 + (uint64_t)r];
-    std::memcpy(host + (uint64_t)r * bps_, src_rows_ + src * bps_, bps_);
+    for (int64_t r = 0; r < (int64_t)n; ++r) {
+      const uint64_t src = ident ? j0 + (uint64_t)r : (uint64_t)perm[j0 + (uint64_t)r];
+      std::memcpy(host + (uint64_t)r * bps_, src_rows_ + src * bps_, bps_);
+    }
   }
   PCAONE_HIP(hipMemcpyAsync(raw_.get(), host, bytes, hipMemcpyHostToDevice, stream));
   ++stream_loads;

@@ -55,6 +55,30 @@ def golden_inputs(tmp_path_factory):
     return out
 
 
+@pytest.fixture(scope="session")
+def golden_pgen(golden_inputs, tmp_path_factory):
+    """case -> prefix of the .pgen/.pvar/.psam set the reference read for that
+    golden case, written again from the committed .bed by pgen_writer and
+    checked against the recorded SHA-256."""
+    import hashlib
+    sys.path.insert(0, str(GOLDEN))
+    import pgen_writer
+    tmp = tmp_path_factory.mktemp("golden_pgen")
+    made = {}
+
+    def get(case):
+        if case not in made:
+            meta = load_meta(case)
+            opts = {k: v for k, v in meta["pgen"].items() if k != "sha256"}
+            prefix = str(tmp / case)
+            pgen_writer.bed_to_pgen(golden_inputs[meta["input"]], prefix, **opts)
+            digest = hashlib.sha256(Path(prefix + ".pgen").read_bytes()).hexdigest()
+            assert digest == meta["pgen"]["sha256"], f"{case}: .pgen does not regenerate bit-identically"
+            made[case] = prefix
+        return made[case]
+    return get
+
+
 def params_from_args(args: str, **extra):
     """Reference CLI flags (src/Cmd.cpp) -> oracle Params."""
     from oracle.pcaone_oracle import Params
@@ -65,7 +89,7 @@ def params_from_args(args: str, **extra):
              "--seed": ("seed", int), "--tol-rsvd": ("tol", float), "--maf": ("maf", float), "-w": ("bands", int),
              "--maxiter": ("maxiter", int), "--oversamples": ("oversamples", int), "--tol-em": ("tolem", float),
              "--ld-stats": ("ld_stats", int),
-             "-m": ("memory", float), "--buffer": ("buffer", int)}
+             "-m": ("memory", float), "--buffer": ("buffer", int), "-n": ("threads", int)}
     printv = False
     while i < len(toks):
         t = toks[i]
@@ -82,6 +106,8 @@ def params_from_args(args: str, **extra):
             kw["noshuffle"] = True
         elif t == "--emu":
             kw["emu"] = True
+        elif t == "--hardcall":
+            kw["hardcall"] = True
         else:
             raise ValueError(f"unknown flag {t}")
         i += 1

@@ -27,6 +27,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parents[1]
 sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(HERE))
 
 import numpy as np  # noqa: E402
 
@@ -53,6 +54,11 @@ INPUTS = {
     "f8": dict(spec=synth.Spec(seed=2008, nsamples=256, nsnps=150, npops=4), commit=True),
     # 1,008 SNPs in 64 windows of 16: window 63 starts at SNP 1,008 = M, an empty window
     "f9": dict(spec=synth.Spec(seed=7, nsamples=150, nsnps=1008, npops=3), commit=True),
+    # LD structure for the .pgen record types: SNP 2i + 1 repeats SNP 2i with
+    # 0 / 5 / 40 / 80 entries changed, every third pair with ALT and REF
+    # swapped; N = 700 (two-byte sample indices, difference lists of two groups)
+    "f10": dict(spec=synth.Spec(seed=2010, nsamples=700, nsnps=1200, npops=3, miss_rate=0.03), commit=True,
+                edit="ld"),
 }
 
 
@@ -88,6 +94,31 @@ def edge_rows(rows, n):
     fill(41, [1] * 5 + [0] + [1] * (n - 6))
     fill(100, [2] * n)
     return rows
+
+def ld_rows(rows, n, seed=10):
+    """Packed rows -> rows whose odd SNPs copy the SNP before them with a few
+    entries redrawn (and ALT/REF swapped for every third pair), plus SNPs that
+    are homozygous everywhere or missing nearly everywhere."""
+    import pgen_writer as W
+    rng = np.random.default_rng(seed)
+    g = W.bed_to_geno(rows, n)
+    for i in range(g.shape[0] // 2):
+        a = g[2 * i].copy()
+        c = (0, 5, 40, 80)[i % 4]
+        idx = rng.choice(n, size=c, replace=False)
+        a[idx] = (a[idx] + 1 + rng.integers(0, 3, size=c)) % 4
+        g[2 * i + 1] = W.invert(a) if i % 3 == 1 else a
+    g[20] = 0
+    g[21] = 2
+    g[22] = 3
+    g[22, :30] = 1
+    g[23] = 2
+    g[23, rng.choice(n, size=70, replace=False)] = 0
+    g[26] = 3
+    g[26, rng.choice(n, size=12, replace=False)] = 2
+    inv = np.array([3, 2, 0, 1], dtype=np.uint8)  # ALT count -> .bed code
+    return np.stack([np.frombuffer(W.pack2(inv[r]), dtype=np.uint8) for r in g])
+
 
 CASES = {
     "f1_d1": ("f1", "-d 1 -k 5 --maxp 7 -V"),
@@ -137,6 +168,19 @@ CASES = {
     "f7_d0_ld": ("f7", "-d 0 -k 3 -D"),
     # winSVD whose last window is empty (src/Halko.cpp:191-194 with a zero width)
     "f9_d2_empty": ("f9", "-d 2 -w 64 -k 4 --maxp 7 --tol-rsvd 0 -V"),
+    # PLINK 2 input (-p/--pgen, src/FilePgen.cpp): the .pgen written from the
+    # input by pgen_writer with the options given (not committed; its SHA-256
+    # is recorded, and the tests write it again from the committed .bed)
+    "f10_pg_d1": ("f10", "-d 1 -k 5 --maxp 7 -V", dict(mode="var", policy="cycle")),
+    "f10_pg_d2": ("f10", "-d 2 -w 16 -k 5 --maxp 7 -V",
+                  dict(mode="var", policy="plink2", nonref=3, len_bytes=2, fid=False)),
+    "f1_pg_ooc": ("f1", "-d 2 -w 16 -k 5 --maxp 7 -V -m 0.00068 -n 4", dict(mode="var", policy="cycle")),
+    "f1_pg_p1": ("f1", "-d 1 -k 5 --maxp 7 -V", dict(mode="plink1")),
+    "f1_pg_dos": ("f1", "-d 1 -k 5 --maxp 7 --hardcall -V", dict(mode="var", policy="cycle", dosage=True,
+                                                                   len_bytes=3)),
+    "f4_pg_emu": ("f4", "-d 1 -k 3 --emu --maxiter 5 -V", dict(mode="fixed", fid=False)),
+    "f4_pg_emu_ooc": ("f4", "-d 2 -w 16 -k 3 --emu --maxiter 5 -V -m 0.0009 -n 4", dict(mode="var", policy="plink2")),
+    "f5_pg_maf": ("f5", "-d 1 -k 4 --maf 0.02 --maxp 5 -V", dict(mode="var", policy="plink2")),
 }
 
 EPOCH_RE = re.compile(r"running of epoch =\s*(\d+), diff = ([-0-9.e+]+)")
@@ -162,6 +206,11 @@ def make_inputs(tmp: Path) -> dict:
             raw = bed.read_bytes()
             rows = np.frombuffer(raw[3:], dtype=np.uint8).reshape(spec.nsnps, -1).copy()
             bed.write_bytes(raw[:3] + edge_rows(rows, spec.nsamples).tobytes())
+        if d.get("edit") == "ld":
+            bed = Path(str(prefix) + ".bed")
+            raw = bed.read_bytes()
+            rows = np.frombuffer(raw[3:], dtype=np.uint8).reshape(spec.nsnps, -1)
+            bed.write_bytes(raw[:3] + ld_rows(rows, spec.nsamples).tobytes())
         if d.get("edit", "").startswith("dup"):
             reps = int(d["edit"][3:])
             bed = Path(str(prefix) + ".bed")
@@ -175,15 +224,26 @@ def make_inputs(tmp: Path) -> dict:
     return out
 
 
-def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
+def run_case(name: str, inp: dict, args: str, tmp: Path, pgen: dict | None = None) -> None:
     dest = HERE / name
     if dest.exists():
         shutil.rmtree(dest)
     dest.mkdir()
     outp = tmp / name
-    full = "-d 0" in args or "-d 3" in args
-    cmd = [str(REF_FULL if full else REF), "-b", inp["prefix"], *args.split(), "-o", str(outp), "-v", "1"]
+    full = "-d 0" in args or "-d 3" in args or pgen is not None
+    if pgen is not None:
+        import pgen_writer as W
+        pin = str(tmp / (name + "_in"))
+        W.bed_to_pgen(inp["prefix"], pin, **pgen)
+        cmd = [str(REF_FULL), "-p", pin, *args.split(), "-o", str(outp), "-v", "1"]
+    else:
+        cmd = [str(REF_FULL if full else REF), "-b", inp["prefix"], *args.split(), "-o", str(outp), "-v", "1"]
     env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    if "-n " in args:
+        # FilePgen keeps one reader state per -n thread and indexes them by the
+        # OpenMP thread number (src/FilePgen.hpp:30-36, src/FilePgen.cpp:165-168),
+        # so the OpenMP team must not be larger than -n
+        env["OMP_NUM_THREADS"] = args.split("-n ")[1].split()[0]
     res = subprocess.run(cmd, capture_output=True, text=True, env=env, check=True)
     log = res.stdout
     trace = [float(m.group(2)) for m in EPOCH_RE.finditer(log)]
@@ -198,6 +258,8 @@ def run_case(name: str, inp: dict, args: str, tmp: Path) -> None:
         if src.exists():
             shutil.copy(src, dest / suf[1:])
     meta = dict(input=inp_name_of(inp), args=args, trace=trace, stop_epochs=[int(s) for s in stop], em_trace=em)
+    if pgen is not None:
+        meta["pgen"] = dict(pgen, sha256=sha256(Path(pin + ".pgen")))
     if plan:
         meta["block_plan"] = plan[-1]  # [blocksize, nblocks, factor] after any winSVD adjustment
     (dest / "meta.json").write_text(json.dumps(meta, indent=1) + "\n")
@@ -222,9 +284,9 @@ def main() -> None:
         manifest = {n: dict(sha256=i["sha256"], spec=i["spec"], committed=i["committed"]) for n, i in inputs.items()}
         (HERE / "inputs.json").write_text(json.dumps(manifest, indent=1) + "\n")
         only = set(sys.argv[1:])  # optional: regenerate only the named cases
-        for name, (inp, args) in CASES.items():
+        for name, (inp, args, *pg) in CASES.items():
             if not only or name in only:
-                run_case(name, inputs[inp], args, tmp)
+                run_case(name, inputs[inp], args, tmp, pg[0] if pg else None)
 
 
 if __name__ == "__main__":

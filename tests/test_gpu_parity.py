@@ -244,22 +244,28 @@ def test_thin_q_paths_match_oracle(ctx, golden_inputs, k):
 
 
 # ---------------------------------------------------------------- whole runs vs golden
-def run_case(ctx, golden_inputs, case, form):
+def run_case(ctx, golden_inputs, case, form, golden_pgen=None):
+    """One golden case on the device: the .bed read, or for a --pgen case the
+    .pgen the reference read (its -m runs then take the .pgen shuffle)."""
     meta = load_meta(case)
-    params, printv = params_from_args(meta["args"])
+    params, printv = params_from_args(meta["args"], pgen="pgen" in meta)
     say(f"golden {case} form={form} args='{meta['args']}'")
-    ctx.read_bed(golden_inputs[meta["input"]], maf=params.maf)
+    if "pgen" in meta:
+        ctx.read_pgen(golden_pgen(case), maf=params.maf, hardcall=params.hardcall)
+    else:
+        ctx.read_bed(golden_inputs[meta["input"]], maf=params.maf)
     p = api.default_params(svd=params.svd, k=params.k, oversamples=params.oversamples, maxp=params.maxp,
                            tol=params.tol, seed=params.seed, rand=params.rand, bands=params.bands,
                            noshuffle=int(params.noshuffle), emu=int(params.emu), maxiter=params.maxiter,
                            tolem=params.tolem, memory=params.memory, buffer=params.buffer, ld=int(params.ld),
-                           ld_stats=params.ld_stats, precision=form)
+                           ld_stats=params.ld_stats, threads=params.threads, hardcall=int(params.hardcall),
+                           precision=form)
     return meta, params, printv, ctx.run(p, want_v=printv)
 
 
 @pytest.mark.parametrize("case", golden_cases())
-def test_golden_fp64(ctx, golden_inputs, case):
-    meta, params, printv, r = run_case(ctx, golden_inputs, case, "fp64")
+def test_golden_fp64(ctx, golden_inputs, golden_pgen, case):
+    meta, params, printv, r = run_case(ctx, golden_inputs, case, "fp64", golden_pgen)
     d = GOLDEN / case
     if meta["trace"]:
         np.testing.assert_allclose(r.trace, meta["trace"], atol=1.5e-6)
@@ -290,6 +296,8 @@ REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f5_d1", "f5_d2", "f5_emu
 REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f6_d1", "f6_d2")]
 # the out-of-core schedule (block windows, permute_plink order, per-pass imputation)
 REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f1_d2_ooc", "f4_emu_d2_ooc")]
+# .pgen input (every record type; the .pgen -m shuffle)
+REDUCED += [(c, f) for f in ("fp32", "bf16") for c in ("f10_pg_d1", "f1_pg_ooc")]
 # Single-piece forms round Omega (or G) to 8-11 bits: the eigenvalues meet the
 # bar, the unconverged trailing PCs move by up to ~1e-3 in correlation.
 PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "int8": 0.9999, "fp8": 0.9999, "fp16": 0.999, "bf16mix": 0.999,
@@ -297,8 +305,8 @@ PC_CORR_MIN = {"fp32": 0.9999, "bf16": 0.9999, "int8": 0.9999, "fp8": 0.9999, "f
 
 
 @pytest.mark.parametrize("case,form", REDUCED)
-def test_golden_reduced_precision(ctx, golden_inputs, case, form):
-    meta, params, printv, r = run_case(ctx, golden_inputs, case, form)
+def test_golden_reduced_precision(ctx, golden_inputs, golden_pgen, case, form):
+    meta, params, printv, r = run_case(ctx, golden_inputs, case, form, golden_pgen)
     ref = O.run_plink(golden_inputs[meta["input"]], params)
     rel = np.abs(r.E - ref.E) / ref.E
     corr = O.pc_correlations(ref.U, r.U)

@@ -131,8 +131,10 @@ def _outputs_match(case, res, data, printv, tol_vec=2e-6):
 
 @pytest.mark.parametrize("case", golden_cases())
 def test_oracle_matches_reference_binary(case, golden_inputs):
+    """The oracle on the golden input (for a .pgen case: the .bed it was
+    written from, read in the .pgen's -m order) against the reference binary."""
     meta = load_meta(case)
-    params, printv = params_from_args(meta["args"])
+    params, printv = params_from_args(meta["args"], pgen="pgen" in meta)
     packed, n, _ = O.read_plink(golden_inputs[meta["input"]])
     data = O.Data.from_packed(packed, n, params)
     res = O.run_pca(data)
@@ -250,6 +252,22 @@ def _plant_ooc_incore_shuffle(monkeypatch):
     monkeypatch.setattr(O, "permute_plink_order", lambda n, m, gb, w: O.shuffle_indices(m))
 
 
+def _plant_pgen_one_thread(monkeypatch):
+    orig = O.pgen_perm_order
+    monkeypatch.setattr(O, "pgen_perm_order", lambda m, nb, bs, nt, seed: orig(m, nb, bs, 1, seed))
+
+
+def _plant_pgen_default_seed(monkeypatch):
+    # the partitions shuffled by a default-seeded engine, as the in-core shuffle draws
+    orig = O.pgen_perm_order
+    monkeypatch.setattr(O, "pgen_perm_order", lambda m, nb, bs, nt, seed: orig(m, nb, bs, nt, 1))
+
+
+def _plant_pgen_plink_order(monkeypatch):
+    monkeypatch.setattr(O, "pgen_perm_order",
+                        lambda m, nb, bs, nt, seed: O.permute_plink_order(200, m, 2, nb))
+
+
 def _plant_ooc_no_flip(monkeypatch):
     monkeypatch.setattr(O, "flip_omg", lambda a, b: b)
 
@@ -283,6 +301,9 @@ def test_block_plan_matches_reference_log(case):
     ("f1_d2_ooc", _plant_ooc_band_factor_one),
     ("f1_d2_ooc", _plant_ooc_incore_shuffle),
     ("f1_d2_ooc", _plant_ooc_no_flip),
+    ("f1_pg_ooc", _plant_pgen_one_thread),
+    ("f1_pg_ooc", _plant_pgen_default_seed),
+    ("f1_pg_ooc", _plant_pgen_plink_order),
     ("f4_emu_d1_ooc", _plant_ooc_impute_unclamped),
     ("f1_d2", _plant_no_flip),
     ("f1_d1_p0", _plant_omega_row_major),
@@ -300,7 +321,7 @@ def test_block_plan_matches_reference_log(case):
 def test_planted_mistake_is_caught(case, plant, golden_inputs, monkeypatch):
     """A plausible mistake planted in the oracle must fail the golden comparison."""
     meta = load_meta(case)
-    params, printv = params_from_args(meta["args"])
+    params, printv = params_from_args(meta["args"], pgen="pgen" in meta)
     packed, n, _ = O.read_plink(golden_inputs[meta["input"]])
     plant(monkeypatch)
     data = O.Data.from_packed(packed, n, params)
