@@ -10,8 +10,8 @@
 //                       write_eigs_files (src/Halko.cpp:46-97, 271-345)
 //   PCAONE_AMD_PRECISION=<PCAONE_PREC_*> (default 0, FP64)
 //
-// Dispatch as src/Main.cpp:48-58, :168-179, :222-231, for -b input, in-core
-// --svd 1/2 (level 1 also takes -m).
+// Dispatch as src/Main.cpp:48-58, :168-179, :222-231, for -b or -p input,
+// in-core --svd 1/2 (level 1 also takes -m).
 #define _DECLARE_TOOLBOX_HERE
 
 #include <cstdlib>
@@ -28,7 +28,8 @@ int main(int argc, char* argv[]) {
   cao.cao.open(params.fileout + ".log");
   cao.is_screen = params.verbose > 0;
   cao.print(params.ss.str());
-  if (params.file_t != FileType::PLINK) cao.error("this driver only reads PLINK (-b) input");
+  if (params.file_t != FileType::PLINK && params.file_t != FileType::PGEN)
+    cao.error("this driver only reads PLINK (-b) or PLINK 2 (-p) input");
   if (params.svd_t != SvdType::PCAoneAlg1 && params.svd_t != SvdType::PCAoneAlg2)
     cao.error("this driver only runs --svd 1 or --svd 2");
   const char* lv = std::getenv("PCAONE_AMD_LEVEL");
@@ -42,13 +43,19 @@ int main(int argc, char* argv[]) {
     if (params.out_of_core) cao.error("level 2 of this driver runs in-core only");
     pcaone_ctx* ctx = nullptr;
     if (pcaone_open(0, &ctx) != PCAONE_OK) cao.error(pcaone_last_error(nullptr));
-    Data* data = new pcaone_amd_binding::AmdFileBed(params, ctx);
+    const bool pgen = params.file_t == FileType::PGEN;
+    Data* data = pgen ? (Data*)new pcaone_amd_binding::AmdFilePgen(params, ctx)
+                      : (Data*)new pcaone_amd_binding::AmdFileBed(params, ctx);
     data->prepare();
     pcaone_amd_binding::run_pca_with_halko_level2(data, params, ctx, precision);
     delete data;
     pcaone_close(ctx);
-    make_plink2_eigenvec_file(params.k, params.fileout + ".eigvecs2", params.fileout + ".eigvecs",
-                              params.filein + ".fam");
+    if (pgen)
+      make_plink2_eigenvec_from_psam(params.k, params.fileout + ".eigvecs2", params.fileout + ".eigvecs",
+                                     params.filein + ".psam");
+    else
+      make_plink2_eigenvec_file(params.k, params.fileout + ".eigvecs2", params.fileout + ".eigvecs",
+                                params.filein + ".fam");
   }
   cao.print(tick.date(), "total elapsed wall time:", tick.abstime(), " seconds");
   return 0;

@@ -9,7 +9,8 @@
 // Level 2: AmdNormalRsvdOpData / AmdFancyRsvdOpData override computeGandH
 //   (src/Halko.cpp:99-224) and keep the reference's computeUSV
 //   (src/Halko.cpp:46-97) with its host Householder QR and JacobiSVD;
-//   AmdFileBed replaces FileBed (src/FilePlink.hpp:10-30) as the Data provider.
+//   AmdFileBed replaces FileBed (src/FilePlink.hpp:10-30) and AmdFilePgen
+//   replaces FilePgen (src/FilePgen.hpp:14-48) as the Data provider.
 #pragma once
 
 #include <cstdint>
@@ -50,18 +51,24 @@ inline pcaone_params to_amd(const Param& p, int precision = PCAONE_PREC_BF16) {
   a.verbose = (int)p.verbose;
   a.memory = p.memory;
   a.buffer = p.buffer;
+  a.hardcall = p.hardcall;
+  a.threads = p.threads;
   a.precision = precision;
   return a;
 }
 
-// Level 1: replaces FileBed::read_all + run_pca_with_halko + write_eigs_files
-// (+ make_plink2_eigenvec_file) for PLINK input with --svd 1/2.
+// Level 1: replaces FileBed/FilePgen::read_all + run_pca_with_halko +
+// write_eigs_files (+ make_plink2_eigenvec_file / _from_psam) for PLINK or
+// PLINK 2 input with --svd 1/2.
 inline void run_pca_with_halko_amd(const Param& params, int precision = PCAONE_PREC_BF16) {
   pcaone_ctx* ctx = nullptr;
   if (pcaone_open(0, &ctx) != PCAONE_OK) cao.error(pcaone_last_error(nullptr));
   const pcaone_params a = to_amd(params, precision);
   pcaone_run_info info;
-  AMD_CHECK(ctx, pcaone_run_bfile(ctx, params.filein.c_str(), params.fileout.c_str(), &a, &info));
+  if (params.file_t == FileType::PGEN)
+    AMD_CHECK(ctx, pcaone_run_pgen(ctx, params.filein.c_str(), params.fileout.c_str(), &a, &info));
+  else
+    AMD_CHECK(ctx, pcaone_run_bfile(ctx, params.filein.c_str(), params.fileout.c_str(), &a, &info));
   cao.print(tick.date(), "device time of the products:", info.g_ms + info.h_ms, "ms");
   pcaone_close(ctx);
 }
@@ -90,6 +97,34 @@ class AmdFileBed : public Data {
   }
   void check_file_offset_first_var() override {}
   // With -m the whole payload is resident in HBM: blocks need no reading.
+  void read_block_initial(uint64, uint64, bool) override {}
+  void read_block_update(uint64, uint64, const Mat2D&, const Mat1D&, const Mat2D&, bool) override {}
+  pcaone_ctx* ctx;
+};
+
+// The PLINK 2 Data provider: the .pgen records are decoded on the host into
+// .bed rows and sent to HBM; the rest as AmdFileBed.
+class AmdFilePgen : public Data {
+ public:
+  AmdFilePgen(const Param& p, pcaone_ctx* c) : Data(p), ctx(c) {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    AMD_CHECK(nullptr, pcaone_pgen_dims(params.filein.c_str(), &n, &m));  // .psam lines, .pgen header
+    nsamples = n;
+    nsnps = (uint)m;
+  }
+  void read_all() override {
+    uint32_t n = 0;
+    uint64_t m = 0, miss = 0;
+    AMD_CHECK(ctx, pcaone_pgen_read(ctx, params.filein.c_str(), params.maf, params.hardcall, &n, &m));
+    nsamples = n;
+    nsnps = (uint)m;
+    F.resize(nsnps);
+    AMD_CHECK(ctx, pcaone_allele_freq(ctx, F.data()));
+    AMD_CHECK(ctx, pcaone_dims(ctx, &n, &m, &miss));
+    p_miss = (double)miss / ((double)n * (double)m);
+  }
+  void check_file_offset_first_var() override {}
   void read_block_initial(uint64, uint64, bool) override {}
   void read_block_update(uint64, uint64, const Mat2D&, const Mat1D&, const Mat2D&, bool) override {}
   pcaone_ctx* ctx;

@@ -548,8 +548,8 @@ void Context::omega_pieces() {
   if (form_colscaled(form_g))
     launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, nullptr, wg_.get(), wg_pitch_,
-                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
-  launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0),
+                     nullptr, nullptr, red_.get(), red_cap_, colsum_.get());
   PCAONE_HIP(hipStreamSynchronize(stream));
 }
 
@@ -645,8 +645,8 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     if (form_colscaled(form_h))
       launch_col_scales(form_h, gwin, nwin, lp, sc_.get() + j0, i8max_.get(), i8_scale(1), i8_mul(1), stream);
     launch_make_pieces(form_h, gwin, nwin, nwin, lp, sc_.get() + j0, nullptr, wh_.get() + off * wb, wh_pitch_,
-                       (uint64_t)lp * wh_pitch_, gwin, nullptr, stream, i8_scale(1));
-    launch_colsum(gt_.get(), j0, j1, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
+                       (uint64_t)lp * wh_pitch_, gwin, nullptr, stream, i8_scale(1), sc_.get() + j0, cF_.get() + j0,
+                       red_.get(), red_cap_, bvec_.get());
   }
 
   h_half(j0, j1, hmode);
@@ -719,8 +719,7 @@ void Context::h_from_g(int hmode) {
   const uint64_t klen = round_up(M, kKStep);
   launch_zero_ranges(wh_.get(), wh_pitch_ * wb, (uint64_t)P * lp, 0, 0, M * wb, (klen - M) * wb, stream);
   launch_make_pieces(form_h, gt_.get(), M, M, lp, sc_.get(), nullptr, wh_.get(), wh_pitch_, (uint64_t)lp * wh_pitch_,
-                     gt_.get(), nullptr, stream, nullptr);
-  launch_colsum(gt_.get(), 0, M, lp, sc_.get(), cF_.get(), red_.get(), bvec_.get(), stream);
+                     gt_.get(), nullptr, stream, nullptr, sc_.get(), cF_.get(), red_.get(), red_cap_, bvec_.get());
   if (lnmiss_) update_missing_weights();
   h_half(0, M, hmode);
 }
@@ -820,8 +819,8 @@ void Context::omega_from(int src, bool flip) {
     if (form_colscaled(form_g))
       launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
     launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
-                       (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
-    launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+                       (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0),
+                       nullptr, nullptr, red_.get(), red_cap_, colsum_.get());
     return;
   }
   const la::Mat& G1 = gram_h_;  // h_thinq_safe() left the Gram of this H there
@@ -859,8 +858,8 @@ void Context::omega_from(int src, bool flip) {
   if (form_colscaled(form_g))
     launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
-                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
-  launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0),
+                     nullptr, nullptr, red_.get(), red_cap_, colsum_.get());
   PCAONE_HIP(hipStreamSynchronize(stream));
 }
 
@@ -914,8 +913,8 @@ void Context::omega_householder(bool flip) {
   if (form_colscaled(form_g))
     launch_col_scales(form_g, omega_c_.get(), N, lp, nullptr, i8max_.get(), i8_scale(0), i8_mul(0), stream);
   launch_make_pieces(form_g, omega_c_.get(), N, ks_, lp, nullptr, dsign, wg_.get(), wg_pitch_,
-                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0));
-  launch_colsum(omega_r_.get(), 0, N, lp, nullptr, nullptr, red_.get(), colsum_.get(), stream);
+                     (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0),
+                     nullptr, nullptr, red_.get(), red_cap_, colsum_.get());
   PCAONE_HIP(hipStreamSynchronize(stream));
 }
 

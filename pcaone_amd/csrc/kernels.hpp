@@ -45,11 +45,17 @@ void launch_missing_lists(const uint32_t* x, uint64_t pitch, uint64_t nrows, uin
 // Splits X (rows x lp, row-major FP64) into MFMA operand pieces, transposed:
 // Wt[p][l][r] for r < rows_pad (rows >= nreal written as 0).  Row r is first
 // multiplied by rowscale[r] (null = 1) and column l by colsign[l] (null = 1).
-// rounded (nullable, row-major rows x lp) receives sum(pieces) / rowscale.
+// rounded (nullable, row-major rows x lp) receives sum(pieces) / rowscale
+// (in place over X allowed); signed_out (nullable) X with the column signs.
 // The I8 and F8 forms take their column scales from colscale (launch_col_scales).
+// With bvec: bvec[l] = sum_r w_r rounded[r][l], w_r = bsc[r] (1/2 - bcf[r])
+// (1 without bsc), from per-tile partials in bpart (bpart_cap elements; a
+// separate column sum over rounded when they do not fit).
 void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
-                        double* signed_out, hipStream_t s, const double* colscale = nullptr);
+                        double* signed_out, hipStream_t s, const double* colscale = nullptr,
+                        const double* bsc = nullptr, const double* bcf = nullptr, double* bpart = nullptr,
+                        uint64_t bpart_cap = 0, double* bvec = nullptr);
 // Column scales of X for a column-scaled form (rows [0, nreal), each row
 // times rowscale[r], null = 1), from m_l = max_r |X[r][l] rowscale[r]|:
 // I8: scale[l] = m_l / kI8Max, mul[l] = 2 scale[l] (the genotype operand is

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bf16.hpp"
 #include "device.hpp"
@@ -23,69 +24,169 @@ inline unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 65535u
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (n); t += (uint64_t)gridDim.x * blockDim.x)
 
 
+// Operand pieces of one form for each value y = X[r][l] colsign[l] rowscale[r]
+// (F8 and I8 also divided by the column scale), and the value they sum to.
 template <Form F>
-__global__ void k_make_pieces(const double* __restrict__ X, uint64_t nreal, uint64_t rows_pad, int lp,
-                              const double* __restrict__ rowscale, const double* __restrict__ colsign, void* Wt,
-                              uint64_t wpitch, uint64_t wpiece, double* __restrict__ rounded,
-                              double* __restrict__ signed_out, const double* __restrict__ colscale) {
-  const uint64_t n = rows_pad * (uint64_t)lp;
-  GRID_STRIDE(t, n) {
-    const uint64_t l = t / rows_pad, r = t - l * rows_pad;
-    double x = 0.0;
-    if (r < nreal) {
-      x = X[r * lp + l];
-      if (colsign) x *= colsign[l];
+struct Pieces;
+template <>
+struct Pieces<Form::FP64> {
+  using w_t = double;
+  static constexpr int P = 1;
+  __device__ static double make(double y, double, w_t* pc) { return pc[0] = y; }
+};
+template <>
+struct Pieces<Form::FP32> {
+  using w_t = float;
+  static constexpr int P = 1;
+  __device__ static double make(double y, double, w_t* pc) { return (double)(pc[0] = (float)y); }
+};
+template <>
+struct Pieces<Form::FP16> {
+  using w_t = uint16_t;
+  static constexpr int P = 1;
+  __device__ static double make(double y, double, w_t* pc) {
+    pc[0] = fp16_bits(y);
+    return fp16_value(pc[0]);
+  }
+};
+template <>
+struct Pieces<Form::BF16X1> {
+  using w_t = uint16_t;
+  static constexpr int P = 1;
+  __device__ static double make(double y, double, w_t* pc) {
+    pc[0] = bf16_bits((float)y);
+    return (double)bf16_value(pc[0]);
+  }
+};
+template <>
+struct Pieces<Form::BF16> {
+  using w_t = uint16_t;
+  static constexpr int P = 2;
+  __device__ static double make(double y, double, w_t* pc) {
+    pc[0] = bf16_bits((float)y);
+    const double back = (double)bf16_value(pc[0]);
+    pc[1] = bf16_bits((float)(y - back));
+    return back + (double)bf16_value(pc[1]);
+  }
+};
+template <>
+struct Pieces<Form::F8> {
+  using w_t = uint8_t;
+  static constexpr int P = 3;
+  // y / s (exact: s is a power of two, |y / s| <= 448) as three E4M3
+  // pieces, each the nearest E4M3 to what the previous ones left
+  __device__ static double make(double y, double s, w_t* pc) {
+    const double x1 = y / s;
+    double v1, v2, v3;
+    pc[0] = e4m3_bits(x1, &v1);
+    pc[1] = e4m3_bits(x1 - v1, &v2);
+    pc[2] = e4m3_bits(x1 - v1 - v2, &v3);
+    return (v1 + v2 + v3) * s;
+  }
+};
+template <>
+struct Pieces<Form::I8> {
+  using w_t = uint8_t;
+  static constexpr int P = 2;
+  // q = rint(y / s) within +-kI8Max; q = 256 q_hi + q_lo, q_lo in [-128, 127]
+  __device__ static double make(double y, double s, w_t* pc) {
+    const int q = (int)fmin(fmax(rint(y / s), -(double)kI8Max), (double)kI8Max);
+    const int hi = (q + 128) >> 8, lo = q - 256 * hi;
+    pc[0] = (uint8_t)(int8_t)hi;
+    pc[1] = (uint8_t)(int8_t)lo;
+    return (double)q * s;
+  }
+};
+
+// Rows per workgroup of the row-tiled kernels below and the LDS row pitch of
+// their transposed pieces (element size wsz bytes): padded so that the
+// column groups of one store land in different banks.
+constexpr int kEpiRows = 64;
+constexpr int epi_pitch(size_t wsz) { return kEpiRows + (wsz == 2 ? 2 : 1); }
+
+// Operand pieces of X (rows x lp, row-major FP64), transposed to
+// Wt[p][l][r], for a tile of 64 rows per workgroup.  Each thread owns 4
+// adjacent columns of every RG-th row of the tile (16-byte loads along the
+// row); the pieces go through LDS so that the transposed stores are coalesced
+// along the rows.  Optional outputs: signed_out (X colsign), rounded (the
+// pieces' sum / rowscale; in place over X allowed: every element is read and
+// written by the same thread), and the per-tile column sums of
+// w_r rounded[r][l] (w_r = bsc[r] (1/2 - bcf[r]), or 1 without bsc) in bpart.
+template <Form F>
+__global__ void __launch_bounds__(256) k_make_pieces(const double* X, uint64_t nreal, uint64_t rows_pad, int lp,
+                                                     const double* __restrict__ rowscale,
+                                                     const double* __restrict__ colsign, void* Wt, uint64_t wpitch,
+                                                     uint64_t wpiece, double* rounded, double* __restrict__ signed_out,
+                                                     const double* __restrict__ colscale,
+                                                     const double* __restrict__ bsc, const double* __restrict__ bcf,
+                                                     double* __restrict__ bpart) {
+  using PF = Pieces<F>;
+  using w_t = typename PF::w_t;
+  constexpr int P = PF::P;
+  constexpr int PITCH = epi_pitch(sizeof(w_t));
+  extern __shared__ __align__(16) unsigned char sm[];
+  w_t* pt = reinterpret_cast<w_t*>(sm);  // P x lp rows of PITCH
+  double* bred = reinterpret_cast<double*>(sm + ((size_t)P * lp * PITCH * sizeof(w_t) + 15) / 16 * 16);
+  const int CG = lp / 4, RG = 256 / CG;
+  const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG;
+  const bool act = rg < RG;
+  const uint64_t r0 = (uint64_t)blockIdx.x * kEpiRows;
+  const int nr = (int)(rows_pad - r0 < (uint64_t)kEpiRows ? rows_pad - r0 : kEpiRows);
+  double b4[4] = {0.0, 0.0, 0.0, 0.0};
+  if (act) {
+    double cs[4] = {1.0, 1.0, 1.0, 1.0}, sl[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (colsign) cs[q] = colsign[4 * cg + q];
+      if constexpr (F == Form::F8 || F == Form::I8) sl[q] = colscale[4 * cg + q];
     }
-    if (signed_out) signed_out[r * lp + l] = x;
-    const double rs = rowscale ? rowscale[r] : 1.0;
-    const double y = x * rs;
-    double back;
-    if constexpr (F == Form::FP64) {
-      static_cast<double*>(Wt)[l * wpitch + r] = y;
-      back = y;
-    } else if constexpr (F == Form::FP32) {
-      const float yf = (float)y;
-      static_cast<float*>(Wt)[l * wpitch + r] = yf;
-      back = (double)yf;
-    } else if constexpr (F == Form::FP16) {
-      const uint16_t h = fp16_bits(y);
-      static_cast<uint16_t*>(Wt)[l * wpitch + r] = h;
-      back = fp16_value(h);
-    } else if constexpr (F == Form::F8) {
-      // y / s_l (exact: s_l is a power of two, |y / s_l| <= 448) as three
-      // E4M3 pieces, each the nearest E4M3 to what the previous ones left
-      const double sl = colscale[l];
-      const double x1 = y / sl;
-      double v1, v2, v3;
-      const uint8_t b1 = e4m3_bits(x1, &v1);
-      const uint8_t b2 = e4m3_bits(x1 - v1, &v2);
-      const uint8_t b3 = e4m3_bits(x1 - v1 - v2, &v3);
-      uint8_t* w = static_cast<uint8_t*>(Wt);
-      w[l * wpitch + r] = b1;
-      w[wpiece + l * wpitch + r] = b2;
-      w[2 * wpiece + l * wpitch + r] = b3;
-      back = (v1 + v2 + v3) * sl;
-    } else if constexpr (F == Form::I8) {
-      // q = rint(y / s_l) within +-kI8Max; q = 256 q_hi + q_lo, q_lo in [-128, 127]
-      const double sl = colscale[l];
-      const int q = (int)fmin(fmax(rint(y / sl), -(double)kI8Max), (double)kI8Max);
-      const int hi = (q + 128) >> 8, lo = q - 256 * hi;
-      uint8_t* w = static_cast<uint8_t*>(Wt);
-      w[l * wpitch + r] = (uint8_t)(int8_t)hi;
-      w[wpiece + l * wpitch + r] = (uint8_t)(int8_t)lo;
-      back = (double)q * sl;
-    } else {
-      const uint16_t hi = bf16_bits((float)y);
-      uint16_t* w = static_cast<uint16_t*>(Wt);
-      w[l * wpitch + r] = hi;
-      back = (double)bf16_value(hi);
-      if constexpr (F == Form::BF16) {
-        const uint16_t lo = bf16_bits((float)(y - back));
-        w[wpiece + l * wpitch + r] = lo;
-        back += (double)bf16_value(lo);
+    for (int r = rg; r < nr; r += RG) {
+      const uint64_t j = r0 + r;
+      const uint64_t t = j * lp + 4 * cg;
+      double x[4] = {0.0, 0.0, 0.0, 0.0};
+      if (j < nreal) {
+        const double2 a = *reinterpret_cast<const double2*>(X + t);
+        const double2 b = *reinterpret_cast<const double2*>(X + t + 2);
+        x[0] = a.x * cs[0], x[1] = a.y * cs[1], x[2] = b.x * cs[2], x[3] = b.y * cs[3];
+      }
+      if (signed_out) {
+        *reinterpret_cast<double2*>(signed_out + t) = double2{x[0], x[1]};
+        *reinterpret_cast<double2*>(signed_out + t + 2) = double2{x[2], x[3]};
+      }
+      const double rs = rowscale ? rowscale[j] : 1.0;
+      const double w = j < nreal ? (bsc ? bsc[j] * (0.5 - bcf[j]) : 1.0) : 0.0;
+      double rd[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        w_t pc[P];
+        const double back = PF::make(x[q] * rs, sl[q], pc);
+        rd[q] = (j < nreal && rs != 0.0) ? back / rs : 0.0;
+        b4[q] += w * rd[q];
+#pragma unroll
+        for (int p = 0; p < P; ++p) pt[(p * lp + 4 * cg + q) * PITCH + r] = pc[p];
+      }
+      if (rounded) {
+        *reinterpret_cast<double2*>(rounded + t) = double2{rd[0], rd[1]};
+        *reinterpret_cast<double2*>(rounded + t + 2) = double2{rd[2], rd[3]};
       }
     }
-    if (rounded) rounded[r * lp + l] = (r < nreal && rs != 0.0) ? back / rs : 0.0;
+    if (bpart) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bred[rg * lp + 4 * cg + q] = b4[q];
+    }
+  }
+  __syncthreads();
+  w_t* wt = static_cast<w_t*>(Wt);
+  for (int v = threadIdx.x; v < P * lp * kEpiRows; v += 256) {
+    const int r = v % kEpiRows, pl = v / kEpiRows, p = pl / lp, l = pl - p * lp;
+    if (r < nr) wt[p * wpiece + l * wpitch + r0 + r] = pt[pl * PITCH + r];
+  }
+  if (bpart) {
+    for (int l = threadIdx.x; l < lp; l += 256) {
+      double sum = 0.0;
+      for (int q = 0; q < RG; ++q) sum += bred[q * lp + l];
+      bpart[(uint64_t)blockIdx.x * lp + l] = sum;
+    }
   }
 }
 
@@ -112,6 +213,39 @@ __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t s
   }
 }
 
+// k_epi_g for 32-bit partials (FP32 or int32), 4 adjacent columns per thread
+// (16-byte partial loads, 32-byte stores).
+template <class PT>
+__global__ void k_epi_g_v4(const PT* __restrict__ part, uint64_t nchunks, uint64_t stride, uint64_t nreal,
+                           uint64_t rows_pad, int lp, const double* __restrict__ sc, const double* __restrict__ cf,
+                           const double* __restrict__ colsum, double* __restrict__ out,
+                           const double* __restrict__ colmul) {
+  using v4 = typename std::conditional<std::is_same<PT, float>::value, float4, int4>::type;
+  const uint64_t n4 = rows_pad * (uint64_t)lp / 4;
+  GRID_STRIDE(t4, n4) {
+    const uint64_t t = 4 * t4;
+    const uint64_t j = t / lp;
+    const int l = (int)(t - j * lp);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < nreal) {
+      double y[4] = {0.0, 0.0, 0.0, 0.0};
+      for (uint64_t c = 0; c < nchunks; ++c) {
+        const v4 a = *reinterpret_cast<const v4*>(part + c * stride + t);
+        y[0] += (double)a.x, y[1] += (double)a.y, y[2] += (double)a.z, y[3] += (double)a.w;
+      }
+      const double s = sc[j], beta = s * (0.5 - cf[j]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (colmul) y[q] *= colmul[l + q];
+        v[q] = 0.25 * s * y[q] + beta * colsum[l + q];
+      }
+    }
+    double2* o = reinterpret_cast<double2*>(out + t);
+    o[0] = double2{v[0], v[1]};
+    o[1] = double2{v[2], v[3]};
+  }
+}
+
 // Fused G epilogue for data without missing entries.  A workgroup takes 64
 // SNP rows of the window: reduces the split-K partials, applies the affine
 // correction (G_j = sc_j/4 y_j + sc_j (1/2 - cF_j) colsum), splits sc_j G_j
@@ -120,10 +254,6 @@ __global__ void k_epi_g(const P* __restrict__ part, uint64_t nchunks, uint64_t s
 // column sums of beta_j G~_j with beta_j = sc_j (1/2 - cF_j).  Each thread
 // owns 4 adjacent columns (16-byte partial loads) of every RG-th row; its
 // beta sums stay in registers and are combined across row groups in LDS.
-constexpr int kEpiRows = 64;
-// LDS row pitch of the transposed pieces (element size wsz bytes): padded so
-// that the column groups of one store land in different banks.
-constexpr int epi_pitch(size_t wsz) { return kEpiRows + (wsz == 2 ? 2 : 1); }
 
 template <Form F, class PT>
 __global__ void __launch_bounds__(256) k_epi_g_fused(const PT* __restrict__ part, uint64_t nchunks, uint64_t stride,
@@ -1265,42 +1395,56 @@ void launch_rows_to_colmajor(const double* src, uint64_t n, int lp, int k, doubl
   check_launch("rows_to_colmajor");
 }
 
+template <Form F>
+static void make_pieces_t(const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
+                          const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
+                          double* signed_out, const double* colscale, const double* bsc, const double* bcf,
+                          double* bpart, hipStream_t s) {
+  using w_t = typename Pieces<F>::w_t;
+  constexpr int P = Pieces<F>::P;
+  const auto lds_for = [](int l) {
+    return ((size_t)epi_pitch(sizeof(w_t)) * l * P * sizeof(w_t) + 15) / 16 * 16 + (size_t)1024 * 8;
+  };
+  static bool attr = false;
+  if (!attr) {
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_make_pieces<F>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_for(128)));
+    attr = true;
+  }
+  const uint64_t nb = (rows_pad + kEpiRows - 1) / kEpiRows;
+  hipLaunchKernelGGL(k_make_pieces<F>, dim3((unsigned)nb), dim3(256), lds_for(lp), s, X, nreal, rows_pad, lp, rowscale,
+                     colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bpart);
+}
+
 void launch_make_pieces(Form f, const double* X, uint64_t nreal, uint64_t rows_pad, int lp, const double* rowscale,
                         const double* colsign, void* Wt, uint64_t wpitch, uint64_t wpiece, double* rounded,
-                        double* signed_out, hipStream_t s, const double* colscale) {
-  const unsigned g = grid_for(rows_pad * lp);
+                        double* signed_out, hipStream_t s, const double* colscale, const double* bsc,
+                        const double* bcf, double* bpart, uint64_t bpart_cap, double* bvec) {
   require(!form_colscaled(f) || colscale, kErrInternal, "make_pieces: this form needs column scales");
+  require(lp >= 16 && lp <= 128 && lp % 4 == 0, kErrArgument, "make_pieces: bad lp");
+  if (!rows_pad) return;
+  const uint64_t nb = (rows_pad + kEpiRows - 1) / kEpiRows;
+  // the column sums ride along when their per-tile partials fit the scratch
+  const bool fuse = bvec && bpart && nb * (uint64_t)lp <= bpart_cap;
+  double* bp = fuse ? bpart : nullptr;
   switch (f) {
-    case Form::FP64:
-      hipLaunchKernelGGL(k_make_pieces<Form::FP64>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
-    case Form::FP32:
-      hipLaunchKernelGGL(k_make_pieces<Form::FP32>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
-    case Form::BF16:
-      hipLaunchKernelGGL(k_make_pieces<Form::BF16>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
-    case Form::BF16X1:
-      hipLaunchKernelGGL(k_make_pieces<Form::BF16X1>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
-    case Form::FP16:
-      hipLaunchKernelGGL(k_make_pieces<Form::FP16>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
-    case Form::I8:
-      hipLaunchKernelGGL(k_make_pieces<Form::I8>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
-    case Form::F8:
-      hipLaunchKernelGGL(k_make_pieces<Form::F8>, dim3(g), dim3(256), 0, s, X, nreal, rows_pad, lp, rowscale,
-                         colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale);
-      break;
+    case Form::FP64: make_pieces_t<Form::FP64>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
+    case Form::FP32: make_pieces_t<Form::FP32>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
+    case Form::BF16: make_pieces_t<Form::BF16>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
+    case Form::BF16X1: make_pieces_t<Form::BF16X1>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
+    case Form::FP16: make_pieces_t<Form::FP16>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
+    case Form::I8: make_pieces_t<Form::I8>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
+    case Form::F8: make_pieces_t<Form::F8>(X, nreal, rows_pad, lp, rowscale, colsign, Wt, wpitch, wpiece, rounded, signed_out, colscale, bsc, bcf, bp, s); break;
   }
   check_launch("make_pieces");
+  if (!bvec) return;
+  if (fuse) {
+    sum_partials(bpart, nb, (uint64_t)lp, bvec, s);
+    check_launch("make_pieces_colsum");
+  } else {
+    require(rounded != nullptr && bpart != nullptr, kErrInternal, "make_pieces: column sums need rounded and scratch");
+    launch_colsum(rounded, 0, nreal, lp, bsc, bcf, bpart, bvec, s);
+  }
 }
 
 void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride, uint64_t nreal, uint64_t rows_pad,
@@ -1311,7 +1455,16 @@ void launch_epi_g(Form f, const void* P, uint64_t nchunks, uint64_t chunk_stride
   if (f == Form::FP64)
     hipLaunchKernelGGL(k_epi_g<double>, dim3(g), dim3(256), 0, s, static_cast<const double*>(P), nchunks,
                        chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
-  else if (f == Form::I8)
+  else if (lp % 4 == 0 && chunk_stride % 4 == 0 && reinterpret_cast<uintptr_t>(P) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(out) % 16 == 0) {
+    const unsigned g4 = grid_for(rows_pad * lp / 4);
+    if (f == Form::I8)
+      hipLaunchKernelGGL(k_epi_g_v4<int32_t>, dim3(g4), dim3(256), 0, s, static_cast<const int32_t*>(P), nchunks,
+                         chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
+    else
+      hipLaunchKernelGGL(k_epi_g_v4<float>, dim3(g4), dim3(256), 0, s, static_cast<const float*>(P), nchunks,
+                         chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
+  } else if (f == Form::I8)
     hipLaunchKernelGGL(k_epi_g<int32_t>, dim3(g), dim3(256), 0, s, static_cast<const int32_t*>(P), nchunks,
                        chunk_stride, nreal, rows_pad, lp, sc, cf, colsum, out, colmul);
   else

@@ -232,6 +232,39 @@ bool NormalRsvdOpData::enqueue_next(int pi) {
   return true;
 }
 
+namespace {
+
+// Consecutive windows that use one Omega and add into one of H1 / H2 form a
+// single product over their SNP range: G_b = A_b Omega for every window b of
+// the range and H_x += sum_b A_b^T G_b are the same numbers whether the
+// windows go one by one (src/Halko.cpp:195-203) or as one range, up to the
+// order of the summation.  Windows are merged only on resident genotypes.
+class Segments {
+ public:
+  Segments(Context& ctx, bool merge) : ctx_(ctx), merge_(merge) {}
+  void add(uint64_t j0, uint64_t j1, int hmode) {
+    if (merge_ && j1_ > j0_ && hmode == hmode_ && j0 == j1_) {
+      j1_ = j1;
+      return;
+    }
+    flush();
+    j0_ = j0, j1_ = j1, hmode_ = hmode;
+    if (!merge_) flush();
+  }
+  void flush() {
+    if (j1_ > j0_) ctx_.sketch(j0_, j1_, hmode_);
+    j0_ = j1_ = 0;
+  }
+
+ private:
+  Context& ctx_;
+  bool merge_;
+  uint64_t j0_ = 0, j1_ = 0;
+  int hmode_ = 0;
+};
+
+}  // namespace
+
 void FancyRsvdOpData::computeGandH(int pi) {
   // src/Halko.cpp:155-224 (in-core).  Windows starting past the last SNP
   // (possible when M < bands^2) are treated as empty.
@@ -260,13 +293,15 @@ void FancyRsvdOpData::computeGandH(int pi) {
   }
   bandsize_ = std::min<uint64_t>(bandsize_ * 2, bands);
   const uint64_t M = rows();
+  Segments seg(ctx_, !ctx_.streamed());
   for (uint64_t b = 0, i = 1; b < bands; ++b, ++i) {
     const uint64_t start = b * blocksize_;
     const uint64_t stop = (b + 1) * blocksize_ >= M ? M - 1 : (b + 1) * blocksize_ - 1;
-    if (start <= stop) ctx_.sketch(start, stop + 1, i <= bandsize_ / 2 ? 1 : 2);
+    if (start <= stop) seg.add(start, stop + 1, i <= bandsize_ / 2 ? 1 : 2);
     const bool adjacent = (pi > 0 && (double)(b + 1) == std::pow(2, pi - 1) && std::pow(2, pi) < (double)bands);
     if ((b + 1) < bandsize_ && !adjacent) continue;
     if (!((i == bandsize_) || (i == bandsize_ / 2) || adjacent)) continue;
+    seg.flush();
     ctx_.omega_from(1);
     ++omega_updates;
     if (i == bandsize_) {
@@ -276,6 +311,7 @@ void FancyRsvdOpData::computeGandH(int pi) {
       ctx_.zero_h(2);
     }
   }
+  seg.flush();
 }
 
 void FancyRsvdOpData::computeGandH_blocks(int pi) {
@@ -289,12 +325,14 @@ void FancyRsvdOpData::computeGandH_blocks(int pi) {
     bandsize_ = bp.band_factor;
   }
   bandsize_ = std::min<uint64_t>(bandsize_ * 2, bp.nblocks);
+  Segments seg(ctx_, !ctx_.streamed());  // streamed blocks are loaded one at a time
   for (uint64_t b = 0, i = 1; b < bp.nblocks; ++b, ++i) {
-    ctx_.sketch(bp.start[b], bp.stop[b] + 1, i <= bandsize_ / 2 ? 1 : 2);
+    seg.add(bp.start[b], bp.stop[b] + 1, i <= bandsize_ / 2 ? 1 : 2);
     const bool adjacent = (pi > 0 && (double)(b + 1) == std::pow(2, pi - 1) * (double)bp.band_factor &&
                            std::pow(2, pi) < (double)bands);
     if ((b + 1) < bandsize_ && !adjacent) continue;
     if (!((i == bandsize_) || (i == bandsize_ / 2) || adjacent)) continue;
+    seg.flush();
     ctx_.omega_from(1);
     ++omega_updates;
     if (i == bandsize_) {
@@ -304,6 +342,7 @@ void FancyRsvdOpData::computeGandH_blocks(int pi) {
       ctx_.zero_h(2);
     }
   }
+  seg.flush();
 }
 
 PcaResult run_pca(Context& ctx, const Param& params, RunInfo* info, double* U_out) {

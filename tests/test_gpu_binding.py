@@ -22,13 +22,15 @@ BIN = ROOT / "oracle" / "_ref" / "pcaone_ref_amd"
 
 @pytest.mark.skipif(not BIN.exists(), reason="oracle/_ref/pcaone_ref_amd is built only where the reference sources are")
 @pytest.mark.parametrize("case,level", [("f1_d1", 2), ("f1_d2", 2), ("f4_emu_d1", 2), ("f6_d1", 2),
-                                        ("f1_d1", 1), ("f1_d2_ooc", 1)])
-def test_binding_in_the_reference_path(golden_inputs, case, level, tmp_path):
+                                        ("f1_d1", 1), ("f1_d2_ooc", 1), ("f10_pg_d1", 2), ("f4_pg_emu", 2),
+                                        ("f1_pg_ooc", 1)])
+def test_binding_in_the_reference_path(golden_inputs, golden_pgen, case, level, tmp_path):
     meta = load_meta(case)
     out = str(tmp_path / "run")
     print(f"binding level {level}: {case} '{meta['args']}'", flush=True)
     env = dict(os.environ, PCAONE_AMD_LEVEL=str(level), PCAONE_AMD_PRECISION="0", OMP_NUM_THREADS="8")
-    cmd = [str(BIN), "-b", golden_inputs[meta["input"]], *meta["args"].split(), "-o", out, "-v", "1"]
+    src = ["-p", golden_pgen(case)] if "pgen" in meta else ["-b", golden_inputs[meta["input"]]]
+    cmd = [str(BIN), *src, *meta["args"].split(), "-o", out, "-v", "1"]
     res = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=120)
     assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
     d = GOLDEN / case
