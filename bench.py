@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: time to the top-k PCs of PCAone's single-pass randomized SVD
-(--svd 1) on one MI355X, BF16 form against the plain FP32 form at equal error.
+(--svd 1) on one MI355X, FP8 form (the headline; BF16 hi/lo and INT8 beside
+it) against the plain FP32 form at equal error.
 
 Workload (the precision form): 50,000 samples x 1,000,000 SNPs of synthetic
 Balding-Nichols genotypes (8 populations, Fst 0.1, p ~ U(0.05, 0.5)),
@@ -9,8 +10,9 @@ oversampling 20 raised to 40 by the reference rule (L = 80); 7 power
 iterations forced with --tol-rsvd 0 (8 passes over the data).  One step is
 one complete randomized SVD (top 40 PCs) with the genotypes already resident.
 
-Printed: one JSON line (rank 0).  Errors of the BF16 and FP32 forms are
-measured against the FP64 form of the same run (same Omega).  The CPU
+Printed: one JSON line (rank 0).  Errors of the reduced forms are measured
+against the FP64 form of the same run (same Omega) and against the same
+workload in FP64 iterated to convergence.  The CPU
 baseline is the reference's own binary (oracle/_ref/pcaone_ref, built from
 the reference sources) on a bounded sample of the same workload.
 """
@@ -47,7 +49,9 @@ FORM_TEXT = {
 }
 DTYPE = {"bf16": "bf16", "bf16x1": "bf16", "bf16mix": "bf16", "fp16": "fp16", "fp16mix": "fp16", "int8": "int8",
          "fp8": "fp8"}
-SIDE_FORMS = ("int8", "fp8")  # reported beside the headline with their errors
+# operand pieces per value of W (Omega or G): executed MFMA work = pieces x algorithmic
+PIECES = {"bf16": 2, "bf16x1": 1, "bf16mix": 2, "fp16": 1, "fp16mix": 2, "int8": 2, "fp8": 3, "fp32": 1, "fp64": 1}
+SIDE_FORMS = ("bf16", "fp8", "int8")  # reported beside the headline (when not the headline) with their errors
 
 
 def log(msg: str) -> None:
@@ -154,7 +158,7 @@ def cpu_baseline(args, spec_full, k, os_, maxp):
 
 CONFIG_RUNS = {
     # the headline workload through the out-of-core schedule (only with --memory)
-    3: dict(params=dict(svd=1, k=40, oversamples=20, maxp=7, tol=0.0), forms=("bf16", "fp32", "fp64"),
+    3: dict(params=dict(svd=1, k=40, oversamples=20, maxp=7, tol=0.0), forms=("fp8", "bf16", "fp32", "fp64"),
             what="--svd 1, k=40, L=80, 7 power iterations"),
     # winSVD, 64 windows, 10 epochs (--maxp 9 --tol-rsvd 0), k = 20 (L = 40)
     2: dict(params=dict(svd=2, k=20, oversamples=10, bands=64, maxp=9, tol=0.0),
@@ -330,7 +334,8 @@ def main() -> int:
     ap.add_argument("--wm-h", type=int, default=0, help="GEMM wave tile of the H product (0 = tuned default)")
     ap.add_argument("--waves-g", type=int, default=0, help="waves per workgroup of the G product (0 = default)")
     ap.add_argument("--waves-h", type=int, default=0, help="waves per workgroup of the H product (0 = default)")
-    ap.add_argument("--head", default="bf16", choices=sorted(DTYPE), help="headline form (default: bf16 hi/lo)")
+    ap.add_argument("--head", default="fp8", choices=sorted(DTYPE),
+                    help="headline form (default: fp8, three E4M3 pieces; bf16 = hi/lo)")
     ap.add_argument("--config", type=int, default=3,
                     help="3: the headline workload; 2 (winSVD), 4 (batched cohorts) or 5 (EMU): a timing/error "
                          "line for that configuration at full size instead")
@@ -401,10 +406,10 @@ def main() -> int:
         return r, ts
 
     head = args.head
-    r_bf16, ts_bf16 = (None, [])
+    r_head, ts_head = (None, [])
     if only is None or head in only:
-        r_bf16, ts_bf16 = timed(head, args.steps, args.warmup)
-        results[head] = r_bf16
+        r_head, ts_head = timed(head, args.steps, args.warmup)
+        results[head] = r_head
         # live kernel timing (HIP events on the context stream), separate run
         ctx.time_kernels(True)
         rk = ctx.run(params(head))
@@ -414,10 +419,14 @@ def main() -> int:
         r_fp32, ts_fp32 = timed("fp32", 1, 1 if args.warmup else 0)
         results["fp32"] = r_fp32
     ts_side = {}
+    side_kt = {}
     for side in SIDE_FORMS:
         if only is None and head != side:
-            # reported beside the headline with its errors
+            # reported beside the headline with its errors and its product launch times
             results[side], ts_side[side] = timed(side, 1, 1 if args.warmup else 0)
+            ctx.time_kernels(True)
+            side_kt[side] = ctx.run(params(side)).info
+            ctx.time_kernels(False)
     if (only is None and not args.no_fp64) or (only and "fp64" in only):
         r64, ts64 = timed("fp64", 1, 0)
         results["fp64"] = r64
@@ -425,7 +434,7 @@ def main() -> int:
         print(json.dumps({f: {"seconds": results[f].info["seconds"]} for f in results}))
         return 0
     if only is not None:
-        print(json.dumps({"form": head, "seconds": float(np.mean(ts_bf16)), "g_ms": kt["g_ms"] / kt["g_launches"],
+        print(json.dumps({"form": head, "seconds": float(np.mean(ts_head)), "g_ms": kt["g_ms"] / kt["g_launches"],
                           "h_ms": kt["h_ms"] / kt["h_launches"], "wm_g": args.wm_g, "wm_h": args.wm_h,
                           "waves_g": args.waves_g, "waves_h": args.waves_h,
                           "kchunk_g": args.kchunk_g, "kchunk_h": args.kchunk_h}), flush=True)
@@ -438,7 +447,7 @@ def main() -> int:
     dom, dom_ms = ("H = A^T G", h_avg) if h_avg >= g_avg else ("G = A Omega", g_avg)
     achieved = flops_launch / (dom_ms * 1e-3) / 1e12
     traffic = recorded_traffic(N, M, L, head, "H" if dom.startswith("H") else "G")
-    value = float(np.mean(ts_bf16))
+    value = float(np.mean(ts_head))
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -462,6 +471,7 @@ def main() -> int:
                      "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
                      "avg_launch_ms": round(dom_ms, 3),
                      "algorithmic_flop_per_launch": flops_launch,
+                     "executed_frac": round(PIECES[head] * achieved / PEAK_TFLOPS[head], 4),
                      "other_product_avg_ms": round(g_avg if dom.startswith("H") else h_avg, 3)},
         "setup_s": round(setup_s, 2),
         # every step draws Omega again on the host (libstdc++ engine, parallel), as the reference's
@@ -472,8 +482,15 @@ def main() -> int:
         t32 = float(np.mean(ts_fp32))
         out["fp32_form"] = {"seconds": round(t32, 4), f"speedup_{head}_over_fp32": round(t32 / value, 2)}
         for side, ts in ts_side.items():
+            kt_s = side_kt[side]
+            g_s = kt_s["g_ms"] / max(kt_s["g_launches"], 1)
             out[f"{side}_form"] = {"seconds": round(float(np.mean(ts)), 4),
                                    f"speedup_{side}_over_fp32": round(t32 / float(np.mean(ts)), 2),
+                                   "g_launch_ms": round(g_s, 3),
+                                   "h_launch_ms": round(kt_s["h_ms"] / max(kt_s["h_launches"], 1), 3),
+                                   "algorithmic_tflops": round(flops_launch / (g_s * 1e-3) / 1e12, 1),
+                                   "peak": PEAK_TFLOPS[side],
+                                   "frac": round(flops_launch / (g_s * 1e-3) / 1e12 / PEAK_TFLOPS[side], 4),
                                    "form": FORM_TEXT[side]}
     if "fp64" in results:
         ref = results["fp64"]

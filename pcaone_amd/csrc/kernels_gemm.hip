@@ -105,6 +105,8 @@ template <>
 struct FT<Form::F8> {
   using w_t = uint8_t;
   using acc_t = float;
+  // (a 128-code step with two 4-wave workgroups per CU was measured slower:
+  // 8.66 against 8.21 ms per launch at 50,000 x 1,000,000, L = 80)
   static constexpr int P = 3, KS = 256, NACC = 1;
 };
 template <>
@@ -150,7 +152,8 @@ using aword_t = typename std::conditional<FT<F>::KS == 256, u32x4_t, u32x2_t>::t
 // keep the 16-row period.
 template <Form F>
 __device__ __forceinline__ int swz(int row, int v) {
-  return v ^ (row & (sizeof(typename FT<F>::w_t) == 2 ? 7 : 15));
+  constexpr int VPR = FT<F>::KS * (int)sizeof(typename FT<F>::w_t) / 16;  // vectors per W-tile row
+  return v ^ (row & (sizeof(typename FT<F>::w_t) == 2 ? 7 : (VPR < 16 ? VPR - 1 : 15)));
 }
 
 // All MFMAs of one K-step for one wave.
@@ -189,13 +192,15 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
     }
   } else if constexpr (F == Form::F8) {
     // 16x16x128 FP8 MFMAs, E4M3 on both sides: substep m takes words 2m and
-    // 2m + 1 of the lane group's four (codes 64 q + 32 m + [0, 32), in order),
-    // decoded to E4M3 (+2 = 0x40, 0, -2 = 0xC0: the same bytes as the BF16
-    // top bytes); the B fragment of a piece is the same 32 K slots of its W
-    // row, two 16-byte vectors.  The pieces accumulate into one FP32 set.
-    i32x8_t af[2][WM];
+    // 2m + 1 of the lane group's KS / 64 (codes (KS / 4) q + 32 m + [0, 32),
+    // in order), decoded to E4M3 (+2 = 0x40, 0, -2 = 0xC0: the same bytes as
+    // the BF16 top bytes); the B fragment of a piece is the same 32 K slots of
+    // its W row, two 16-byte vectors.  The pieces accumulate into one FP32 set.
+    constexpr int NSUB = T::KS / 128;
+    constexpr int VQ = T::KS / 64;  // 16-byte W vectors per lane group
+    i32x8_t af[NSUB][WM];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < NSUB; ++m)
 #pragma unroll
       for (int rb = 0; rb < WM; ++rb)
 #pragma unroll
@@ -204,15 +209,15 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
           for (int t = 0; t < 4; ++t) af[m][rb][4 * i + t] = (int)top_bytes(spread(aw[rb][2 * m + i], t));
     auto read_b = [&](int m, int cb, int p) {
       const w_t* row = wl + (p * LP + cb * 16 + r16) * LDSP;
-      const u32x4_t v0 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, 4 * q + 2 * m) * 16);
-      const u32x4_t v1 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, 4 * q + 2 * m + 1) * 16);
+      const u32x4_t v0 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, VQ * q + 2 * m) * 16);
+      const u32x4_t v1 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, VQ * q + 2 * m + 1) * 16);
       i32x8_t b;
 #pragma unroll
       for (int e = 0; e < 4; ++e) b[e] = (int)v0[e], b[4 + e] = (int)v1[e];
       return b;
     };
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < NSUB; ++m)
 #pragma unroll
       for (int cb = 0; cb < NC; ++cb)
 #pragma unroll
