@@ -111,6 +111,8 @@ typedef struct pcaone_run_info {
   double omega_ms;   /* host time of the Omega draws (initOmg), included in seconds */
   uint64_t householder_fallbacks; /* thin-Q of H by Householder QR (H near rank deficient) */
   uint64_t gram_second_passes;    /* QR of G with a second pass over G (ill conditioned)   */
+  uint64_t epoch_rollbacks;       /* winSVD epochs rerun because an Omega update's device
+                                     thin-Q met a near-singular Gram of H                   */
 } pcaone_run_info;
 
 /* Synthetic Balding-Nichols PLINK data (test and benchmark input; no

@@ -147,6 +147,7 @@ void fill_info(pcaone_run_info* info, const RunInfo& r, const Context& c, int ns
   info->omega_ms = r.omega_ms;
   info->householder_fallbacks = c.householder_fallbacks;
   info->gram_second_passes = c.gram_second_passes;
+  info->epoch_rollbacks = c.epoch_rollbacks;
 }
 
 }  // namespace

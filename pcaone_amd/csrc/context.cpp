@@ -476,7 +476,7 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
     PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
   }
   ktimes = KernelTimes{};
-  householder_fallbacks = gram_second_passes = 0;
+  householder_fallbacks = gram_second_passes = epoch_rollbacks = 0;
 }
 
 void Context::set_t(const la::Mat& T) {
@@ -786,7 +786,8 @@ void Context::omega_from(int src, bool flip) {
     ++h_version_;
   }
   const double* X = H_.get();
-  if (!h_thinq_safe()) {
+  const bool deferred = deferred_thinq && deferred_ok();
+  if (!deferred && !h_thinq_safe()) {
     // H is (nearly) rank deficient: CholeskyQR2 could break down or lose
     // orthogonality.  The thin Q comes from a Householder QR instead, the
     // reference's own factorization (src/Halko.cpp:121-122), which gives L
@@ -804,7 +805,7 @@ void Context::omega_from(int src, bool flip) {
       launch_gram(X, nullptr, nullptr, N, lp, red_.get(), red_cap_, gh, stream);
       gram_dev_version_ = h_version_;
     }
-    launch_thinq_first(gh, L, lp, r1, small_.get(), lstatus_.get(), stream);
+    launch_thinq_first(gh, L, lp, r1, small_.get(), lstatus_.get(), stream, deferred ? kThinqMinPivot : 0.0);
     launch_apply(X, small_.get(), N, lp, tmp_n_.get(), stream);
     launch_gram(tmp_n_.get(), nullptr, nullptr, N, lp, red_.get(), red_cap_, g2, stream);
     launch_thinq_second(g2, r1, X, L, lp, small_.get(), lstatus_.get(), stream);
@@ -861,6 +862,51 @@ void Context::omega_from(int src, bool flip) {
                      (uint64_t)lp * wg_pitch_, omega_r_.get(), omega_.get(), stream, i8_scale(0),
                      nullptr, nullptr, red_.get(), red_cap_, colsum_.get());
   PCAONE_HIP(hipStreamSynchronize(stream));
+}
+
+bool Context::deferred_ok() const { return L > 0 && thinq_on_device(lp) && !streamed(); }
+
+namespace {
+template <class T>
+void copy_dev(DevBuf<T>& dst, const DevBuf<T>& src, hipStream_t s) {
+  if (dst.size() < src.size()) dst.resize(src.size());
+  if (src.size()) PCAONE_HIP(hipMemcpyAsync(dst.get(), src.get(), src.bytes(), hipMemcpyDeviceToDevice, s));
+}
+template <class T>
+void restore_dev(DevBuf<T>& dst, const DevBuf<T>& src, hipStream_t s) {
+  if (dst.size()) PCAONE_HIP(hipMemcpyAsync(dst.get(), src.get(), dst.bytes(), hipMemcpyDeviceToDevice, s));
+}
+}  // namespace
+
+void Context::epoch_checkpoint() {
+  copy_dev(ck_H1_, H1_, stream);
+  copy_dev(ck_H2_, H2_, stream);
+  copy_dev(ck_omega_, omega_, stream);
+  copy_dev(ck_omega_r_, omega_r_, stream);
+  copy_dev(ck_colsum_, colsum_, stream);
+  copy_dev(ck_i8s_, i8s_, stream);
+  copy_dev(ck_wg_, wg_, stream);
+  if (lstatus_.get()) PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
+}
+
+bool Context::epoch_thinq_failed() {
+  int st = 0;
+  if (lstatus_.get()) PCAONE_HIP(hipMemcpyAsync(&st, lstatus_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  return st != 0;
+}
+
+void Context::epoch_rollback() {
+  restore_dev(H1_, ck_H1_, stream);
+  restore_dev(H2_, ck_H2_, stream);
+  restore_dev(omega_, ck_omega_, stream);
+  restore_dev(omega_r_, ck_omega_r_, stream);
+  restore_dev(colsum_, ck_colsum_, stream);
+  restore_dev(i8s_, ck_i8s_, stream);
+  restore_dev(wg_, ck_wg_, stream);
+  if (lstatus_.get()) PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
+  ++h_version_;  // every cached Gram of H is stale
+  ++epoch_rollbacks;
 }
 
 la::Mat Context::from_pinned_gram(int which) const {

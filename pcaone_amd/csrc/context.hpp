@@ -191,6 +191,16 @@ class Context {
   // how often the orthonormalization safeguards ran (Householder thin-Q of H,
   // second pass over G~)
   uint64_t householder_fallbacks = 0, gram_second_passes = 0;
+  // winSVD epochs: with deferred_thinq, every Omega update of the epoch runs the
+  // device thin-Q with its safety test on the device (no host round trip); an
+  // epoch in which one update failed the test is rolled back to its
+  // checkpoint and run again with the host test before each update.
+  bool deferred_thinq = false;
+  bool deferred_ok() const;          // the device thin-Q serves this configuration
+  void epoch_checkpoint();           // H1, H2 and the Omega state; clears the status
+  bool epoch_thinq_failed();         // synchronizes
+  void epoch_rollback();
+  uint64_t epoch_rollbacks = 0;
 
  private:
   // streamed genotypes
@@ -269,6 +279,8 @@ class Context {
   bool res_as_read_ = false;
   DevBuf<double> ldev_;
   DevBuf<int> lstatus_;
+  DevBuf<double> ck_H1_, ck_H2_, ck_omega_, ck_omega_r_, ck_colsum_, ck_i8s_;  // epoch checkpoint
+  DevBuf<unsigned char> ck_wg_;
   uint64_t gram_dev_version_ = ~0ull;
 
   void build_derived();  // xt + missing lists from xs

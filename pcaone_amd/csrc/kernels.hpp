@@ -99,7 +99,10 @@ void launch_flip_norms(const double* O2, const double* Q, uint64_t nrows, int lp
 //           Householder signs of Xtop (R2 R1)^-1, Xtop the first n rows of X.
 // A Cholesky breakdown sets *status = 1 (the outputs are then zero).
 bool thinq_on_device(int lp);
-void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s);
+// min_ratio > 0: also set *status = 2 when the smallest Cholesky pivot is at
+// most min_ratio times the largest diagonal entry (H near rank deficient).
+void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s,
+                        double min_ratio = 0.0);
 void launch_thinq_second(const double* G2, const double* R1, const double* Xtop, int n, int lp, double* T,
                          int* status, hipStream_t s);
 // flipOmg's column signs from the norms of launch_flip_norms: -1 where

@@ -156,6 +156,56 @@ __device__ __forceinline__ int swz(int row, int v) {
   return v ^ (row & (sizeof(typename FT<F>::w_t) == 2 ? 7 : (VPR < 16 ? VPR - 1 : 15)));
 }
 
+// FP8 form, 16x16x128 MFMAs with E4M3 on both sides: substep m takes words 2m
+// and 2m + 1 of the lane group's KS / 64 (codes (KS / 4) q + 32 m + [0, 32), in
+// order), decoded to E4M3 (+2 = 0x40, 0, -2 = 0xC0: the same bytes as the BF16
+// top bytes); the B fragment of a piece is the same 32 K slots of its W row,
+// two 16-byte vectors.  The pieces accumulate into one FP32 set.  The decode
+// and the MFMAs are separate so that the main loop can issue the next step's
+// loads between them.
+template <int WM>
+__device__ __forceinline__ void f8_decode(i32x8_t (&af)[FT<Form::F8>::KS / 128][WM],
+                                          const aword_t<Form::F8> (&aw)[WM]) {
+#pragma unroll
+  for (int m = 0; m < FT<Form::F8>::KS / 128; ++m)
+#pragma unroll
+    for (int rb = 0; rb < WM; ++rb)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) af[m][rb][4 * i + t] = (int)top_bytes(spread(aw[rb][2 * m + i], t));
+}
+
+template <int WM, int NC>
+__device__ __forceinline__ void f8_mfma(f32x4_t (&acc)[WM][NC], const i32x8_t (&af)[FT<Form::F8>::KS / 128][WM],
+                                        const uint8_t* wl, int q, int r16) {
+  using T = FT<Form::F8>;
+  constexpr int LP = NC * 16;
+  constexpr int LDSP = T::KS;
+  constexpr int NSUB = T::KS / 128;
+  constexpr int VQ = T::KS / 64;  // 16-byte W vectors per lane group
+  auto read_b = [&](int m, int cb, int p) {
+    const uint8_t* row = wl + (p * LP + cb * 16 + r16) * LDSP;
+    const u32x4_t v0 = *reinterpret_cast<const u32x4_t*>(row + swz<Form::F8>(r16, VQ * q + 2 * m) * 16);
+    const u32x4_t v1 = *reinterpret_cast<const u32x4_t*>(row + swz<Form::F8>(r16, VQ * q + 2 * m + 1) * 16);
+    i32x8_t b;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) b[e] = (int)v0[e], b[4 + e] = (int)v1[e];
+    return b;
+  };
+#pragma unroll
+  for (int m = 0; m < NSUB; ++m)
+#pragma unroll
+    for (int cb = 0; cb < NC; ++cb)
+#pragma unroll
+      for (int p = 0; p < T::P; ++p) {
+        const i32x8_t bv = read_b(m, cb, p);
+#pragma unroll
+        for (int rb = 0; rb < WM; ++rb)  // scale operands 0: the plain (unscaled) product
+          acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[m][rb], bv, acc[rb][cb], 0, 0, 0, 0, 0, 0);
+      }
+}
+
 // All MFMAs of one K-step for one wave.
 template <Form F, int WM, int NC>
 __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][NC], const aword_t<F> (&aw)[WM],
@@ -191,43 +241,9 @@ __device__ __forceinline__ void compute_step(accv_t<F> (&acc)[FT<F>::NACC][WM][N
                 af[rb], __builtin_bit_cast(i32x4_t, bv[cb * T::P + p]), acc[p][rb][cb], 0, 0, 0);
     }
   } else if constexpr (F == Form::F8) {
-    // 16x16x128 FP8 MFMAs, E4M3 on both sides: substep m takes words 2m and
-    // 2m + 1 of the lane group's KS / 64 (codes (KS / 4) q + 32 m + [0, 32),
-    // in order), decoded to E4M3 (+2 = 0x40, 0, -2 = 0xC0: the same bytes as
-    // the BF16 top bytes); the B fragment of a piece is the same 32 K slots of
-    // its W row, two 16-byte vectors.  The pieces accumulate into one FP32 set.
-    constexpr int NSUB = T::KS / 128;
-    constexpr int VQ = T::KS / 64;  // 16-byte W vectors per lane group
-    i32x8_t af[NSUB][WM];
-#pragma unroll
-    for (int m = 0; m < NSUB; ++m)
-#pragma unroll
-      for (int rb = 0; rb < WM; ++rb)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int t = 0; t < 4; ++t) af[m][rb][4 * i + t] = (int)top_bytes(spread(aw[rb][2 * m + i], t));
-    auto read_b = [&](int m, int cb, int p) {
-      const w_t* row = wl + (p * LP + cb * 16 + r16) * LDSP;
-      const u32x4_t v0 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, VQ * q + 2 * m) * 16);
-      const u32x4_t v1 = *reinterpret_cast<const u32x4_t*>(row + swz<F>(r16, VQ * q + 2 * m + 1) * 16);
-      i32x8_t b;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) b[e] = (int)v0[e], b[4 + e] = (int)v1[e];
-      return b;
-    };
-#pragma unroll
-    for (int m = 0; m < NSUB; ++m)
-#pragma unroll
-      for (int cb = 0; cb < NC; ++cb)
-#pragma unroll
-        for (int p = 0; p < T::P; ++p) {
-          const i32x8_t bv = read_b(m, cb, p);
-#pragma unroll
-          for (int rb = 0; rb < WM; ++rb)  // scale operands 0: the plain (unscaled) product
-            acc[0][rb][cb] =
-                __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[m][rb], bv, acc[0][rb][cb], 0, 0, 0, 0, 0, 0);
-        }
+    i32x8_t af[T::KS / 128][WM];
+    f8_decode<WM>(af, aw);
+    f8_mfma<WM, NC>(acc[0], af, wl, q, r16);
   } else if constexpr (F == Form::BF16 || F == Form::BF16X1 || F == Form::FP16) {
     // Fragment j of substep s + 1 is read from LDS into the registers of
     // fragment j of substep s right after that fragment's last MFMA, and the A
@@ -519,6 +535,9 @@ __device__ __forceinline__ void gemm_tile(const uint32_t* __restrict__ X, uint64
 #pragma unroll
         for (int rb = 0; rb < WM; ++rb) acur[rb] = anext[rb];
       }
+      // (Letting the next step's code words cross the barrier, a bare barrier
+      // after a wait for the tile DMA only, measured the same on one box: FP8
+      // 8.65-8.73 ms, BF16 10.89-11.03 ms per launch either way.)
       if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);  // this wave's tile loads have landed
       __syncthreads();
       buf ^= 1;

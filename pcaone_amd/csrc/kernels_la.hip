@@ -1081,7 +1081,9 @@ __device__ __forceinline__ void rm_store(const RegMat<NB>& m, double* dst, int l
 // upper R with A = R^T R; rows and columns >= n and the lower triangle end
 // zero.  Returns false (in every thread) on breakdown.  rowbuf: 2 lp.
 template <int NB>
-__device__ bool rb_chol_upper(RegMat<NB>& a, int n, int lp, double* rowbuf, int ty, int tx) {
+__device__ bool rb_chol_upper(RegMat<NB>& a, int n, int lp, double* rowbuf, int ty, int tx,
+                              double* min_pivot = nullptr) {
+  double dmin = INFINITY;
   if (ty == 0) {
 #pragma unroll
     for (int q = 0; q < NB; ++q) rowbuf[tx + 16 * q] = a.v[0][q];
@@ -1095,6 +1097,7 @@ __device__ bool rb_chol_upper(RegMat<NB>& a, int n, int lp, double* rowbuf, int 
       ok = false;
       break;
     }
+    dmin = fmin(dmin, d);
     const double rjj = sqrt(d), inv = 1.0 / rjj;
     double si[NB], sk[NB];
 #pragma unroll
@@ -1129,6 +1132,7 @@ __device__ bool rb_chol_upper(RegMat<NB>& a, int n, int lp, double* rowbuf, int 
       const int i = ty + 16 * p, k = tx + 16 * q;
       if (!ok || k < i || i >= n || k >= n) a.v[p][q] = 0.0;
     }
+  if (min_pivot) *min_pivot = dmin;
   return ok;
 }
 
@@ -1261,15 +1265,28 @@ __device__ void rb_householder_signs(RegMat<NB>& w, int n, int lp, double* buf, 
 template <int NB>
 __global__ void __launch_bounds__(kSmallThreads) k_thinq_first(const double* __restrict__ G1, int n, int lp,
                                                                double* __restrict__ R1, double* __restrict__ T1,
-                                                               int* __restrict__ status) {
+                                                               int* __restrict__ status, double min_ratio) {
   __shared__ double buf[5 * 16 * kThinqMaxNB];
+  __shared__ double dmax_s;
   const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
   RegMat<NB> a, x;
   rm_load(a, G1, n, lp, ty, tx);
-  const bool ok = rb_chol_upper(a, n, lp, buf, ty, tx);
+  if (min_ratio > 0.0) {
+    // largest diagonal entry of the Gram (before the factorization overwrites it)
+    if (threadIdx.x == 0) {
+      double m = 0.0;
+      for (int i = 0; i < n; ++i) m = fmax(m, G1[i * lp + i]);
+      dmax_s = m;
+    }
+  }
+  double dmin = 0.0;
+  const bool ok = rb_chol_upper(a, n, lp, buf, ty, tx, &dmin);
   if (ok) rb_inv_upper(a, x, n, lp, buf, ty, tx);
   else rm_zero(x);
   if (!ok && threadIdx.x == 0) *status = 1;
+  // the host's safety test of this Gram (smallest pivot / largest diagonal),
+  // when the caller defers it to the device: a near rank-deficient H is marked
+  if (ok && min_ratio > 0.0 && threadIdx.x == 0 && !(dmin > min_ratio * dmax_s)) *status = 2;
   rm_store(a, R1, lp, ty, tx);
   rm_store(x, T1, lp, ty, tx);
 }
@@ -1722,14 +1739,15 @@ void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
 
 bool thinq_on_device(int lp) { return lp % 16 == 0 && lp <= 16 * kThinqMaxNB; }
 
-void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s) {
+void launch_thinq_first(const double* G1, int n, int lp, double* R1, double* T1, int* status, hipStream_t s,
+                        double min_ratio) {
   require(thinq_on_device(lp) && n <= lp, kErrInternal, "thinq: lp too large for the device path");
   switch (lp / 16) {
-    case 1: hipLaunchKernelGGL(k_thinq_first<1>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
-    case 2: hipLaunchKernelGGL(k_thinq_first<2>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
-    case 3: hipLaunchKernelGGL(k_thinq_first<3>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
-    case 4: hipLaunchKernelGGL(k_thinq_first<4>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
-    default: hipLaunchKernelGGL(k_thinq_first<5>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status); break;
+    case 1: hipLaunchKernelGGL(k_thinq_first<1>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status, min_ratio); break;
+    case 2: hipLaunchKernelGGL(k_thinq_first<2>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status, min_ratio); break;
+    case 3: hipLaunchKernelGGL(k_thinq_first<3>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status, min_ratio); break;
+    case 4: hipLaunchKernelGGL(k_thinq_first<4>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status, min_ratio); break;
+    default: hipLaunchKernelGGL(k_thinq_first<5>, dim3(1), dim3(kSmallThreads), 0, s, G1, n, lp, R1, T1, status, min_ratio); break;
   }
   check_launch("thinq_first");
 }

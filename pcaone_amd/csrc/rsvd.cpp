@@ -266,31 +266,59 @@ class Segments {
 }  // namespace
 
 void FancyRsvdOpData::computeGandH(int pi) {
-  // src/Halko.cpp:155-224 (in-core).  Windows starting past the last SNP
-  // (possible when M < bands^2) are treated as empty.
+  // src/Halko.cpp:155-224 (in-core) and :225-268 (out-of-core).  Windows
+  // starting past the last SNP (possible when M < bands^2) are treated as
+  // empty.
   const uint64_t bands = params_.bands;
   if (pi == 0) initOmg();
   if (std::pow(2, pi) >= (double)bands) {
     ctx_.zero_h(1);
     ctx_.zero_h(2);
   }
-  if (plan) {
-    computeGandH_blocks(pi);
-    return;
-  }
   if (pi == 0) {
     if (update) fit_with_pi();
     if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
-    bandsize_ = 1;
-    blocksize_ = (uint64_t)std::ceil((double)rows() / bands);
-    if (params_.perm && !(update && params_.emu)) {
-      std::vector<int> p(rows());
-      std::iota(p.begin(), p.end(), 0);
-      auto rng = std::default_random_engine{};
-      std::shuffle(p.data(), p.data() + p.size(), rng);  // src/RSVD.hpp:61-70
-      ctx_.permute_columns(p);
+    if (plan) {
+      bandsize_ = plan->band_factor;
+    } else {
+      bandsize_ = 1;
+      blocksize_ = (uint64_t)std::ceil((double)rows() / bands);
+      if (params_.perm && !(update && params_.emu)) {
+        std::vector<int> p(rows());
+        std::iota(p.begin(), p.end(), 0);
+        auto rng = std::default_random_engine{};
+        std::shuffle(p.data(), p.data() + p.size(), rng);  // src/RSVD.hpp:61-70
+        ctx_.permute_columns(p);
+      }
     }
   }
+  // The epoch's Omega updates run with their safety test on the device; if
+  // one fails it (H near rank deficient), the epoch is run again from its
+  // checkpoint with the host test, which takes the Householder fallback.
+  const bool defer = ctx_.deferred_ok();
+  const uint64_t bandsize0 = bandsize_, updates0 = omega_updates;
+  if (defer) {
+    ctx_.epoch_checkpoint();
+    ctx_.deferred_thinq = true;
+  }
+  auto run = [&] { plan ? windows_blocks(pi) : windows(pi); };
+  try {
+    run();
+  } catch (...) {
+    ctx_.deferred_thinq = false;
+    throw;
+  }
+  ctx_.deferred_thinq = false;
+  if (defer && ctx_.epoch_thinq_failed()) {
+    ctx_.epoch_rollback();
+    bandsize_ = bandsize0;
+    omega_updates = updates0;
+    run();
+  }
+}
+
+void FancyRsvdOpData::windows(int pi) {
+  const uint64_t bands = params_.bands;
   bandsize_ = std::min<uint64_t>(bandsize_ * 2, bands);
   const uint64_t M = rows();
   Segments seg(ctx_, !ctx_.streamed());
@@ -314,16 +342,11 @@ void FancyRsvdOpData::computeGandH(int pi) {
   seg.flush();
 }
 
-void FancyRsvdOpData::computeGandH_blocks(int pi) {
+void FancyRsvdOpData::windows_blocks(int pi) {
   // src/Halko.cpp:225-268: the windows are the plan's blocks and the bands
-  // grow from band_factor blocks; (update, standardize) as for the Normal op.
+  // grow from band_factor blocks.
   const BlockPlan& bp = *plan;
   const uint64_t bands = params_.bands;
-  if (pi == 0) {
-    if (update) fit_with_pi();
-    if (standardize && params_.scale == -9) ctx_.standardize(params_.ploidy);
-    bandsize_ = bp.band_factor;
-  }
   bandsize_ = std::min<uint64_t>(bandsize_ * 2, bp.nblocks);
   Segments seg(ctx_, !ctx_.streamed());  // streamed blocks are loaded one at a time
   for (uint64_t b = 0, i = 1; b < bp.nblocks; ++b, ++i) {

@@ -119,7 +119,8 @@ class FancyRsvdOpData : public RsvdOpData {
 
  private:
   uint64_t bandsize_ = 1, blocksize_ = 0;
-  void computeGandH_blocks(int pi);
+  void windows(int pi);         // the epoch's window schedule (in-core)
+  void windows_blocks(int pi);  // the same over the out-of-core plan's blocks
 };
 
 struct PcaResult {
