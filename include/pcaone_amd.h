@@ -97,6 +97,9 @@ typedef struct pcaone_params {
                         /* stores dosages (without it such a file is rejected)      */
   uint32_t threads;     /* -n/--threads (12): with -m, a .pgen's shuffle depends on */
                         /* min(threads, host cores) (src/Main.cpp:53-54,169-172)    */
+  int miss_gather;      /* 1: missing-entry corrections gather their source rows    */
+                        /* per entry from L2 (the CSR form) even where the tiled    */
+                        /* form applies (reduced precisions with L <= 32)           */
 } pcaone_params;
 
 typedef struct pcaone_run_info {

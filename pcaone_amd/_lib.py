@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("waves_h", C.c_int), ("kchunk_g", C.c_uint64),
                 ("kchunk_h", C.c_uint64), ("memory", C.c_double), ("buffer", C.c_uint32), ("ld", C.c_int),
                 ("ld_stats", C.c_int), ("imaxiter", C.c_uint32), ("itol", C.c_double), ("stream", C.c_int),
-                ("hardcall", C.c_int), ("threads", C.c_uint32)]
+                ("hardcall", C.c_int), ("threads", C.c_uint32), ("miss_gather", C.c_int)]
 
 
 class RunInfo(C.Structure):

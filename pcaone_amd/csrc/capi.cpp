@@ -115,6 +115,7 @@ Param to_param(const pcaone_params* p) {
   q.tuning.waves_h = p->waves_h;
   q.tuning.kchunk_g = p->kchunk_g;
   q.tuning.kchunk_h = p->kchunk_h;
+  q.tuning.miss_gather = p->miss_gather != 0;
   q.memory = p->memory;
   q.buffer = p->buffer;
   q.ld = p->ld != 0;

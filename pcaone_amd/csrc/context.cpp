@@ -236,6 +236,97 @@ void Context::build_lists(uint64_t rows, uint64_t ncodes_t) {
   const uint64_t nt = lists(xt_.get(), pitch_t_, N, ncodes_t, rp_t_, ix_t_);
   require(ns == nt && (streamed() || ns == nmiss), kErrInternal, "missing-entry lists disagree with the code counts");
   lnmiss_ = ns;
+  mt_s_.built = mt_t_.built = false;
+}
+
+void Context::build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf<uint32_t>& ix, uint64_t nrows,
+                          uint64_t kext, uint32_t span) {
+  // (1) entries per (row block, granule) on the device; (2) the tiles of each
+  // block on the host: runs of granules from the first one holding entries,
+  // as long as they hold at most kMissEMax entries and span at most `span`
+  // indexes; (3) row offsets and local indexes on the device.
+  require(kext <= 0xffffffffull && span >= (uint32_t)kMissKG, kErrInternal, "build_tiles: extent");
+  const uint64_t nb = (nrows + kMissRB - 1) / kMissRB;
+  const uint64_t ngran = (kext + kMissKG - 1) / kMissKG;
+  DevBuf<uint32_t> cnt(std::max<uint64_t>(nb * ngran, 1));
+  PCAONE_HIP(hipMemsetAsync(cnt.get(), 0, cnt.bytes(), stream));
+  launch_miss_hist(rp.get(), ix.get(), nrows, (uint32_t)ngran, cnt.get(), stream);
+  std::vector<uint32_t> h(nb * ngran);
+  if (!h.empty()) PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), h.size() * 4, hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  t.h_tiles.clear();
+  t.h_tb.assign(nb + 1, 0);
+  uint64_t off = 0;
+  const uint64_t gspan = span / kMissKG;
+  for (uint64_t b = 0; b < nb; ++b) {
+    const uint32_t* c = h.data() + b * ngran;
+    uint64_t g = 0;
+    while (true) {
+      while (g < ngran && c[g] == 0) ++g;
+      if (g == ngran) break;
+      const uint64_t g0 = g;
+      uint64_t n = 0;
+      while (g < ngran && g - g0 < gspan && n + c[g] <= (uint64_t)kMissEMax) n += c[g++];
+      MissTile tl{};
+      tl.e0 = off;
+      tl.k0 = (uint32_t)(g0 * kMissKG);
+      tl.k1 = (uint32_t)std::min<uint64_t>(g * kMissKG, kext);
+      tl.ne = (uint32_t)n;
+      t.h_tiles.push_back(tl);
+      off += round_up(n, 8);
+    }
+    t.h_tb[b + 1] = (uint32_t)t.h_tiles.size();
+  }
+  const uint64_t ntiles = t.h_tiles.size();
+  t.tiles.resize(std::max<uint64_t>(ntiles, 1));
+  t.tb.resize(nb + 1);
+  t.trow.resize(std::max<uint64_t>(ntiles, 1) * kMissRowStride);
+  t.eix.resize(std::max<uint64_t>(off, 8));
+  t.ew.resize(std::max<uint64_t>(off, 8));
+  if (ntiles)
+    PCAONE_HIP(hipMemcpyAsync(t.tiles.get(), t.h_tiles.data(), ntiles * sizeof(MissTile), hipMemcpyHostToDevice,
+                              stream));
+  PCAONE_HIP(hipMemcpyAsync(t.tb.get(), t.h_tb.data(), (nb + 1) * 4, hipMemcpyHostToDevice, stream));
+  DevBuf<int> st(1);
+  PCAONE_HIP(hipMemsetAsync(st.get(), 0, sizeof(int), stream));
+  launch_miss_fill(rp.get(), ix.get(), nrows, t.tiles.get(), t.tb.get(), nb, t.trow.get(), t.eix.get(), st.get(),
+                   stream);
+  int bad = 0;
+  PCAONE_HIP(hipMemcpyAsync(&bad, st.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));  // the host copies and cnt go out of scope
+  require(!bad, kErrInternal, "missing-entry tiles disagree with their planned entry counts");
+  t.nrows = nrows;
+  t.kext = kext;
+  t.nblocks = nb;
+  t.span = span;
+  t.built = true;
+}
+
+int Context::miss_parts(const MissTiles& t, uint64_t r_lo, uint64_t r_hi, uint64_t k_lo, uint64_t k_hi) const {
+  // Workgroups per row block (one workgroup per CU at a time): the number of
+  // parts that best fills the last round of workgroups, a larger count taken
+  // only for 2 % more filled slots; at least 8 tiles per part (the first
+  // block's tiles in the index range stand for all), and the partial rows
+  // must fit their buffer.
+  const uint64_t nblk = (r_hi - 1) / kMissRB - r_lo / kMissRB + 1;
+  const uint64_t b = r_lo / kMissRB;
+  uint64_t nt = 0;
+  for (uint32_t i = t.h_tb[b]; i < t.h_tb[b + 1]; ++i)
+    nt += t.h_tiles[i].k1 > k_lo && t.h_tiles[i].k0 < k_hi;
+  const uint64_t slots = std::max<uint64_t>(cus_, 1);
+  const uint64_t cap = mpart_.size() / std::max<uint64_t>((r_hi - r_lo) * (uint64_t)L, 1);
+  const uint64_t pmax = std::min<uint64_t>({std::max<uint64_t>(1, nt / 8), std::max<uint64_t>(cap, 1), 64});
+  uint64_t best = 1;
+  double best_fill = 0.0;
+  for (uint64_t p = 1; p <= pmax; ++p) {
+    const uint64_t total = nblk * p;
+    const double fill = (double)total / (double)(((total + slots - 1) / slots) * slots);
+    if (fill > best_fill + 0.02) {
+      best = p;
+      best_fill = fill;
+    }
+  }
+  return (int)best;
 }
 
 void Context::filter_snps(const std::vector<uint32_t>& keep) {
@@ -457,6 +548,15 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
     csh_.resize(rows_s_ * cpitch_ * esz);
     mpart_.resize(kMissPartialElems);
   }
+  // the tiled corrections take FP32 sources of at most kMissMaxCp columns;
+  // tiles are planned for the span that this L's source rows allow
+  const bool tiled = nmiss && miss_f32_ && cp_ <= kMissMaxCp && !tuning.miss_gather;
+  if (tiled != miss_tiled_) weights_dirty_ = true;
+  miss_tiled_ = tiled;
+  if (tiled && mt_s_.span != miss_tile_span(cp_)) {
+    mt_s_.built = mt_t_.built = false;
+    weights_dirty_ = true;
+  }
   fixed_g_ = tuning.wm_g || tuning.waves_g;
   fixed_h_ = tuning.wm_h || tuning.waves_h;
   tuning.kchunk_g = round_up(tuning.kchunk_g, kKStep);  // 0 stays 0 (planned per launch)
@@ -618,7 +718,28 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     launch_epi_g(form_g, part_.get(), nch_g, g.ychunk, nwin, nwin, lp, sc_.get() + j0, cF_.get() + j0,
                  colsum_.get(), gwin, stream, i8_mul(0));
   }
-  if (lnmiss_) {
+  if (lnmiss_ && miss_tiled_) {
+    update_missing_weights();
+    launch_compact_rows(omega_r_.get(), N, lp, L, cpitch_, true, csg_.get(), stream);
+    MissTiledSpmmArgs m{};
+    m.t = mt_s_.list();
+    m.ew = mt_s_.ew.get();
+    m.r_lo = jl;
+    m.r_hi = jl + nwin;
+    m.k_lo = 0;
+    m.k_hi = N;
+    m.src = reinterpret_cast<const float*>(csg_.get());
+    m.src_k0 = 0;
+    m.cp = cp_;
+    m.cpitch = cpitch_;
+    m.ncols = L;
+    m.parts = miss_parts(mt_s_, m.r_lo, m.r_hi, m.k_lo, m.k_hi);
+    m.partial = mpart_.get();
+    m.partial_cap = mpart_.size();
+    m.out = gwin;
+    m.lp = lp;
+    launch_missing_spmm_tiled(m, stream);
+  } else if (lnmiss_) {
     update_missing_weights();
     launch_compact_rows(omega_r_.get(), N, lp, L, cpitch_, miss_f32_, csg_.get(), stream);
     MissSpmmArgs m{};
@@ -684,7 +805,27 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
   double* target = (hmode == 0 || hmode == 3) ? H_.get() : (hmode == 1 ? H1_.get() : H2_.get());
   launch_epi_h(form_h, part_.get(), nch_h, h.ychunk, N, rows_t_, lp, bvec_.get(), target, hmode == 0 ? 0 : 1, stream,
                i8_mul(1));
-  if (lnmiss_) {
+  if (lnmiss_ && miss_tiled_) {
+    launch_compact_rows(gwin, nwin, lp, L, cpitch_, true, csh_.get(), stream);
+    MissTiledSpmmArgs m{};
+    m.t = mt_t_.list();
+    m.ew = mt_t_.ew.get();
+    m.r_lo = 0;
+    m.r_hi = N;
+    m.k_lo = jl;
+    m.k_hi = jl + nwin;
+    m.src = reinterpret_cast<const float*>(csh_.get());
+    m.src_k0 = jl;
+    m.cp = cp_;
+    m.cpitch = cpitch_;
+    m.ncols = L;
+    m.parts = miss_parts(mt_t_, m.r_lo, m.r_hi, m.k_lo, m.k_hi);
+    m.partial = mpart_.get();
+    m.partial_cap = mpart_.size();
+    m.out = target;
+    m.lp = lp;
+    launch_missing_spmm_tiled(m, stream);
+  } else if (lnmiss_) {
     launch_compact_rows(gwin, nwin, lp, L, cpitch_, miss_f32_, csh_.get(), stream);
     MissSpmmArgs m{};
     m.rp = rp_t_.get();
@@ -735,10 +876,42 @@ uint64_t Context::miss_span(uint64_t nrows, uint64_t kext) const {
 
 void Context::update_missing_weights() {
   if (!lnmiss_ || !weights_dirty_) return;
+  const uint64_t xo = xoff();
+  if (miss_tiled_) {
+    const uint32_t span = miss_tile_span(cp_);
+    if (!mt_s_.built || mt_s_.span != span) build_tiles(mt_s_, rp_s_, ix_s_, lrows_, N, span);
+    if (!mt_t_.built || mt_t_.span != span) build_tiles(mt_t_, rp_t_, ix_t_, N, lrows_, span);
+    if (emu_k_) launch_pack_col_state(vimp_.get(), M, emu_kp_, emu_k_, sc_.get(), cF_.get(), dF_.get(), stream);
+    const auto weights = [&](MissTiles& t, bool by_snp) {
+      MissTiledWeightArgs a{};
+      a.t = t.list();
+      a.ew = t.ew.get();
+      a.nrows = t.nrows;
+      a.nblocks = t.nblocks;
+      a.by_snp = by_snp;
+      a.sc = sc_.get() + xo;
+      a.cf = cF_.get() + xo;
+      a.Fclamp = emu_k_ ? dF_.get() + xo : nullptr;
+      a.US = us_.get();
+      a.V = emu_k_ ? vimp_.get() + xo * emu_kp_ : nullptr;
+      a.kk = emu_k_;
+      a.kp = emu_kp_;
+      const uint64_t per_block = t.nblocks ? t.h_tiles.size() / t.nblocks : 0;
+      a.parts = (int)std::min<uint64_t>({std::max<uint64_t>(1, (2 * std::max<uint64_t>(cus_, 1) + t.nblocks - 1) /
+                                                                   std::max<uint64_t>(t.nblocks, 1)),
+                                         std::max<uint64_t>(1, per_block / 4), 65535});
+      launch_missing_weights_tiled(a, stream);
+    };
+    weights(mt_s_, true);
+    weights(mt_t_, false);
+    weights_dirty_ = false;
+    weights_tiled_ = true;
+    return;
+  }
+  weights_tiled_ = false;
   w_s_.resize(lnmiss_);
   w_t_.resize(lnmiss_);
   // the lists index SNPs from the first loaded row: the per-SNP state too
-  const uint64_t xo = xoff();
   MissWeightArgs a{};
   a.sc = sc_.get() + xo;
   a.cf = cF_.get() + xo;

@@ -6,7 +6,8 @@
 //   xt  sample-major interleaved words, rows_t x pitch_t (product H = A^T * G)
 //   cF, sc, F   per-column centering frequency, scale, and allele frequency
 //               in the reference's (unpermuted) SNP order
-//   miss_s/t    CSR lists of missing entries by SNP and by sample
+//   miss_s/t    CSR lists of missing entries by SNP and by sample, and for the
+//               reduced forms the same lists in LDS-sized tiles
 //   omega       Omega (FP64, N x lp, row-major) == Omg2 of the reference
 //   omega_r     the Omega actually multiplied (sum of operand pieces)
 //   wg          Omega operand pieces, transposed [piece][col][sample]
@@ -23,6 +24,7 @@
 
 #include "device.hpp"
 #include "gemm.hpp"
+#include "kernels.hpp"
 #include "linalg_host.hpp"
 
 namespace pcaone {
@@ -43,6 +45,7 @@ struct GemmTuning {
   int waves_h = 0;       // waves per workgroup of the H product
   uint64_t kchunk_g = 0; // split-K chunk of the G product in codes (0 = auto)
   uint64_t kchunk_h = 0; // split-K chunk of the H product in codes (0 = auto)
+  bool miss_gather = false;  // missing-entry corrections from the CSR lists even where the tiles apply
 };
 
 struct EpochResult {
@@ -241,6 +244,27 @@ class Context {
   DevBuf<double> w_s_, w_t_;   // per-entry weights of the missing-entry corrections
   bool weights_dirty_ = true;
   void update_missing_weights();
+  // The missing-entry lists in tiles (FP32 weights in tile order) for the
+  // reduced forms with L <= kMissMaxCp: a tile's source rows are read once
+  // per tile from LDS instead of once per entry from L2.
+  struct MissTiles {
+    std::vector<MissTile> h_tiles;
+    std::vector<uint32_t> h_tb;
+    DevBuf<MissTile> tiles;
+    DevBuf<uint32_t> tb;
+    DevBuf<uint16_t> trow, eix;
+    DevBuf<float> ew;
+    uint64_t nrows = 0, kext = 0, nblocks = 0;
+    uint32_t span = 0;
+    bool built = false;
+    MissTileList list() const { return {tiles.get(), tb.get(), trow.get(), eix.get()}; }
+  };
+  MissTiles mt_s_, mt_t_;
+  bool miss_tiled_ = false;     // this run's corrections use the tiles
+  bool weights_tiled_ = false;  // the current weights are the tiles' (else the CSR lists')
+  void build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf<uint32_t>& ix, uint64_t nrows,
+                   uint64_t kext, uint32_t span);
+  int miss_parts(const MissTiles& t, uint64_t r_lo, uint64_t r_hi, uint64_t k_lo, uint64_t k_hi) const;
   // compact sources of the sparse corrections (Omega~ rows, G~ window rows)
   DevBuf<unsigned char> csg_, csh_;
   DevBuf<double> mpart_;

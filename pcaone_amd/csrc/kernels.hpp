@@ -224,5 +224,89 @@ struct MissSpmmArgs {
 // Adds the corrections to out; the source rows are gathered from L2 in index
 // chunks of kspan rows (partial rows summed in chunk order).
 void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s);
+// out[r][l] += sum_z partial[z][r][l] in z order (nz x nrows x ncols partials; out pitch lp).
+void launch_sum_chunks(const double* partial, uint64_t nz, uint64_t nrows, int ncols, double* out, int lp,
+                       hipStream_t s);
+
+// ---- tiled missing-entry corrections (kernels_miss.hip) -------------------
+// A missing-entry list (CSR rows, sorted indexes) re-laid in tiles: row blocks
+// of kMissRB list rows, each split along the index range into tiles of whole
+// kMissKG-index granules that hold at most kMissEMax entries and span at most
+// `span` indexes.  A tile's entries are stored together (row by row, indexes
+// ascending within a row) as 16-bit indexes local to the tile and FP32 weights,
+// with kMissRowStride 16-bit row offsets per tile.  One workgroup stages a
+// whole tile - row offsets, entries and the source rows of its index range -
+// into LDS and every lane group owns fixed rows of the block, so a source row
+// is read from HBM or L2 once per tile rather than once per entry.
+constexpr int kMissRB = 384;
+constexpr int kMissKG = 16;
+constexpr int kMissEMax = kMissRB * kMissKG;  // one granule always fits
+constexpr int kMissRowStride = 392;           // kMissRB + 1 offsets, padded to 16 bytes
+constexpr int kMissThreads = 1024;
+constexpr int kMissMaxCp = 32;                // FP32 source rows of at most 32 columns
+constexpr uint32_t kMissLdsBuf = 80 * 1024;   // bytes of one staging buffer (two per workgroup)
+
+struct MissTile {
+  uint64_t e0;      // first entry (a multiple of 8)
+  uint32_t k0, k1;  // index range [k0, k1)
+  uint32_t ne;      // entries
+  uint32_t pad;
+};
+
+// Largest index span of a tile whose FP32 source rows of cp columns fit a
+// staging buffer beside the row offsets and kMissEMax entries.
+uint32_t miss_tile_span(int cp);
+// Entry counts per (row block, granule): cnt[b * ngran + g] += entries of the
+// block's rows with index in granule g (cnt zeroed by the caller).
+void launch_miss_hist(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, uint32_t ngran, uint32_t* cnt,
+                      hipStream_t s);
+// Fills the tiles' row offsets and local indexes from the CSR list (tiles and
+// tb = first tile of each row block, planned on the host from the counts);
+// *status = 1 if a tile's rows hold other than its planned entry count.
+void launch_miss_fill(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const MissTile* tiles,
+                      const uint32_t* tb, uint64_t nblocks, uint16_t* trow, uint16_t* eix, int* status,
+                      hipStream_t s);
+
+struct MissTileList {
+  const MissTile* tiles;
+  const uint32_t* tb;   // nblocks + 1
+  const uint16_t* trow;
+  const uint16_t* eix;
+};
+// Weights of a tiled list, as MissWeightArgs describes them, in tile order
+// (FP32).  EMU: the row side's k-vectors of a block and the index side's of a
+// tile (`sub` rows at a time) sit in LDS.
+struct MissTiledWeightArgs {
+  MissTileList t;
+  float* ew;
+  uint64_t nrows, nblocks;
+  bool by_snp;
+  const double* sc;
+  const double* cf;
+  const double* Fclamp;  // null = no EMU
+  const double* US;
+  const double* V;
+  int kk, kp;
+  int parts;             // workgroups per row block (tile ranges)
+};
+void launch_missing_weights_tiled(const MissTiledWeightArgs& a, hipStream_t s);
+// out[r - r_lo][:ncols] (+)= sum over the entries of list rows r in [r_lo,
+// r_hi) with index k in [k_lo, k_hi) of w * src[k - src_k0][:ncols]; src are
+// FP32 compact rows (cp columns, pitch cpitch).  With parts > 1 each part
+// writes partial rows ([part][r - r_lo][ncols]) that are summed in part order.
+struct MissTiledSpmmArgs {
+  MissTileList t;
+  const float* ew;
+  uint64_t r_lo, r_hi, k_lo, k_hi;
+  const float* src;
+  uint64_t src_k0;
+  int cp, cpitch, ncols;
+  int parts;
+  double* partial;
+  uint64_t partial_cap;
+  double* out;
+  int lp;
+};
+void launch_missing_spmm_tiled(const MissTiledSpmmArgs& a, hipStream_t s);
 
 }  // namespace pcaone

@@ -1730,11 +1730,15 @@ void launch_missing_spmm(const MissSpmmArgs& a, hipStream_t s) {
   if (a.f32) hipLaunchKernelGGL((k_missing_spmm<float>), dim3(gx, 1, gz), block, 0, s, a);
   else hipLaunchKernelGGL((k_missing_spmm<double>), dim3(gx, 1, gz), block, 0, s, a);
   check_launch("missing_spmm");
-  if (nz > 1) {
-    hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(a.nrows * (uint64_t)a.ncols)), dim3(256), 0, s, a.partial, nz,
-                       a.nrows, a.ncols, a.out, a.lp);
-    check_launch("sum_chunks");
-  }
+  if (nz > 1) launch_sum_chunks(a.partial, nz, a.nrows, a.ncols, a.out, a.lp, s);
+}
+
+void launch_sum_chunks(const double* partial, uint64_t nz, uint64_t nrows, int ncols, double* out, int lp,
+                       hipStream_t s) {
+  if (!nrows || !ncols) return;
+  hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(nrows * (uint64_t)ncols)), dim3(256), 0, s, partial, nz, nrows,
+                     ncols, out, lp);
+  check_launch("sum_chunks");
 }
 
 bool thinq_on_device(int lp) { return lp % 16 == 0 && lp <= 16 * kThinqMaxNB; }

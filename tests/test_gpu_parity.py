@@ -226,6 +226,68 @@ def test_missing_entries_with_chunked_sources(ctx, emu):
     np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-8)
 
 
+TILE_SPECS = {  # 1,000 x 3,000: 8 row blocks of SNPs, 3 of samples
+    # dense: tiles bounded by the entry cap (about 30 indexes each), several per row block
+    "dense": dict(seed=31, nsamples=1000, nsnps=3000, npops=3, miss_rate=0.5, high_miss_frac=0.05,
+                  high_miss_rate=0.9),
+    # sparse: tiles bounded by the index span the source rows may take in LDS
+    "sparse": dict(seed=32, nsamples=1000, nsnps=3000, npops=3, miss_rate=0.02),
+}
+
+
+@pytest.mark.parametrize("density", ["dense", "sparse"])
+@pytest.mark.parametrize("L", [4, 12, 20, 32])
+def test_tiled_corrections_match_oracle(ctx, density, L):
+    """The reduced forms' missing-entry corrections run on tiles of the two
+    lists (row blocks of 384 rows; source rows, indexes and FP32 weights
+    staged in LDS).  L = 4, 12, 20, 32 take 1, 3, 5 and 8 lanes per entry.
+    The products of whole and cut windows (a window starting inside a row
+    block, tiles cut by the window's SNP range, several parts per row block)
+    equal A Omega and A^T G of the FP64 oracle, where missing entries are 0
+    after centering."""
+    from pcaone_amd import synth
+    spec = synth.Spec(**TILE_SPECS[density])
+    say(f"tiled corrections {density} L={L}")
+    ctx.generate(spec)
+    n, m = spec.nsamples, spec.nsnps
+    d = O.Data.from_packed(synth.packed(spec), n, O.Params(svd=1, k=2))
+    d.standardize_E()
+    om = O.omega(112, True, n, L)
+    ctx.configure("fp32", L)
+    ctx.standardize(2)  # scales the columns once: sc *= sqrt(2) / sd
+    for form, tol in (("fp32", 1e-5), ("bf16", 1e-4)):
+        ctx.configure(form, L)
+        ctx.set_omega(om)
+        for j0, j1 in ((0, m), (100, 357), (383, 1900)):
+            ctx.sketch(j0, j1, 0)
+            G = d.G[:, j0:j1].T @ om
+            H = d.G[:, j0:j1] @ G
+            np.testing.assert_allclose(ctx.get_g(j0, j1, L), G, rtol=0, atol=tol * np.abs(G).max())
+            np.testing.assert_allclose(ctx.get_h(0, L), H, rtol=0, atol=tol * np.abs(H).max())
+
+
+@pytest.mark.parametrize("svd", [1, 2])
+def test_tiled_emu_runs_match_gather_and_oracle(ctx, svd):
+    """EMU runs with the tiled corrections and tiled imputation weights (U S
+    and V rows staged in LDS) against the per-entry gather form of the same
+    precision and against the FP64 oracle."""
+    from pcaone_amd import synth
+    spec = synth.Spec(seed=33, nsamples=1000, nsnps=3000, npops=3, miss_rate=0.2, high_miss_frac=0.05,
+                      high_miss_rate=0.6)
+    say(f"tiled EMU run svd={svd}")
+    ctx.generate(spec)
+    kw = dict(svd=svd, k=4, oversamples=6, maxp=4, tol=0.0, bands=4, emu=True, maxiter=3, tolem=0.0)
+    ref = O.run_pca_with_halko(O.Data.from_packed(synth.packed(spec), spec.nsamples, O.Params(**kw)))
+    kw["emu"] = 1
+    tiled = ctx.run(api.default_params(precision="fp32", **kw))
+    gather = ctx.run(api.default_params(precision="fp32", miss_gather=1, **kw))
+    assert tiled.info["em_iters"] == gather.info["em_iters"] == len(ref.em_trace)
+    np.testing.assert_allclose(tiled.E, gather.E, rtol=1e-6)
+    assert O.pc_correlations(gather.U, tiled.U).min() > 1 - 1e-6
+    assert np.max(np.abs(tiled.E - ref.E) / ref.E) <= 1e-4
+    assert O.pc_correlations(ref.U, tiled.U).min() >= 0.9999
+
+
 @pytest.mark.parametrize("k", [8, 20, 30, 40, 50])
 def test_thin_q_paths_match_oracle(ctx, golden_inputs, k):
     """The Omega update (thin-Q of H with Householder signs, then flipOmg) runs
