@@ -2,6 +2,7 @@
 #include "context.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 
@@ -240,7 +241,7 @@ void Context::build_lists(uint64_t rows, uint64_t ncodes_t) {
 }
 
 void Context::build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf<uint32_t>& ix, uint64_t nrows,
-                          uint64_t kext, uint32_t span) {
+                          uint64_t kext, uint32_t span, bool balance) {
   // (1) entries per (row block, granule) on the device; (2) the tiles of each
   // block on the host: runs of granules from the first one holding entries,
   // as long as they hold at most kMissEMax entries and span at most `span`
@@ -248,9 +249,34 @@ void Context::build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf
   require(kext <= 0xffffffffull && span >= (uint32_t)kMissKG, kErrInternal, "build_tiles: extent");
   const uint64_t nb = (nrows + kMissRB - 1) / kMissRB;
   const uint64_t ngran = (kext + kMissKG - 1) / kMissKG;
+  t.rperm.release();
+  t.rpos.release();
+  mwork_.clear();
+  if (balance && nrows > 1) {
+    // the lane groups of a workgroup own fixed rows: rows of like entry
+    // counts share a block, so no group waits for one with a much longer row
+    std::vector<uint64_t> hp(nrows + 1);
+    PCAONE_HIP(hipMemcpyAsync(hp.data(), rp.get(), hp.size() * 8, hipMemcpyDeviceToHost, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+    std::vector<uint32_t> order(nrows);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return hp[x + 1] - hp[x] > hp[y + 1] - hp[y]; });
+    std::vector<uint32_t> perm(nb * kMissRB, 0xffffffffu), pos(nrows);
+    for (uint64_t p = 0; p < nrows; ++p) {
+      perm[p] = order[p];
+      pos[order[p]] = (uint32_t)p;
+    }
+    t.rperm.resize(perm.size());
+    t.rpos.resize(nrows);
+    PCAONE_HIP(hipMemcpyAsync(t.rperm.get(), perm.data(), perm.size() * 4, hipMemcpyHostToDevice, stream));
+    PCAONE_HIP(hipMemcpyAsync(t.rpos.get(), pos.data(), nrows * 4, hipMemcpyHostToDevice, stream));
+    PCAONE_HIP(hipStreamSynchronize(stream));
+  }
   DevBuf<uint32_t> cnt(std::max<uint64_t>(nb * ngran, 1));
   PCAONE_HIP(hipMemsetAsync(cnt.get(), 0, cnt.bytes(), stream));
-  launch_miss_hist(rp.get(), ix.get(), nrows, (uint32_t)ngran, cnt.get(), stream);
+  launch_miss_hist(rp.get(), ix.get(), nrows, t.rpos.size() ? t.rpos.get() : nullptr, (uint32_t)ngran, cnt.get(),
+                   stream);
   std::vector<uint32_t> h(nb * ngran);
   if (!h.empty()) PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), h.size() * 4, hipMemcpyDeviceToHost, stream));
   PCAONE_HIP(hipStreamSynchronize(stream));
@@ -273,7 +299,7 @@ void Context::build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf
       tl.k1 = (uint32_t)std::min<uint64_t>(g * kMissKG, kext);
       tl.ne = (uint32_t)n;
       t.h_tiles.push_back(tl);
-      off += round_up(n, 8);
+      off += miss_stored((uint32_t)n);
     }
     t.h_tb[b + 1] = (uint32_t)t.h_tiles.size();
   }
@@ -283,14 +309,17 @@ void Context::build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf
   t.trow.resize(std::max<uint64_t>(ntiles, 1) * kMissRowStride);
   t.eix.resize(std::max<uint64_t>(off, 8));
   t.ew.resize(std::max<uint64_t>(off, 8));
+  // the padding entries stay zero (the fill and the weights write [0, ne))
+  PCAONE_HIP(hipMemsetAsync(t.eix.get(), 0, t.eix.bytes(), stream));
+  PCAONE_HIP(hipMemsetAsync(t.ew.get(), 0, t.ew.bytes(), stream));
   if (ntiles)
     PCAONE_HIP(hipMemcpyAsync(t.tiles.get(), t.h_tiles.data(), ntiles * sizeof(MissTile), hipMemcpyHostToDevice,
                               stream));
   PCAONE_HIP(hipMemcpyAsync(t.tb.get(), t.h_tb.data(), (nb + 1) * 4, hipMemcpyHostToDevice, stream));
   DevBuf<int> st(1);
   PCAONE_HIP(hipMemsetAsync(st.get(), 0, sizeof(int), stream));
-  launch_miss_fill(rp.get(), ix.get(), nrows, t.tiles.get(), t.tb.get(), nb, t.trow.get(), t.eix.get(), st.get(),
-                   stream);
+  launch_miss_fill(rp.get(), ix.get(), nrows, t.rperm.size() ? t.rperm.get() : nullptr, t.tiles.get(), t.tb.get(),
+                   nb, t.trow.get(), t.eix.get(), st.get(), stream);
   int bad = 0;
   PCAONE_HIP(hipMemcpyAsync(&bad, st.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
   PCAONE_HIP(hipStreamSynchronize(stream));  // the host copies and cnt go out of scope
@@ -302,31 +331,61 @@ void Context::build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf
   t.built = true;
 }
 
-int Context::miss_parts(const MissTiles& t, uint64_t r_lo, uint64_t r_hi, uint64_t k_lo, uint64_t k_hi) const {
-  // Workgroups per row block (one workgroup per CU at a time): the number of
-  // parts that best fills the last round of workgroups, a larger count taken
-  // only for 2 % more filled slots; at least 8 tiles per part (the first
-  // block's tiles in the index range stand for all), and the partial rows
-  // must fit their buffer.
-  const uint64_t nblk = (r_hi - 1) / kMissRB - r_lo / kMissRB + 1;
-  const uint64_t b = r_lo / kMissRB;
-  uint64_t nt = 0;
-  for (uint32_t i = t.h_tb[b]; i < t.h_tb[b + 1]; ++i)
-    nt += t.h_tiles[i].k1 > k_lo && t.h_tiles[i].k0 < k_hi;
-  const uint64_t slots = std::max<uint64_t>(cus_, 1);
-  const uint64_t cap = mpart_.size() / std::max<uint64_t>((r_hi - r_lo) * (uint64_t)L, 1);
-  const uint64_t pmax = std::min<uint64_t>({std::max<uint64_t>(1, nt / 8), std::max<uint64_t>(cap, 1), 64});
-  uint64_t best = 1;
-  double best_fill = 0.0;
-  for (uint64_t p = 1; p <= pmax; ++p) {
-    const uint64_t total = nblk * p;
-    const double fill = (double)total / (double)(((total + slots - 1) / slots) * slots);
-    if (fill > best_fill + 0.02) {
-      best = p;
-      best_fill = fill;
-    }
+const Context::MissWorkSet& Context::miss_work(const MissTiles& t, uint64_t r_lo, uint64_t r_hi, uint64_t k_lo,
+                                               uint64_t k_hi) {
+  const std::array<uint64_t, 6> key{(uint64_t)(&t == &mt_s_ ? 0 : 1), r_lo, r_hi, k_lo, k_hi, (uint64_t)L};
+  auto it = mwork_.find(key);
+  if (it != mwork_.end()) return it->second;
+  // tiles of each block in the index range
+  const uint64_t b0 = r_lo / kMissRB, b1 = (r_hi - 1) / kMissRB;
+  std::vector<uint32_t> ta(b1 - b0 + 1), tz(b1 - b0 + 1);
+  uint64_t total = 0;
+  for (uint64_t b = b0; b <= b1; ++b) {
+    const MissTile* lo = t.h_tiles.data() + t.h_tb[b];
+    const MissTile* hi = t.h_tiles.data() + t.h_tb[b + 1];
+    const MissTile* x0 = std::partition_point(lo, hi, [&](const MissTile& m) { return m.k1 <= k_lo; });
+    const MissTile* x1 = std::partition_point(x0, hi, [&](const MissTile& m) { return m.k0 < k_hi; });
+    ta[b - b0] = (uint32_t)(x0 - t.h_tiles.data());
+    tz[b - b0] = (uint32_t)(x1 - t.h_tiles.data());
+    total += x1 - x0;
   }
-  return (int)best;
+  // shares of about `per` tiles: about sixteen rounds of workgroups (one
+  // per CU at a time), at least 8 tiles a share, the partial rows within
+  // their buffer
+  const uint64_t cus = std::max<uint64_t>(cus_, 1);
+  uint64_t per = std::max<uint64_t>(8, (total + 16 * cus - 1) / (16 * cus));
+  const uint64_t cap = mpart_.size() / std::max<uint64_t>((r_hi - r_lo) * (uint64_t)L, 1);
+  std::vector<uint32_t> ns(ta.size());
+  uint32_t slots = 1;
+  while (true) {
+    slots = 1;
+    for (size_t i = 0; i < ta.size(); ++i) {
+      ns[i] = (uint32_t)std::max<uint64_t>(1, (tz[i] - ta[i] + per - 1) / per);
+      slots = std::max(slots, ns[i]);
+    }
+    if (slots == 1 || slots <= cap) break;
+    per *= 2;
+  }
+  std::vector<MissWork> work;
+  for (size_t i = 0; i < ta.size(); ++i) {
+    const uint32_t n = tz[i] - ta[i];
+    for (uint32_t z = 0; z < ns[i]; ++z)
+      work.push_back(MissWork{(uint32_t)(b0 + i), ta[i] + (uint32_t)((uint64_t)n * z / ns[i]),
+                              ta[i] + (uint32_t)((uint64_t)n * (z + 1) / ns[i]), z});
+  }
+  // the longest shares first, so that the short ones fill the last round
+  std::stable_sort(work.begin(), work.end(),
+                   [](const MissWork& x, const MissWork& y) { return x.t1 - x.t0 > y.t1 - y.t0; });
+  MissWorkSet& ws = mwork_[key];
+  ws.work.resize(work.size());
+  ws.bslots.resize(ns.size());
+  PCAONE_HIP(hipMemcpyAsync(ws.work.get(), work.data(), work.size() * sizeof(MissWork), hipMemcpyHostToDevice,
+                            stream));
+  PCAONE_HIP(hipMemcpyAsync(ws.bslots.get(), ns.data(), ns.size() * 4, hipMemcpyHostToDevice, stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));
+  ws.nwork = (uint32_t)work.size();
+  ws.slots = slots;
+  return ws;
 }
 
 void Context::filter_snps(const std::vector<uint32_t>& keep) {
@@ -733,7 +792,12 @@ void Context::sketch(uint64_t j0, uint64_t j1, int hmode) {
     m.cp = cp_;
     m.cpitch = cpitch_;
     m.ncols = L;
-    m.parts = miss_parts(mt_s_, m.r_lo, m.r_hi, m.k_lo, m.k_hi);
+    const MissWorkSet& ws = miss_work(mt_s_, m.r_lo, m.r_hi, m.k_lo, m.k_hi);
+    m.work = ws.work.get();
+    m.nwork = ws.nwork;
+    m.slots = ws.slots;
+    m.bslots = ws.bslots.get();
+    m.rpos = mt_s_.rpos.size() ? mt_s_.rpos.get() : nullptr;
     m.partial = mpart_.get();
     m.partial_cap = mpart_.size();
     m.out = gwin;
@@ -819,7 +883,12 @@ void Context::h_half(uint64_t j0, uint64_t j1, int hmode) {
     m.cp = cp_;
     m.cpitch = cpitch_;
     m.ncols = L;
-    m.parts = miss_parts(mt_t_, m.r_lo, m.r_hi, m.k_lo, m.k_hi);
+    const MissWorkSet& ws = miss_work(mt_t_, m.r_lo, m.r_hi, m.k_lo, m.k_hi);
+    m.work = ws.work.get();
+    m.nwork = ws.nwork;
+    m.slots = ws.slots;
+    m.bslots = ws.bslots.get();
+    m.rpos = mt_t_.rpos.size() ? mt_t_.rpos.get() : nullptr;
     m.partial = mpart_.get();
     m.partial_cap = mpart_.size();
     m.out = target;
@@ -879,8 +948,8 @@ void Context::update_missing_weights() {
   const uint64_t xo = xoff();
   if (miss_tiled_) {
     const uint32_t span = miss_tile_span(cp_);
-    if (!mt_s_.built || mt_s_.span != span) build_tiles(mt_s_, rp_s_, ix_s_, lrows_, N, span);
-    if (!mt_t_.built || mt_t_.span != span) build_tiles(mt_t_, rp_t_, ix_t_, N, lrows_, span);
+    if (!mt_s_.built || mt_s_.span != span) build_tiles(mt_s_, rp_s_, ix_s_, lrows_, N, span, false);
+    if (!mt_t_.built || mt_t_.span != span) build_tiles(mt_t_, rp_t_, ix_t_, N, lrows_, span, true);
     if (emu_k_) launch_pack_col_state(vimp_.get(), M, emu_kp_, emu_k_, sc_.get(), cF_.get(), dF_.get(), stream);
     const auto weights = [&](MissTiles& t, bool by_snp) {
       MissTiledWeightArgs a{};
@@ -896,10 +965,10 @@ void Context::update_missing_weights() {
       a.V = emu_k_ ? vimp_.get() + xo * emu_kp_ : nullptr;
       a.kk = emu_k_;
       a.kp = emu_kp_;
-      const uint64_t per_block = t.nblocks ? t.h_tiles.size() / t.nblocks : 0;
-      a.parts = (int)std::min<uint64_t>({std::max<uint64_t>(1, (2 * std::max<uint64_t>(cus_, 1) + t.nblocks - 1) /
-                                                                   std::max<uint64_t>(t.nblocks, 1)),
-                                         std::max<uint64_t>(1, per_block / 4), 65535});
+      if (!t.nrows) return;
+      const MissWorkSet& ws = miss_work(t, 0, t.nrows, 0, t.kext);
+      a.work = ws.work.get();
+      a.nwork = ws.nwork;
       launch_missing_weights_tiled(a, stream);
     };
     weights(mt_s_, true);

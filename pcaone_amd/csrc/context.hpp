@@ -17,7 +17,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -254,17 +256,31 @@ class Context {
     DevBuf<uint32_t> tb;
     DevBuf<uint16_t> trow, eix;
     DevBuf<float> ew;
+    DevBuf<uint32_t> rperm, rpos;  // rows by descending entry count (balance), else empty
     uint64_t nrows = 0, kext = 0, nblocks = 0;
     uint32_t span = 0;
     bool built = false;
-    MissTileList list() const { return {tiles.get(), tb.get(), trow.get(), eix.get()}; }
+    MissTileList list() const {
+      return {tiles.get(), tb.get(), trow.get(), eix.get(), rperm.size() ? rperm.get() : nullptr};
+    }
   };
   MissTiles mt_s_, mt_t_;
   bool miss_tiled_ = false;     // this run's corrections use the tiles
   bool weights_tiled_ = false;  // the current weights are the tiles' (else the CSR lists')
+  // balance: place the rows in the row blocks by descending entry count (the
+  // by-sample list, whose corrections always take all rows)
   void build_tiles(MissTiles& t, const DevBuf<uint64_t>& rp, const DevBuf<uint32_t>& ix, uint64_t nrows,
-                   uint64_t kext, uint32_t span);
-  int miss_parts(const MissTiles& t, uint64_t r_lo, uint64_t r_hi, uint64_t k_lo, uint64_t k_hi) const;
+                   uint64_t kext, uint32_t span, bool balance);
+  // Work items of a tiled correction over list rows [r_lo, r_hi) and indexes
+  // [k_lo, k_hi): each row block's tiles in shares of about equal tile
+  // counts, kept per range (the windows of an epoch repeat every epoch).
+  struct MissWorkSet {
+    DevBuf<MissWork> work;
+    DevBuf<uint32_t> bslots;
+    uint32_t nwork = 0, slots = 1;
+  };
+  std::map<std::array<uint64_t, 6>, MissWorkSet> mwork_;
+  const MissWorkSet& miss_work(const MissTiles& t, uint64_t r_lo, uint64_t r_hi, uint64_t k_lo, uint64_t k_hi);
   // compact sources of the sparse corrections (Omega~ rows, G~ window rows)
   DevBuf<unsigned char> csg_, csh_;
   DevBuf<double> mpart_;

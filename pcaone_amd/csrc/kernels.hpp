@@ -238,13 +238,18 @@ void launch_sum_chunks(const double* partial, uint64_t nz, uint64_t nrows, int n
 // whole tile - row offsets, entries and the source rows of its index range -
 // into LDS and every lane group owns fixed rows of the block, so a source row
 // is read from HBM or L2 once per tile rather than once per entry.
-constexpr int kMissRB = 384;
-constexpr int kMissKG = 16;
+constexpr int kMissRB = 192;
+constexpr int kMissKG = 32;
 constexpr int kMissEMax = kMissRB * kMissKG;  // one granule always fits
-constexpr int kMissRowStride = 392;           // kMissRB + 1 offsets, padded to 16 bytes
+constexpr int kMissRowStride = 200;           // kMissRB + 1 offsets, padded to 16 bytes
 constexpr int kMissThreads = 1024;
 constexpr int kMissMaxCp = 32;                // FP32 source rows of at most 32 columns
 constexpr uint32_t kMissLdsBuf = 80 * 1024;   // bytes of one staging buffer (two per workgroup)
+
+// Entries stored per tile: its ne entries, zeros up to a multiple of 8 and 8
+// more zeros (index 0, weight 0), so that a step of 8 entries that starts
+// inside the tile reads only initialized entries.
+__host__ __device__ constexpr uint32_t miss_stored(uint32_t ne) { return (ne + 7) / 8 * 8 + 8; }
 
 struct MissTile {
   uint64_t e0;      // first entry (a multiple of 8)
@@ -256,22 +261,31 @@ struct MissTile {
 // Largest index span of a tile whose FP32 source rows of cp columns fit a
 // staging buffer beside the row offsets and kMissEMax entries.
 uint32_t miss_tile_span(int cp);
+// The rows of a list may be placed in the row blocks in another order:
+// position p = b kMissRB + i holds list row rperm[p] (>= nrows: none), and
+// rpos[r] is row r's position (both null: row r at position r).
 // Entry counts per (row block, granule): cnt[b * ngran + g] += entries of the
 // block's rows with index in granule g (cnt zeroed by the caller).
-void launch_miss_hist(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, uint32_t ngran, uint32_t* cnt,
-                      hipStream_t s);
+void launch_miss_hist(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const uint32_t* rpos, uint32_t ngran,
+                      uint32_t* cnt, hipStream_t s);
 // Fills the tiles' row offsets and local indexes from the CSR list (tiles and
 // tb = first tile of each row block, planned on the host from the counts);
 // *status = 1 if a tile's rows hold other than its planned entry count.
-void launch_miss_fill(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const MissTile* tiles,
-                      const uint32_t* tb, uint64_t nblocks, uint16_t* trow, uint16_t* eix, int* status,
-                      hipStream_t s);
+void launch_miss_fill(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const uint32_t* rperm,
+                      const MissTile* tiles, const uint32_t* tb, uint64_t nblocks, uint16_t* trow, uint16_t* eix,
+                      int* status, hipStream_t s);
 
+// A workgroup's share of a tiled correction: tiles [t0, t1) of row block b,
+// the slot-th of the block's nslot shares.
+struct MissWork {
+  uint32_t b, t0, t1, slot;
+};
 struct MissTileList {
   const MissTile* tiles;
-  const uint32_t* tb;   // nblocks + 1
+  const uint32_t* tb;     // nblocks + 1
   const uint16_t* trow;
   const uint16_t* eix;
+  const uint32_t* rperm;  // list row of each position (null: the position itself)
 };
 // Weights of a tiled list, as MissWeightArgs describes them, in tile order
 // (FP32).  EMU: the row side's k-vectors of a block and the index side's of a
@@ -287,13 +301,16 @@ struct MissTiledWeightArgs {
   const double* US;
   const double* V;
   int kk, kp;
-  int parts;             // workgroups per row block (tile ranges)
+  const MissWork* work;  // one workgroup per item
+  uint32_t nwork;
 };
 void launch_missing_weights_tiled(const MissTiledWeightArgs& a, hipStream_t s);
 // out[r - r_lo][:ncols] (+)= sum over the entries of list rows r in [r_lo,
 // r_hi) with index k in [k_lo, k_hi) of w * src[k - src_k0][:ncols]; src are
-// FP32 compact rows (cp columns, pitch cpitch).  With parts > 1 each part
-// writes partial rows ([part][r - r_lo][ncols]) that are summed in part order.
+// FP32 compact rows (cp columns, pitch cpitch).  One workgroup per work item;
+// with several shares per block (slots > 1) each writes partial rows
+// ([slot][r - r_lo][ncols]) that are summed in slot order, the block's nslot
+// (bslots[b - r_lo / kMissRB]) of them.
 struct MissTiledSpmmArgs {
   MissTileList t;
   const float* ew;
@@ -301,7 +318,11 @@ struct MissTiledSpmmArgs {
   const float* src;
   uint64_t src_k0;
   int cp, cpitch, ncols;
-  int parts;
+  const MissWork* work;
+  uint32_t nwork;
+  uint32_t slots;           // largest share count of a block
+  const uint32_t* bslots;   // shares of each block in the range
+  const uint32_t* rpos;     // position of each list row (null: the row itself)
   double* partial;
   uint64_t partial_cap;
   double* out;

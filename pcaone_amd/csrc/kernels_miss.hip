@@ -33,11 +33,12 @@ constexpr int kMissOffBytes = kMissRowStride * 2;  // row offsets at the start o
 // indexes ascend, so each run of equal granules within 64 entries is counted
 // by its first lane with one atomic.
 __global__ void __launch_bounds__(256) k_miss_hist(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ ix,
-                                                   uint64_t nrows, uint32_t ngran, uint32_t* __restrict__ cnt) {
+                                                   uint64_t nrows, const uint32_t* __restrict__ rpos,
+                                                   uint32_t ngran, uint32_t* __restrict__ cnt) {
   const uint64_t r = blockIdx.x * 4ull + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (r >= nrows) return;  // wave-uniform
-  uint32_t* c = cnt + (r / kMissRB) * (uint64_t)ngran;
+  uint32_t* c = cnt + ((rpos ? rpos[r] : r) / kMissRB) * (uint64_t)ngran;
   const uint64_t e1 = rp[r + 1];
   for (uint64_t eb = rp[r]; eb < e1; eb += 64) {
     const int nb = (int)min<uint64_t>(64, e1 - eb);
@@ -60,13 +61,14 @@ __global__ void __launch_bounds__(256) k_miss_hist(const uint64_t* __restrict__ 
 // than planned is not written past its end and sets *status.
 __global__ void __launch_bounds__(kMissRB) k_miss_fill(const uint64_t* __restrict__ rp,
                                                       const uint32_t* __restrict__ ix, uint64_t nrows,
+                                                      const uint32_t* __restrict__ rperm,
                                                       const MissTile* __restrict__ tiles,
                                                       const uint32_t* __restrict__ tb, uint16_t* __restrict__ trow,
                                                       uint16_t* __restrict__ eix, int* __restrict__ status) {
   __shared__ uint32_t wsum[kMissRB / 64];
   const uint64_t b = blockIdx.x;
   const int i = threadIdx.x, lane = i & 63, wave = i >> 6;
-  const uint64_t r = b * kMissRB + i;
+  const uint64_t r = rperm ? (uint64_t)rperm[b * kMissRB + i] : b * kMissRB + i;
   uint64_t cur = 0, end = 0;
   if (r < nrows) {
     cur = rp[r];
@@ -98,43 +100,21 @@ __global__ void __launch_bounds__(kMissRB) k_miss_fill(const uint64_t* __restric
   }
 }
 
-// The tiles [t0, t1) of row block b that meet the index range [k_lo, k_hi),
-// split into gridDim.y parts; this workgroup's part.
-__device__ __forceinline__ void part_tiles(const MissTile* tiles, const uint32_t* tb, uint64_t b, uint64_t k_lo,
-                                           uint64_t k_hi, uint32_t& t0, uint32_t& t1) {
-  uint32_t lo = tb[b], hi = tb[b + 1];
-  while (lo < hi) {  // first tile with k1 > k_lo
-    const uint32_t mid = (lo + hi) / 2;
-    if (tiles[mid].k1 <= k_lo) lo = mid + 1;
-    else hi = mid;
-  }
-  const uint32_t ta = lo;
-  hi = tb[b + 1];
-  while (lo < hi) {  // first tile with k0 >= k_hi
-    const uint32_t mid = (lo + hi) / 2;
-    if (tiles[mid].k0 < k_hi) lo = mid + 1;
-    else hi = mid;
-  }
-  const uint32_t nt = lo - ta, per = (nt + gridDim.y - 1) / gridDim.y;
-  t0 = min(lo, ta + blockIdx.y * per);
-  t1 = min(lo, t0 + per);
-}
-
 // Sparse correction on tiles.  Lane group g of a wave (LPE = cp / 4 lanes, 4
 // FP32 columns each) owns rows RPG g' .. RPG g' + RPG - 1 of the row block (g'
 // = the group's index in the workgroup).  Per tile the staging buffer holds,
 // in 16-byte slots: the row offsets, the local indexes, the weights and the
 // source rows of [max(k0, k_lo), min(k1, k_hi)) at their offset from k0.
 template <int RPG>
-__global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSpmmArgs a, uint64_t b0, int lpe) {
+__global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSpmmArgs a, int lpe) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int G = 64 / lpe, g = lane / lpe, cl = lane - g * lpe;
   const bool act = g < G;
   const int row0 = (wave * G + g) * RPG;
-  const uint64_t b = b0 + blockIdx.x;
-  uint32_t t0, t1;
-  part_tiles(a.t.tiles, a.t.tb, b, a.k_lo, a.k_hi, t0, t1);
+  const MissWork wk = a.work[blockIdx.x];
+  const uint64_t b = wk.b;
+  const uint32_t t0 = wk.t0, t1 = wk.t1;
   const int cpv = a.cp / 4;  // 16-byte slots per source row
 
   // A staging buffer is filled from registers: each thread loads its
@@ -148,8 +128,8 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
   uint4 reg[kSlots];
   auto load = [&](uint32_t t, const MissTile& T) {
     const uint32_t sA = kMissOffBytes / 16;
-    const uint32_t sB = sA + (T.ne + 7) / 8;
-    const uint32_t sC = sB + (T.ne + 3) / 4;
+    const uint32_t sB = sA + miss_stored(T.ne) / 8;
+    const uint32_t sC = sB + miss_stored(T.ne) / 4;
     const uint64_t klo = max<uint64_t>(T.k0, a.k_lo), khi = min<uint64_t>(T.k1, a.k_hi);
     const uint32_t sD0 = sC + (uint32_t)(klo - T.k0) * cpv, sD1 = sC + (uint32_t)(khi - T.k0) * cpv;
     const unsigned char* pa = reinterpret_cast<const unsigned char*>(a.t.trow + (uint64_t)t * kMissRowStride);
@@ -176,7 +156,7 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
   // The local indexes are stored as the 16-byte slot offsets of their source
   // rows (k cp / 4 < 65536 for the spans planned).
   auto store = [&](unsigned char* buf, uint32_t nslots, uint32_t ne) {
-    const uint32_t sA = kMissOffBytes / 16, sB = sA + (ne + 7) / 8;
+    const uint32_t sA = kMissOffBytes / 16, sB = sA + miss_stored(ne) / 8;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
       const uint32_t s = tid + i * kMissThreads;
@@ -197,18 +177,18 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[q][c] = 0.0;
 
-  // A row's entries in the tile are taken U at a time; the last, partial
-  // step (every step of a tile that the index range cuts) masks its entries
-  // past the row's end (or outside the range) to the source row at `olo`
-  // (always staged) with weight 0.  Reads past the row's end stay inside the
-  // staging buffer.  The row's sum within the tile is taken in FP32 and added
-  // to its FP64 total.
+  // A row's entries in the tile are taken U at a time.  Reads past the row's
+  // end meet the next rows' entries or the tile's zero padding, all indexes
+  // of staged rows, so the last, partial step only zeroes their weights; in a
+  // tile that the index range cuts, every step also moves entries outside the
+  // range to the source row at `olo` (always staged).  The row's sum within
+  // the tile is taken in FP32 and added to its FP64 total.
   auto process = [&](const unsigned char* buf, const MissTile& T, auto edge_tag) {
     constexpr bool EDGE = decltype(edge_tag)::value;
     const uint16_t* ro = reinterpret_cast<const uint16_t*>(buf);
     const uint16_t* ix = reinterpret_cast<const uint16_t*>(buf + kMissOffBytes);
-    const float* w = reinterpret_cast<const float*>(buf + kMissOffBytes + ((T.ne + 7) / 8) * 16);
-    const unsigned char* src = reinterpret_cast<const unsigned char*>(w + ((T.ne + 3) / 4) * 4) + cl * 16;
+    const float* w = reinterpret_cast<const float*>(buf + kMissOffBytes + miss_stored(T.ne) * 2);
+    const unsigned char* src = reinterpret_cast<const unsigned char*>(w + miss_stored(T.ne)) + cl * 16;
     const uint32_t olo = EDGE ? (uint32_t)(max<uint64_t>(T.k0, a.k_lo) - T.k0) * cpv : 0u;
     const uint32_t ohi = EDGE ? (uint32_t)(min<uint64_t>(T.k1, a.k_hi) - T.k0) * cpv : 0u;
     constexpr int U = RPG >= 3 ? 4 : 8;  // three rows' sums leave registers for four entries a step
@@ -219,10 +199,12 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
       for (int u = 0; u < U; ++u) {
         o[u] = ix[e + u];
         wv[u] = w[e + u];
-        if (masked) {
-          const bool ok = e + u < ee && (!EDGE || (o[u] >= olo && o[u] < ohi));
+        if (EDGE) {
+          const bool ok = e + u < ee && o[u] >= olo && o[u] < ohi;
           o[u] = ok ? o[u] : olo;
           wv[u] = ok ? wv[u] : 0.0f;
+        } else if (masked) {
+          wv[u] = e + u < ee ? wv[u] : 0.0f;
         }
       }
       float4 x[U];
@@ -275,19 +257,21 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
     }
   }
 
-  const bool partial = gridDim.y > 1;
+  const bool partial = a.slots > 1;
   const uint64_t nr = a.r_hi - a.r_lo;
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
     const int r = row0 + q;
-    const uint64_t gr = b * kMissRB + r;
-    if (!act || r >= kMissRB || gr < a.r_lo || gr >= a.r_hi) continue;
+    if (!act || r >= kMissRB) continue;
+    const uint64_t pos = b * kMissRB + r;
+    const uint64_t gr = a.t.rperm ? (uint64_t)a.t.rperm[pos] : pos;
+    if (gr < a.r_lo || gr >= a.r_hi) continue;
     const uint64_t orow = gr - a.r_lo;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int col = cl * 4 + c;
       if (col >= a.ncols) continue;
-      if (partial) a.partial[((uint64_t)blockIdx.y * nr + orow) * a.ncols + col] = acc[q][c];
+      if (partial) a.partial[((uint64_t)wk.slot * nr + orow) * a.ncols + col] = acc[q][c];
       else a.out[orow * a.lp + col] += acc[q][c];
     }
   }
@@ -302,7 +286,8 @@ template <bool EMU>
 __global__ void __launch_bounds__(kMissThreads) k_missing_weights_tiled(MissTiledWeightArgs a, uint32_t sub) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int tid = threadIdx.x;
-  const uint64_t b = blockIdx.x;
+  const MissWork wk = a.work[blockIdx.x];
+  const uint64_t b = wk.b;
   const int kk = a.kk;
   const int vr = a.by_snp ? kk + 3 : kk;  // doubles per row-side vector
   const int vi = a.by_snp ? kk : kk + 3;  // doubles per index-side vector
@@ -315,13 +300,12 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_weights_tiled(MissTile
     const double* rs = a.by_snp ? a.V : a.US;
     for (int t = tid; t < kMissRB * vr; t += kMissThreads) {
       const int i = t / vr, c = t - i * vr;
-      const uint64_t r = b * kMissRB + i;
+      const uint64_t pos = b * kMissRB + i;
+      const uint64_t r = a.t.rperm ? (uint64_t)a.t.rperm[pos] : pos;
       rowvec[t] = r < a.nrows ? rs[r * a.kp + c] : 0.0;
     }
   }
-  uint32_t t0, t1;
-  part_tiles(a.t.tiles, a.t.tb, b, 0, ~0ull, t0, t1);
-  for (uint32_t t = t0; t < t1; ++t) {
+  for (uint32_t t = wk.t0; t < wk.t1; ++t) {
     const MissTile T = a.t.tiles[t];
     __syncthreads();  // the previous tile's buffers are free
     for (int i = tid; i <= kMissRB; i += kMissThreads) ro[i] = a.t.trow[(uint64_t)t * kMissRowStride + i];
@@ -356,12 +340,29 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_weights_tiled(MissTile
           const double sc = st[0], cf = st[1], f = st[2];
           w = -sc * (0.5 - cf) + sc * fmin(fmax(p, -f), 1.0 - f);
         } else {
-          const uint64_t c = a.by_snp ? b * kMissRB + i : (uint64_t)T.k0 + x;
+          const uint64_t pos = b * kMissRB + i;
+          const uint64_t c = a.by_snp ? (a.t.rperm ? (uint64_t)a.t.rperm[pos] : pos) : (uint64_t)T.k0 + x;
           w = -a.sc[c] * (0.5 - a.cf[c]);
         }
         a.ew[T.e0 + e] = (float)w;
       }
     }
+  }
+}
+
+// out[r][l] += sum of the first nslot partial rows of r's block, in slot order.
+__global__ void k_sum_slots(const double* __restrict__ partial, uint64_t nr, int ncols, uint64_t r_lo,
+                            const uint32_t* __restrict__ rpos, const uint32_t* __restrict__ bslots,
+                            double* __restrict__ out, int lp) {
+  const uint64_t n = nr * (uint64_t)ncols;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = t / ncols;
+    const int l = (int)(t - r * ncols);
+    const uint64_t pos = rpos ? rpos[r_lo + r] : r_lo + r;
+    const uint32_t ns = bslots[pos / kMissRB - r_lo / kMissRB];
+    double v = 0.0;
+    for (uint32_t z = 0; z < ns; ++z) v += partial[z * n + t];
+    out[r * lp + l] += v;
   }
 }
 
@@ -374,31 +375,32 @@ void allow_lds(K kernel, size_t bytes) {
 }  // namespace
 
 uint32_t miss_tile_span(int cp) {
-  const uint32_t fixed = kMissOffBytes + kMissEMax * 2 + kMissEMax * 4;
+  const uint32_t fixed = kMissOffBytes + miss_stored(kMissEMax) * 6;
   const uint32_t rows = (kMissLdsBuf - fixed) / (uint32_t)(cp * 4);
   return rows / kMissKG * kMissKG;
 }
 
-void launch_miss_hist(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, uint32_t ngran, uint32_t* cnt,
-                      hipStream_t s) {
+void launch_miss_hist(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const uint32_t* rpos, uint32_t ngran,
+                      uint32_t* cnt, hipStream_t s) {
   if (!nrows) return;
-  hipLaunchKernelGGL(k_miss_hist, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, s, rp, ix, nrows, ngran, cnt);
+  hipLaunchKernelGGL(k_miss_hist, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, s, rp, ix, nrows, rpos, ngran,
+                     cnt);
   check_launch("miss_hist");
 }
 
-void launch_miss_fill(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const MissTile* tiles,
-                      const uint32_t* tb, uint64_t nblocks, uint16_t* trow, uint16_t* eix, int* status,
-                      hipStream_t s) {
+void launch_miss_fill(const uint64_t* rp, const uint32_t* ix, uint64_t nrows, const uint32_t* rperm,
+                      const MissTile* tiles, const uint32_t* tb, uint64_t nblocks, uint16_t* trow, uint16_t* eix,
+                      int* status, hipStream_t s) {
   if (!nblocks) return;
-  hipLaunchKernelGGL(k_miss_fill, dim3((unsigned)nblocks), dim3(kMissRB), 0, s, rp, ix, nrows, tiles, tb, trow,
-                     eix, status);
+  hipLaunchKernelGGL(k_miss_fill, dim3((unsigned)nblocks), dim3(kMissRB), 0, s, rp, ix, nrows, rperm, tiles, tb,
+                     trow, eix, status);
   check_launch("miss_fill");
 }
 
 void launch_missing_weights_tiled(const MissTiledWeightArgs& a, hipStream_t s) {
   if (!a.nblocks) return;
   const bool emu = a.Fclamp != nullptr;
-  const size_t cap = 2 * (size_t)kMissLdsBuf;
+  const size_t cap = kMissLdsBuf;  // two workgroups per CU: one stages a tile while the other computes
   size_t lds = kMissOffBytes + 2 * (size_t)kMissEMax * 2;
   uint32_t sub = 0;
   if (emu) {
@@ -410,8 +412,9 @@ void launch_missing_weights_tiled(const MissTiledWeightArgs& a, hipStream_t s) {
     sub = (uint32_t)((cap - lds) / ((size_t)vi * 8));
     lds += (size_t)sub * vi * 8;
   }
-  require(a.parts >= 1 && a.parts < 65536, kErrInternal, "missing_weights_tiled: bad parts");
-  const dim3 grid((unsigned)a.nblocks, (unsigned)a.parts);
+  if (!a.nwork) return;
+  require(a.work != nullptr, kErrInternal, "missing_weights_tiled: no work items");
+  const dim3 grid(a.nwork);
   if (emu) {
     static bool once = (allow_lds(k_missing_weights_tiled<true>, cap), true);
     (void)once;
@@ -423,28 +426,34 @@ void launch_missing_weights_tiled(const MissTiledWeightArgs& a, hipStream_t s) {
 }
 
 void launch_missing_spmm_tiled(const MissTiledSpmmArgs& a, hipStream_t s) {
-  if (a.r_hi <= a.r_lo || a.k_hi <= a.k_lo) return;
+  if (a.r_hi <= a.r_lo || a.k_hi <= a.k_lo || !a.nwork) return;
   require(a.cp >= 4 && a.cp % 4 == 0 && a.cp <= kMissMaxCp && a.ncols >= 1 && a.ncols <= a.cp &&
               a.cpitch >= a.cp && a.cpitch % 4 == 0 && a.ncols <= a.lp,
           kErrArgument, "missing_spmm_tiled: bad column counts");
   const uint64_t nr = a.r_hi - a.r_lo;
-  require(a.parts >= 1 && a.parts < 65536 && (a.parts == 1 || (a.partial && a.parts * nr * a.ncols <= a.partial_cap)),
-          kErrInternal, "missing_spmm_tiled: bad parts");
+  require(a.slots >= 1 && a.work && a.bslots &&
+              (a.slots == 1 || (a.partial && a.slots * nr * a.ncols <= a.partial_cap)),
+          kErrInternal, "missing_spmm_tiled: bad work split");
   const int lpe = a.cp / 4, groups = 16 * (64 / lpe);
   const int rpg = (kMissRB + groups - 1) / groups;
-  const uint64_t b0 = a.r_lo / kMissRB, b1 = (a.r_hi - 1) / kMissRB;
-  const dim3 grid((unsigned)(b1 - b0 + 1), (unsigned)a.parts);
+  const dim3 grid(a.nwork);
   const size_t lds = 2 * (size_t)kMissLdsBuf;
   static bool once = (allow_lds(k_missing_spmm_tiled<1>, lds), allow_lds(k_missing_spmm_tiled<2>, lds),
                       allow_lds(k_missing_spmm_tiled<3>, lds), true);
   (void)once;
   switch (rpg) {
-    case 1: hipLaunchKernelGGL(k_missing_spmm_tiled<1>, grid, dim3(kMissThreads), lds, s, a, b0, lpe); break;
-    case 2: hipLaunchKernelGGL(k_missing_spmm_tiled<2>, grid, dim3(kMissThreads), lds, s, a, b0, lpe); break;
-    default: hipLaunchKernelGGL(k_missing_spmm_tiled<3>, grid, dim3(kMissThreads), lds, s, a, b0, lpe); break;
+    case 1: hipLaunchKernelGGL(k_missing_spmm_tiled<1>, grid, dim3(kMissThreads), lds, s, a, lpe); break;
+    case 2: hipLaunchKernelGGL(k_missing_spmm_tiled<2>, grid, dim3(kMissThreads), lds, s, a, lpe); break;
+    default: hipLaunchKernelGGL(k_missing_spmm_tiled<3>, grid, dim3(kMissThreads), lds, s, a, lpe); break;
   }
   check_launch("missing_spmm_tiled");
-  if (a.parts > 1) launch_sum_chunks(a.partial, a.parts, nr, a.ncols, a.out, a.lp, s);
+  if (a.slots > 1) {
+    const uint64_t n = nr * (uint64_t)a.ncols;
+    const unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 65535u * 8);
+    hipLaunchKernelGGL(k_sum_slots, dim3(g), dim3(256), 0, s, a.partial, nr, a.ncols, a.r_lo, a.rpos, a.bslots,
+                       a.out, a.lp);
+    check_launch("sum_slots");
+  }
 }
 
 }  // namespace pcaone

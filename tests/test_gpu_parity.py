@@ -226,7 +226,7 @@ def test_missing_entries_with_chunked_sources(ctx, emu):
     np.testing.assert_allclose(O.sign_align(ref.U, r.U), ref.U, atol=1e-8)
 
 
-TILE_SPECS = {  # 1,000 x 3,000: 8 row blocks of SNPs, 3 of samples
+TILE_SPECS = {  # 1,000 x 3,000: 16 row blocks of SNPs, 6 of samples
     # dense: tiles bounded by the entry cap (about 30 indexes each), several per row block
     "dense": dict(seed=31, nsamples=1000, nsnps=3000, npops=3, miss_rate=0.5, high_miss_frac=0.05,
                   high_miss_rate=0.9),
@@ -239,7 +239,7 @@ TILE_SPECS = {  # 1,000 x 3,000: 8 row blocks of SNPs, 3 of samples
 @pytest.mark.parametrize("L", [4, 12, 20, 32])
 def test_tiled_corrections_match_oracle(ctx, density, L):
     """The reduced forms' missing-entry corrections run on tiles of the two
-    lists (row blocks of 384 rows; source rows, indexes and FP32 weights
+    lists (row blocks of 192 rows; source rows, indexes and FP32 weights
     staged in LDS).  L = 4, 12, 20, 32 take 1, 3, 5 and 8 lanes per entry.
     The products of whole and cut windows (a window starting inside a row
     block, tiles cut by the window's SNP range, several parts per row block)
