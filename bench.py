@@ -192,7 +192,7 @@ def batch_line(args) -> int:
            "cohorts": len(specs), "total_samples": sum(sp.nsamples for sp in specs),
            "total_snps": sum(sp.nsnps for sp in specs), "algorithmic_flop": flops}
     res, secs = {}, {}
-    forms = [f for f in (args.forms.split(",") if args.forms else ["bf16", "fp32"]) if f != "fp64"] + ["fp64"]
+    forms = [f for f in (args.forms.split(",") if args.forms else ["bf16", "fp8", "fp32"]) if f != "fp64"] + ["fp64"]
     for form in forms:
         p = api.default_params(precision=form, **kw)
         r = ctx.batch_run(p, use_graph=True)  # first run: tuning + graph capture
@@ -219,6 +219,7 @@ def batch_line(args) -> int:
         rel = max(float(np.max(np.abs(r["E"][c] - r64["E"][c]) / r64["E"][c])) for c in range(len(specs)))
         corr = min(float(O.pc_correlations(u64[c], us[c]).min()) for c in range(len(specs)))
         errs[f] = {"eig_rel_err_max": rel, "pc_corr_min": corr}
+        errs[f]["meets_bar"] = bar_met(errs[f])
     out["error_vs_fp64"] = errs
     if args.sequential:
         # the same cohorts one after another through the single-matrix path (generation untimed)

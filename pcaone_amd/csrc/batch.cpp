@@ -331,6 +331,8 @@ void Batch::enqueue_epochs(const BatchArgs& a, int p, bool want_v) {
   for (int pi = 0; pi <= p; ++pi) {
     // computeGandH (src/Halko.cpp:99-127): Omega (pi == 0) or thin-Q(H) is ready
     if (pi == 0) {
+      if (form_colscaled((Form)layout_.form))
+        launch_batch_omega_colmax(a, items_n_.get(), n_items_n_, kBufOmega0, s);
       launch_batch_make_pieces(a, items_n_.get(), n_items_n_, kBufOmega0, false, s);
       launch_batch_colsum(a, items_n_.get(), n_items_n_, part, s);
     }
@@ -359,6 +361,8 @@ void Batch::enqueue_epochs(const BatchArgs& a, int p, bool want_v) {
     launch_batch_apply(a, items_n_.get(), n_items_n_, kBufH, kBufOmegaC, 0, 1, s);
     launch_batch_flip(a, items_n_.get(), n_items_n_, part, s);
     launch_batch_small(a, kSmallFlip, pi, s);
+    if (form_colscaled((Form)layout_.form))
+      launch_batch_omega_colmax(a, items_n_.get(), n_items_n_, kBufOmegaC, s);
     launch_batch_make_pieces(a, items_n_.get(), n_items_n_, kBufOmegaC, true, s);
     launch_batch_colsum(a, items_n_.get(), n_items_n_, part, s);
   }
@@ -416,7 +420,7 @@ void Batch::run(const Param& p, bool want_v, bool use_graph, BatchResult& out) {
             "batch: k + oversamples exceeds the dimensions of cohort " + std::to_string(c));
   const ProductForms pf = forms_of(p.precision);
   require(pf.g == pf.h, kErrArgument, "batch: mixed operand forms are not supported");
-  require(!form_colscaled(pf.g), kErrArgument, "batch: the int8 and fp8 forms run on the single-matrix path only");
+  require(pf.g != Form::I8, kErrArgument, "batch: the int8 form runs on the single-matrix path only");
   const int form = (int)pf.g;
   // configuration: explicit (tuning fields) or autotuned once per (form, L)
   const GemmTuning& t = p.tuning;

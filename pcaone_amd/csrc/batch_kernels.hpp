@@ -33,6 +33,11 @@ struct BatchSmall {
   double colsign[kBatchLp];
   double ab[2 * kBatchLp];             // flipOmg column norms
   double trace[kBatchMaxEpochs];       // 1 - mev per epoch
+  // FP8 form: column maxima (bit patterns of non-negative doubles) of the
+  // Omega that is split (G side) and of diag(sc) G~ (H side); each column's
+  // power-of-two scale follows from its maximum
+  unsigned long long cmax_g[kBatchLp];
+  unsigned long long cmax_h[kBatchLp];
 };
 
 // Which per-cohort FP64 matrix a grouped kernel reads or writes.
@@ -107,7 +112,12 @@ void launch_batch_make_pieces(const BatchArgs& a, const RowItem* items_n, uint32
 // Column sums of Omega~ into colsum (partials per item, then per cohort).
 void launch_batch_colsum(const BatchArgs& a, const RowItem* items_n, uint32_t nitems, double* partial,
                          hipStream_t s);
+// FP8 form: the column maxima of the Omega in buffer src (N rows) that the
+// next make_pieces splits.
+void launch_batch_omega_colmax(const BatchArgs& a, const RowItem* items_n, uint32_t nitems, int src, hipStream_t s);
 // G epilogue (fused with the operand split of diag(sc) G~ and the b-vector).
+// FP8 form: G~ and the column maxima of diag(sc) G~ first, then the split
+// with the column scales.
 void launch_batch_epi_g(const BatchArgs& a, const RowItem* items_e, uint32_t nitems, double* partial,
                         hipStream_t s);
 // H epilogue: H = sum_c partials / 4 + b.

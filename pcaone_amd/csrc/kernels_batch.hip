@@ -19,16 +19,37 @@ typedef double f64x4_t __attribute__((ext_vector_type(4)));
 
 template <Form F>
 using piece_t = typename std::conditional<
-    F == Form::FP64, double, typename std::conditional<F == Form::FP32, float, uint16_t>::type>::type;
+    F == Form::FP64, double,
+    typename std::conditional<F == Form::FP32, float,
+                              typename std::conditional<F == Form::F8, uint8_t, uint16_t>::type>::type>::type;
 template <Form F>
 constexpr int pieces() {
-  return F == Form::BF16 ? 2 : 1;
+  return F == Form::BF16 ? 2 : (F == Form::F8 ? 3 : 1);
 }
 
-// x -> operand pieces; returns the value they sum to.
+// FP8 column scale: the smallest power of two >= max / 448 (the column
+// maximum lands in (224, 448]), 1 for an all-zero column.
+__device__ __forceinline__ double f8_col_scale(unsigned long long bits) {
+  const double mx = __longlong_as_double((long long)bits);
+  if (!(mx > 0.0)) return 1.0;
+  int e;
+  const double f = frexp(mx / 448.0, &e);
+  return ldexp(1.0, f == 0.5 ? e - 1 : e);
+}
+
+// x -> operand pieces; returns the value they sum to.  FP8: three E4M3
+// pieces of x / cs (cs the column's power-of-two scale), each the nearest
+// E4M3 to what the previous ones left.
 template <Form F>
-__device__ __forceinline__ double split(double x, piece_t<F> (&pc)[2]) {
-  if constexpr (F == Form::FP64) {
+__device__ __forceinline__ double split(double x, piece_t<F> (&pc)[3], double cs = 1.0) {
+  if constexpr (F == Form::F8) {
+    const double x1 = x / cs;
+    double v1, v2, v3;
+    pc[0] = e4m3_bits(x1, &v1);
+    pc[1] = e4m3_bits(x1 - v1, &v2);
+    pc[2] = e4m3_bits(x1 - v1 - v2, &v3);
+    return (v1 + v2 + v3) * cs;
+  } else if constexpr (F == Form::FP64) {
     pc[0] = x;
     return x;
   } else if constexpr (F == Form::FP32) {
@@ -77,10 +98,11 @@ __global__ void __launch_bounds__(256) k_b_make_pieces(BatchArgs a, const RowIte
   const uint64_t wpiece = (uint64_t)LP * c.wg_pitch;
   const int l = threadIdx.x % LP, r0 = threadIdx.x / LP;
   const double sgn = (signed_ && l < a.L) ? c.st->colsign[l] : 1.0;
+  const double cs = F == Form::F8 ? f8_col_scale(c.st->cmax_g[l]) : 1.0;
   for (uint32_t t0 = 0; t0 < it.nrows; t0 += TR) {
     const int nr = (int)min<uint32_t>(TR, it.nrows - t0);
     for (int r = r0; r < TR; r += 256 / LP) {
-      w_t pc[2] = {w_t(0), w_t(0)};
+      w_t pc[3] = {w_t(0), w_t(0), w_t(0)};
       if (r < nr) {
         const uint64_t t = (it.row0 + t0 + r) * LP + l;
         double x = 0.0;
@@ -88,7 +110,7 @@ __global__ void __launch_bounds__(256) k_b_make_pieces(BatchArgs a, const RowIte
           x = src[t];
           if (signed_) x *= sgn;
         }
-        const double back = split<F>(x, pc);
+        const double back = split<F>(x, pc, cs);
         o2[t] = x;
         orr[t] = back;
       }
@@ -182,7 +204,12 @@ __device__ __forceinline__ void add4(const double* p, double (&y)[4]) {
 // diag(sc) G~ and this item's share of the b-vector.  Each thread owns 4
 // adjacent columns of every (256 / (LP / 4))-th row; its b terms stay in
 // registers and the row slots are combined in order at the end.
-template <Form F, class PT>
+// MODE 0 (fused): G~ = sc (y / 4 + (1/2 - cF) colsum) from the partial
+// products, split into the operand pieces of diag(sc) G~ (rounded G~ = the
+// pieces' sum / sc) with the b-vector partials.  The FP8 form takes two
+// passes around the column maxima of diag(sc) G~: MODE 1 writes G~ (y times
+// the Omega column scale), MODE 2 splits it with the H-side column scales.
+template <Form F, class PT, int MODE>
 __global__ void __launch_bounds__(256) k_b_epi_g(BatchArgs a, const RowItem* __restrict__ items,
                                                  double* __restrict__ bpart) {
   constexpr int P = pieces<F>();
@@ -198,39 +225,67 @@ __global__ void __launch_bounds__(256) k_b_epi_g(BatchArgs a, const RowItem* __r
   double* gout = c.mat[kBufG];
   const int nr = (int)it.nrows;
   const int l0 = 4 * (threadIdx.x % CG), rs = threadIdx.x / CG;
-  double cs[4], bs[4] = {0.0, 0.0, 0.0, 0.0};
+  double cs[4], ms[4], bs[4] = {0.0, 0.0, 0.0, 0.0}, mx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) cs[q] = c.colsum[l0 + q];
+  for (int q = 0; q < 4; ++q) {
+    cs[q] = c.colsum[l0 + q];
+    ms[q] = 1.0;
+    if (F == Form::F8) ms[q] = f8_col_scale(MODE == 2 ? c.st->cmax_h[l0 + q] : c.st->cmax_g[l0 + q]);
+  }
   for (int r = rs; r < kBatchEpiRows; r += RG) {
-    w_t pc[4][2];
+    w_t pc[4][3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) pc[q][0] = pc[q][1] = w_t(0);
+    for (int q = 0; q < 4; ++q) pc[q][0] = pc[q][1] = pc[q][2] = w_t(0);
     if (r < nr) {
       const uint64_t j = it.row0 + r;
       const uint64_t t = j * LP + l0;
-      double y[4] = {0.0, 0.0, 0.0, 0.0};
-      for (uint64_t ch = 0; ch < c.nch_g; ++ch) add4(part + ch * c.ychunk_g + t, y);
       const double s = c.sc[j];
       const double beta = s * (0.5 - c.cf[j]);
+      double g[4];
+      if (MODE == 2) {
+        const double2 u = *reinterpret_cast<const double2*>(gout + t);
+        const double2 v = *reinterpret_cast<const double2*>(gout + t + 2);
+        g[0] = u.x, g[1] = u.y, g[2] = v.x, g[3] = v.y;
+      } else {
+        double y[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint64_t ch = 0; ch < c.nch_g; ++ch) add4(part + ch * c.ychunk_g + t, y);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) g[q] = 0.25 * s * (y[q] * (MODE == 1 ? ms[q] : 1.0)) + beta * cs[q];
+      }
       double gr[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double g = 0.25 * s * y[q] + beta * cs[q];
-        const double back = split<F>(g * s, pc[q]);
-        gr[q] = s != 0.0 ? back / s : 0.0;
-        bs[q] += beta * gr[q];
+        if (MODE == 1) {
+          gr[q] = g[q];
+          mx[q] = fmax(mx[q], fabs(g[q] * s));
+        } else {
+          const double back = split<F>(g[q] * s, pc[q], MODE == 2 ? ms[q] : 1.0);
+          gr[q] = s != 0.0 ? back / s : 0.0;
+          bs[q] += beta * gr[q];
+        }
       }
       *reinterpret_cast<double2*>(gout + t) = double2{gr[0], gr[1]};
       *reinterpret_cast<double2*>(gout + t + 2) = double2{gr[2], gr[3]};
     }
+    if (MODE != 1) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int p = 0; p < P; ++p) pt[(p * LP + l0 + q) * TRP + r] = pc[q][p];
+        for (int p = 0; p < P; ++p) pt[(p * LP + l0 + q) * TRP + r] = pc[q][p];
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) bred[rs * LP + l0 + q] = bs[q];
+  for (int q = 0; q < 4; ++q) bred[rs * LP + l0 + q] = MODE == 1 ? mx[q] : bs[q];
   __syncthreads();
+  if (MODE == 1) {  // the items' column maxima of diag(sc) G~, one atomic per column
+    if (threadIdx.x < LP) {
+      const int l = threadIdx.x;
+      double m = 0.0;
+      for (int q = 0; q < RG; ++q) m = fmax(m, bred[q * LP + l]);
+      if (m > 0.0) atomicMax(&c.st->cmax_h[l], (unsigned long long)__double_as_longlong(m));
+    }
+    return;
+  }
   w_t* w = static_cast<w_t*>(c.wh);
   const uint64_t wpiece = (uint64_t)LP * c.wh_pitch;
   for (int v = threadIdx.x; v < P * LP * kBatchEpiRows; v += 256) {
@@ -239,10 +294,37 @@ __global__ void __launch_bounds__(256) k_b_epi_g(BatchArgs a, const RowItem* __r
   }
   if (threadIdx.x < LP) {
     const int l = threadIdx.x;
-    double s = 0.0;
-    for (int q = 0; q < RG; ++q) s += bred[q * LP + l];
-    bpart[(uint64_t)blockIdx.x * LP + l] = s;
+    double sm = 0.0;
+    for (int q = 0; q < RG; ++q) sm += bred[q * LP + l];
+    bpart[(uint64_t)blockIdx.x * LP + l] = sm;
   }
+}
+
+// FP8 form: column maxima of |X[r][l]| over the items' rows into cmax_g, one
+// atomic per column and item.
+__global__ void __launch_bounds__(256) k_b_colmax(BatchArgs a, const RowItem* __restrict__ items, int srcsel) {
+  __shared__ double m[256];
+  const RowItem it = items[blockIdx.x];
+  const BatchCohort& c = a.coh[it.cohort];
+  if (c.st->state != 1) return;
+  const double* X = c.mat[srcsel];
+  const int l = threadIdx.x % LP, r0 = threadIdx.x / LP;
+  double mx = 0.0;
+  if (l < a.L)
+    for (uint32_t r = r0; r < it.nrows; r += 256 / LP) mx = fmax(mx, fabs(X[(it.row0 + r) * LP + l]));
+  m[threadIdx.x] = mx;
+  __syncthreads();
+  if (threadIdx.x < LP) {
+    for (int q = 1; q < 256 / LP; ++q) mx = fmax(mx, m[q * LP + l]);
+    if (mx > 0.0) atomicMax(&c.st->cmax_g[l], (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// Clears cmax_g or cmax_h of the running cohorts (one workgroup per cohort).
+__global__ void k_b_zero_cmax(BatchArgs a, int h_side) {
+  const BatchCohort& c = a.coh[blockIdx.x];
+  if (c.st->state != 1 || threadIdx.x >= LP) return;
+  (h_side ? c.st->cmax_h : c.st->cmax_g)[threadIdx.x] = 0ull;
 }
 
 // H = y / 4 + b over the item's rows, 4 adjacent elements per thread.
@@ -258,12 +340,18 @@ __global__ void __launch_bounds__(256) k_b_epi_h(BatchArgs a, const RowItem* __r
   double b[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) b[q] = c.bvec[l0 + q];
+  // FP8 form: the products are of diag(sc) G~ / scale_l: times the H-side scale
+  double m[4] = {0.25, 0.25, 0.25, 0.25};
+  if (a.form == (int)Form::F8) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = 0.25 * f8_col_scale(c.st->cmax_h[l0 + q]);
+  }
   for (int v = threadIdx.x; v < n4; v += 256) {
     const uint64_t t = it.row0 * LP + 4 * (uint64_t)v;
     double y[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint64_t ch = 0; ch < c.nch_h; ++ch) add4(part + ch * c.ychunk_h + t, y);
-    *reinterpret_cast<double2*>(h + t) = double2{0.25 * y[0] + b[0], 0.25 * y[1] + b[1]};
-    *reinterpret_cast<double2*>(h + t + 2) = double2{0.25 * y[2] + b[2], 0.25 * y[3] + b[3]};
+    *reinterpret_cast<double2*>(h + t) = double2{m[0] * y[0] + b[0], m[1] * y[1] + b[1]};
+    *reinterpret_cast<double2*>(h + t + 2) = double2{m[2] * y[2] + b[2], m[3] * y[3] + b[3]};
   }
 }
 
@@ -737,10 +825,17 @@ void launch_batch_make_pieces(const BatchArgs& a, const RowItem* items_n, uint32
     case Form::BF16: hipLaunchKernelGGL(k_b_make_pieces<Form::BF16>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::BF16X1: hipLaunchKernelGGL(k_b_make_pieces<Form::BF16X1>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::FP16: hipLaunchKernelGGL(k_b_make_pieces<Form::FP16>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
+    case Form::F8: hipLaunchKernelGGL(k_b_make_pieces<Form::F8>, dim3(nitems), dim3(256), 0, s, a, items_n, src, sg); break;
     case Form::I8: fail(kErrArgument, "batch: no int8 form");
-    case Form::F8: fail(kErrArgument, "batch: no fp8 form");
   }
   check_launch("batch_make_pieces");
+}
+
+void launch_batch_omega_colmax(const BatchArgs& a, const RowItem* items_n, uint32_t nitems, int src, hipStream_t s) {
+  if (!nitems) return;
+  hipLaunchKernelGGL(k_b_zero_cmax, dim3(nblocks(a.ncoh)), dim3(LP), 0, s, a, 0);
+  hipLaunchKernelGGL(k_b_colmax, dim3(nitems), dim3(256), 0, s, a, items_n, src);
+  check_launch("batch_omega_colmax");
 }
 
 void launch_batch_colsum(const BatchArgs& a, const RowItem* items_n, uint32_t nitems, double* partial,
@@ -764,14 +859,19 @@ void launch_batch_flip(const BatchArgs& a, const RowItem* items_n, uint32_t nite
 void launch_batch_epi_g(const BatchArgs& a, const RowItem* items_e, uint32_t nitems, double* partial,
                         hipStream_t s) {
   if (!nitems) return;
+  const dim3 g(nitems), b(256);
   switch ((Form)a.form) {
-    case Form::FP64: hipLaunchKernelGGL((k_b_epi_g<Form::FP64, double>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
-    case Form::FP32: hipLaunchKernelGGL((k_b_epi_g<Form::FP32, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
-    case Form::BF16: hipLaunchKernelGGL((k_b_epi_g<Form::BF16, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
-    case Form::BF16X1: hipLaunchKernelGGL((k_b_epi_g<Form::BF16X1, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
-    case Form::FP16: hipLaunchKernelGGL((k_b_epi_g<Form::FP16, float>), dim3(nitems), dim3(256), 0, s, a, items_e, partial); break;
+    case Form::FP64: hipLaunchKernelGGL((k_b_epi_g<Form::FP64, double, 0>), g, b, 0, s, a, items_e, partial); break;
+    case Form::FP32: hipLaunchKernelGGL((k_b_epi_g<Form::FP32, float, 0>), g, b, 0, s, a, items_e, partial); break;
+    case Form::BF16: hipLaunchKernelGGL((k_b_epi_g<Form::BF16, float, 0>), g, b, 0, s, a, items_e, partial); break;
+    case Form::BF16X1: hipLaunchKernelGGL((k_b_epi_g<Form::BF16X1, float, 0>), g, b, 0, s, a, items_e, partial); break;
+    case Form::FP16: hipLaunchKernelGGL((k_b_epi_g<Form::FP16, float, 0>), g, b, 0, s, a, items_e, partial); break;
+    case Form::F8:
+      hipLaunchKernelGGL(k_b_zero_cmax, dim3(nblocks(a.ncoh)), dim3(LP), 0, s, a, 1);
+      hipLaunchKernelGGL((k_b_epi_g<Form::F8, float, 1>), g, b, 0, s, a, items_e, partial);
+      hipLaunchKernelGGL((k_b_epi_g<Form::F8, float, 2>), g, b, 0, s, a, items_e, partial);
+      break;
     case Form::I8: fail(kErrArgument, "batch: no int8 form");
-    case Form::F8: fail(kErrArgument, "batch: no fp8 form");
   }
   hipLaunchKernelGGL(k_b_sum_items, dim3(nblocks(a.ncoh)), dim3(256), 0, s, a, partial, LP, (int)kListE,
                      (int)kDestBvec, 1);

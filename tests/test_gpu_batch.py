@@ -74,7 +74,7 @@ def test_batch_fp64_matches_single_path(ctx, reference):
         np.testing.assert_allclose(res["E"][c], res["S"][c] ** 2 / sp.nsnps)
 
 
-@pytest.mark.parametrize("form", ["bf16", "fp32"])
+@pytest.mark.parametrize("form", ["bf16", "fp32", "fp8"])
 def test_batch_reduced_precision_meets_bar(ctx, reference, form):
     say(f"batch {form} vs fp64 single path")
     ctx.batch_generate(SPECS)
@@ -87,10 +87,26 @@ def test_batch_reduced_precision_meets_bar(ctx, reference, form):
         assert corr.min() >= 0.9999
 
 
-def test_batch_graph_replays_match_eager(ctx):
-    say("batch graph replays vs eager")
+def test_batch_fp8_matches_single_path_fp8(ctx):
+    """The FP8 form batched: per-cohort power-of-two column scales of Omega
+    (before its split) and of diag(sc) G~ (between the G epilogue and the
+    split); every cohort equals the single path's FP8 run of its matrix up to
+    the summation order of the FP32 partials."""
+    say("batch fp8 vs single-path fp8")
+    ref = single_runs(ctx, SPECS, k=5, oversamples=5, maxp=5, tol=0.0, precision="fp8")
     ctx.batch_generate(SPECS)
-    p = api.default_params(svd=1, k=5, oversamples=5, maxp=4, tol=0.0, precision="bf16", wm_g=4, waves_g=8,
+    res = ctx.batch_run(api.default_params(svd=1, k=5, oversamples=5, maxp=5, tol=0.0, precision="fp8"))
+    for c, r in enumerate(ref):
+        assert res["status"][c] == 0
+        np.testing.assert_allclose(res["S"][c], r.S[:5], rtol=1e-6)
+        assert O.pc_correlations(r.U, ctx.batch_get_u(c)).min() > 1 - 1e-6
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp8"])
+def test_batch_graph_replays_match_eager(ctx, form):
+    say(f"batch graph replays vs eager {form}")
+    ctx.batch_generate(SPECS)
+    p = api.default_params(svd=1, k=5, oversamples=5, maxp=4, tol=0.0, precision=form, wm_g=4, waves_g=8,
                            wm_h=4, waves_h=8, kchunk_g=2048, kchunk_h=4096)
     eager = ctx.batch_run(p, use_graph=False)
     U0 = [ctx.batch_get_u(c) for c in range(len(SPECS))]
