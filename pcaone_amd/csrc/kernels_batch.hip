@@ -288,9 +288,26 @@ __global__ void __launch_bounds__(256) k_b_epi_g(BatchArgs a, const RowItem* __r
   }
   w_t* w = static_cast<w_t*>(c.wh);
   const uint64_t wpiece = (uint64_t)LP * c.wh_pitch;
-  for (int v = threadIdx.x; v < P * LP * kBatchEpiRows; v += 256) {
-    const int r = v % kBatchEpiRows, pl = v / kBatchEpiRows, p = pl / LP, l = pl - p * LP;
-    if (r < nr) w[p * wpiece + l * c.wh_pitch + it.row0 + r] = pt[pl * TRP + r];
+  if constexpr (sizeof(w_t) == 1) {
+    // four rows' bytes per 32-bit store (rows start on 4-byte boundaries: the
+    // pitch and row0 are multiples of 128)
+    constexpr int R4 = kBatchEpiRows / 4;
+    for (int v = threadIdx.x; v < P * LP * R4; v += 256) {
+      const int r = 4 * (v % R4), pl = v / R4, p = pl / LP, l = pl - p * LP;
+      if (r >= nr) continue;
+      const w_t* src = pt + pl * TRP + r;
+      w_t* dst = w + p * wpiece + l * c.wh_pitch + it.row0 + r;
+      if (r + 4 <= nr)
+        *reinterpret_cast<uint32_t*>(dst) = (uint32_t)src[0] | ((uint32_t)src[1] << 8) | ((uint32_t)src[2] << 16) |
+                                            ((uint32_t)src[3] << 24);
+      else
+        for (int q = 0; q < nr - r; ++q) dst[q] = src[q];
+    }
+  } else {
+    for (int v = threadIdx.x; v < P * LP * kBatchEpiRows; v += 256) {
+      const int r = v % kBatchEpiRows, pl = v / kBatchEpiRows, p = pl / LP, l = pl - p * LP;
+      if (r < nr) w[p * wpiece + l * c.wh_pitch + it.row0 + r] = pt[pl * TRP + r];
+    }
   }
   if (threadIdx.x < LP) {
     const int l = threadIdx.x;
