@@ -120,54 +120,72 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
   // A staging buffer is filled from registers: each thread loads its
   // 16-byte slots of the next tile before processing the current one and
   // writes them to LDS after it, so the loads' latency is covered by the
-  // processing.  Tile descriptors are likewise read a step ahead.  A source
-  // row's slot is split into (row, part) by a multiply with a 16-bit
-  // reciprocal, exact for the slot counts of a tile.
+  // processing.  Tile descriptors are likewise read a step ahead.  The
+  // buffer's regions sit at fixed slots (row offsets, indexes, weights,
+  // source rows; sized for kMissEMax entries), so each thread decodes its
+  // slots once: a region, a byte offset within the region's tile data and,
+  // for a source row slot, the row it belongs to.
   constexpr int kSlots = (int)(kMissLdsBuf / 16 / kMissThreads);  // slots per thread
-  const uint32_t rcp = (65536u + cpv - 1) / cpv;
+  constexpr uint32_t sB = kMissOffBytes / 16, sC = sB + miss_stored(kMissEMax) / 8,
+                     sD = sC + miss_stored(kMissEMax) / 4;
+  int reg_of[kSlots];      // 0 row offsets, 1 indexes, 2 weights, 3 source rows
+  uint32_t off_of[kSlots]; // byte offset within the region's data
+  uint32_t row_of[kSlots]; // source row (region 3), or the slot within its region
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) {
+    const uint32_t s = tid + i * kMissThreads;
+    if (s < sB) {
+      reg_of[i] = 0, row_of[i] = s, off_of[i] = s * 16;
+    } else if (s < sC) {
+      reg_of[i] = 1, row_of[i] = s - sB, off_of[i] = (s - sB) * 16;
+    } else if (s < sD) {
+      reg_of[i] = 2, row_of[i] = s - sC, off_of[i] = (s - sC) * 16;
+    } else {
+      const uint32_t d = s - sD, j = d / cpv, p = d - j * cpv;
+      reg_of[i] = 3, row_of[i] = j, off_of[i] = (j * (uint32_t)a.cpitch + p * 4) * 4;
+    }
+  }
   uint4 reg[kSlots];
   auto load = [&](uint32_t t, const MissTile& T) {
-    const uint32_t sA = kMissOffBytes / 16;
-    const uint32_t sB = sA + miss_stored(T.ne) / 8;
-    const uint32_t sC = sB + miss_stored(T.ne) / 4;
-    const uint64_t klo = max<uint64_t>(T.k0, a.k_lo), khi = min<uint64_t>(T.k1, a.k_hi);
-    const uint32_t sD0 = sC + (uint32_t)(klo - T.k0) * cpv, sD1 = sC + (uint32_t)(khi - T.k0) * cpv;
-    const unsigned char* pa = reinterpret_cast<const unsigned char*>(a.t.trow + (uint64_t)t * kMissRowStride);
-    const unsigned char* pb = reinterpret_cast<const unsigned char*>(a.t.eix + T.e0) - (uint64_t)sA * 16;
-    const unsigned char* pc = reinterpret_cast<const unsigned char*>(a.ew + T.e0) - (uint64_t)sB * 16;
-    // source row k of the tile is compact row k - src_k0 (>= 0 where staged)
-    const int64_t rbase = (int64_t)T.k0 - (int64_t)a.src_k0;
+    const uint32_t nb = miss_stored(T.ne) / 8, nc = miss_stored(T.ne) / 4;
+    const uint32_t jlo = (uint32_t)(max<uint64_t>(T.k0, a.k_lo) - T.k0);
+    const uint32_t jhi = (uint32_t)(min<uint64_t>(T.k1, a.k_hi) - T.k0);
+    const uint64_t pa = reinterpret_cast<uint64_t>(a.t.trow + (uint64_t)t * kMissRowStride);
+    const uint64_t pb = reinterpret_cast<uint64_t>(a.t.eix + T.e0);
+    const uint64_t pc = reinterpret_cast<uint64_t>(a.ew + T.e0);
+    // source row k of the tile is compact row k - src_k0 (>= 0 where staged);
+    // the base is formed in integers, only rows at or above src_k0 are read
+    const uint64_t pd = reinterpret_cast<uint64_t>(a.src) +
+                        (uint64_t)((int64_t)T.k0 - (int64_t)a.src_k0) * (uint64_t)a.cpitch * 4ull;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
-      const uint32_t s = tid + i * kMissThreads;
-      const bool live = s < sC || (s >= sD0 && s < sD1);
-      const unsigned char* src;
-      if (s < sC) {
-        src = (s < sA ? pa : (s < sB ? pb : pc)) + (uint64_t)s * 16;
-      } else {
-        const uint32_t d = s - sC, j = __umul24(d, rcp) >> 16, p = d - j * cpv;
-        const uint64_t row = live ? (uint64_t)(rbase + j) : 0ull;
-        src = reinterpret_cast<const unsigned char*>(a.src + row * a.cpitch + p * 4);
+      const int rg = reg_of[i];
+      const uint32_t r = row_of[i];
+      const bool live = rg == 0 || (rg == 1 && r < nb) || (rg == 2 && r < nc) || (rg == 3 && r >= jlo && r < jhi);
+      const uint64_t base = rg == 0 ? pa : (rg == 1 ? pb : (rg == 2 ? pc : pd));
+      using gptr = const __attribute__((address_space(1))) uint32_t*;  // global loads, not flat ones
+      uint4 v{0u, 0u, 0u, 0u};
+      if (live) {
+        const gptr g = reinterpret_cast<gptr>(base + off_of[i]);
+        v = uint4{g[0], g[1], g[2], g[3]};
       }
-      reg[i] = live ? *reinterpret_cast<const uint4*>(src) : uint4{0u, 0u, 0u, 0u};
+      reg[i] = v;
     }
-    return sD1;
   };
   // The local indexes are stored as the 16-byte slot offsets of their source
-  // rows (k cp / 4 < 65536 for the spans planned).
-  auto store = [&](unsigned char* buf, uint32_t nslots, uint32_t ne) {
-    const uint32_t sA = kMissOffBytes / 16, sB = sA + miss_stored(ne) / 8;
+  // rows (k cp / 4 < 65536 for the spans planned).  Slots the tile does not
+  // fill are written too (zeros); nothing reads them.
+  auto store = [&](unsigned char* buf) {
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
-      const uint32_t s = tid + i * kMissThreads;
       uint4 v = reg[i];
-      if (s >= sA && s < sB) {
+      if (reg_of[i] == 1) {
         uint32_t* h = &v.x;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           h[j] = (__umul24(h[j] & 0xffffu, cpv) & 0xffffu) | (__umul24(h[j] >> 16, cpv) << 16);
       }
-      if (s < nslots) *reinterpret_cast<uint4*>(buf + (uint64_t)s * 16) = v;
+      *reinterpret_cast<uint4*>(buf + (uint64_t)(tid + i * kMissThreads) * 16) = v;
     }
   };
 
@@ -186,9 +204,9 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
   auto process = [&](const unsigned char* buf, const MissTile& T, auto edge_tag) {
     constexpr bool EDGE = decltype(edge_tag)::value;
     const uint16_t* ro = reinterpret_cast<const uint16_t*>(buf);
-    const uint16_t* ix = reinterpret_cast<const uint16_t*>(buf + kMissOffBytes);
-    const float* w = reinterpret_cast<const float*>(buf + kMissOffBytes + miss_stored(T.ne) * 2);
-    const unsigned char* src = reinterpret_cast<const unsigned char*>(w + miss_stored(T.ne)) + cl * 16;
+    const uint16_t* ix = reinterpret_cast<const uint16_t*>(buf + sB * 16);
+    const float* w = reinterpret_cast<const float*>(buf + sC * 16);
+    const unsigned char* src = buf + sD * 16 + cl * 16;
     const uint32_t olo = EDGE ? (uint32_t)(max<uint64_t>(T.k0, a.k_lo) - T.k0) * cpv : 0u;
     const uint32_t ohi = EDGE ? (uint32_t)(min<uint64_t>(T.k1, a.k_hi) - T.k0) * cpv : 0u;
     constexpr int U = RPG >= 3 ? 4 : 8;  // three rows' sums leave registers for four entries a step
@@ -240,16 +258,18 @@ __global__ void __launch_bounds__(kMissThreads) k_missing_spmm_tiled(MissTiledSp
     unsigned char* buf1 = lds + kMissLdsBuf;
     MissTile Tc = a.t.tiles[t0];
     MissTile Tn = a.t.tiles[min(t0 + 1, t1 - 1)];
-    store(buf0, load(t0, Tc), Tc.ne);
+    load(t0, Tc);
+    store(buf0);
     __syncthreads();
     int cur = 0;
     for (uint32_t t = t0; t < t1; ++t) {
       const MissTile Tnn = a.t.tiles[min(t + 2, t1 - 1)];
-      const uint32_t nslots = t + 1 < t1 ? load(t + 1, Tn) : 0u;
+      const bool more = t + 1 < t1;
+      if (more) load(t + 1, Tn);
       if (Tc.k0 < a.k_lo || Tc.k1 > a.k_hi) process(cur ? buf1 : buf0, Tc, std::true_type{});
       else process(cur ? buf1 : buf0, Tc, std::false_type{});
       // the other buffer was last read before the previous barrier
-      store(cur ? buf0 : buf1, nslots, Tn.ne);
+      if (more) store(cur ? buf0 : buf1);
       __syncthreads();
       cur ^= 1;
       Tc = Tn;
