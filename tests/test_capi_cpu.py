@@ -33,6 +33,34 @@ def test_binding_covers_header():
     assert set(header_functions()) == set(_lib._SIGS)
 
 
+STRUCTS = [("pcaone_params", _lib.Params), ("pcaone_run_info", _lib.RunInfo),
+           ("pcaone_synth_spec", _lib.SynthSpec), ("pcaone_batch_info", _lib.BatchInfo)]
+
+
+def test_ctypes_structs_match_header_layout(tmp_path):
+    """Every ctypes structure has the size and field offsets that a C compiler
+    gives the header's struct (a caller's buffer of the wrong size would be
+    overrun by pcaone_params_default and the run-info writers)."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "pcaone_amd.h"', "int main(void) {"]
+    for cname, py in STRUCTS:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{HEADER.parent}", str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, key, val = line.split()
+        got[(cname, key)] = int(val)
+    for cname, py in STRUCTS:
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, f"{cname}.{f}"
+
+
 def test_params_default_match_reference():
     p = api.default_params()
     # src/Cmd.hpp defaults and src/Cmd.cpp:41-45 (-d defaults to 2)
