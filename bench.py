@@ -306,6 +306,12 @@ def config_line(args) -> int:
             # the two packed layouts of one block on the device, against both layouts of the whole matrix
             "device_genotype_bytes": int(2 * blk * bps),
             "resident_genotype_bytes": int(2 * spec.nsnps * bps),
+            # per run of each form: blocks loaded, the host wall time of the loads and their rate
+            # (every pass loads all blocks, spec.nsnps rows of bps bytes)
+            "block_loads": {f: int(r.info["stream_loads"]) for f, r in res.items()},
+            "load_s": {f: round(r.info["stream_load_ms"] / 1e3, 3) for f, r in res.items()},
+            "load_gb_per_s": {f: round(r.info["stream_loads"] / out["block_plan"]["nblocks"] * spec.nsnps * bps
+                                       / max(r.info["stream_load_ms"], 1e-9) / 1e6, 1) for f, r in res.items()},
             "what": "packed rows in host memory; every product loads its block (gather, DMA, interleave, "
                     "transpose, missing lists) into block-sized layouts"}
     ctx.time_kernels(True)

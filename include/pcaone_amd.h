@@ -116,6 +116,8 @@ typedef struct pcaone_run_info {
   uint64_t gram_second_passes;    /* QR of G with a second pass over G (ill conditioned)   */
   uint64_t epoch_rollbacks;       /* winSVD epochs rerun because an Omega update's device
                                      thin-Q met a near-singular Gram of H                   */
+  uint64_t stream_loads;          /* streamed genotypes: blocks moved host -> device        */
+  double stream_load_ms;          /* host wall time of those loads (gather, DMA, interleave) */
 } pcaone_run_info;
 
 /* Synthetic Balding-Nichols PLINK data (test and benchmark input; no

@@ -46,7 +46,8 @@ class RunInfo(C.Structure):
                 ("n_sv", C.c_int), ("seconds", C.c_double), ("g_ms", C.c_double), ("h_ms", C.c_double),
                 ("g_launches", C.c_uint64), ("h_launches", C.c_uint64),
                 ("omega_ms", C.c_double), ("householder_fallbacks", C.c_uint64),
-                ("gram_second_passes", C.c_uint64), ("epoch_rollbacks", C.c_uint64)]
+                ("gram_second_passes", C.c_uint64), ("epoch_rollbacks", C.c_uint64),
+                ("stream_loads", C.c_uint64), ("stream_load_ms", C.c_double)]
 
 
 _lib = None

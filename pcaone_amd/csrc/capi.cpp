@@ -149,6 +149,8 @@ void fill_info(pcaone_run_info* info, const RunInfo& r, const Context& c, int ns
   info->householder_fallbacks = c.householder_fallbacks;
   info->gram_second_passes = c.gram_second_passes;
   info->epoch_rollbacks = c.epoch_rollbacks;
+  info->stream_loads = c.stream_loads;
+  info->stream_load_ms = c.stream_load_ms;
 }
 
 }  // namespace

@@ -635,7 +635,8 @@ void Context::configure(int prec, int Lr, const GemmTuning& t) {
     PCAONE_HIP(hipMemsetAsync(lstatus_.get(), 0, sizeof(int), stream));
   }
   ktimes = KernelTimes{};
-  householder_fallbacks = gram_second_passes = epoch_rollbacks = 0;
+  householder_fallbacks = gram_second_passes = epoch_rollbacks = stream_loads = 0;
+  stream_load_ms = 0.0;
 }
 
 void Context::set_t(const la::Mat& T) {

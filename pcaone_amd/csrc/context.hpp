@@ -221,7 +221,7 @@ class Context {
   void stream_setup(uint32_t nsamples, uint64_t nsnps, uint64_t block_snps);
   void stream_close();
   void load_block(uint64_t j0, uint64_t j1);
-  void block_rows_to_device(uint64_t j0, uint64_t n);  // rows of the current order -> raw_ (device)
+  void block_rows_to_device(uint64_t j0, uint64_t n);  // rows of the current order -> xs_ (device)
   void build_lists(uint64_t rows, uint64_t ncodes_t);  // missing lists of the loaded layouts
   uint64_t xoff() const { return streamed() ? blk0_ : 0; }
   // data layout

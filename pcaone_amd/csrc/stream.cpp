@@ -8,7 +8,8 @@
 // whole matrix.  A block load is a parallel host gather of its rows (in the
 // current column order, so a permuted order reads the file's rows in that
 // order, as the reference reads its rewritten file) into pinned memory, the
-// DMA, the interleave, the transpose and the missing lists.
+// DMA and the interleave, chunk by chunk with the gather of one chunk beside
+// the DMA of the one before, then the transpose and the missing lists.
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -69,8 +70,6 @@ void Context::stream_setup(uint32_t nsamples, uint64_t nsnps, uint64_t block_snp
   for (uint64_t j0 = 0; j0 < M; j0 += sblock_) {
     const uint64_t n = std::min(sblock_, M - j0);
     block_rows_to_device(j0, n);
-    launch_fill_rows(xs_.get(), pitch_s_, 0, rows_s_, stream);
-    launch_interleave(raw_.get(), n, bps_, N, xs_.get(), pitch_s_, stream);
     launch_code_counts(xs_.get(), pitch_s_, n, N, cnt.get(), stream);
     PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.get(), 4 * n * 4, hipMemcpyDeviceToHost, stream));
     PCAONE_HIP(hipStreamSynchronize(stream));
@@ -147,23 +146,39 @@ void Context::stream_pgen(std::shared_ptr<const PgenSource> src, uint64_t block_
 }
 
 void Context::block_rows_to_device(uint64_t j0, uint64_t n) {
-  // rows j0..j0+n of the current column order (file rows perm[j]) -> raw_
+  // rows j0..j0+n of the current column order (file rows perm[j]) -> the
+  // block's SNP-major layout xs_.  Two staging slots (pinned host + raw
+  // device rows) of about 32 MB: the host gathers chunk c+1 into its pinned
+  // slot while the DMA and the interleave of chunk c run.
   const auto t0 = std::chrono::steady_clock::now();
-  const uint64_t bytes = n * bps_;
-  pinned_.resize(std::max<uint64_t>(bytes, 1));
-  raw_.resize(std::max<uint64_t>(bytes, 1));
-  unsigned char* host = pinned_.get();
+  launch_fill_rows(xs_.get(), pitch_s_, 0, rows_s_, stream);
+  const uint64_t chunk = std::max<uint64_t>(1, (32ull << 20) / std::max<uint64_t>(bps_, 1));
+  const uint64_t slot = std::max<uint64_t>(std::min(n, chunk), 1) * bps_;
+  pinned_.resize(2 * slot);
+  raw_.resize(2 * slot);
   const bool ident = perm_identity_;
-  if (pgen_) {
-    pgen_->decode_rows(j0, n, ident ? nullptr : perm.data(), host);
-  } else {
+  Event done[2];
+  for (uint64_t c = 0, r = 0; r < n; ++c, r += chunk) {
+    const int s = (int)(c & 1);
+    const uint64_t m = std::min(chunk, n - r), bytes = m * bps_;
+    if (c >= 2) PCAONE_HIP(hipEventSynchronize(done[s].get()));  // slot s's DMA and interleave are done
+    unsigned char* host = pinned_.get() + s * slot;
+    if (pgen_) {
+      pgen_->decode_rows(j0 + r, m, ident ? nullptr : perm.data(), host);
+    } else {
 #pragma omp parallel for schedule(static)
-    for (int64_t r = 0; r < (int64_t)n; ++r) {
-      const uint64_t src = ident ? j0 + (uint64_t)r : (uint64_t)perm[j0 + (uint64_t)r];
-      std::memcpy(host + (uint64_t)r * bps_, src_rows_ + src * bps_, bps_);
+      for (int64_t i = 0; i < (int64_t)m; ++i) {
+        const uint64_t j = j0 + r + (uint64_t)i;
+        const uint64_t src = ident ? j : (uint64_t)perm[j];
+        std::memcpy(host + (uint64_t)i * bps_, src_rows_ + src * bps_, bps_);
+      }
     }
+    PCAONE_HIP(hipMemcpyAsync(raw_.get() + s * slot, host, bytes, hipMemcpyHostToDevice, stream));
+    launch_interleave(raw_.get() + s * slot, m, bps_, N, xs_.get() + r * pitch_s_, pitch_s_, stream);
+    PCAONE_HIP(hipEventRecord(done[s].get(), stream));
   }
-  PCAONE_HIP(hipMemcpyAsync(raw_.get(), host, bytes, hipMemcpyHostToDevice, stream));
+  // the pinned slots are refilled by the next load: wait for their DMAs
+  PCAONE_HIP(hipStreamSynchronize(stream));
   ++stream_loads;
   stream_load_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -175,8 +190,6 @@ void Context::load_block(uint64_t j0, uint64_t j1) {
               "-SNP device block");
   const uint64_t n = j1 - j0;
   block_rows_to_device(j0, n);
-  launch_fill_rows(xs_.get(), pitch_s_, 0, rows_s_, stream);
-  launch_interleave(raw_.get(), n, bps_, N, xs_.get(), pitch_s_, stream);
   launch_transpose_codes(xs_.get(), pitch_s_, n, N, xt_.get(), pitch_t_, rows_t_, stream);
   blk0_ = j0;
   blk1_ = j1;

@@ -10,7 +10,7 @@ import pytest
 
 from conftest import GOLDEN, load_meta, params_from_args
 from oracle import pcaone_oracle as O
-from pcaone_amd import api
+from pcaone_amd import api, synth
 from pcaone_amd._lib import PcaoneError
 
 pytestmark = pytest.mark.gpu
@@ -73,6 +73,31 @@ def test_streamed_host_rows_reduced_precision(ctx, golden_inputs, case, form):
     corr = O.pc_correlations(ref.U, r.U)
     say(f"  eig rel err max {rel.max():.2e}  min PC corr {corr.min():.8f}")
     assert rel.max() <= 1e-4 and corr.min() >= 0.9999
+
+
+@pytest.mark.parametrize("svd,bands,memory,nblocks", [(1, 64, 3, 3), (2, 4, 2, 4)])
+def test_streamed_blocks_of_several_chunks_match_resident(ctx, svd, bands, memory, nblocks):
+    """A block load stages its rows through two pinned slots of about 32 MB,
+    so blocks of 40,000 samples (10,000 bytes a row, 3,355 rows a slot) load
+    in two to four chunks, the last one ragged; the winSVD case gathers its
+    rows in the shuffled order across the chunk bounds.  Streamed runs equal
+    the resident run of the same rows in FP64."""
+    spec = synth.Spec(seed=23, nsamples=40000, nsnps=24000, miss_rate=0.02)
+    packed = synth.packed(spec)
+    p = api.default_params(svd=svd, k=5, maxp=3, bands=bands, memory=memory, precision="fp64")
+    plan = O.block_plan(p, spec.nsamples, spec.nsnps)
+    say(f"several chunks: svd {svd} bands {bands} -m {memory}: {plan.nblocks} blocks of {plan.blocksize}")
+    assert plan.nblocks == nblocks and plan.blocksize * spec.bytes_per_snp > 32 << 20
+    ctx.stream_host(packed, spec.nsamples, plan.blocksize)
+    rs = ctx.run(p)
+    assert rs.info["stream_loads"] >= 2 * nblocks  # every pass loads every block
+    ctx.upload_packed(packed, spec.nsamples)
+    rr = ctx.run(p)
+    assert rr.info["stream_loads"] == 0
+    rel = np.abs(rs.E - rr.E) / rr.E
+    corr = O.pc_correlations(rr.U, rs.U)
+    say(f"  eig rel diff max {rel.max():.2e}  min PC corr {corr.min():.12f}")
+    assert rel.max() <= 1e-9 and corr.min() >= 1 - 1e-9
 
 
 def test_streamed_argument_errors(ctx, golden_inputs):
