@@ -38,6 +38,10 @@ Context::~Context() {
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
     if (gram_ev_) (void)hipEventDestroy(gram_ev_);
+    if (copy_stream_) {
+      (void)hipStreamSynchronize(copy_stream_);
+      (void)hipStreamDestroy(copy_stream_);
+    }
     (void)hipStreamDestroy(stream);
   }
 }

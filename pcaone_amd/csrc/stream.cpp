@@ -149,19 +149,24 @@ void Context::block_rows_to_device(uint64_t j0, uint64_t n) {
   // rows j0..j0+n of the current column order (file rows perm[j]) -> the
   // block's SNP-major layout xs_.  Two staging slots (pinned host + raw
   // device rows) of about 32 MB: the host gathers chunk c+1 into its pinned
-  // slot while the DMA and the interleave of chunk c run.
+  // slot while the DMA of chunk c runs on the copy stream, and the
+  // interleave of chunk c on the context stream runs beside the DMA of
+  // chunk c+1.  A slot is refilled once its DMA has read the pinned rows
+  // (host wait) and its interleave has read the raw rows (copy stream wait).
   const auto t0 = std::chrono::steady_clock::now();
+  if (!copy_stream_) PCAONE_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
   launch_fill_rows(xs_.get(), pitch_s_, 0, rows_s_, stream);
   const uint64_t chunk = std::max<uint64_t>(1, (32ull << 20) / std::max<uint64_t>(bps_, 1));
   const uint64_t slot = std::max<uint64_t>(std::min(n, chunk), 1) * bps_;
   pinned_.resize(2 * slot);
   raw_.resize(2 * slot);
   const bool ident = perm_identity_;
-  Event done[2];
+  // the raw slots are free here: each load ends with the interleaves done
+  Event dma[2], done[2];
   for (uint64_t c = 0, r = 0; r < n; ++c, r += chunk) {
     const int s = (int)(c & 1);
     const uint64_t m = std::min(chunk, n - r), bytes = m * bps_;
-    if (c >= 2) PCAONE_HIP(hipEventSynchronize(done[s].get()));  // slot s's DMA and interleave are done
+    if (c >= 2) PCAONE_HIP(hipEventSynchronize(dma[s].get()));  // chunk c-2's DMA has read pinned slot s
     unsigned char* host = pinned_.get() + s * slot;
     if (pgen_) {
       pgen_->decode_rows(j0 + r, m, ident ? nullptr : perm.data(), host);
@@ -173,11 +178,15 @@ void Context::block_rows_to_device(uint64_t j0, uint64_t n) {
         std::memcpy(host + (uint64_t)i * bps_, src_rows_ + src * bps_, bps_);
       }
     }
-    PCAONE_HIP(hipMemcpyAsync(raw_.get() + s * slot, host, bytes, hipMemcpyHostToDevice, stream));
+    if (c >= 2) PCAONE_HIP(hipStreamWaitEvent(copy_stream_, done[s].get(), 0));  // chunk c-2's interleave
+    PCAONE_HIP(hipMemcpyAsync(raw_.get() + s * slot, host, bytes, hipMemcpyHostToDevice, copy_stream_));
+    PCAONE_HIP(hipEventRecord(dma[s].get(), copy_stream_));
+    PCAONE_HIP(hipStreamWaitEvent(stream, dma[s].get(), 0));
     launch_interleave(raw_.get() + s * slot, m, bps_, N, xs_.get() + r * pitch_s_, pitch_s_, stream);
     PCAONE_HIP(hipEventRecord(done[s].get(), stream));
   }
-  // the pinned slots are refilled by the next load: wait for their DMAs
+  // every interleave waited for its DMA: after this the pinned and raw
+  // slots are free for the next load
   PCAONE_HIP(hipStreamSynchronize(stream));
   ++stream_loads;
   stream_load_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
