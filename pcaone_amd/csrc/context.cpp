@@ -74,16 +74,19 @@ void Context::upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n) {
   require(j0 + n <= M, kErrArgument, "upload beyond the reserved SNP count");
   if (!n) return;
   // Two staging slots (pinned host + raw device rows): the host copies chunk
-  // c+1 into its pinned slot while the DMA and the interleave of chunk c run.
+  // c+1 into its pinned slot while the DMA of chunk c runs on the copy
+  // stream, and the interleave of chunk c runs beside the DMA of chunk c+1
+  // (the host waits for a slot's DMA, the copy stream for its interleave).
+  if (!copy_stream_) PCAONE_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
   const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / std::max<uint64_t>(bps_, 1));
   const uint64_t slot = std::min(n, chunk) * bps_;
   raw_.resize(2 * slot);
   pinned_.resize(2 * slot);
-  Event done[2];
+  Event dma[2], done[2];
   for (uint64_t c = 0, r = 0; r < n; ++c, r += chunk) {
     const int s = (int)(c & 1);
     const uint64_t m = std::min(chunk, n - r), bytes = m * bps_;
-    if (c >= 2) PCAONE_HIP(hipEventSynchronize(done[s].get()));
+    if (c >= 2) PCAONE_HIP(hipEventSynchronize(dma[s].get()));
     unsigned char* host = pinned_.get() + s * slot;
     const uint8_t* src = rows + r * bps_;
     const int64_t pieces = 16;
@@ -92,11 +95,14 @@ void Context::upload_rows(const uint8_t* rows, uint64_t j0, uint64_t n) {
       const uint64_t b0 = bytes * q / pieces, b1 = bytes * (q + 1) / pieces;
       std::memcpy(host + b0, src + b0, b1 - b0);
     }
-    PCAONE_HIP(hipMemcpyAsync(raw_.get() + s * slot, host, bytes, hipMemcpyHostToDevice, stream));
+    if (c >= 2) PCAONE_HIP(hipStreamWaitEvent(copy_stream_, done[s].get(), 0));
+    PCAONE_HIP(hipMemcpyAsync(raw_.get() + s * slot, host, bytes, hipMemcpyHostToDevice, copy_stream_));
+    PCAONE_HIP(hipEventRecord(dma[s].get(), copy_stream_));
+    PCAONE_HIP(hipStreamWaitEvent(stream, dma[s].get(), 0));
     launch_interleave(raw_.get() + s * slot, m, bps_, N, xs_.get() + (j0 + r) * pitch_s_, pitch_s_, stream);
     PCAONE_HIP(hipEventRecord(done[s].get(), stream));
   }
-  PCAONE_HIP(hipStreamSynchronize(stream));
+  PCAONE_HIP(hipStreamSynchronize(stream));  // every interleave waited for its DMA
 }
 
 void synth_rows(const synth::Spec& spec, uint32_t* xs, uint64_t pitch, hipStream_t stream) {

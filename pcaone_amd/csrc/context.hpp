@@ -211,7 +211,7 @@ class Context {
   // streamed genotypes
   const uint8_t* src_rows_ = nullptr;  // packed rows in file order (host)
   void* map_ = nullptr;                // mapped .bed file (stream_bed)
-  hipStream_t copy_stream_ = nullptr;  // block DMAs beside the interleaves (created on first load)
+  hipStream_t copy_stream_ = nullptr;  // upload and block DMAs beside the interleaves (created on first use)
   std::shared_ptr<const PgenSource> pgen_;  // .pgen records decoded per block (stream_pgen)
   size_t map_len_ = 0;
   uint64_t cus_ = 0;                   // compute units of the device (split-K planning)
